@@ -1,0 +1,125 @@
+"""GPU path vs the CPU oracle on the same inputs (run on an MI355X).
+
+Bars (from the path's specification):
+  neighbor list and active-bond set  - bit-exact, canonical order
+  energy                             - 1e-9 relative
+  forces                             - 1e-6 kcal/mol/A max-abs
+  charges                            - 1e-7 e at QEq tolerance 1e-10 (tests use 1e-12)
+  per-atom bond properties           - 1e-8; bond counts exact
+"""
+
+import numpy as np
+import pytest
+
+from oracle import reaxff as orc
+from tests import cases
+from xreac_amd import Calculator, ForceField, workloads
+
+pytestmark = pytest.mark.gpu
+
+E_REL = 1e-9
+F_ABS = 1e-6
+Q_ABS = 1e-7
+
+
+def _systems():
+    out = {}
+    for name, (s, x) in cases.water_cases().items():
+        out["water_" + name] = ("ffield.reax.HO.2015", s, x, None, None, 0.0)
+    for name, (s, x) in cases.ZNO.items():
+        out["zno_" + name] = ("ffield.reax.ZnOH.2010", s, x, None, None, 0.0)
+    for name, (s, x) in cases.carbon_cases().items():
+        out["cho_" + name] = ("ffield.reax.CHO.2008", s, x, None, None, 0.0)
+    out["periodic_carbon_chain"] = (*cases.periodic_carbon_chain(), 0.0)
+    for name, (ffn, s, x, cell, pbc) in cases.small_cell_cases().items():
+        out["small_" + name] = (ffn, s, x, cell, pbc, 0.0)
+    for name, (s, x, cell, pbc) in cases.periodic_cases(include_bulk=True).items():
+        out["periodic_" + name] = ("ffield.reax.HO.2015", s, x, cell, pbc, 0.0)
+    for name, (s, x, cell, pbc, q) in cases.charged_cases().items():
+        out["charged_" + name] = ("ffield.reax.HO.2015", s, x, cell, pbc, q)
+    s, x = workloads.water_cluster()
+    out["water_cluster_192"] = ("ffield.reax.HO.2015", s, x, None, None, 0.0)
+    s, x, cell = workloads.cho_mixture(counts=(("CH4", 12), ("O2", 12), ("H2O", 6), ("CH3OH", 4)), length=14.0, seed=7)
+    out["cho_mixture_small"] = ("ffield.reax.CHO.2008", s, x, cell, [True] * 3, 0.0)
+    return out
+
+
+SYSTEMS = _systems()
+_CALCS = {}
+
+
+def _calc(ffname):
+    if ffname not in _CALCS:
+        _CALCS[ffname] = Calculator(ForceField.bundled(ffname), qeq_tolerance=1e-12, max_expanded_atoms=10**9)
+    return _CALCS[ffname]
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_parity(name):
+    ffname, s, x, cell, pbc, q = SYSTEMS[name]
+    print(f"[parity] {name}: n={len(s)}", flush=True)
+    calc = _calc(ffname)
+    gpu = calc.evaluate(s, x, cell=cell, pbc=pbc, total_charge=q)
+    ref = orc.evaluate(calc.force_field, s, x, cell=cell, pbc=pbc, total_charge=q)
+    model = ref["model"]
+    # neighbor list: identical canonical (i, j, S)
+    eng = calc.engine
+    m = len(model.i)
+    row_ptr, j, S = eng.neighbors(len(s), m)
+    np.testing.assert_array_equal(row_ptr, model.edges.row_ptr)
+    np.testing.assert_array_equal(j, model.j)
+    np.testing.assert_array_equal(S, model.edges.shifts)
+    # active bonds: the candidate edges whose raw bond order passes the cutoff
+    active = model._bond_candidates[model._bond_mask]
+    bi, bj, bS, bo = eng.bonds(len(active))
+    np.testing.assert_array_equal(bi, model.i[active])
+    np.testing.assert_array_equal(bj, model.j[active])
+    np.testing.assert_array_equal(bS, model.edges.shifts[active])
+    e_ref = ref["energy"]
+    assert abs(gpu.energy - e_ref) <= E_REL * max(1.0, abs(e_ref)), (gpu.energy, e_ref)
+    for k, v in ref["components"].items():
+        assert abs(gpu.components[k] - v) <= E_REL * max(1.0, abs(e_ref)), (k, gpu.components[k], v)
+    np.testing.assert_allclose(gpu.forces, ref["forces"], rtol=0, atol=F_ABS)
+    np.testing.assert_allclose(gpu.charges, ref["charges"], rtol=0, atol=Q_ABS)
+    np.testing.assert_allclose(gpu.total_bond_orders, ref["total_bond_orders"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(gpu.lone_pairs, ref["lone_pairs"], rtol=0, atol=1e-8)
+    np.testing.assert_array_equal(gpu.bond_counts, ref["bond_counts"])
+    np.testing.assert_allclose(gpu.bond_orders, ref["bond_orders"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(gpu.dipole, ref["dipole"], rtol=0, atol=1e-7)
+    assert abs(gpu.charges.sum() - q) < 1e-10
+
+
+def test_golden_monomer_on_gpu():
+    calc = _calc("ffield.reax.HO.2015")
+    r = calc.evaluate(["O", "H", "H"], [[0, 0, 0], [0.97, 0, 0], [-0.243, 0.94, 0]])
+    assert abs(r.energy - (-248.34420920633704)) < 1e-8
+    assert abs(r.charges.sum()) < 1e-12
+    ion = calc.evaluate(["O", "H"], [[0, 0, 0], [0.97, 0, 0]], total_charge=-1)
+    assert abs(ion.charges.sum() + 1) < 1e-12
+
+
+def test_golden_small_water_hbond_on_gpu():
+    _, s, x, cell, pbc = cases.small_cell_cases()["water_4A"]
+    r = _calc("ffield.reax.HO.2015").evaluate(s, x, cell=cell, pbc=pbc)
+    assert r.components["hydrogen_bond"] == pytest.approx(-0.3700520183, abs=1e-9)
+
+
+def test_errors_on_gpu():
+    water = _calc("ffield.reax.HO.2015")
+    with pytest.raises(ValueError, match="Coincident"):
+        water.evaluate(["O", "H"], [[0.0, 0, 0], [12.0, 0, 0]], cell=[12.0] * 3)
+    with pytest.raises(ValueError, match="absent from the force field"):
+        water.evaluate(["Zn"], [[0, 0, 0]])
+    cho = _calc("ffield.reax.CHO.2008")
+    with pytest.raises(ValueError, match="Collinear atoms in an active torsion"):
+        cho.evaluate(["C"] * 4, [[0, 0, 0], [1.5, 0, 0], [3, 0, 0], [4.5, 0.3, 0]])
+
+
+def test_deterministic_repeat():
+    s, x = workloads.water_cluster()
+    calc = _calc("ffield.reax.HO.2015")
+    a = calc.evaluate(s, x)
+    b = calc.evaluate(s, x)
+    assert a.energy == b.energy
+    np.testing.assert_array_equal(a.forces, b.forces)
+    np.testing.assert_array_equal(a.charges, b.charges)

@@ -1,0 +1,111 @@
+"""ctypes binding of the HIP library ``libxreac_hip.so`` (C ABI in include/xreac.h).
+
+The library is built in-tree by ``xreac_amd.build``.  There is no fallback: if
+the library is missing or fails to load, every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libxreac_hip.so")
+
+XREAC_OK, XREAC_INVALID, XREAC_DEVICE, XREAC_NOMEM = 0, 1, 2, 3
+NCOMP = 14
+
+EXPORTED = (
+    "xreac_ff_create",
+    "xreac_ff_destroy",
+    "xreac_ctx_create",
+    "xreac_ctx_destroy",
+    "xreac_error_message",
+    "xreac_evaluate",
+    "xreac_copy_neighbors",
+    "xreac_copy_bonds",
+)
+
+
+class FFTables(C.Structure):
+    _fields_ = [
+        ("ntypes", C.c_int32),
+        ("vdw_type", C.c_int32),
+        ("general", C.c_void_p),
+        ("atom", C.c_void_p),
+        ("pair", C.c_void_p),
+        ("angle_offsets", C.c_void_p),
+        ("angle_rows", C.c_void_p),
+        ("torsion", C.c_void_p),
+        ("torsion_on", C.c_void_p),
+        ("hbond", C.c_void_p),
+        ("hbond_on", C.c_void_p),
+    ]
+
+
+class System(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("positions", C.c_void_p),
+        ("types", C.c_void_p),
+        ("cell", C.c_void_p),
+        ("pbc", C.c_int32 * 3),
+        ("total_charge", C.c_double),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("qeq_tolerance", C.c_double),
+        ("qeq_max_iterations", C.c_int32),
+        ("lammps_angles", C.c_int32),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("components", C.c_double * NCOMP),
+        ("forces", C.c_void_p),
+        ("charges", C.c_void_p),
+        ("total_bond_orders", C.c_void_p),
+        ("lone_pairs", C.c_void_p),
+        ("bond_counts", C.c_void_p),
+        ("num_edges", C.c_int64),
+        ("num_bonds", C.c_int64),
+        ("qeq_iterations", C.c_int32),
+    ]
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def load():
+    """Load and type the library once; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc, gfx950)")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.xreac_ff_create.argtypes = [C.POINTER(FFTables), C.POINTER(vp)]
+    lib.xreac_ff_create.restype = C.c_int
+    lib.xreac_ff_destroy.argtypes = [vp]
+    lib.xreac_ff_destroy.restype = None
+    lib.xreac_ctx_create.argtypes = [C.c_int, vp, C.POINTER(vp)]
+    lib.xreac_ctx_create.restype = C.c_int
+    lib.xreac_ctx_destroy.argtypes = [vp]
+    lib.xreac_ctx_destroy.restype = None
+    lib.xreac_error_message.argtypes = [vp]
+    lib.xreac_error_message.restype = C.c_char_p
+    lib.xreac_evaluate.argtypes = [vp, vp, C.POINTER(System), C.POINTER(Options), C.POINTER(Result)]
+    lib.xreac_evaluate.restype = C.c_int
+    lib.xreac_copy_neighbors.argtypes = [vp, vp, vp, vp, C.c_int64]
+    lib.xreac_copy_neighbors.restype = C.c_int
+    lib.xreac_copy_bonds.argtypes = [vp, vp, vp, vp, vp, C.c_int64]
+    lib.xreac_copy_bonds.restype = C.c_int
+    _LIB = lib
+    return lib

@@ -1,0 +1,591 @@
+// C ABI and host orchestration of one evaluation (include/xreac.h).
+//
+// Pipeline on the context's stream (host synchronisation points marked *):
+//   bounds*  (only with nonperiodic directions)  -> bin, scatter
+//   neighbor count -> scan*  (M)  -> neighbor fill (canonical rows)
+//   edge pass (H values, active-bond flags) -> scan*  (B) -> bond compaction
+//   QEq CG (two RHS)*  -> nonbonded energy/forces
+//   bond orders, lone pair, over/under, special terms
+//   angles, torsions, hydrogen bonds   (forward + adjoint deposits)
+//   bond-order adjoint -> forces -> energy reduction*
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/xreac.h"
+#include "launch.h"
+#include "qeq.h"
+#include "state.h"
+
+static_assert(XREAC_ATOM_PARAMS == AP_COUNT, "atom table layout");
+static_assert(XREAC_PAIR_PARAMS == PP_COUNT, "pair table layout");
+static_assert(XREAC_COMPONENTS == NCOMP, "component count");
+
+struct xreac_ff {
+  FFDev dev{};
+  std::vector<void*> allocations;
+  double cutoff = 0.0;
+};
+
+namespace {
+
+// Grow-only device buffer.
+struct Buffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= bytes) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    const size_t grow = need + need / 4 + 256;
+    hipError_t e = hipMalloc(&ptr, grow);
+    if (e == hipSuccess) bytes = grow;
+    return e;
+  }
+  ~Buffer() {
+    if (ptr) (void)hipFree(ptr);
+  }
+};
+
+// Sequential carving of one buffer into aligned typed arrays.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+template <class T>
+size_t carve_size(size_t& off, size_t count) {
+  off = (off + 255) & ~size_t(255);
+  off += count * sizeof(T);
+  return off;
+}
+
+}  // namespace
+
+struct xreac_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string message;
+  Buffer atom_buf, bin_buf, edge_buf, bond_buf;
+  unsigned* err = nullptr;       // device
+  unsigned* err_host = nullptr;  // pinned
+  double* comps = nullptr;       // device [NCOMP]
+  // last evaluation
+  int n = 0;
+  int64_t m = 0, nb = 0;
+  int* row_ptr = nullptr;
+  int* nbr_j = nullptr;
+  int* nbr_s = nullptr;
+  int* bond_ptr = nullptr;
+  int* bond_i = nullptr;
+  int* bond_j = nullptr;
+  int* bond_code = nullptr;
+  double* bond_BO = nullptr;
+};
+
+namespace {
+
+constexpr int kBondScalars = 27;  // per-bond double arrays carved below
+constexpr int kAtomScalars = 21;  // per-atom double arrays carved below
+
+int fail(xreac_ctx* c, int code, const std::string& msg) {
+  c->message = msg;
+  return code;
+}
+
+int hip_fail(xreac_ctx* c, hipError_t e, const char* where) {
+  c->message = std::string("HIP error in ") + where + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? XREAC_NOMEM : XREAC_DEVICE;
+}
+
+#define HIPCHK(expr, where)                          \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, where); \
+  } while (0)
+
+std::string error_text(unsigned bits) {
+  if (bits & XR_ERR_COINCIDENT) return "Coincident atoms are not supported";
+  if (bits & XR_ERR_SHIFT_RANGE)
+    return "Positions lie more than 511 periodic images from the cell; wrap them before evaluating";
+  if (bits & XR_ERR_ROW_TOO_LONG) return "More than 2048 neighbors within the cutoff for one atom";
+  if (bits & XR_ERR_BOC3) return "Invalid p_boc3 mixing for an active bond-order correction";
+  if (bits & XR_ERR_BOC4) return "Invalid p_boc4 mixing for an active bond-order correction";
+  if (bits & XR_ERR_BOC5) return "Invalid p_boc5 mixing for an active bond-order correction";
+  if (bits & XR_ERR_COLLINEAR)
+    return "Collinear atoms in an active torsion: the dihedral derivative is undefined; perturb the geometry";
+  if (bits & XR_ERR_REVERSE) return "Internal error: bond without a reverse edge";
+  return "Non-finite energy, charges, or forces; check the geometry";
+}
+
+void invert3(const double* a, double* inv) {
+  const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                     a[2] * (a[3] * a[7] - a[4] * a[6]);
+  inv[0] = (a[4] * a[8] - a[5] * a[7]) / det;
+  inv[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+  inv[3] = (a[5] * a[6] - a[3] * a[8]) / det;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+  inv[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+  inv[6] = (a[3] * a[7] - a[4] * a[6]) / det;
+  inv[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+template <class T>
+hipError_t upload(xreac_ff* ff, const T* src, size_t count, const T** dst) {
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) return e;
+  ff->allocations.push_back(p);
+  if (src) {
+    e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+  } else {
+    e = hipMemset(p, 0, count * sizeof(T));
+  }
+  *dst = static_cast<const T*>(p);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xreac_ff_create(const xreac_ff_tables* t, xreac_ff** out) {
+  if (!t || !out || t->ntypes <= 0 || !t->general || !t->atom || !t->pair || !t->angle_offsets) return XREAC_INVALID;
+  auto* ff = new xreac_ff();
+  const size_t nt = (size_t)t->ntypes;
+  ff->dev.nt = t->ntypes;
+  ff->dev.vdw_type = t->vdw_type;
+  ff->cutoff = t->general[12];
+  hipError_t e = hipSuccess;
+  const size_t nang = (size_t)t->angle_offsets[nt * nt * nt];
+  if (e == hipSuccess) e = upload(ff, t->general, NGEN, &ff->dev.gen);
+  if (e == hipSuccess) e = upload(ff, t->atom, nt * AP_COUNT, &ff->dev.atom);
+  if (e == hipSuccess) e = upload(ff, t->pair, nt * nt * PP_COUNT, &ff->dev.pair);
+  if (e == hipSuccess) e = upload(ff, t->angle_offsets, nt * nt * nt + 1, &ff->dev.ang_off);
+  if (e == hipSuccess) e = upload(ff, t->angle_rows, nang * ANG_W, &ff->dev.ang);
+  if (e == hipSuccess) e = upload(ff, t->torsion, nt * nt * nt * nt * TOR_W, &ff->dev.tor);
+  if (e == hipSuccess) e = upload(ff, t->torsion_on, nt * nt * nt * nt, &ff->dev.tor_on);
+  if (e == hipSuccess) e = upload(ff, t->hbond, nt * nt * nt * HB_W, &ff->dev.hb);
+  if (e == hipSuccess) e = upload(ff, t->hbond_on, nt * nt * nt, &ff->dev.hb_on);
+  if (e != hipSuccess) {
+    xreac_ff_destroy(ff);
+    return e == hipErrorOutOfMemory ? XREAC_NOMEM : XREAC_DEVICE;
+  }
+  *out = ff;
+  return XREAC_OK;
+}
+
+void xreac_ff_destroy(xreac_ff* ff) {
+  if (!ff) return;
+  for (void* p : ff->allocations) (void)hipFree(p);
+  delete ff;
+}
+
+int xreac_ctx_create(int device, void* stream, xreac_ctx** out) {
+  if (!out) return XREAC_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return XREAC_DEVICE;
+  auto* c = new xreac_ctx();
+  c->device = device;
+  c->stream = static_cast<hipStream_t>(stream);
+  if (hipMalloc(&c->err, 64 * sizeof(double)) != hipSuccess ||
+      hipHostMalloc(&c->err_host, 64 * sizeof(double)) != hipSuccess) {
+    xreac_ctx_destroy(c);
+    return XREAC_NOMEM;
+  }
+  c->comps = reinterpret_cast<double*>(c->err) + 8;
+  *out = c;
+  return XREAC_OK;
+}
+
+void xreac_ctx_destroy(xreac_ctx* c) {
+  if (!c) return;
+  if (c->err) (void)hipFree(c->err);
+  if (c->err_host) (void)hipHostFree(c->err_host);
+  delete c;
+}
+
+const char* xreac_error_message(const xreac_ctx* c) { return c ? c->message.c_str() : "null context"; }
+
+int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
+                   xreac_result* out) {
+  if (!ctx || !ffh || !sys || !opt || !out) return XREAC_INVALID;
+  ctx->message.clear();
+  const int n = sys->n;
+  if (n <= 0 || !sys->positions || !sys->types || !out->forces || !out->charges)
+    return fail(ctx, XREAC_INVALID, "A nonempty system with device positions, types, forces and charges is required");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t s = ctx->stream;
+  const FFDev& ff = ffh->dev;
+  const double rc = ffh->cutoff;
+  const double bond_cut = std::min(5.0, rc);
+
+  // ---------------------------------------------------------------- cell
+  CellDev cell{};
+  const bool periodic = sys->cell && (sys->pbc[0] || sys->pbc[1] || sys->pbc[2]);
+  if (periodic) {
+    std::memcpy(cell.L, sys->cell, sizeof(cell.L));
+  } else {
+    for (int k = 0; k < 9; ++k) cell.L[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  }
+  invert3(cell.L, cell.inv);
+  for (int k = 0; k < 3; ++k) cell.pbc[k] = periodic ? (sys->pbc[k] != 0) : 0;
+  cell.periodic = periodic ? 1 : 0;
+  double height[3];
+  for (int k = 0; k < 3; ++k) {
+    const double c0 = cell.inv[k], c1 = cell.inv[3 + k], c2 = cell.inv[6 + k];
+    height[k] = 1.0 / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  }
+
+  // ------------------------------------------------------ atom workspace
+  const int cap = (n + 3) / 4 + 1;  // energy partial slots per channel
+  size_t off = 0;
+  const size_t scan_ints = xr_scan_scratch_ints(std::max(n, 1 << 16));
+  carve_size<int>(off, 3 * (size_t)n);   // home
+  carve_size<int>(off, n);               // bin
+  carve_size<int>(off, n);               // counts
+  carve_size<int>(off, n + 1);           // row_ptr
+  carve_size<int>(off, n);               // bond_count
+  carve_size<int>(off, n + 1);           // bond_ptr
+  carve_size<int>(off, scan_ints);       // scan scratch
+  carve_size<int>(off, n);               // binned
+  carve_size<double2>(off, 6 * (size_t)n);  // CG vectors
+  carve_size<double>(off, n);            // inv_diag
+  carve_size<double>(off, xr_cg_partials(n));
+  carve_size<int>(off, 4);               // cg state
+  carve_size<double>(off, 2);            // mu
+  carve_size<double>(off, (size_t)kAtomScalars * n);  // atom scalars
+  carve_size<int>(off, n);               // bond counts (>0.3)
+  carve_size<double>(off, 6 * (size_t)n);   // F_nb, F_hb
+  carve_size<double>(off, (size_t)NCH * cap);
+  HIPCHK(ctx->atom_buf.ensure(off + 4096), "atom workspace");
+  Carver cv{static_cast<char*>(ctx->atom_buf.ptr)};
+  int* home = cv.take<int>(3 * (size_t)n);
+  int* bin = cv.take<int>(n);
+  int* counts = cv.take<int>(n);
+  int* row_ptr = cv.take<int>(n + 1);
+  int* bond_count = cv.take<int>(n);
+  int* bond_ptr = cv.take<int>(n + 1);
+  int* scan_tmp = cv.take<int>(scan_ints);
+  int* binned = cv.take<int>(n);
+  double2* cgv = cv.take<double2>(6 * (size_t)n);
+  double* inv_diag = cv.take<double>(n);
+  double* cg_part = cv.take<double>(xr_cg_partials(n));
+  int* cg_state = cv.take<int>(4);
+  double* mu = cv.take<double>(2);
+  double* scal = cv.take<double>((size_t)kAtomScalars * n);
+  double* const scal_end = scal + (size_t)kAtomScalars * n;
+  int* bcount03 = cv.take<int>(n);
+  double* fbuf = cv.take<double>(6 * (size_t)n);
+  double* epart = cv.take<double>((size_t)NCH * cap);
+
+  Atoms at{};
+  at.n = n;
+  at.x = sys->positions;
+  at.type = sys->types;
+  double* sp = scal;
+  auto nxt = [&]() {
+    double* p = sp;
+    sp += n;
+    return p;
+  };
+  at.totp = nxt();
+  at.total = nxt();
+  at.d = nxt();
+  at.db = nxt();
+  at.nlp = nxt();
+  at.vlpex = nxt();
+  at.dlp = nxt();
+  at.dlpt = nxt();
+  at.sbo = nxt();
+  at.sbo2 = nxt();
+  at.prod = nxt();
+  at.ov2 = nxt();
+  at.A_total = nxt();
+  at.A_d = nxt();
+  at.A_db = nxt();
+  at.A_nlp = nxt();
+  at.A_dlpt = nxt();
+  at.A_dlp = nxt();
+  at.A_sbo2 = nxt();
+  at.a_ov2 = nxt();
+  at.a_totp = nxt();
+  if (sp > scal_end) return fail(ctx, XREAC_INVALID, "internal error: per-atom workspace layout");
+  at.bond_count = bcount03;
+  at.F_nb = fbuf;
+  at.F_hb = fbuf + 3 * (size_t)n;
+  at.q = out->charges;
+  EnergyAcc acc{epart, cap};
+
+  HIPCHK(hipMemsetAsync(ctx->err, 0, sizeof(unsigned), s), "memset");
+
+  // -------------------------------------------------------- binning grid
+  BinGeom g{};
+  const double rcb = rc * (1 + 1e-8) + 1e-12;
+  bool need_bounds = false;
+  for (int k = 0; k < 3; ++k) need_bounds |= !cell.pbc[k];
+  double bounds[6] = {0, 0, 0, 0, 0, 0};
+  if (need_bounds) {
+    double* dev_bounds = reinterpret_cast<double*>(ctx->err) + 32;
+    HIPCHK(xr_coord_bounds(n, sys->positions, cell, dev_bounds, s), "bounds");
+    HIPCHK(hipMemcpyAsync(bounds, dev_bounds, sizeof(bounds), hipMemcpyDeviceToHost, s), "bounds copy");
+    HIPCHK(hipStreamSynchronize(s), "bounds sync");
+    for (int k = 0; k < 6; ++k)
+      if (!std::isfinite(bounds[k]))
+        return fail(ctx, XREAC_INVALID, "positions must be a finite (N, 3) array in Angstrom");
+  }
+  long long nbins = 1;
+  for (int k = 0; k < 3; ++k) {
+    g.pbc[k] = cell.pbc[k];
+    if (cell.pbc[k]) {
+      int nbk = (int)std::floor(height[k] / rcb);
+      nbk = std::max(1, std::min(nbk, 64));
+      g.nb[k] = nbk;
+      g.w[k] = 1.0 / nbk;
+      g.lo[k] = 0.0;
+      const double h = height[k] / nbk;
+      g.reach[k] = h >= rcb ? 1 : (int)std::floor(rcb / h) + 1;
+    } else {
+      double w = rcb / height[k];
+      const double span = bounds[3 + k] - bounds[k];
+      long long nbk = (long long)std::floor(span / w) + 1;
+      if (nbk > 64) {
+        w = span / 63.0;
+        nbk = 64;
+      }
+      g.nb[k] = (int)nbk;
+      g.w[k] = w;
+      g.lo[k] = bounds[k];
+      g.reach[k] = 1;
+    }
+    nbins *= g.nb[k];
+  }
+  off = 0;
+  carve_size<int>(off, nbins + 1);
+  carve_size<int>(off, nbins + 1);
+  carve_size<int>(off, nbins + 1);
+  carve_size<int>(off, xr_scan_scratch_ints((int)nbins));
+  HIPCHK(ctx->bin_buf.ensure(off + 4096), "bin workspace");
+  Carver cb{static_cast<char*>(ctx->bin_buf.ptr)};
+  int* bin_count = cb.take<int>(nbins + 1);
+  int* bin_start = cb.take<int>(nbins + 1);
+  int* bin_fill = cb.take<int>(nbins + 1);
+  int* bin_scan = cb.take<int>(xr_scan_scratch_ints((int)nbins));
+  HIPCHK(hipMemsetAsync(bin_count, 0, sizeof(int) * (nbins + 1), s), "memset");
+  HIPCHK(hipMemsetAsync(bin_fill, 0, sizeof(int) * (nbins + 1), s), "memset");
+  HIPCHK(xr_bin_atoms(n, sys->positions, cell, g, home, bin, bin_count, s), "bin");
+  HIPCHK(xr_exclusive_scan(bin_count, (int)nbins, bin_start, bin_scan, s), "bin scan");
+  HIPCHK(xr_bin_scatter(n, bin, bin_start, bin_fill, binned, s), "bin scatter");
+
+  // --------------------------------------------------------- neighbor rows
+  HIPCHK(xr_neighbor_count(n, sys->positions, cell, g, home, bin, bin_start, binned, rc, 2048, counts, ctx->err, s),
+         "neighbor count");
+  HIPCHK(xr_exclusive_scan(counts, n, row_ptr, scan_tmp, s), "neighbor scan");
+  int m_host = 0;
+  HIPCHK(hipMemcpyAsync(&m_host, row_ptr + n, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+  HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, sizeof(unsigned), hipMemcpyDeviceToHost, s), "copy");
+  HIPCHK(hipStreamSynchronize(s), "neighbor sync");
+  if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
+  const int64_t m = m_host;
+
+  off = 0;
+  carve_size<int>(off, m);
+  carve_size<int>(off, m);
+  carve_size<double>(off, m);
+  carve_size<unsigned char>(off, m);
+  HIPCHK(ctx->edge_buf.ensure(off + 4096), "edge workspace");
+  Carver ce{static_cast<char*>(ctx->edge_buf.ptr)};
+  int* nbr_j = ce.take<int>(m);
+  int* nbr_s = ce.take<int>(m);
+  double* hval = ce.take<double>(m);
+  unsigned char* bflag = ce.take<unsigned char>(m);
+  HIPCHK(xr_neighbor_fill(n, sys->positions, cell, g, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s,
+                          ctx->err, s),
+         "neighbor fill");
+
+  // ------------------------------------------------------- edges and bonds
+  HIPCHK(xr_edge_prep(n, sys->positions, sys->types, cell, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, bflag,
+                      bond_count, s),
+         "edge pass");
+  HIPCHK(xr_exclusive_scan(bond_count, n, bond_ptr, scan_tmp, s), "bond scan");
+  int nb_host = 0;
+  HIPCHK(hipMemcpyAsync(&nb_host, bond_ptr + n, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+  HIPCHK(hipStreamSynchronize(s), "bond sync");
+  const int64_t nbond = nb_host;
+
+  off = 0;
+  for (int k = 0; k < 5; ++k) carve_size<int>(off, nbond);  // edge, bi, bj, code, rev
+  carve_size<double>(off, 3 * (size_t)nbond);               // vec
+  carve_size<double>(off, 3 * (size_t)nbond);               // G
+  carve_size<double>(off, (size_t)kBondScalars * nbond);    // per-bond scalars
+  HIPCHK(ctx->bond_buf.ensure(off + 4096), "bond workspace");
+  Carver cbd{static_cast<char*>(ctx->bond_buf.ptr)};
+  Bonds bd{};
+  bd.nb = (int)nbond;
+  bd.ptr = bond_ptr;
+  int* b_edge = cbd.take<int>(nbond);
+  int* b_i = cbd.take<int>(nbond);
+  bd.edge = b_edge;
+  bd.bi = b_i;
+  bd.bj = cbd.take<int>(nbond);
+  bd.code = cbd.take<int>(nbond);
+  bd.rev = cbd.take<int>(nbond);
+  bd.vec = cbd.take<double>(3 * (size_t)nbond);
+  bd.G = cbd.take<double>(3 * (size_t)nbond);
+  double* bs = cbd.take<double>((size_t)kBondScalars * nbond);
+  double* const bs_end = bs + (size_t)kBondScalars * nbond;
+  auto bn = [&]() {
+    double* p = bs;
+    bs += nbond;
+    return p;
+  };
+  bd.r = bn();
+  bd.bop = bn();
+  bd.pim = bn();
+  bd.ppm = bn();
+  bd.dsig = bn();
+  bd.dpi = bn();
+  bd.dpp = bn();
+  bd.f1 = bn();
+  bd.f4 = bn();
+  bd.f5 = bn();
+  bd.boc = bn();
+  bd.pic = bn();
+  bd.ppc = bn();
+  bd.BO = bn();
+  bd.PI = bn();
+  bd.PP = bn();
+  bd.SIG = bn();
+  bd.A_BO = bn();
+  bd.A_PI = bn();
+  bd.A_PP = bn();
+  bd.A_SIG = bn();
+  bd.A_totfar = bn();
+  bd.a_bop = bn();
+  bd.a_pim = bn();
+  bd.a_ppm = bn();
+  bd.c_i = bn();
+  bd.c_j = bn();
+  if (bs > bs_end) return fail(ctx, XREAC_INVALID, "internal error: per-bond workspace layout");
+  if (nbond > 0) HIPCHK(xr_bond_fill(n, row_ptr, bflag, bond_ptr, b_edge, b_i, s), "bond fill");
+
+  // ------------------------------------------------------------------ QEq
+  CgArgs cg{};
+  cg.n = n;
+  cg.type = sys->types;
+  cg.ff = ff;
+  cg.row_ptr = row_ptr;
+  cg.nbr_j = nbr_j;
+  cg.hval = hval;
+  cg.X = cgv;
+  cg.R = cgv + n;
+  cg.Z = cgv + 2 * (size_t)n;
+  cg.P0 = cgv + 3 * (size_t)n;
+  cg.P1 = cgv + 4 * (size_t)n;
+  cg.AP = cgv + 5 * (size_t)n;
+  cg.inv_diag = inv_diag;
+  cg.partials = cg_part;
+  cg.state = cg_state;
+  cg.tol = opt->qeq_tolerance > 0 ? opt->qeq_tolerance : 1e-10;
+  cg.max_iter = opt->qeq_max_iterations > 0 ? opt->qeq_max_iterations : 2000;
+  cg.first_check = 8;
+  cg.check_every = 4;
+  cg.total_charge = sys->total_charge;
+  cg.q = out->charges;
+  cg.mu = mu;
+  int iters = 0;
+  HIPCHK(xr_cg_solve(cg, s, &iters), "QEq");
+  if (iters < 0) return fail(ctx, XREAC_INVALID, "QEq conjugate gradient did not converge; check the geometry");
+  out->qeq_iterations = iters;
+
+  // ----------------------------------------------------- energy and forces
+  HIPCHK(hipMemsetAsync(epart, 0, sizeof(double) * NCH * cap, s), "memset");
+  HIPCHK(hipMemsetAsync(at.F_hb, 0, sizeof(double) * 3 * (size_t)n, s), "memset");
+  HIPCHK(xr_nonbonded(at, cell, ff, row_ptr, nbr_j, nbr_s, acc, s), "nonbonded");
+  HIPCHK(xr_bond_forward(at, bd, cell, ff, nbr_j, nbr_s, acc, ctx->err, s), "bond orders");
+  HIPCHK(xr_manybody(at, bd, cell, ff, row_ptr, nbr_j, nbr_s, opt->lammps_angles, acc, ctx->err, s), "many-body");
+  HIPCHK(xr_bond_backward(at, bd, ff, out->forces, s), "adjoint");
+  HIPCHK(xr_energy_reduce(acc, ctx->comps, s), "energy");
+  if (out->total_bond_orders)
+    HIPCHK(hipMemcpyAsync(out->total_bond_orders, at.total, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
+  if (out->lone_pairs)
+    HIPCHK(hipMemcpyAsync(out->lone_pairs, at.nlp, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
+  if (out->bond_counts)
+    HIPCHK(hipMemcpyAsync(out->bond_counts, at.bond_count, sizeof(int) * n, hipMemcpyDeviceToDevice, s), "copy");
+  double* host_comps = reinterpret_cast<double*>(ctx->err_host) + 8;
+  HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, 64 * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+  HIPCHK(hipStreamSynchronize(s), "final sync");
+  if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
+  std::memcpy(out->components, host_comps, sizeof(double) * NCOMP);
+  out->num_edges = m;
+  out->num_bonds = nbond;
+
+  ctx->n = n;
+  ctx->m = m;
+  ctx->nb = nbond;
+  ctx->row_ptr = row_ptr;
+  ctx->nbr_j = nbr_j;
+  ctx->nbr_s = nbr_s;
+  ctx->bond_ptr = bond_ptr;
+  ctx->bond_i = b_i;
+  ctx->bond_j = bd.bj;
+  ctx->bond_code = bd.code;
+  ctx->bond_BO = bd.BO;
+  return XREAC_OK;
+}
+
+int xreac_copy_neighbors(xreac_ctx* ctx, int32_t* row_ptr, int32_t* j, int32_t* shifts, int64_t capacity) {
+  if (!ctx || !row_ptr || !j || !shifts) return XREAC_INVALID;
+  if (capacity < ctx->m) return fail(ctx, XREAC_INVALID, "neighbor capacity too small");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  std::vector<int> codes(ctx->m);
+  HIPCHK(hipMemcpy(row_ptr, ctx->row_ptr, sizeof(int) * (ctx->n + 1), hipMemcpyDeviceToHost), "copy");
+  if (ctx->m) {
+    HIPCHK(hipMemcpy(j, ctx->nbr_j, sizeof(int) * ctx->m, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(codes.data(), ctx->nbr_s, sizeof(int) * ctx->m, hipMemcpyDeviceToHost), "copy");
+  }
+  for (int64_t e = 0; e < ctx->m; ++e) {
+    int sx, sy, sz;
+    xr_unpack_shift(codes[e], sx, sy, sz);
+    shifts[3 * e] = sx;
+    shifts[3 * e + 1] = sy;
+    shifts[3 * e + 2] = sz;
+  }
+  return XREAC_OK;
+}
+
+int xreac_copy_bonds(xreac_ctx* ctx, int32_t* i, int32_t* j, int32_t* shifts, double* bo, int64_t capacity) {
+  if (!ctx || !i || !j || !shifts || !bo) return XREAC_INVALID;
+  if (capacity < ctx->nb) return fail(ctx, XREAC_INVALID, "bond capacity too small");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  std::vector<int> codes(ctx->nb);
+  if (ctx->nb) {
+    HIPCHK(hipMemcpy(i, ctx->bond_i, sizeof(int) * ctx->nb, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(j, ctx->bond_j, sizeof(int) * ctx->nb, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(codes.data(), ctx->bond_code, sizeof(int) * ctx->nb, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(bo, ctx->bond_BO, sizeof(double) * ctx->nb, hipMemcpyDeviceToHost), "copy");
+  }
+  for (int64_t e = 0; e < ctx->nb; ++e) {
+    int sx, sy, sz;
+    xr_unpack_shift(codes[e], sx, sy, sz);
+    shifts[3 * e] = sx;
+    shifts[3 * e + 1] = sy;
+    shifts[3 * e + 2] = sz;
+  }
+  return XREAC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,446 @@
+// Valence-angle, torsion and hydrogen-bond terms with their adjoints.
+//
+// Items are enumerated on the device from the bond rows, in the reference's
+// selection rules (src/xreac/energy.py:220, 248-370):
+//   angles    - centre j, unordered pairs of bonds with BO > 0.001 whose
+//               product exceeds 1e-5, every parameter row with |v1| > 0.001;
+//   torsions  - half bond j->k with BO > 0.001, left bond at j, right bond at
+//               k, excluding the middle edge, its reverse, 3-rings through the
+//               same image and products <= 0.001;
+//   H-bonds   - hydrogen j, donor bond to a heavy atom with BO >= 0.01 and any
+//               heavy-atom edge within min(7.5, rc) other than that same edge.
+// An item's derivative is deposited only into slots owned by the thread that
+// enumerates it (its own bond row and atom).  Torsions are enumerated from
+// both ends of the middle bond and H-bonds from both the hydrogen and the
+// acceptor, so each end writes its own share without atomics.
+#include "state.h"
+
+namespace {
+
+constexpr double THB_CUT = 0.001;
+constexpr double THB_PRODUCT_CUT = 0.00001;
+constexpr double DEG = 3.14159265 / 180;
+
+__device__ inline const double* AP(const FFDev& ff, int t) { return ff.atom + t * AP_COUNT; }
+__device__ inline double dot3(const double* a, const double* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+__device__ inline void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// ------------------------------------------------------------------ angles
+__global__ void __launch_bounds__(256) k_angles(Atoms at, Bonds bd, FFDev ff, int lammps, EnergyAcc acc) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double e[3] = {0.0, 0.0, 0.0};
+  if (j < at.n) {
+    const double* g = ff.gen;
+    const int tj = at.type[j];
+    const double* aj = AP(ff, tj);
+    const int nt = ff.nt;
+    const double db = at.db[j], d = at.d[j], totj = at.total[j], sbo2 = at.sbo2[j];
+    const double p3 = aj[AP_PVAL3], p5 = aj[AP_PVAL5];
+    // centre-only factors
+    const double ex3 = exp(-g[20] * d), ex4 = exp(g[21] * d);
+    const double Q = (2 + ex3) / (1 + ex3 + ex4);
+    const double dQ = (-g[20] * ex3 * (1 + ex3 + ex4) - (2 + ex3) * (-g[20] * ex3 + g[21] * ex4)) /
+                      ((1 + ex3 + ex4) * (1 + ex3 + ex4));
+    const double ecj = exp(g[2] * (totj - aj[AP_VALVAL]));
+    const double Cj = 1 / (1 + ecj);
+    const double dCj = -g[2] * ecj * Cj * Cj;
+    const double esb = exp(-g[17] * (2 - sbo2));
+    double A_db = 0.0, A_d = 0.0, A_tot = 0.0, A_sbo2 = 0.0;
+    const int b0 = bd.ptr[j], b1 = bd.ptr[j + 1];
+    for (int a = b0; a < b1; ++a) {
+      const double Ba = bd.BO[a];
+      if (!(Ba > THB_CUT)) continue;
+      for (int c = a + 1; c < b1; ++c) {
+        const double Bc = bd.BO[c];
+        if (!(Bc > THB_CUT)) continue;
+        if (Ba * Bc <= THB_PRODUCT_CUT) continue;
+        const int ia = bd.bj[a], kc = bd.bj[c];
+        const int key = (at.type[ia] * nt + tj) * nt + at.type[kc];
+        const int r0 = ff.ang_off[key], r1 = ff.ang_off[key + 1];
+        if (r0 == r1) continue;
+        const double* va = bd.vec + 3 * a;
+        const double* vc = bd.vec + 3 * c;
+        const double ra = bd.r[a], rc = bd.r[c];
+        const double cosv = dot3(va, vc) / (ra * rc);
+        double theta, dth;
+        if (lammps) {
+          const double cc = fmin(fmax(cosv, -1.0), 1.0);
+          theta = acos(cc);
+          dth = -1.0 / fmax(sin(theta), 1e-5);
+        } else {
+          const double lo = -1 + 1e-14, hi = 1 - 1e-14;
+          const double cc = fmin(fmax(cosv, lo), hi);
+          theta = acos(cc);
+          dth = (cc == lo || cc == hi) ? 0.0 : -1.0 / sqrt(1 - cc * cc);
+        }
+        const double bij = Ba - THB_CUT, bjk = Bc - THB_CUT;
+        const double tota = at.total[ia], totc = at.total[kc];
+        double gA = 0.0, gC = 0.0, dE_dcos = 0.0, a_ta = 0.0, a_tc = 0.0;
+        for (int row = r0; row < r1; ++row) {
+          const double* pr = ff.ang + row * ANG_W;
+          const double th00 = pr[0], v1 = pr[1], v2 = pr[2], coa1 = pr[3], v7 = pr[4], pen1 = pr[5], v4 = pr[6];
+          const double theta0 = (180 - th00 * (1 - esb)) * DEG;
+          const double pa = pow(bij, v4), pc = pow(bjk, v4);
+          const double ea = exp(-p3 * pa), ec = exp(-p3 * pc);
+          const double Fa = 1 - ea, Fc = 1 - ec;
+          const double f7 = Fa * Fc;
+          const double ex6 = exp(g[14] * db), ex7 = exp(-v7 * db);
+          const double den8 = 1 + ex6 + ex7;
+          const double R = (2 + ex6) / den8;
+          const double f8 = p5 - (p5 - 1) * R;
+          const double dR = (g[14] * ex6 * den8 - (2 + ex6) * (g[14] * ex6 - v7 * ex7)) / (den8 * den8);
+          const double dt = theta0 - theta;
+          const double exang = exp(-v2 * (dt * dt));
+          const double s = v1 >= 0 ? 1 - exang : -exang;
+          e[0] += f7 * f8 * v1 * s;
+          const double dE_dth = -f7 * f8 * v1 * 2 * v2 * dt * exang;
+          dE_dcos += dE_dth * dth;
+          A_sbo2 += -dE_dth * th00 * g[17] * esb * DEG;
+          const double dE_df7 = f8 * v1 * s, dE_df8 = f7 * v1 * s;
+          gA += dE_df7 * Fc * p3 * v4 * (bij > 0 ? pa / bij : 0.0) * ea;
+          gC += dE_df7 * Fa * p3 * v4 * (bjk > 0 ? pc / bjk : 0.0) * ec;
+          A_db += dE_df8 * (-(p5 - 1) * dR);
+          // penalty
+          const double Ebb = exp(-g[19] * ((bij - 2) * (bij - 2) + (bjk - 2) * (bjk - 2)));
+          const double Ep = pen1 * Q * Ebb;
+          e[1] += Ep;
+          A_d += pen1 * dQ * Ebb;
+          gA += Ep * (-2 * g[19] * (bij - 2));
+          gC += Ep * (-2 * g[19] * (bjk - 2));
+          // angle conjugation
+          const double E1 = exp(-g[38] * ((tota - bij) * (tota - bij) + (totc - bjk) * (totc - bjk)));
+          const double E2 = exp(-g[30] * ((bij - 1.5) * (bij - 1.5) + (bjk - 1.5) * (bjk - 1.5)));
+          const double Ec = coa1 * Cj * E1 * E2;
+          e[2] += Ec;
+          A_tot += coa1 * dCj * E1 * E2;
+          a_ta += Ec * (-2 * g[38] * (tota - bij));
+          a_tc += Ec * (-2 * g[38] * (totc - bjk));
+          gA += Ec * (2 * g[38] * (tota - bij) - 2 * g[30] * (bij - 1.5));
+          gC += Ec * (2 * g[38] * (totc - bjk) - 2 * g[30] * (bjk - 1.5));
+        }
+        bd.A_BO[a] += gA;
+        bd.A_BO[c] += gC;
+        bd.A_totfar[a] += a_ta;
+        bd.A_totfar[c] += a_tc;
+        const double inv = 1.0 / (ra * rc);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          bd.G[3 * a + k] += dE_dcos * (vc[k] * inv - cosv * va[k] / (ra * ra));
+          bd.G[3 * c + k] += dE_dcos * (va[k] * inv - cosv * vc[k] / (rc * rc));
+        }
+      }
+    }
+    at.A_db[j] += A_db;
+    at.A_d[j] += A_d;
+    at.A_total[j] += A_tot;
+    at.A_sbo2[j] += A_sbo2;
+  }
+  double ea[1] = {e[0]}, ep[1] = {e[1]}, ec[1] = {e[2]};
+  xr_block_partials<1>(ea, acc.partials + CH_ANGLE * acc.cap, gridDim.x);
+  xr_block_partials<1>(ep, acc.partials + CH_PEN * acc.cap, gridDim.x);
+  xr_block_partials<1>(ec, acc.partials + CH_COA * acc.cap, gridDim.x);
+}
+
+// ---------------------------------------------------------------- torsions
+struct TorsionGrad {
+  double E_tor, E_conj;
+  double Gu[3], Gv[3], Gw[3];
+  double dB1, dB2, dB3, dPI, dDS;
+};
+
+__device__ inline bool torsion_eval(const Bonds& bd, const FFDev& ff, const double* prm, int a, int m, int c,
+                                    double dbj, double dbk, TorsionGrad& o) {
+  const double* g = ff.gen;
+  const double* u = bd.vec + 3 * a;
+  const double* v = bd.vec + 3 * m;
+  const double* w = bd.vec + 3 * c;
+  const double ra = bd.r[a], rm = bd.r[m], rcw = bd.r[c];
+  double n1[3], n2[3];
+  cross3(u, v, n1);
+  cross3(w, v, n2);
+  const double A1 = dot3(n1, n1), A2 = dot3(n2, n2);
+  const double s1 = A1 / ((ra * rm) * (ra * rm));
+  const double s2 = A2 / ((rm * rcw) * (rm * rcw));
+  if (s1 < 1e-20 || s2 < 1e-20) return false;
+  const double sp = sqrt(s1 * s2);
+  const double P = dot3(n1, n2);
+  const double sq = sqrt(A1 * A2);
+  const double cw = P / sq;
+  const double v1 = prm[0], v2 = prm[1], v3 = prm[2], tor1 = prm[3], cot1 = prm[4];
+  const double b1 = bd.BO[a] - THB_CUT, b2 = bd.BO[m] - THB_CUT, b3 = bd.BO[c] - THB_CUT;
+  const double ds = dbj + dbk;
+  const double ex3 = exp(-g[24] * ds), ex4 = exp(g[25] * ds);
+  const double den11 = 1 + ex3 + ex4;
+  const double f11 = (2 + ex3) / den11;
+  const double df11 = (-g[24] * ex3 * den11 - (2 + ex3) * (-g[24] * ex3 + g[25] * ex4)) / (den11 * den11);
+  const double PIm = bd.PI[m];
+  const double z = 2 - PIm - f11;
+  const double ex1 = exp(tor1 * (z * z));
+  const double e1 = exp(-g[23] * b1), e2 = exp(-g[23] * b2), e3 = exp(-g[23] * b3);
+  const double fn10 = (1 - e1) * (1 - e2) * (1 - e3);
+  const double cv = 0.5 * (v1 * (1 + cw) + v2 * ex1 * (2 - 2 * cw * cw) + v3 * (1 + 4 * (cw * cw * cw) - 3 * cw));
+  o.E_tor = fn10 * sp * cv;
+  const double q1 = b1 - 1.5, q2 = b2 - 1.5, q3 = b3 - 1.5;
+  const double fn12 = exp(-g[27] * ((q1 * q1 + q2 * q2) + q3 * q3));
+  o.E_conj = cot1 * fn12 * (1 + (cw * cw - 1) * sp);
+  const double dcv = 0.5 * (v1 - 4 * v2 * ex1 * cw + v3 * (12 * cw * cw - 3));
+  const double dE_dcw = fn10 * sp * dcv + cot1 * fn12 * 2 * cw * sp;
+  const double dE_dsp = fn10 * cv + cot1 * fn12 * (cw * cw - 1);
+  const double dE_dex1 = fn10 * sp * 0.5 * v2 * (2 - 2 * cw * cw);
+  const double dE_dfn10 = sp * cv;
+  const double dE_dfn12 = cot1 * (1 + (cw * cw - 1) * sp);
+  o.dB1 = dE_dfn10 * g[23] * e1 * (1 - e2) * (1 - e3) + dE_dfn12 * fn12 * (-2 * g[27] * q1);
+  o.dB2 = dE_dfn10 * g[23] * e2 * (1 - e1) * (1 - e3) + dE_dfn12 * fn12 * (-2 * g[27] * q2);
+  o.dB3 = dE_dfn10 * g[23] * e3 * (1 - e1) * (1 - e2) + dE_dfn12 * fn12 * (-2 * g[27] * q3);
+  const double dex1_dz = ex1 * tor1 * 2 * z;  // d ex1 / d z, z = 2 - PI - f11
+  o.dPI = -dE_dex1 * dex1_dz;
+  o.dDS = -dE_dex1 * dex1_dz * df11;
+  const double EP = dE_dcw / sq;
+  const double EA1 = dE_dcw * (-0.5 * cw / A1) + dE_dsp * (0.5 * sp / A1);
+  const double EA2 = dE_dcw * (-0.5 * cw / A2) + dE_dsp * (0.5 * sp / A2);
+  const double Era = -dE_dsp * sp / ra, Erm = -2 * dE_dsp * sp / rm, Erc = -dE_dsp * sp / rcw;
+  double vn2[3], vn1[3], n2u[3], n1w[3], n1u[3], n2w[3];
+  cross3(v, n2, vn2);
+  cross3(v, n1, vn1);
+  cross3(n2, u, n2u);
+  cross3(n1, w, n1w);
+  cross3(n1, u, n1u);
+  cross3(n2, w, n2w);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o.Gu[k] = EP * vn2[k] + 2 * EA1 * vn1[k] + Era * u[k] / ra;
+    o.Gv[k] = EP * (n2u[k] + n1w[k]) + 2 * EA1 * n1u[k] + 2 * EA2 * n2w[k] + Erm * v[k] / rm;
+    o.Gw[k] = EP * vn1[k] + 2 * EA2 * vn2[k] + Erc * w[k] / rcw;
+  }
+  return true;
+}
+
+// Walk the torsions around middle bond m = (j -> k) (half, BO > 0.001).
+// role 0: called by j's thread; role 1: called by k's thread with mk = rev(m).
+template <class Fn>
+__device__ inline void for_each_torsion(const Atoms& at, const Bonds& bd, const FFDev& ff, int m, unsigned& err,
+                                        Fn fn) {
+  const int j = bd.bi[m], k = bd.bj[m], mr = bd.rev[m];
+  const int nt = ff.nt;
+  const int tj = at.type[j], tk = at.type[k];
+  int msx, msy, msz;
+  xr_unpack_shift(bd.code[m], msx, msy, msz);
+  for (int a = bd.ptr[j]; a < bd.ptr[j + 1]; ++a) {
+    if (a == m || !(bd.BO[a] > THB_CUT)) continue;
+    const int i = bd.bj[a];
+    const int base = ((at.type[i] * nt + tj) * nt + tk) * nt;
+    int asx, asy, asz;
+    xr_unpack_shift(bd.code[a], asx, asy, asz);
+    for (int c = bd.ptr[k]; c < bd.ptr[k + 1]; ++c) {
+      if (!(bd.BO[c] > THB_CUT)) continue;
+      const int l = bd.bj[c];
+      const int key = base + at.type[l];
+      if (!ff.tor_on[key]) continue;
+      if (c == mr || bd.BO[a] * bd.BO[m] * bd.BO[c] <= THB_CUT) continue;
+      if (i == l) {
+        int csx, csy, csz;
+        xr_unpack_shift(bd.code[c], csx, csy, csz);
+        if (asx == msx + csx && asy == msy + csy && asz == msz + csz) continue;
+      }
+      TorsionGrad o;
+      if (!torsion_eval(bd, ff, ff.tor + key * TOR_W, a, m, c, at.db[j], at.db[k], o)) {
+        err |= XR_ERR_COLLINEAR;
+        continue;
+      }
+      fn(a, c, o);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc,
+                                                  unsigned* __restrict__ err_out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  double e[2] = {0.0, 0.0};
+  unsigned err = 0;
+  if (x < at.n) {
+    double A_db = 0.0;
+    for (int mb = bd.ptr[x]; mb < bd.ptr[x + 1]; ++mb) {
+      if (!(bd.BO[mb] > THB_CUT)) continue;
+      const int rv = bd.rev[mb];
+      const bool half = bd.edge[mb] < bd.edge[rv];
+      if (half) {
+        // x is the j end: owns left bonds a, middle m and db_j.
+        double gm[3] = {0, 0, 0}, aBm = 0.0, aPIm = 0.0;
+        for_each_torsion(at, bd, ff, mb, err, [&](int a, int c, const TorsionGrad& o) {
+          e[0] += o.E_tor;
+          e[1] += o.E_conj;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            bd.G[3 * a + k] += o.Gu[k];
+            gm[k] += o.Gv[k];
+          }
+          bd.A_BO[a] += o.dB1;
+          aBm += o.dB2;
+          aPIm += o.dPI;
+          A_db += o.dDS;
+        });
+#pragma unroll
+        for (int k = 0; k < 3; ++k) bd.G[3 * mb + k] += gm[k];
+        bd.A_BO[mb] += aBm;
+        bd.A_PI[mb] += aPIm;
+      } else {
+        // x is the k end of the half bond rv: owns right bonds c and db_k.
+        for_each_torsion(at, bd, ff, rv, err, [&](int a, int c, const TorsionGrad& o) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) bd.G[3 * c + k] += o.Gw[k];
+          bd.A_BO[c] += o.dB3;
+          A_db += o.dDS;
+        });
+      }
+    }
+    at.A_db[x] += A_db;
+  }
+  if (err) atomicOr(err_out, err);
+  double et[1] = {e[0]}, ec[1] = {e[1]};
+  xr_block_partials<1>(et, acc.partials + CH_TOR * acc.cap, gridDim.x);
+  xr_block_partials<1>(ec, acc.partials + CH_CONJ * acc.cap, gridDim.x);
+}
+
+// ---------------------------------------------------------------- H-bonds
+struct HbTerm {
+  double E, dBO;
+  double Gd[3], Ga[3];
+};
+
+__device__ inline void hbond_eval(const double* prm, double BOd, const double* vd, double rd, const double* va,
+                                  double ra, HbTerm& o) {
+  const double r0 = prm[0], hb1 = prm[1], hb2 = prm[2], hb3 = prm[3];
+  const double cosv = dot3(vd, va) / (rd * ra);
+  const double cc = fmin(fmax(cosv, -1.0), 1.0);
+  const double sin4 = 0.25 * ((1 - cc) * (1 - cc));
+  const double eb = exp(-hb2 * BOd);
+  const double Ex = exp(-hb3 * (r0 / ra + ra / r0 - 2));
+  o.E = hb1 * (1 - eb) * Ex * sin4;
+  o.dBO = hb1 * hb2 * eb * Ex * sin4;
+  const double dE_ddist = hb1 * (1 - eb) * Ex * sin4 * (-hb3) * (-r0 / (ra * ra) + 1 / r0);
+  const double dE_dcos = (cc == -1.0 || cc == 1.0) ? 0.0 : hb1 * (1 - eb) * Ex * (-0.5 * (1 - cc));
+  const double inv = 1.0 / (rd * ra);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    o.Gd[k] = dE_dcos * (va[k] * inv - cosv * vd[k] / (rd * rd));
+    o.Ga[k] = dE_dcos * (vd[k] * inv - cosv * va[k] / (ra * ra)) + dE_ddist * va[k] / ra;
+  }
+}
+
+// One wave per hydrogen: donor-side energy and derivatives.
+__global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, CellDev cell, FFDev ff,
+                                                  const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
+                                                  const int* __restrict__ nbr_s, EnergyAcc acc) {
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double etot[1] = {0.0};
+  if (j < at.n && AP(ff, at.type[j])[AP_HBOND] == 1.0) {
+    const int nt = ff.nt, tj = at.type[j];
+    const double hcut = fmin(7.5, ff.gen[12]);
+    double F[3] = {0, 0, 0};
+    for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
+      const int kd = bd.bj[d];
+      if (AP(ff, at.type[kd])[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
+      const double* vd = bd.vec + 3 * d;
+      const double rd = bd.r[d], BOd = bd.BO[d];
+      const int ed = bd.edge[d];
+      const int kbase = (at.type[kd] * nt + tj) * nt;
+      double Gd[3] = {0, 0, 0}, dBO = 0.0;
+      for (int e = row_ptr[j] + lane; e < row_ptr[j + 1]; e += 64) {
+        const int k = nbr_j[e];
+        const int tk = at.type[k];
+        if (e == ed || AP(ff, tk)[AP_HBOND] != 2.0 || !ff.hb_on[kbase + tk]) continue;
+        double va[3];
+        const double ra = xr_edge(at.x, cell, j, k, nbr_s[e], va);
+        if (!(ra <= hcut)) continue;
+        HbTerm o;
+        hbond_eval(ff.hb + (kbase + tk) * HB_W, BOd, vd, rd, va, ra, o);
+        etot[0] += o.E;
+        dBO += o.dBO;
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) {
+          Gd[k2] += o.Gd[k2];
+          F[k2] += o.Ga[k2];
+        }
+      }
+      dBO = xr_wave_sum(dBO);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) Gd[k2] = xr_wave_sum(Gd[k2]);
+      if (lane == 0) {
+        bd.A_BO[d] += dBO;
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) bd.G[3 * d + k2] += Gd[k2];
+      }
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
+    if (lane == 0) {
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) at.F_hb[3 * j + k2] += F[k2];
+    }
+  }
+  xr_block_partials<1>(etot, acc.partials + CH_HB * acc.cap, gridDim.x);
+}
+
+// One wave per heavy atom: force from the acceptor-vector derivative.
+__global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, CellDev cell, FFDev ff,
+                                                     const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
+                                                     const int* __restrict__ nbr_s) {
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= at.n) return;
+  const int tk = at.type[k];
+  if (AP(ff, tk)[AP_HBOND] != 2.0) return;
+  const int nt = ff.nt;
+  const double hcut = fmin(7.5, ff.gen[12]);
+  double F[3] = {0, 0, 0};
+  for (int e = row_ptr[k] + lane; e < row_ptr[k + 1]; e += 64) {
+    const int j = nbr_j[e];
+    const int tj = at.type[j];
+    if (AP(ff, tj)[AP_HBOND] != 1.0) continue;
+    const int code = nbr_s[e];
+    double vk[3];
+    const double ra = xr_edge(at.x, cell, k, j, code, vk);
+    if (!(ra <= hcut)) continue;
+    const double va[3] = {-vk[0], -vk[1], -vk[2]};
+    const int back = xr_neg_shift(code);
+    for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
+      const int kd = bd.bj[d];
+      const int td = at.type[kd];
+      if (AP(ff, td)[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
+      if (kd == k && bd.code[d] == back) continue;  // the acceptor edge is the donor edge
+      const int key = (td * nt + tj) * nt + tk;
+      if (!ff.hb_on[key]) continue;
+      HbTerm o;
+      hbond_eval(ff.hb + key * HB_W, bd.BO[d], bd.vec + 3 * d, bd.r[d], va, ra, o);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) F[k2] -= o.Ga[k2];
+    }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k2 = 0; k2 < 3; ++k2) at.F_hb[3 * k + k2] += F[k2];
+  }
+}
+
+inline dim3 grid(int count) { return dim3(count > 0 ? (count + 255) / 256 : 1); }
+
+}  // namespace
+
+hipError_t xr_manybody(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* row_ptr,
+                       const int* nbr_j, const int* nbr_s, int lammps_angles, EnergyAcc acc, unsigned* err,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_angles, grid(at.n), dim3(256), 0, s, at, bd, ff, lammps_angles, acc);
+  hipLaunchKernelGGL(k_torsions, grid(at.n), dim3(256), 0, s, at, bd, ff, acc, err);
+  const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
+  hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, cell, ff, row_ptr, nbr_j, nbr_s, acc);
+  hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, cell, ff, row_ptr, nbr_j, nbr_s);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}

@@ -1,0 +1,264 @@
+// Cell-list neighbor build producing the canonical directed list.
+//
+// Membership is the reference's exact test (src/xreac/neighbors.py:41-46):
+// (i, j, S) is an edge iff ((x_j - x_i) + S@L)^2 <= rc^2 and (i, j, S) is not
+// (i, i, 0).  Binning only proposes candidates; the test itself is evaluated on
+// the caller's (possibly unwrapped) coordinates with the reference rounding.
+//
+// Bins live in fractional coordinates for periodic directions (each bin at
+// least rc thick measured along the face normal) and in scaled coordinates for
+// nonperiodic ones.  Rows are sorted in LDS by the 64-bit key
+// (j << 32) | packed(S), which is the reference's (i, j, Sx, Sy, Sz)
+// lexicographic order within row i (neighbors.py:65-98).
+#include "launch.h"
+
+namespace {
+
+constexpr int ROW_CAP = 2048;  // max neighbors per atom held in LDS for sorting
+
+__device__ inline void frac_of(const CellDev& c, const double* x, int i, double f[3]) {
+  const double a = x[3 * i], b = x[3 * i + 1], d = x[3 * i + 2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) f[k] = a * c.inv[k] + b * c.inv[3 + k] + d * c.inv[6 + k];
+}
+
+__device__ inline int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+__global__ void k_coord_bounds(int n, const double* __restrict__ x, CellDev cell, double* __restrict__ out) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double f[3];
+    frac_of(cell, x, i, f);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = fmin(lo[k], f[k]);
+      hi[k] = fmax(hi[k], f[k]);
+    }
+  }
+  __shared__ double red[6][256];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    red[k][threadIdx.x] = lo[k];
+    red[3 + k][threadIdx.x] = hi[k];
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+        red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_bin_atoms(int n, const double* __restrict__ x, CellDev cell, BinGeom g, int* __restrict__ home,
+                            int* __restrict__ bin, int* __restrict__ bin_count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[3];
+  frac_of(cell, x, i, f);
+  int b[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double u;
+    if (g.pbc[k]) {
+      const double h = floor(f[k]);
+      home[3 * i + k] = (int)h;
+      u = (f[k] - h) / g.w[k];
+    } else {
+      home[3 * i + k] = 0;
+      u = (f[k] - g.lo[k]) / g.w[k];
+    }
+    int bk = (int)floor(u);
+    b[k] = bk < 0 ? 0 : (bk >= g.nb[k] ? g.nb[k] - 1 : bk);
+  }
+  const int id = (b[0] * g.nb[1] + b[1]) * g.nb[2] + b[2];
+  bin[i] = id;
+  atomicAdd(&bin_count[id], 1);
+}
+
+__global__ void k_bin_scatter(int n, const int* __restrict__ bin, const int* __restrict__ bin_start,
+                              int* __restrict__ bin_fill, int* __restrict__ binned) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = bin[i];
+  binned[bin_start[b] + atomicAdd(&bin_fill[b], 1)] = i;
+}
+
+// Visit every candidate image (j, T) of atom i in the surrounding bins; calls
+// fn(j, packed_shift, r2) for those passing the exact distance test.  Lanes
+// stride over the atoms of each bin (lane = t, stride = nthreads).
+template <class Fn>
+__device__ inline void for_each_neighbor(int i, const double* __restrict__ x, const CellDev& cell, const BinGeom& g,
+                                         const int* __restrict__ home, const int* __restrict__ bin,
+                                         const int* __restrict__ bin_start, const int* __restrict__ binned,
+                                         double rc2, int t, int nthreads, unsigned& err, Fn fn) {
+  const int id = bin[i];
+  const int bz = id % g.nb[2], by = (id / g.nb[2]) % g.nb[1], bx = id / (g.nb[2] * g.nb[1]);
+  const int hi0 = home[3 * i], hi1 = home[3 * i + 1], hi2 = home[3 * i + 2];
+  const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+  for (int o0 = -g.reach[0]; o0 <= g.reach[0]; ++o0) {
+    int c0 = bx + o0, t0 = 0;
+    if (g.pbc[0]) {
+      t0 = floordiv(c0, g.nb[0]);
+      c0 -= t0 * g.nb[0];
+    } else if (c0 < 0 || c0 >= g.nb[0]) {
+      continue;
+    }
+    for (int o1 = -g.reach[1]; o1 <= g.reach[1]; ++o1) {
+      int c1 = by + o1, t1 = 0;
+      if (g.pbc[1]) {
+        t1 = floordiv(c1, g.nb[1]);
+        c1 -= t1 * g.nb[1];
+      } else if (c1 < 0 || c1 >= g.nb[1]) {
+        continue;
+      }
+      for (int o2 = -g.reach[2]; o2 <= g.reach[2]; ++o2) {
+        int c2 = bz + o2, t2 = 0;
+        if (g.pbc[2]) {
+          t2 = floordiv(c2, g.nb[2]);
+          c2 -= t2 * g.nb[2];
+        } else if (c2 < 0 || c2 >= g.nb[2]) {
+          continue;
+        }
+        const int cb = (c0 * g.nb[1] + c1) * g.nb[2] + c2;
+        const int s = bin_start[cb], e = bin_start[cb + 1];
+        for (int p = s + t; p < e; p += nthreads) {
+          const int j = binned[p];
+          const int sx = g.pbc[0] ? t0 - home[3 * j] + hi0 : 0;
+          const int sy = g.pbc[1] ? t1 - home[3 * j + 1] + hi1 : 0;
+          const int sz = g.pbc[2] ? t2 - home[3 * j + 2] + hi2 : 0;
+          if (sx < -511 || sx > 511 || sy < -511 || sy > 511 || sz < -511 || sz > 511) {
+            err |= XR_ERR_SHIFT_RANGE;
+            continue;
+          }
+          const int code = xr_pack_shift(sx, sy, sz);
+          double off[3];
+          xr_offset(cell, code, off);
+          const double v0 = (x[3 * j] - xi0) + off[0];
+          const double v1 = (x[3 * j + 1] - xi1) + off[1];
+          const double v2 = (x[3 * j + 2] - xi2) + off[2];
+          const double r2 = (v0 * v0 + v1 * v1) + v2 * v2;
+          if (r2 <= rc2 && !(j == i && sx == 0 && sy == 0 && sz == 0)) fn(j, code, r2);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
+                                                   const int* __restrict__ home, const int* __restrict__ bin,
+                                                   const int* __restrict__ bin_start, const int* __restrict__ binned,
+                                                   double rc2, int row_cap, int* __restrict__ counts,
+                                                   unsigned* __restrict__ err_out) {
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  unsigned err = 0;
+  int cnt = 0;
+  for_each_neighbor(i, x, cell, g, home, bin, bin_start, binned, rc2, lane, 64, err,
+                    [&](int j, int code, double r2) {
+                      ++cnt;
+                      if (j != i && r2 < 1e-12) err |= XR_ERR_COINCIDENT;
+                    });
+  cnt = xr_wave_sum_int(cnt);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) err |= __shfl_down(err, off, 64);
+  if (lane == 0) {
+    counts[i] = cnt;
+    if (cnt > row_cap) err |= XR_ERR_ROW_TOO_LONG;
+    if (err) atomicOr(err_out, err);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_nbr_fill(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
+                                                  const int* __restrict__ home, const int* __restrict__ bin,
+                                                  const int* __restrict__ bin_start, const int* __restrict__ binned,
+                                                  double rc2, const int* __restrict__ row_ptr, int* __restrict__ nbr_j,
+                                                  int* __restrict__ nbr_s) {
+  __shared__ unsigned long long keys[ROW_CAP];
+  __shared__ int cursor;
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int begin = row_ptr[i], count = row_ptr[i + 1] - begin;
+  if (count > ROW_CAP) return;  // reported by the count pass
+  if (threadIdx.x == 0) cursor = 0;
+  __syncthreads();
+  unsigned err = 0;
+  for_each_neighbor(i, x, cell, g, home, bin, bin_start, binned, rc2, threadIdx.x, blockDim.x, err,
+                    [&](int j, int code, double) {
+                      const int p = atomicAdd(&cursor, 1);
+                      if (p < ROW_CAP) keys[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
+                    });
+  __syncthreads();
+  int size = 1;
+  while (size < count) size <<= 1;
+  for (int t = count + threadIdx.x; t < size; t += blockDim.x) keys[t] = ~0ull;
+  __syncthreads();
+  for (int k = 2; k <= size; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int t = threadIdx.x; t < size; t += blockDim.x) {
+        const int partner = t ^ jj;
+        if (partner > t) {
+          const unsigned long long a = keys[t], b = keys[partner];
+          const bool up = (t & k) == 0;
+          if ((a > b) == up) {
+            keys[t] = b;
+            keys[partner] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < count; t += blockDim.x) {
+    const unsigned long long key = keys[t];
+    nbr_j[begin + t] = (int)(key >> 32);
+    nbr_s[begin + t] = (int)(key & 0xffffffffu);
+  }
+}
+
+}  // namespace
+
+hipError_t xr_coord_bounds(int n, const double* x, CellDev cell, double* bounds6, hipStream_t s) {
+  hipLaunchKernelGGL(k_coord_bounds, dim3(1), dim3(256), 0, s, n, x, cell, bounds6);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_bin_atoms(int n, const double* x, CellDev cell, BinGeom g, int* home, int* bin, int* bin_count,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_atoms, dim3((n + 255) / 256), dim3(256), 0, s, n, x, cell, g, home, bin, bin_count);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_bin_scatter(int n, const int* bin, const int* bin_start, int* bin_fill, int* binned, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, bin, bin_start, bin_fill, binned);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_neighbor_count(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
+                             const int* bin_start, const int* binned, double cutoff, int row_cap, int* counts,
+                             unsigned* err, hipStream_t s) {
+  const int cap = row_cap < ROW_CAP ? row_cap : ROW_CAP;
+  hipLaunchKernelGGL(k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start, binned,
+                     cutoff * cutoff, cap, counts, err);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_neighbor_fill(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
+                            const int* bin_start, const int* binned, double cutoff, const int* row_ptr, int* nbr_j,
+                            int* nbr_s, unsigned* err, hipStream_t s) {
+  (void)err;
+  hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start, binned,
+                     cutoff * cutoff, row_ptr, nbr_j, nbr_s);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}

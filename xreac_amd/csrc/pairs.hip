@@ -1,0 +1,135 @@
+// Tapered van der Waals + shielded Coulomb over the full directed edge list,
+// QEq self energy, and the final energy reduction.
+//
+// Energies follow src/xreac/energy.py:213-219: per directed edge
+//   vdW  = 0.5 * taper * D * (ev^2 - 2 ev) [+ 0.5 * taper * ecore exp(acore (1 - r/rcore))]
+//   Coul = 0.5 * 332.06371 * q_i q_j * shield
+// One wave owns one atom's row, so the force on atom i is the row sum
+// F_i = sum_e phi'(r_e) v_e / r_e (both directions of a pair contribute half
+// of phi each), with no scatter to neighbours.
+#include "state.h"
+
+namespace {
+
+constexpr double C_ELE = 332.06371;
+constexpr double SELF_CONVERSION = 23.02;
+
+__global__ void __launch_bounds__(256) k_nonbonded(Atoms at, CellDev cell, FFDev ff, const int* __restrict__ row_ptr,
+                                                   const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
+                                                   EnergyAcc acc) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  double e[3] = {0.0, 0.0, 0.0};  // vdw, coulomb, qeq
+  if (i < at.n) {
+    const double* g = ff.gen;
+    const double rc = g[12];
+    const double pw = g[28];
+    const int ti = at.type[i];
+    const double qi = at.q[i];
+    const bool shielded = ff.vdw_type == 1 || ff.vdw_type == 3;
+    const bool wall = ff.vdw_type == 2 || ff.vdw_type == 3;
+    double ev_sum = 0.0, ec_sum = 0.0, F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    for (int ed = row_ptr[i] + lane; ed < row_ptr[i + 1]; ed += 64) {
+      const int j = nbr_j[ed];
+      const int tj = at.type[j];
+      const double* p = ff.pair + (ti * ff.nt + tj) * PP_COUNT;
+      double v[3];
+      const double r = xr_edge(at.x, cell, i, j, nbr_s[ed], v);
+      double t, dt;
+      xr_taper(r, rc, t, dt);
+      if (t == 0.0 && dt == 0.0) continue;
+      double f13, df13;
+      if (shielded) {
+        const double rp = pow(r, pw);
+        const double base = rp + pow(1 / p[PP_GAMMAW], pw);
+        f13 = pow(base, 1 / pw);
+        df13 = rp / r * f13 / base;
+      } else {
+        f13 = r;
+        df13 = 1.0;
+      }
+      const double ex = exp(0.5 * p[PP_ALPHA] * (1 - f13 / p[PP_RVDW]));
+      const double ev = p[PP_D] * (ex * ex - 2 * ex);
+      const double dev = p[PP_D] * (2 * ex - 2) * ex * (-0.5 * p[PP_ALPHA] / p[PP_RVDW]) * df13;
+      double phi = t * ev;  // per-direction vdW energy (times 2)
+      double dphi = dt * ev + t * dev;
+      if (wall) {
+        const double ecr = p[PP_ECORE] * exp(p[PP_ACORE] * (1 - r / p[PP_RCORE]));
+        phi += t * ecr;
+        dphi += dt * ecr + t * ecr * (-p[PP_ACORE] / p[PP_RCORE]);
+      }
+      double s, ds;
+      xr_shield(r, rc, p[PP_GAMMA], s, ds);
+      const double qq = qi * at.q[j];
+      ev_sum += phi;
+      ec_sum += qq * s;
+      const double dtot = (dphi + C_ELE * qq * ds) / r;
+      F0 += dtot * v[0];
+      F1 += dtot * v[1];
+      F2 += dtot * v[2];
+    }
+    ev_sum = xr_wave_sum(ev_sum);
+    ec_sum = xr_wave_sum(ec_sum);
+    F0 = xr_wave_sum(F0);
+    F1 = xr_wave_sum(F1);
+    F2 = xr_wave_sum(F2);
+    if (lane == 0) {
+      const double* a = ff.atom + ti * AP_COUNT;
+      e[0] = 0.5 * ev_sum;
+      e[1] = 0.5 * C_ELE * ec_sum;
+      e[2] = SELF_CONVERSION * (a[AP_CHI] * qi + 0.5 * a[AP_ETA] * qi * qi);
+      at.F_nb[3 * i] = F0;
+      at.F_nb[3 * i + 1] = F1;
+      at.F_nb[3 * i + 2] = F2;
+    }
+  }
+  double ev[1] = {e[0]}, ec[1] = {e[1]}, eq[1] = {e[2]};
+  xr_block_partials<1>(ev, acc.partials + CH_VDW * acc.cap, gridDim.x);
+  xr_block_partials<1>(ec, acc.partials + CH_COUL * acc.cap, gridDim.x);
+  xr_block_partials<1>(eq, acc.partials + CH_QEQ * acc.cap, gridDim.x);
+}
+
+__global__ void k_energy_reduce(EnergyAcc acc, double* __restrict__ comps) {
+  __shared__ double ch[NCH];
+  const int lane = threadIdx.x;
+  for (int c = 0; c < NCH; ++c) {
+    double s = 0.0;
+    for (int b = lane; b < acc.cap; b += 64) s += acc.partials[c * acc.cap + b];
+    s = xr_wave_sum(s);
+    if (lane == 0) ch[c] = s;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double out[NCOMP];
+    for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+    out[C_BOND] = ch[CH_BOND] + ch[CH_TRIPLE];
+    out[C_ATOM] = ch[CH_ATOM];
+    out[C_LONE] = ch[CH_LP] + ch[CH_C2];
+    out[C_ANGLE] = ch[CH_ANGLE];
+    out[C_PENALTY] = ch[CH_PEN];
+    out[C_COA] = ch[CH_COA];
+    out[C_HBOND] = ch[CH_HB];
+    out[C_TORSION] = ch[CH_TOR];
+    out[C_CONJ] = ch[CH_CONJ];
+    out[C_VDW] = ch[CH_VDW];
+    out[C_COULOMB] = ch[CH_COUL];
+    out[C_QEQ] = ch[CH_QEQ];
+    for (int k = 0; k < NCOMP; ++k) comps[k] = out[k];
+  }
+}
+
+}  // namespace
+
+hipError_t xr_nonbonded(const Atoms& at, CellDev cell, FFDev ff, const int* row_ptr, const int* nbr_j,
+                        const int* nbr_s, EnergyAcc acc, hipStream_t s) {
+  const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
+  hipLaunchKernelGGL(k_nonbonded, grid, dim3(256), 0, s, at, cell, ff, row_ptr, nbr_j, nbr_s, acc);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_energy_reduce(EnergyAcc acc, double* comps, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(64), 0, s, acc, comps);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}

@@ -1,0 +1,64 @@
+// Device state of one evaluation: per-bond and per-atom arrays, forward values
+// and their adjoints.  All arrays are carved from one workspace allocation.
+#pragma once
+#include "common.h"
+
+// Energy accumulation channels; several channels feed one reported component.
+enum Channel {
+  CH_BOND, CH_TRIPLE, CH_LP, CH_C2, CH_ATOM, CH_ANGLE, CH_PEN, CH_COA, CH_HB, CH_TOR, CH_CONJ, CH_VDW, CH_COUL,
+  CH_QEQ, NCH
+};
+
+struct EnergyAcc {
+  double* partials;  // [NCH][cap]
+  int cap;           // partial slots per channel (zeroed before each evaluation)
+};
+
+struct Bonds {
+  int nb;                  // active directed bonds
+  const int* ptr;          // [n+1]
+  const int* edge;         // [nb] canonical edge index
+  const int* bi;           // [nb] row atom
+  int* bj;                 // [nb] far atom
+  int* code;               // [nb] packed image shift
+  int* rev;                // [nb] reverse bond slot
+  double* vec;             // [nb*3] edge vector x_j - x_i + S@L
+  double* r;               // [nb]
+  double* bop;             // [nb] uncorrected bo' = raw - cutoff
+  double* pim;             // [nb] uncorrected pi'
+  double* ppm;             // [nb] uncorrected pipi'
+  double* dsig, *dpi, *dpp;   // d/dr of sigma', pi', pipi'
+  double* f1, *f4, *f5;    // correction factors
+  double* boc, *pic, *ppc; // corrected, before the 1e-10 threshold
+  double* BO, *PI, *PP, *SIG; // thresholded outputs
+  // adjoints
+  double* A_BO, *A_PI, *A_PP, *A_SIG, *A_totfar;
+  double* G;               // [nb*3] dE/d(edge vector)
+  double* a_bop, *a_pim, *a_ppm, *c_i, *c_j;
+};
+
+struct Atoms {
+  int n;
+  const double* x;
+  const int* type;
+  double* totp;            // uncorrected total bond order
+  double* total, *d, *db, *nlp, *vlpex, *dlp, *dlpt, *sbo, *sbo2, *prod, *ov2;
+  int* bond_count;         // bonds with BO > 0.3
+  double *A_total, *A_d, *A_db, *A_nlp, *A_dlpt, *A_dlp, *A_sbo2, *a_ov2, *a_totp;
+  double* F_nb;            // [n*3] nonbonded forces
+  double* F_hb;            // [n*3] hydrogen-bond acceptor-vector forces
+  double* q;               // charges
+};
+
+// bondorder.hip
+hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* nbr_j,
+                           const int* nbr_s, EnergyAcc acc, unsigned* err, hipStream_t s);
+hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, double* forces, hipStream_t s);
+// manybody.hip
+hipError_t xr_manybody(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* row_ptr,
+                       const int* nbr_j, const int* nbr_s, int lammps_angles, EnergyAcc acc, unsigned* err,
+                       hipStream_t s);
+// pairs.hip
+hipError_t xr_nonbonded(const Atoms& at, CellDev cell, FFDev ff, const int* row_ptr, const int* nbr_j,
+                        const int* nbr_s, EnergyAcc acc, hipStream_t s);
+hipError_t xr_energy_reduce(EnergyAcc acc, double* comps, hipStream_t s);

@@ -1,0 +1,164 @@
+"""Device engine: one force field bound to one GPU context and stream.
+
+Thin owner of the C handles of ``libxreac_hip.so``.  Inputs and outputs are
+device tensors; PyTorch only supplies device memory and the stream.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .tables import build_tables
+
+COMPONENTS = (
+    "bond",
+    "atom",
+    "lone_pair",
+    "molecule",
+    "angle",
+    "penalty",
+    "angle_conjugation",
+    "hydrogen_bond",
+    "torsion",
+    "conjugation",
+    "vdw",
+    "coulomb",
+    "electric_field",
+    "qeq",
+)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data if isinstance(a, np.ndarray) else int(a.data_ptr()))
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+class Engine:
+    def __init__(self, force_field, device=0):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("xreac_amd evaluates on an AMD GPU; no ROCm device is visible")
+        self.torch = torch
+        self.lib = _lib.load()
+        self.device = torch.device("cuda", device)
+        self.tables = build_tables(force_field)
+        t = self.tables
+        keep = [
+            t.general, t.atom, t.pair, t.angle_offsets, t.angle_rows if len(t.angle_rows) else np.zeros((1, 7)),
+            t.torsion, t.torsion_on, t.hbond, t.hbond_on,
+        ]
+        tab = _lib.FFTables(
+            len(t.labels), t.vdw_type, *[C.c_void_p(a.ctypes.data) for a in keep]
+        )
+        self._ff = C.c_void_p()
+        self._check(self.lib.xreac_ff_create(C.byref(tab), C.byref(self._ff)), None)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        self._ctx = C.c_void_p()
+        with torch.cuda.device(self.device):
+            self._check(self.lib.xreac_ctx_create(device, C.c_void_p(stream), C.byref(self._ctx)), None)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self.lib.xreac_ctx_destroy(self._ctx)
+            self._ctx = None
+        if getattr(self, "_ff", None):
+            self.lib.xreac_ff_destroy(self._ff)
+            self._ff = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, status, ctx):
+        if status == _lib.XREAC_OK:
+            return
+        msg = self.lib.xreac_error_message(ctx).decode() if ctx is not None else "xreac call failed"
+        if status == _lib.XREAC_INVALID:
+            raise ValueError(msg)
+        raise DeviceError(msg)
+
+    def type_ids(self, symbols):
+        idx = self.tables.index
+        return np.array([idx[s] for s in symbols], dtype=np.int32)
+
+    def evaluate(self, positions, types, cell=None, pbc=(False, False, False), total_charge=0.0, *, tol=1e-10,
+                 max_iter=2000, lammps_angles=True, properties=True):
+        """Evaluate one system whose positions (n,3 float64) and types (n int32) are device tensors.
+
+        Returns a dict of device tensors (forces, charges and optionally
+        total_bond_orders, lone_pairs, bond_counts), the 14 components (host
+        numpy) and the edge/bond/iteration counts.
+        """
+        torch = self.torch
+        n = int(positions.shape[0])
+        assert positions.dtype == torch.float64 and positions.is_contiguous() and positions.shape == (n, 3)
+        assert types.dtype == torch.int32 and types.is_contiguous() and types.shape == (n,)
+        dev = positions.device
+        forces = torch.empty((n, 3), dtype=torch.float64, device=dev)
+        charges = torch.empty(n, dtype=torch.float64, device=dev)
+        tbo = lp = bc = None
+        if properties:
+            tbo = torch.empty(n, dtype=torch.float64, device=dev)
+            lp = torch.empty(n, dtype=torch.float64, device=dev)
+            bc = torch.empty(n, dtype=torch.int32, device=dev)
+        cell_arr = None
+        if cell is not None:
+            cell_arr = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
+        sysd = _lib.System(
+            n,
+            _ptr(positions),
+            _ptr(types),
+            C.c_void_p(cell_arr.ctypes.data) if cell_arr is not None else None,
+            (C.c_int32 * 3)(*[int(bool(v)) for v in pbc]),
+            float(total_charge),
+        )
+        opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0)
+        res = _lib.Result()
+        res.forces = _ptr(forces)
+        res.charges = _ptr(charges)
+        res.total_bond_orders = _ptr(tbo) if tbo is not None else None
+        res.lone_pairs = _ptr(lp) if lp is not None else None
+        res.bond_counts = _ptr(bc) if bc is not None else None
+        with torch.cuda.device(dev):
+            status = self.lib.xreac_evaluate(self._ctx, self._ff, C.byref(sysd), C.byref(opt), C.byref(res))
+        self._check(status, self._ctx)
+        return dict(
+            forces=forces,
+            charges=charges,
+            total_bond_orders=tbo,
+            lone_pairs=lp,
+            bond_counts=bc,
+            components=np.array(res.components[:], dtype=np.float64),
+            num_edges=int(res.num_edges),
+            num_bonds=int(res.num_bonds),
+            qeq_iterations=int(res.qeq_iterations),
+            n=n,
+        )
+
+    def neighbors(self, n, m):
+        """Canonical (row_ptr, j, S) of the last evaluation, host arrays."""
+        row_ptr = np.zeros(n + 1, dtype=np.int32)
+        j = np.zeros(max(m, 1), dtype=np.int32)
+        S = np.zeros((max(m, 1), 3), dtype=np.int32)
+        self._check(self.lib.xreac_copy_neighbors(self._ctx, _ptr(row_ptr), _ptr(j), _ptr(S), max(m, 1)), self._ctx)
+        return row_ptr, j[:m], S[:m]
+
+    def bonds(self, nb):
+        """Active directed bonds (i, j, S, BO) of the last evaluation, host arrays."""
+        i = np.zeros(max(nb, 1), dtype=np.int32)
+        j = np.zeros(max(nb, 1), dtype=np.int32)
+        S = np.zeros((max(nb, 1), 3), dtype=np.int32)
+        bo = np.zeros(max(nb, 1), dtype=np.float64)
+        self._check(
+            self.lib.xreac_copy_bonds(self._ctx, _ptr(i), _ptr(j), _ptr(S), _ptr(bo), max(nb, 1)), self._ctx
+        )
+        return i[:nb], j[:nb], S[:nb], bo[:nb]
