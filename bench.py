@@ -1,0 +1,264 @@
+#!/usr/bin/env python
+"""Benchmark: ReaxFF energy + forces + QEq evaluations on one MI355X.
+
+Workload (default): the 4096-water periodic box, 12,288 atoms, ffield.reax.HO.2015,
+plain FP64, QEq relative residual 1e-10.  Geometry: 16^3 jittered lattice with
+random orientations (seed 1), then 200 steepest-descent steps on the GPU
+(x += 0.002 F, per-atom step capped at 0.05 A) before timing.
+
+A step is one complete evaluation (neighbor build, QEq, all energy terms,
+fixed-charge forces, bond properties) with positions already resident in HBM.
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name, n, m, nb):
+    """Algorithmic HBM bytes per launch of a timed kernel class.
+
+    cg_direction: the implicit QEq matrix read once in half storage (value 8 B
+    + column 4 B per unordered pair, P = M/2) plus 72 B per atom (read z and
+    previous p, write p and A p as FP64 pairs, row pointer).
+    """
+    P = m / 2
+    if name == "cg_direction":
+        return 12 * P + 72 * n
+    if name == "cg_update":
+        return 112 * n  # read/write x, r (FP64 pairs), read p, Ap, write z, diag
+    if name == "nonbonded":
+        return 16 * P + 56 * n  # half list (j + image code) + positions, types, charges, forces
+    if name == "edge":
+        return 8 * P + 12 * P + 40 * n  # half list read, H write, positions
+    if name == "neighbor_fill":
+        return 16 * P + 32 * n
+    return None
+
+
+def build_workload(args, eng, torch):
+    from xreac_amd import workloads
+
+    symbols, x, cell = workloads.water_box(side=args.side, seed=1)
+    types = torch.as_tensor(eng.type_ids(symbols), device=eng.device)
+    pbc = (True, True, True)
+    cache = args.geometry_cache
+    if cache and os.path.exists(cache):
+        data = np.load(cache)
+        if data["x"].shape == x.shape and int(data["prerelax"]) == args.prerelax:
+            log(f"[bench] loaded pre-relaxed geometry from {cache}")
+            return symbols, data["x"], cell, types, pbc
+
+    def forces(y):
+        xd = torch.as_tensor(y, device=eng.device)
+        out = eng.evaluate(xd, types, cell=cell, pbc=pbc, tol=args.qeq_tol, properties=False)
+        return out["forces"].cpu().numpy()
+
+    t0 = time.time()
+    x = workloads.steepest_descent(forces, x, steps=args.prerelax)
+    log(f"[bench] pre-relaxed {len(symbols)} atoms, {args.prerelax} steepest-descent steps in {time.time() - t0:.1f}s")
+    if cache:
+        np.savez(cache, x=x, prerelax=args.prerelax)
+    return symbols, x, cell, types, pbc
+
+
+def copy_calibration(nbytes, reps=5):
+    """A plain device-to-device copy of known size, for scaling HBM counters."""
+    import torch
+
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "reps": reps}), flush=True)
+
+
+def cpu_baseline(args, ff):
+    """Time the FP64 CPU oracle on a bounded sample of the same workload."""
+    import torch
+
+    from oracle import reaxff as orc
+    from xreac_amd import workloads
+
+    symbols, x, cell = workloads.water_box(side=args.cpu_side, seed=1)
+    t0 = time.time()
+    reps = 0
+    while True:
+        orc.evaluate(ff, symbols, x, cell=cell, pbc=True)
+        reps += 1
+        if time.time() - t0 > args.cpu_seconds or reps >= 5:
+            break
+    dt = (time.time() - t0) / reps
+    return {
+        "value": len(symbols) / dt,
+        "unit": "atom-evals/s",
+        "cores": int(torch.get_num_threads()),
+        "kind": "port",
+        "sample": (
+            f"{reps} evaluation(s) of a {len(symbols)}-atom water box ({args.cpu_side}^3 waters, same generator and "
+            f"density, no pre-relax), E+F+QEq with the FP64 CPU oracle (NumPy/torch-CPU, dense LAPACK QEq); "
+            f"{dt:.2f} s per evaluation"
+        ),
+    }
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc, from profiles/, if recorded."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data.get(name, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--side", type=int, default=16, help="waters per box edge (16 -> 12,288 atoms)")
+    ap.add_argument("--prerelax", type=int, default=200)
+    ap.add_argument("--qeq-tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-side", type=int, default=8, help="CPU sample box edge in waters")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-timers", action="store_true", help="do not bracket kernels with hipEvents")
+    ap.add_argument("--geometry-cache", default=None, help="npz file to reuse the pre-relaxed geometry")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (HBM counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(
+            "bench.py measures one GPU: this build has no multi-GPU path (requests would shard across GPUs "
+            "by whole request, and large systems by spatial domains; not implemented)",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import torch
+
+    from xreac_amd import ForceField
+    from xreac_amd.engine import Engine
+
+    ff = ForceField.bundled("ffield.reax.HO.2015")
+    eng = Engine(ff, 0)
+    symbols, x, cell, types, pbc = build_workload(args, eng, torch)
+    n = len(symbols)
+    xd = torch.as_tensor(x, device=eng.device).contiguous()
+
+    def step():
+        return eng.evaluate(xd, types, cell=cell, pbc=pbc, tol=args.qeq_tol, properties=True)
+
+    for _ in range(args.warmup):
+        out = step()
+    torch.cuda.synchronize()
+    if not args.no_timers:
+        eng.profile(True)
+        eng.timers(reset=True)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        out = step()
+        lat.append(time.perf_counter() - s0)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    timers = eng.timers(reset=True) if not args.no_timers else {}
+    eng.profile(False)
+    m, nb, iters = out["num_edges"], out["num_bonds"], out["qeq_iterations"]
+    value = n * args.steps / elapsed
+
+    kernels = {}
+    for name, (ms, launches) in timers.items():
+        if launches == 0:
+            continue
+        avg_us = 1e3 * ms / launches
+        row = {"total_ms_per_step": ms / args.steps, "launches_per_step": launches / args.steps, "avg_us": avg_us}
+        b = kernel_bytes(name, n, m, nb)
+        if b is not None:
+            row["algorithmic_bytes_per_launch"] = b
+            row["achieved_GBs"] = b / (avg_us * 1e-6) / 1e9
+        kernels[name] = row
+    roofline = None
+    if kernels:
+        dom = max((k for k in kernels if "achieved_GBs" in kernels[k]), key=lambda k: kernels[k]["total_ms_per_step"])
+        r = kernels[dom]
+        roofline = {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": r["achieved_GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": r["achieved_GBs"] / HBM_PEAK_GBS,
+            "traffic": read_traffic(dom),
+            "avg_launch_us": r["avg_us"],
+            "algorithmic_bytes_per_launch": r["algorithmic_bytes_per_launch"],
+        }
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU oracle on a bounded sample")
+        baseline = cpu_baseline(args, ff)
+
+    lat_ms = np.array(lat) * 1e3
+    result = {
+        "metric": "atom-evals/s (E+F+QEq), plain FP64 GPU path",
+        "value": value,
+        "unit": "atom-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"periodic water box, {n} atoms ({args.side}^3 H2O, 0.997 g/cm^3, L={cell[0, 0]:.4f} A), "
+            f"ffield.reax.HO.2015, {args.prerelax}-step steepest-descent pre-relax",
+            "atoms": n,
+            "directed_edges": m,
+            "active_bonds": nb,
+            "qeq_tolerance": args.qeq_tol,
+            "qeq_iterations": iters,
+            "precision": "plain FP64",
+        },
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99))},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

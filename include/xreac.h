@@ -44,6 +44,22 @@ enum {
                              hydrogen_bond torsion conjugation vdw coulomb electric_field qeq */
 };
 
+/* Kernel timers (xreac_profile_read): neighbor count, neighbor fill, edge
+ * pass, CG direction+A p sweep, CG update, nonbonded, bond-order forward,
+ * many-body forward, adjoint. */
+enum {
+  XREAC_TIMER_NEIGHBOR_COUNT = 0,
+  XREAC_TIMER_NEIGHBOR_FILL = 1,
+  XREAC_TIMER_EDGE = 2,
+  XREAC_TIMER_CG_DIRECTION = 3,
+  XREAC_TIMER_CG_UPDATE = 4,
+  XREAC_TIMER_NONBONDED = 5,
+  XREAC_TIMER_BOND_FORWARD = 6,
+  XREAC_TIMER_MANYBODY = 7,
+  XREAC_TIMER_BACKWARD = 8,
+  XREAC_TIMERS = 9
+};
+
 typedef struct xreac_ff xreac_ff;
 typedef struct xreac_ctx xreac_ctx;
 
@@ -103,6 +119,12 @@ const char* xreac_error_message(const xreac_ctx* ctx);
 /* Energy, QEq charges and fixed-charge forces (calculator.py:76-165). */
 int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ff, const xreac_system* sys, const xreac_options* opt,
                    xreac_result* out);
+
+/* Kernel timing with hipEvents on the context stream (no reference
+ * counterpart; measurement only).  Enabled, every launch of a timed kernel is
+ * bracketed by two events and its elapsed time summed per timer. */
+int xreac_profile_enable(xreac_ctx* ctx, int enable);
+int xreac_profile_read(xreac_ctx* ctx, int timer, double* total_ms, int64_t* launches, int reset);
 
 /* Canonical neighbor list of the last evaluation (host arrays): row_ptr [n+1],
  * j [M], shifts [M*3].  capacity is the length of j; returns XREAC_INVALID if

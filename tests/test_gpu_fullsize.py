@@ -1,0 +1,72 @@
+"""Full-size parity on the 12,288-atom water box against committed oracle results.
+
+The CPU oracle needs minutes at this size, so its results were computed once
+by tests/golden/make_water_box_fixture.py and committed; the geometry is
+regenerated from its seed.  Checks: neighbor list and active-bond set by
+SHA-256 and count, energy and components to 1e-9 relative, charges (1e-7 e) and
+forces (1e-6 kcal/mol/A) on 512 seeded atoms, plus the size-independent sums
+(charges sum to the total charge, forces sum to zero).
+"""
+
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from xreac_amd import ForceField, workloads
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "water_box_12288.npz")
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a, dtype=np.int64).tobytes())
+    return h.hexdigest()
+
+
+def test_water_box_12288_matches_oracle():
+    import torch
+
+    from xreac_amd.engine import Engine
+
+    ref = np.load(FIXTURE)
+    ff = ForceField.bundled("ffield.reax.HO.2015")
+    eng = Engine(ff)
+    s, x, cell = workloads.water_box(side=16, seed=1)
+    n = len(s)
+    print(f"[fullsize] water box n={n}", flush=True)
+    out = eng.evaluate(
+        torch.as_tensor(x, device=eng.device),
+        torch.as_tensor(eng.type_ids(s), device=eng.device),
+        cell=cell,
+        pbc=(True, True, True),
+        tol=1e-10,
+    )
+    m, nb = out["num_edges"], out["num_bonds"]
+    assert m == int(ref["num_edges"])
+    row_ptr, j, S = eng.neighbors(n, m)
+    i = np.repeat(np.arange(n), np.diff(row_ptr))
+    assert digest(i, j, S) == str(ref["neighbor_sha256"])
+    assert nb == int(ref["num_bonds"])
+    bi, bj, bS, _ = eng.bonds(nb)
+    assert digest(bi, bj, bS) == str(ref["bond_sha256"])
+    comps = out["components"]
+    e = float(comps.sum())
+    e_ref = float(ref["energy"])
+    assert abs(e - e_ref) <= 1e-9 * abs(e_ref), (e, e_ref)
+    for k, (a, b) in enumerate(zip(comps, ref["components"])):
+        assert abs(a - b) <= 1e-9 * abs(e_ref), (k, a, b)
+    sub = ref["subset"]
+    q = out["charges"].cpu().numpy()
+    f = out["forces"].cpu().numpy()
+    np.testing.assert_allclose(q[sub], ref["charges"], rtol=0, atol=1e-7)
+    np.testing.assert_allclose(f[sub], ref["forces"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(out["total_bond_orders"].cpu().numpy()[sub], ref["total_bond_orders"], atol=1e-8)
+    np.testing.assert_allclose(out["lone_pairs"].cpu().numpy()[sub], ref["lone_pairs"], atol=1e-8)
+    np.testing.assert_array_equal(out["bond_counts"].cpu().numpy()[sub], ref["bond_counts"])
+    assert abs(q.sum()) < 1e-9
+    np.testing.assert_allclose(f.sum(axis=0), 0.0, atol=1e-6)

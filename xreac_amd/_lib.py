@@ -23,6 +23,19 @@ EXPORTED = (
     "xreac_evaluate",
     "xreac_copy_neighbors",
     "xreac_copy_bonds",
+    "xreac_profile_enable",
+    "xreac_profile_read",
+)
+TIMERS = (
+    "neighbor_count",
+    "neighbor_fill",
+    "edge",
+    "cg_direction",
+    "cg_update",
+    "nonbonded",
+    "bond_forward",
+    "manybody",
+    "backward",
 )
 
 
@@ -107,5 +120,9 @@ def load():
     lib.xreac_copy_neighbors.restype = C.c_int
     lib.xreac_copy_bonds.argtypes = [vp, vp, vp, vp, vp, C.c_int64]
     lib.xreac_copy_bonds.restype = C.c_int
+    lib.xreac_profile_enable.argtypes = [vp, C.c_int]
+    lib.xreac_profile_enable.restype = C.c_int
+    lib.xreac_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
+    lib.xreac_profile_read.restype = C.c_int
     _LIB = lib
     return lib

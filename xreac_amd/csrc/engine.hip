@@ -17,12 +17,16 @@
 
 #include "../../include/xreac.h"
 #include "launch.h"
+#include "profile.h"
 #include "qeq.h"
 #include "state.h"
 
 static_assert(XREAC_ATOM_PARAMS == AP_COUNT, "atom table layout");
 static_assert(XREAC_PAIR_PARAMS == PP_COUNT, "pair table layout");
 static_assert(XREAC_COMPONENTS == NCOMP, "component count");
+static_assert(XREAC_TIMERS == XT_COUNT, "timer count");
+
+thread_local XrProfiler* xr_active_profiler = nullptr;
 
 struct xreac_ff {
   FFDev dev{};
@@ -78,6 +82,7 @@ struct xreac_ctx {
   hipStream_t stream = nullptr;
   std::string message;
   Buffer atom_buf, bin_buf, edge_buf, bond_buf;
+  XrProfiler prof;
   unsigned* err = nullptr;       // device
   unsigned* err_host = nullptr;  // pinned
   double* comps = nullptr;       // device [NCOMP]
@@ -230,6 +235,17 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   hipStream_t s = ctx->stream;
   const FFDev& ff = ffh->dev;
+  // Timers are resolved after every synchronisation that may end the call.
+  struct ActiveProfiler {
+    XrProfiler* p;
+    explicit ActiveProfiler(XrProfiler* q) : p(q) { xr_active_profiler = q; }
+    ~ActiveProfiler() {
+      // An early return leaves recorded events unresolved; drop them.
+      p->pending.clear();
+      p->used = 0;
+      xr_active_profiler = nullptr;
+    }
+  } active(&ctx->prof);
   const double rc = ffh->cutoff;
   const double bond_cut = std::min(5.0, rc);
 
@@ -515,9 +531,16 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   HIPCHK(hipMemsetAsync(epart, 0, sizeof(double) * NCH * cap, s), "memset");
   HIPCHK(hipMemsetAsync(at.F_hb, 0, sizeof(double) * 3 * (size_t)n, s), "memset");
   HIPCHK(xr_nonbonded(at, cell, ff, row_ptr, nbr_j, nbr_s, acc, s), "nonbonded");
-  HIPCHK(xr_bond_forward(at, bd, cell, ff, nbr_j, nbr_s, acc, ctx->err, s), "bond orders");
-  HIPCHK(xr_manybody(at, bd, cell, ff, row_ptr, nbr_j, nbr_s, opt->lammps_angles, acc, ctx->err, s), "many-body");
-  HIPCHK(xr_bond_backward(at, bd, ff, out->forces, s), "adjoint");
+  {
+    hipError_t e = hipSuccess;
+    XR_TIMED(XT_BOND_FORWARD, s, e = xr_bond_forward(at, bd, cell, ff, nbr_j, nbr_s, acc, ctx->err, s));
+    HIPCHK(e, "bond orders");
+    XR_TIMED(XT_MANYBODY, s,
+             e = xr_manybody(at, bd, cell, ff, row_ptr, nbr_j, nbr_s, opt->lammps_angles, acc, ctx->err, s));
+    HIPCHK(e, "many-body");
+    XR_TIMED(XT_BACKWARD, s, e = xr_bond_backward(at, bd, ff, out->forces, s));
+    HIPCHK(e, "adjoint");
+  }
   HIPCHK(xr_energy_reduce(acc, ctx->comps, s), "energy");
   if (out->total_bond_orders)
     HIPCHK(hipMemcpyAsync(out->total_bond_orders, at.total, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
@@ -528,6 +551,7 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   double* host_comps = reinterpret_cast<double*>(ctx->err_host) + 8;
   HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, 64 * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
   HIPCHK(hipStreamSynchronize(s), "final sync");
+  ctx->prof.resolve();
   if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
   std::memcpy(out->components, host_comps, sizeof(double) * NCOMP);
   out->num_edges = m;
@@ -544,6 +568,23 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   ctx->bond_j = bd.bj;
   ctx->bond_code = bd.code;
   ctx->bond_BO = bd.BO;
+  return XREAC_OK;
+}
+
+int xreac_profile_enable(xreac_ctx* ctx, int enable) {
+  if (!ctx) return XREAC_INVALID;
+  ctx->prof.on = enable != 0;
+  return XREAC_OK;
+}
+
+int xreac_profile_read(xreac_ctx* ctx, int timer, double* total_ms, int64_t* launches, int reset) {
+  if (!ctx || timer < 0 || timer >= XT_COUNT || !total_ms || !launches) return XREAC_INVALID;
+  *total_ms = ctx->prof.total_ms[timer];
+  *launches = ctx->prof.launches[timer];
+  if (reset) {
+    ctx->prof.total_ms[timer] = 0.0;
+    ctx->prof.launches[timer] = 0;
+  }
   return XREAC_OK;
 }
 

@@ -11,6 +11,7 @@
 // (j << 32) | packed(S), which is the reference's (i, j, Sx, Sy, Sz)
 // lexicographic order within row i (neighbors.py:65-98).
 #include "launch.h"
+#include "profile.h"
 
 namespace {
 
@@ -247,8 +248,9 @@ hipError_t xr_neighbor_count(int n, const double* x, CellDev cell, BinGeom g, co
                              const int* bin_start, const int* binned, double cutoff, int row_cap, int* counts,
                              unsigned* err, hipStream_t s) {
   const int cap = row_cap < ROW_CAP ? row_cap : ROW_CAP;
-  hipLaunchKernelGGL(k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start, binned,
-                     cutoff * cutoff, cap, counts, err);
+  XR_TIMED(XT_NBR_COUNT, s,
+           hipLaunchKernelGGL(k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start,
+                              binned, cutoff * cutoff, cap, counts, err));
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
@@ -257,8 +259,9 @@ hipError_t xr_neighbor_fill(int n, const double* x, CellDev cell, BinGeom g, con
                             const int* bin_start, const int* binned, double cutoff, const int* row_ptr, int* nbr_j,
                             int* nbr_s, unsigned* err, hipStream_t s) {
   (void)err;
-  hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start, binned,
-                     cutoff * cutoff, row_ptr, nbr_j, nbr_s);
+  XR_TIMED(XT_NBR_FILL, s,
+           hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start, binned,
+                              cutoff * cutoff, row_ptr, nbr_j, nbr_s));
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

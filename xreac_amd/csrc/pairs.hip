@@ -8,6 +8,7 @@
 // F_i = sum_e phi'(r_e) v_e / r_e (both directions of a pair contribute half
 // of phi each), with no scatter to neighbours.
 #include "state.h"
+#include "profile.h"
 
 namespace {
 
@@ -123,7 +124,8 @@ __global__ void k_energy_reduce(EnergyAcc acc, double* __restrict__ comps) {
 hipError_t xr_nonbonded(const Atoms& at, CellDev cell, FFDev ff, const int* row_ptr, const int* nbr_j,
                         const int* nbr_s, EnergyAcc acc, hipStream_t s) {
   const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  hipLaunchKernelGGL(k_nonbonded, grid, dim3(256), 0, s, at, cell, ff, row_ptr, nbr_j, nbr_s, acc);
+  XR_TIMED(XT_NONBONDED, s,
+           hipLaunchKernelGGL(k_nonbonded, grid, dim3(256), 0, s, at, cell, ff, row_ptr, nbr_j, nbr_s, acc));
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -1,0 +1,77 @@
+// Optional per-kernel timing with hipEvents on the launching stream.
+//
+// When enabled on a context, launches wrapped in XR_TIMED are bracketed by a
+// pair of pooled events; elapsed times are summed per timer after the
+// evaluation's final synchronisation.  Disabled, XR_TIMED costs one branch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+enum XrTimer {
+  XT_NBR_COUNT, XT_NBR_FILL, XT_EDGE, XT_CG_DIRECTION, XT_CG_UPDATE, XT_NONBONDED, XT_BOND_FORWARD, XT_MANYBODY,
+  XT_BACKWARD, XT_COUNT
+};
+
+struct XrProfiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> pending;
+  double total_ms[XT_COUNT] = {};
+  long long launches[XT_COUNT] = {};
+
+  void begin(int id, hipStream_t s) {
+    if (!on) return;
+    if (used + 2 > pool.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+          on = false;
+          return;
+        }
+        pool.push_back(e);
+      }
+    }
+    pending.emplace_back(id, used);
+    (void)hipEventRecord(pool[used], s);
+    used += 2;
+  }
+  void end(hipStream_t s) {
+    if (!on || pending.empty()) return;
+    (void)hipEventRecord(pool[pending.back().second + 1], s);
+  }
+  // Call after the stream has been synchronised.
+  void resolve() {
+    for (auto& p : pending) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, pool[p.second], pool[p.second + 1]) == hipSuccess) {
+        total_ms[p.first] += ms;
+        launches[p.first] += 1;
+      }
+    }
+    pending.clear();
+    used = 0;
+  }
+  void reset() {
+    for (int k = 0; k < XT_COUNT; ++k) {
+      total_ms[k] = 0.0;
+      launches[k] = 0;
+    }
+  }
+  ~XrProfiler() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// The profiler of the context currently evaluating (set by the engine).
+extern thread_local XrProfiler* xr_active_profiler;
+
+#define XR_TIMED(id, stream, launch)                                  \
+  do {                                                                \
+    XrProfiler* p__ = xr_active_profiler;                             \
+    if (p__ && p__->on) p__->begin((id), (stream));                   \
+    launch;                                                           \
+    if (p__ && p__->on) p__->end(stream);                             \
+  } while (0)

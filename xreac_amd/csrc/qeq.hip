@@ -15,6 +15,7 @@
 // launches are needed and the result is deterministic.
 #include "launch.h"
 #include "qeq.h"
+#include "profile.h"
 
 namespace {
 
@@ -268,8 +269,9 @@ hipError_t xr_edge_prep(int n, const double* x, const int* type, CellDev cell, F
                         const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, unsigned char* bflag,
                         int* bond_count, hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
-  hipLaunchKernelGGL(k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, cell, ff, row_ptr, nbr_j, nbr_s,
-                     bond_cut, hval, bflag, bond_count);
+  XR_TIMED(XT_EDGE, s,
+           hipLaunchKernelGGL(k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, cell, ff, row_ptr, nbr_j,
+                              nbr_s, bond_cut, hval, bflag, bond_count));
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
@@ -306,10 +308,13 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, int* iterations) {
     for (int c = 0; c < chunk && k < a.max_iter; ++c, ++k) {
       double2* pnew = (k & 1) ? a.P1 : a.P0;
       double2* pold = (k & 1) ? a.P0 : a.P1;
-      hipLaunchKernelGGL(k_cg_direction, dim3(na), dim3(256), 0, s, n, k, a.type, a.ff, a.row_ptr, a.nbr_j,
-                         a.hval, a.Z, pold, pnew, a.AP, part_b, part_bb, nb, part_a, tol2, a.state);
-      hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, n, k, a.inv_diag, a.X, a.R, a.Z, pnew, a.AP,
-                         part_a, na, part_b, part_bb, tol2, a.state);
+      XR_TIMED(XT_CG_DIRECTION, s,
+               hipLaunchKernelGGL(k_cg_direction, dim3(na), dim3(256), 0, s, n, k, a.type, a.ff, a.row_ptr,
+                                  a.nbr_j, a.hval, a.Z, pold, pnew, a.AP, part_b, part_bb, nb, part_a, tol2,
+                                  a.state));
+      XR_TIMED(XT_CG_UPDATE, s,
+               hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, n, k, a.inv_diag, a.X, a.R, a.Z, pnew,
+                                  a.AP, part_a, na, part_b, part_bb, tol2, a.state));
     }
     XR_LAUNCH_CHECK();
     hipError_t e = hipMemcpyAsync(host_state, a.state, sizeof(host_state), hipMemcpyDeviceToHost, s);

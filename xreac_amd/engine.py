@@ -144,6 +144,19 @@ class Engine:
             n=n,
         )
 
+    def profile(self, enable=True):
+        """Enable or disable hipEvent timing of the library's kernels."""
+        self._check(self.lib.xreac_profile_enable(self._ctx, 1 if enable else 0), self._ctx)
+
+    def timers(self, reset=True):
+        """{timer: (total_ms, launches)} accumulated since the last reset."""
+        out = {}
+        for k, name in enumerate(_lib.TIMERS):
+            ms, cnt = C.c_double(), C.c_int64()
+            self._check(self.lib.xreac_profile_read(self._ctx, k, C.byref(ms), C.byref(cnt), int(reset)), self._ctx)
+            out[name] = (ms.value, cnt.value)
+        return out
+
     def neighbors(self, n, m):
         """Canonical (row_ptr, j, S) of the last evaluation, host arrays."""
         row_ptr = np.zeros(n + 1, dtype=np.int32)
