@@ -418,31 +418,34 @@ __global__ void k_force(Atoms at, Bonds bd, double* __restrict__ F) {
   F[3 * i + 2] = f2;
 }
 
-inline dim3 grid(int count) { return dim3(count > 0 ? (count + 255) / 256 : 1); }
+// One wave per block: per-atom and per-bond kernels have a few thousand
+// threads, and small blocks spread them over every CU.
+constexpr int BLK = 64;
+inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 1); }
 
 }  // namespace
 
 hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* nbr_j,
                            const int* nbr_s, EnergyAcc acc, unsigned* err, hipStream_t s) {
   if (bd.nb > 0) {
-    hipLaunchKernelGGL(k_bond_geom, grid(bd.nb), dim3(256), 0, s, at, bd, cell, ff, nbr_j, nbr_s, err);
-    hipLaunchKernelGGL(k_bond_reverse, grid(bd.nb), dim3(256), 0, s, bd, err);
+    hipLaunchKernelGGL(k_bond_geom, grid(bd.nb), dim3(BLK), 0, s, at, bd, cell, ff, nbr_j, nbr_s, err);
+    hipLaunchKernelGGL(k_bond_reverse, grid(bd.nb), dim3(BLK), 0, s, bd, err);
   }
-  hipLaunchKernelGGL(k_atom_totalp, grid(at.n), dim3(256), 0, s, at, bd);
-  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_correct, grid(bd.nb), dim3(256), 0, s, at, bd, ff);
-  hipLaunchKernelGGL(k_atom_forward, grid(at.n), dim3(256), 0, s, at, bd, ff, acc);
-  hipLaunchKernelGGL(k_atom_forward2, grid(at.n), dim3(256), 0, s, at, bd, ff, acc);
-  hipLaunchKernelGGL(k_triple, grid(at.n), dim3(256), 0, s, at, bd, ff, acc);
+  hipLaunchKernelGGL(k_atom_totalp, grid(at.n), dim3(BLK), 0, s, at, bd);
+  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_correct, grid(bd.nb), dim3(BLK), 0, s, at, bd, ff);
+  hipLaunchKernelGGL(k_atom_forward, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
+  hipLaunchKernelGGL(k_atom_forward2, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
+  hipLaunchKernelGGL(k_triple, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
 hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, double* forces, hipStream_t s) {
-  hipLaunchKernelGGL(k_atom_backward, grid(at.n), dim3(256), 0, s, at, bd, ff);
-  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_backward1, grid(bd.nb), dim3(256), 0, s, at, bd, ff);
-  hipLaunchKernelGGL(k_atom_backward2, grid(at.n), dim3(256), 0, s, at, bd);
-  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_backward2, grid(bd.nb), dim3(256), 0, s, at, bd);
-  hipLaunchKernelGGL(k_force, grid(at.n), dim3(256), 0, s, at, bd, forces);
+  hipLaunchKernelGGL(k_atom_backward, grid(at.n), dim3(BLK), 0, s, at, bd, ff);
+  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_backward1, grid(bd.nb), dim3(BLK), 0, s, at, bd, ff);
+  hipLaunchKernelGGL(k_atom_backward2, grid(at.n), dim3(BLK), 0, s, at, bd);
+  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_backward2, grid(bd.nb), dim3(BLK), 0, s, at, bd);
+  hipLaunchKernelGGL(k_force, grid(at.n), dim3(BLK), 0, s, at, bd, forces);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -83,6 +83,7 @@ struct xreac_ctx {
   std::string message;
   Buffer atom_buf, bin_buf, edge_buf, bond_buf;
   XrProfiler prof;
+  int last_cg_iters = 0;
   unsigned* err = nullptr;       // device
   unsigned* err_host = nullptr;  // pinned
   double* comps = nullptr;       // device [NCOMP]
@@ -281,7 +282,7 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   carve_size<double2>(off, 6 * (size_t)n);  // CG vectors
   carve_size<double>(off, n);            // inv_diag
   carve_size<double>(off, xr_cg_partials(n));
-  carve_size<int>(off, 4);               // cg state
+  carve_size<char>(off, xr_cg_state_bytes());  // cg scalars
   carve_size<double>(off, 2);            // mu
   carve_size<double>(off, (size_t)kAtomScalars * n);  // atom scalars
   carve_size<int>(off, n);               // bond counts (>0.3)
@@ -300,7 +301,7 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   double2* cgv = cv.take<double2>(6 * (size_t)n);
   double* inv_diag = cv.take<double>(n);
   double* cg_part = cv.take<double>(xr_cg_partials(n));
-  int* cg_state = cv.take<int>(4);
+  void* cg_state = cv.take<char>(xr_cg_state_bytes());
   double* mu = cv.take<double>(2);
   double* scal = cv.take<double>((size_t)kAtomScalars * n);
   double* const scal_end = scal + (size_t)kAtomScalars * n;
@@ -367,25 +368,26 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   for (int k = 0; k < 3; ++k) {
     g.pbc[k] = cell.pbc[k];
     if (cell.pbc[k]) {
-      int nbk = (int)std::floor(height[k] / rcb);
-      nbk = std::max(1, std::min(nbk, 64));
+      // Bins at least rc/2 thick along the face normal, searched two deep.
+      int nbk = (int)std::floor(2.0 * height[k] / rcb);
+      nbk = std::max(1, std::min(nbk, 128));
       g.nb[k] = nbk;
       g.w[k] = 1.0 / nbk;
       g.lo[k] = 0.0;
       const double h = height[k] / nbk;
       g.reach[k] = h >= rcb ? 1 : (int)std::floor(rcb / h) + 1;
     } else {
-      double w = rcb / height[k];
+      double w = 0.5 * rcb / height[k];
       const double span = bounds[3 + k] - bounds[k];
       long long nbk = (long long)std::floor(span / w) + 1;
-      if (nbk > 64) {
-        w = span / 63.0;
-        nbk = 64;
+      if (nbk > 128) {
+        w = span / 127.0;
+        nbk = 128;
       }
       g.nb[k] = (int)nbk;
       g.w[k] = w;
       g.lo[k] = bounds[k];
-      g.reach[k] = 1;
+      g.reach[k] = (int)std::floor(rcb / (w * height[k])) + 1;
     }
     nbins *= g.nb[k];
   }
@@ -517,7 +519,8 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   cg.state = cg_state;
   cg.tol = opt->qeq_tolerance > 0 ? opt->qeq_tolerance : 1e-10;
   cg.max_iter = opt->qeq_max_iterations > 0 ? opt->qeq_max_iterations : 2000;
-  cg.first_check = 8;
+  // Read back convergence first near the previous solve's iteration count.
+  cg.first_check = ctx->last_cg_iters > 0 ? std::max(4, ctx->last_cg_iters - 1) : 16;
   cg.check_every = 4;
   cg.total_charge = sys->total_charge;
   cg.q = out->charges;
@@ -526,6 +529,7 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   HIPCHK(xr_cg_solve(cg, s, &iters), "QEq");
   if (iters < 0) return fail(ctx, XREAC_INVALID, "QEq conjugate gradient did not converge; check the geometry");
   out->qeq_iterations = iters;
+  ctx->last_cg_iters = iters;
 
   // ----------------------------------------------------- energy and forces
   HIPCHK(hipMemsetAsync(epart, 0, sizeof(double) * NCH * cap, s), "memset");

@@ -429,15 +429,18 @@ __global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, CellDev
   }
 }
 
-inline dim3 grid(int count) { return dim3(count > 0 ? (count + 255) / 256 : 1); }
+// One wave per block: per-atom and per-bond kernels have a few thousand
+// threads, and small blocks spread them over every CU.
+constexpr int BLK = 64;
+inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 1); }
 
 }  // namespace
 
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* row_ptr,
                        const int* nbr_j, const int* nbr_s, int lammps_angles, EnergyAcc acc, unsigned* err,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_angles, grid(at.n), dim3(256), 0, s, at, bd, ff, lammps_angles, acc);
-  hipLaunchKernelGGL(k_torsions, grid(at.n), dim3(256), 0, s, at, bd, ff, acc, err);
+  hipLaunchKernelGGL(k_angles, grid(at.n), dim3(BLK), 0, s, at, bd, ff, lammps_angles, acc);
+  hipLaunchKernelGGL(k_torsions, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc, err);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
   hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, cell, ff, row_ptr, nbr_j, nbr_s, acc);
   hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, cell, ff, row_ptr, nbr_j, nbr_s);

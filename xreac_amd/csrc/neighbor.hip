@@ -91,8 +91,11 @@ __global__ void k_bin_scatter(int n, const int* __restrict__ bin, const int* __r
 }
 
 // Visit every candidate image (j, T) of atom i in the surrounding bins; calls
-// fn(j, packed_shift, r2) for those passing the exact distance test.  Lanes
-// stride over the atoms of each bin (lane = t, stride = nthreads).
+// fn(j, packed_shift, r2) for those passing the exact distance test.  Bins
+// with consecutive z index are contiguous in `binned`, so for each (x, y)
+// offset the z-range is walked as one or a few contiguous segments (split only
+// where the periodic image shift changes); lanes stride over each segment
+// (lane = t, stride = nthreads).
 template <class Fn>
 __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, const CellDev& cell, const BinGeom& g,
                                          const int* __restrict__ home, const int* __restrict__ bin,
@@ -102,6 +105,11 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
   const int bz = id % g.nb[2], by = (id / g.nb[2]) % g.nb[1], bx = id / (g.nb[2] * g.nb[1]);
   const int hi0 = home[3 * i], hi1 = home[3 * i + 1], hi2 = home[3 * i + 2];
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+  int zlo = bz - g.reach[2], zhi = bz + g.reach[2];
+  if (!g.pbc[2]) {
+    zlo = zlo < 0 ? 0 : zlo;
+    zhi = zhi >= g.nb[2] ? g.nb[2] - 1 : zhi;
+  }
   for (int o0 = -g.reach[0]; o0 <= g.reach[0]; ++o0) {
     int c0 = bx + o0, t0 = 0;
     if (g.pbc[0]) {
@@ -118,21 +126,27 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
       } else if (c1 < 0 || c1 >= g.nb[1]) {
         continue;
       }
-      for (int o2 = -g.reach[2]; o2 <= g.reach[2]; ++o2) {
-        int c2 = bz + o2, t2 = 0;
+      const int col = (c0 * g.nb[1] + c1) * g.nb[2];
+      const int sx0 = g.pbc[0] ? t0 + hi0 : 0;
+      const int sy0 = g.pbc[1] ? t1 + hi1 : 0;
+      int zz = zlo;
+      while (zz <= zhi) {
+        // one segment: consecutive z bins sharing the same image shift t2
+        const int t2 = g.pbc[2] ? floordiv(zz, g.nb[2]) : 0;
+        const int cz0 = zz - t2 * g.nb[2];
+        int last = zhi;
         if (g.pbc[2]) {
-          t2 = floordiv(c2, g.nb[2]);
-          c2 -= t2 * g.nb[2];
-        } else if (c2 < 0 || c2 >= g.nb[2]) {
-          continue;
+          const int seg_end = (t2 + 1) * g.nb[2] - 1;
+          last = last < seg_end ? last : seg_end;
         }
-        const int cb = (c0 * g.nb[1] + c1) * g.nb[2] + c2;
-        const int s = bin_start[cb], e = bin_start[cb + 1];
+        const int cz1 = last - t2 * g.nb[2];
+        const int s = bin_start[col + cz0], e = bin_start[col + cz1 + 1];
+        const int sz0 = g.pbc[2] ? t2 + hi2 : 0;
         for (int p = s + t; p < e; p += nthreads) {
           const int j = binned[p];
-          const int sx = g.pbc[0] ? t0 - home[3 * j] + hi0 : 0;
-          const int sy = g.pbc[1] ? t1 - home[3 * j + 1] + hi1 : 0;
-          const int sz = g.pbc[2] ? t2 - home[3 * j + 2] + hi2 : 0;
+          const int sx = g.pbc[0] ? sx0 - home[3 * j] : 0;
+          const int sy = g.pbc[1] ? sy0 - home[3 * j + 1] : 0;
+          const int sz = g.pbc[2] ? sz0 - home[3 * j + 2] : 0;
           if (sx < -511 || sx > 511 || sy < -511 || sy > 511 || sz < -511 || sz > 511) {
             err |= XR_ERR_SHIFT_RANGE;
             continue;
@@ -146,6 +160,7 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
           const double r2 = (v0 * v0 + v1 * v1) + v2 * v2;
           if (r2 <= rc2 && !(j == i && sx == 0 && sy == 0 && sz == 0)) fn(j, code, r2);
         }
+        zz = last + 1;
       }
     }
   }
@@ -176,7 +191,7 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
   }
 }
 
-__global__ void __launch_bounds__(256) k_nbr_fill(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
+__global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
                                                   const int* __restrict__ home, const int* __restrict__ bin,
                                                   const int* __restrict__ bin_start, const int* __restrict__ binned,
                                                   double rc2, const int* __restrict__ row_ptr, int* __restrict__ nbr_j,
@@ -260,7 +275,7 @@ hipError_t xr_neighbor_fill(int n, const double* x, CellDev cell, BinGeom g, con
                             int* nbr_s, unsigned* err, hipStream_t s) {
   (void)err;
   XR_TIMED(XT_NBR_FILL, s,
-           hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start, binned,
+           hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(64), 0, s, n, x, cell, g, home, bin, bin_start, binned,
                               cutoff * cutoff, row_ptr, nbr_j, nbr_s));
   XR_LAUNCH_CHECK();
   return hipSuccess;

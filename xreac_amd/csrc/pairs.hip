@@ -91,16 +91,17 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, CellDev cell, FFDev
 }
 
 __global__ void k_energy_reduce(EnergyAcc acc, double* __restrict__ comps) {
+  // One wave per channel, fixed order within each.
   __shared__ double ch[NCH];
-  const int lane = threadIdx.x;
-  for (int c = 0; c < NCH; ++c) {
+  const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
+  if (c < NCH) {
     double s = 0.0;
     for (int b = lane; b < acc.cap; b += 64) s += acc.partials[c * acc.cap + b];
     s = xr_wave_sum(s);
     if (lane == 0) ch[c] = s;
   }
   __syncthreads();
-  if (lane == 0) {
+  if (threadIdx.x == 0) {
     double out[NCOMP];
     for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
     out[C_BOND] = ch[CH_BOND] + ch[CH_TRIPLE];
@@ -131,7 +132,7 @@ hipError_t xr_nonbonded(const Atoms& at, CellDev cell, FFDev ff, const int* row_
 }
 
 hipError_t xr_energy_reduce(EnergyAcc acc, double* comps, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(64), 0, s, acc, comps);
+  hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(64 * NCH), 0, s, acc, comps);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

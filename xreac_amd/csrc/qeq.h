@@ -12,7 +12,7 @@ struct CgArgs {
   double2 *X, *R, *Z, *P0, *P1, *AP;
   double* inv_diag;
   double* partials;  // xr_cg_partials(n) doubles
-  int* state;        // 2 ints: done flag, iterations
+  void* state;       // xr_cg_state_bytes() bytes
   double tol;        // relative residual per right-hand side
   int max_iter;
   int first_check;   // iterations before the first convergence read-back
@@ -28,4 +28,5 @@ hipError_t xr_edge_prep(int n, const double* x, const int* type, CellDev cell, F
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int* bond_edge,
                         int* bond_i, hipStream_t s);
 size_t xr_cg_partials(int n);
+size_t xr_cg_state_bytes();
 hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, int* iterations);

@@ -79,136 +79,183 @@ __global__ void __launch_bounds__(256) k_bond_fill(int n, const int* __restrict_
 }
 
 // --------------------------------------------------------------------- CG
-// Every block reduces `nb` per-block partials (layout [k * nb + blk]) of `nv`
-// values, in the same order.  Result broadcast through LDS.
+// Scalars shared by the CG kernels (device memory).  Each kernel's per-block
+// partial dot products are reduced once, in a fixed order, by whichever block
+// finishes last (integer atomic ticket), which then writes alpha or beta.
+struct CgScalars {
+  double rz[2], rr[2], bb[2], alpha[2], beta[2];
+  int done, iters;
+  unsigned ticket;
+  int pad;
+};
+
+__device__ inline double load_agent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// True in exactly one block: the last to arrive.  Partials stored by any
+// block before the call are visible to it.
+__device__ inline bool last_block(unsigned* ticket) {
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+// Fixed-order sum of NV partial arrays ([k * nb + b]) by the first wave.
 template <int NV>
-__device__ inline void reduce_all(const double* __restrict__ part, int nb, double (&out)[NV]) {
+__device__ inline void reduce_partials(const double* part, int nb, double (&out)[NV]) {
   __shared__ double res[NV];
   if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int b = lane; b < nb; b += 64) s += part[k * nb + b];
+      for (int b = threadIdx.x; b < nb; b += 64) s += load_agent(part + k * nb + b);
       s = xr_wave_sum(s);
-      if (lane == 0) res[k] = s;
+      if (threadIdx.x == 0) res[k] = s;
     }
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = res[k];
-  __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) k_cg_init(int n, const int* __restrict__ type, FFDev ff,
-                                                 double2* __restrict__ X, double2* __restrict__ R,
-                                                 double2* __restrict__ Z, double* __restrict__ inv_diag,
-                                                 double* __restrict__ part_b, double* __restrict__ part_bb, int nblk,
-                                                 int* __restrict__ state) {
+constexpr int UPD_BLOCK = 128;
+
+__global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(int n, const int* __restrict__ type, FFDev ff,
+                                                       double2* __restrict__ X, double2* __restrict__ R,
+                                                       double2* __restrict__ Z, double* __restrict__ inv_diag,
+                                                       double* __restrict__ eta_out, double* __restrict__ part,
+                                                       double tol2, CgScalars* __restrict__ sc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double vals[4] = {0, 0, 0, 0};
-  double bb[2] = {0, 0};
   if (i < n) {
     const double* a = ff.atom + type[i] * AP_COUNT;
-    const double d = 1.0 / a[AP_ETA];
+    const double eta = a[AP_ETA];
+    const double d = 1.0 / eta;
     const double2 r = make_double2(-a[AP_CHI], -1.0);
     X[i] = make_double2(0.0, 0.0);
     R[i] = r;
     const double2 z = make_double2(r.x * d, r.y * d);
     Z[i] = z;
     inv_diag[i] = d;
+    eta_out[i] = eta;
     vals[0] = r.x * z.x;
     vals[1] = r.y * z.y;
     vals[2] = r.x * r.x;
     vals[3] = r.y * r.y;
-    bb[0] = vals[2];
-    bb[1] = vals[3];
   }
-  // slot 1 holds the residual state "before iteration 0"
-  xr_block_partials<4>(vals, part_b + 1 * 4 * nblk, nblk);
-  xr_block_partials<2>(bb, part_bb, nblk);
-  if (i == 0) {
-    state[0] = 0;   // done flag
-    state[1] = -1;  // iterations at convergence
+  xr_block_partials<4>(vals, part, gridDim.x);
+  if (last_block(&sc->ticket)) {
+    double s[4];
+    reduce_partials<4>(part, gridDim.x, s);
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 2; ++k) {
+        sc->rz[k] = s[k];
+        sc->rr[k] = s[2 + k];
+        sc->bb[k] = s[2 + k];  // b = r at x = 0
+        sc->beta[k] = 0.0;
+        sc->alpha[k] = 0.0;
+      }
+      const bool conv = s[2] <= tol2 * s[2] && s[3] <= tol2 * s[3];
+      sc->done = conv ? 1 : 0;
+      sc->iters = conv ? 0 : -1;
+      sc->ticket = 0;
+    }
   }
 }
 
-// Iteration k, first half: beta, new search direction and A p.
-__global__ void __launch_bounds__(256) k_cg_direction(int n, int k, const int* __restrict__ type, FFDev ff,
-                                                      const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                                      const double* __restrict__ hval, const double2* __restrict__ Z,
+// Iteration k, first half: p = z + beta p_old, A p, and alpha = (r.z)/(p.Ap).
+__global__ void __launch_bounds__(256) k_cg_direction(int n, int k, const int* __restrict__ row_ptr,
+                                                      const int* __restrict__ nbr_j, const double* __restrict__ hval,
+                                                      const double* __restrict__ eta, const double2* __restrict__ Z,
                                                       const double2* __restrict__ Pold, double2* __restrict__ Pnew,
-                                                      double2* __restrict__ AP, const double* __restrict__ part_b,
-                                                      const double* __restrict__ part_bb, int nblk_b,
-                                                      double* __restrict__ part_a, double tol2,
-                                                      int* __restrict__ state) {
-  if (state[0]) return;
-  double cur[4], prev[4], bb[2];
-  reduce_all<4>(part_b + ((k - 1) & 1) * 4 * nblk_b, nblk_b, cur);
-  reduce_all<2>(part_bb, nblk_b, bb);
-  const bool conv_s = cur[2] <= tol2 * bb[0];
-  const bool conv_t = cur[3] <= tol2 * bb[1];
-  if (conv_s && conv_t) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      state[0] = 1;
-      state[1] = k;
-    }
-    return;
-  }
-  double beta_s = 0.0, beta_t = 0.0;
-  if (k > 0) {
-    reduce_all<4>(part_b + (k & 1) * 4 * nblk_b, nblk_b, prev);
-    if (!conv_s && prev[0] != 0.0) beta_s = cur[0] / prev[0];
-    if (!conv_t && prev[1] != 0.0) beta_t = cur[1] / prev[1];
-  }
+                                                      double2* __restrict__ AP, double* __restrict__ part,
+                                                      double tol2, CgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const double beta_s = sc->beta[0], beta_t = sc->beta[1];
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double vals[2] = {0.0, 0.0};
   if (i < n) {
-    double acc_s = 0.0, acc_t = 0.0;
-    for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
-      const int j = nbr_j[e];
-      const double2 z = Z[j];
-      const double2 po = k > 0 ? Pold[j] : make_double2(0.0, 0.0);
-      const double h = hval[e];
-      acc_s += h * (z.x + beta_s * po.x);
-      acc_t += h * (z.y + beta_t * po.y);
+    double as0 = 0.0, at0 = 0.0, as1 = 0.0, at1 = 0.0;
+    const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+    int e = e0 + lane;
+    if (k > 0) {
+      for (; e + 64 < e1; e += 128) {
+        const int ja = nbr_j[e], jb = nbr_j[e + 64];
+        const double ha = hval[e], hb = hval[e + 64];
+        const double2 za = Z[ja], zb = Z[jb], pa = Pold[ja], pb = Pold[jb];
+        as0 += ha * (za.x + beta_s * pa.x);
+        at0 += ha * (za.y + beta_t * pa.y);
+        as1 += hb * (zb.x + beta_s * pb.x);
+        at1 += hb * (zb.y + beta_t * pb.y);
+      }
+      if (e < e1) {
+        const int ja = nbr_j[e];
+        const double ha = hval[e];
+        const double2 za = Z[ja], pa = Pold[ja];
+        as0 += ha * (za.x + beta_s * pa.x);
+        at0 += ha * (za.y + beta_t * pa.y);
+      }
+    } else {
+      for (; e + 64 < e1; e += 128) {
+        const int ja = nbr_j[e], jb = nbr_j[e + 64];
+        const double ha = hval[e], hb = hval[e + 64];
+        const double2 za = Z[ja], zb = Z[jb];
+        as0 += ha * za.x;
+        at0 += ha * za.y;
+        as1 += hb * zb.x;
+        at1 += hb * zb.y;
+      }
+      if (e < e1) {
+        const int ja = nbr_j[e];
+        const double ha = hval[e];
+        const double2 za = Z[ja];
+        as0 += ha * za.x;
+        at0 += ha * za.y;
+      }
     }
-    acc_s = xr_wave_sum(acc_s);
-    acc_t = xr_wave_sum(acc_t);
+    const double acc_s = xr_wave_sum(as0 + as1);
+    const double acc_t = xr_wave_sum(at0 + at1);
     if (lane == 0) {
       const double2 z = Z[i];
       const double2 po = k > 0 ? Pold[i] : make_double2(0.0, 0.0);
       const double2 p = make_double2(z.x + beta_s * po.x, z.y + beta_t * po.y);
-      const double eta = ff.atom[type[i] * AP_COUNT + AP_ETA];
-      const double2 ap = make_double2(eta * p.x + acc_s, eta * p.y + acc_t);
+      const double d = eta[i];
+      const double2 ap = make_double2(d * p.x + acc_s, d * p.y + acc_t);
       Pnew[i] = p;
       AP[i] = ap;
       vals[0] = p.x * ap.x;
       vals[1] = p.y * ap.y;
     }
   }
-  xr_block_partials<2>(vals, part_a, gridDim.x);
+  xr_block_partials<2>(vals, part, gridDim.x);
+  if (last_block(&sc->ticket)) {
+    double pap[2];
+    reduce_partials<2>(part, gridDim.x, pap);
+    if (threadIdx.x == 0) {
+      for (int c = 0; c < 2; ++c) {
+        const bool conv = sc->rr[c] <= tol2 * sc->bb[c];
+        sc->alpha[c] = (!conv && pap[c] != 0.0) ? sc->rz[c] / pap[c] : 0.0;
+      }
+      sc->ticket = 0;
+    }
+  }
 }
 
-// Iteration k, second half: alpha, update x, r, z and the residual dots.
-__global__ void __launch_bounds__(256) k_cg_update(int n, int k, const double* __restrict__ inv_diag,
-                                                   double2* __restrict__ X, double2* __restrict__ R,
-                                                   double2* __restrict__ Z, const double2* __restrict__ P,
-                                                   const double2* __restrict__ AP, const double* __restrict__ part_a,
-                                                   int nblk_a, double* __restrict__ part_b,
-                                                   const double* __restrict__ part_bb, double tol2,
-                                                   const int* __restrict__ state) {
-  if (state[0]) return;
-  const int nblk = gridDim.x;
-  double cur[4], pap[2], bb[2];
-  reduce_all<4>(part_b + ((k - 1) & 1) * 4 * nblk, nblk, cur);
-  reduce_all<2>(part_a, nblk_a, pap);
-  reduce_all<2>(part_bb, nblk, bb);
-  const bool conv_s = cur[2] <= tol2 * bb[0];
-  const bool conv_t = cur[3] <= tol2 * bb[1];
-  const double alpha_s = (!conv_s && pap[0] != 0.0) ? cur[0] / pap[0] : 0.0;
-  const double alpha_t = (!conv_t && pap[1] != 0.0) ? cur[1] / pap[1] : 0.0;
+// Iteration k, second half: x += alpha p, r -= alpha A p, z = r / diag,
+// then beta, the residual test and the done flag.
+__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(int n, int k, const double* __restrict__ inv_diag,
+                                                         double2* __restrict__ X, double2* __restrict__ R,
+                                                         double2* __restrict__ Z, const double2* __restrict__ P,
+                                                         const double2* __restrict__ AP, double* __restrict__ part,
+                                                         double tol2, CgScalars* __restrict__ sc) {
+  if (sc->done) return;
+  const double alpha_s = sc->alpha[0], alpha_t = sc->alpha[1];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double vals[4] = {0, 0, 0, 0};
   if (i < n) {
@@ -228,7 +275,26 @@ __global__ void __launch_bounds__(256) k_cg_update(int n, int k, const double* _
     vals[2] = r.x * r.x;
     vals[3] = r.y * r.y;
   }
-  xr_block_partials<4>(vals, part_b + (k & 1) * 4 * nblk, nblk);
+  xr_block_partials<4>(vals, part, gridDim.x);
+  if (last_block(&sc->ticket)) {
+    double s[4];
+    reduce_partials<4>(part, gridDim.x, s);
+    if (threadIdx.x == 0) {
+      bool all = true;
+      for (int c = 0; c < 2; ++c) {
+        const bool conv = s[2 + c] <= tol2 * sc->bb[c];
+        all = all && conv;
+        sc->beta[c] = (!conv && sc->rz[c] != 0.0) ? s[c] / sc->rz[c] : 0.0;
+        sc->rz[c] = s[c];
+        sc->rr[c] = s[2 + c];
+      }
+      if (all) {
+        sc->done = 1;
+        sc->iters = k + 1;
+      }
+      sc->ticket = 0;
+    }
+  }
 }
 
 // q = s + mu t with mu = (Q - sum s) / sum t.
@@ -285,21 +351,26 @@ hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, c
 }
 
 size_t xr_cg_partials(int n) {
-  const int nb = (n + 255) / 256;
+  const int nb = (n + UPD_BLOCK - 1) / UPD_BLOCK;
   const int na = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
-  return (size_t)(2 * 4 * nb + 2 * nb + 2 * na);
+  return (size_t)(4 * nb + 2 * na + 2 * n) + 64;
 }
+
+size_t xr_cg_state_bytes() { return sizeof(CgScalars); }
 
 hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, int* iterations) {
   const int n = a.n;
-  const int nb = (n + 255) / 256;
+  const int nb = (n + UPD_BLOCK - 1) / UPD_BLOCK;
   const int na = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
   double* part_b = a.partials;
-  double* part_bb = part_b + 2 * 4 * nb;
-  double* part_a = part_bb + 2 * nb;
+  double* part_a = part_b + 4 * nb;
+  double* eta = part_a + 2 * na;
+  CgScalars* sc = static_cast<CgScalars*>(a.state);
   const double tol2 = a.tol * a.tol;
-  hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(256), 0, s, n, a.type, a.ff, a.X, a.R, a.Z, a.inv_diag, part_b,
-                     part_bb, nb, a.state);
+  hipError_t e = hipMemsetAsync(&sc->ticket, 0, sizeof(unsigned), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(UPD_BLOCK), 0, s, n, a.type, a.ff, a.X, a.R, a.Z, a.inv_diag, eta,
+                     part_b, tol2, sc);
   XR_LAUNCH_CHECK();
   int k = 0;
   int host_state[2] = {0, -1};
@@ -309,30 +380,18 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, int* iterations) {
       double2* pnew = (k & 1) ? a.P1 : a.P0;
       double2* pold = (k & 1) ? a.P0 : a.P1;
       XR_TIMED(XT_CG_DIRECTION, s,
-               hipLaunchKernelGGL(k_cg_direction, dim3(na), dim3(256), 0, s, n, k, a.type, a.ff, a.row_ptr,
-                                  a.nbr_j, a.hval, a.Z, pold, pnew, a.AP, part_b, part_bb, nb, part_a, tol2,
-                                  a.state));
+               hipLaunchKernelGGL(k_cg_direction, dim3(na), dim3(256), 0, s, n, k, a.row_ptr, a.nbr_j, a.hval, eta,
+                                  a.Z, pold, pnew, a.AP, part_a, tol2, sc));
       XR_TIMED(XT_CG_UPDATE, s,
-               hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(256), 0, s, n, k, a.inv_diag, a.X, a.R, a.Z, pnew,
-                                  a.AP, part_a, na, part_b, part_bb, tol2, a.state));
+               hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(UPD_BLOCK), 0, s, n, k, a.inv_diag, a.X, a.R, a.Z,
+                                  pnew, a.AP, part_b, tol2, sc));
     }
     XR_LAUNCH_CHECK();
-    hipError_t e = hipMemcpyAsync(host_state, a.state, sizeof(host_state), hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(host_state, &sc->done, sizeof(host_state), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
     if (host_state[0]) break;
-  }
-  if (!host_state[0]) {
-    // One more convergence test on the final residual.
-    double2* pnew = (k & 1) ? a.P1 : a.P0;
-    double2* pold = (k & 1) ? a.P0 : a.P1;
-    hipLaunchKernelGGL(k_cg_direction, dim3(na), dim3(256), 0, s, n, k, a.type, a.ff, a.row_ptr, a.nbr_j, a.hval,
-                       a.Z, pold, pnew, a.AP, part_b, part_bb, nb, part_a, tol2, a.state);
-    hipError_t e = hipMemcpyAsync(host_state, a.state, sizeof(host_state), hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
   }
   *iterations = host_state[0] ? host_state[1] : -1;
   hipLaunchKernelGGL(k_qeq_charges, dim3(1), dim3(1024), 0, s, n, a.X, a.total_charge, a.q, a.mu);
