@@ -173,8 +173,10 @@ def main():
     n = len(symbols)
     xd = torch.as_tensor(x, device=eng.device).contiguous()
 
+    bufs = eng.outputs(n, properties=True)
+
     def step():
-        return eng.evaluate(xd, types, cell=cell, pbc=pbc, tol=args.qeq_tol, properties=True)
+        return eng.evaluate(xd, types, cell=cell, pbc=pbc, tol=args.qeq_tol, properties=True, out=bufs)
 
     for _ in range(args.warmup):
         out = step()

@@ -41,6 +41,11 @@ def _systems():
     out["water_cluster_192"] = ("ffield.reax.HO.2015", s, x, None, None, 0.0)
     s, x, cell = workloads.cho_mixture(counts=(("CH4", 12), ("O2", 12), ("H2O", 6), ("CH3OH", 4)), length=14.0, seed=7)
     out["cho_mixture_small"] = ("ffield.reax.CHO.2008", s, x, cell, [True] * 3, 0.0)
+    # ~610 neighbors per atom within 10 A: exercises rows longer than 512
+    rng = np.random.default_rng(3)
+    grid = np.array(np.meshgrid(*[np.arange(10)] * 3, indexing="ij")).reshape(3, -1).T
+    x = grid * 1.9 + rng.normal(0, 0.1, grid.shape)
+    out["dense_oxygen_lattice"] = ("ffield.reax.HO.2015", ["O"] * len(x), x, np.diag([19.0] * 3), [True] * 3, 0.0)
     return out
 
 
@@ -113,6 +118,41 @@ def test_errors_on_gpu():
     cho = _calc("ffield.reax.CHO.2008")
     with pytest.raises(ValueError, match="Collinear atoms in an active torsion"):
         cho.evaluate(["C"] * 4, [[0, 0, 0], [1.5, 0, 0], [3, 0, 0], [4.5, 0.3, 0]])
+
+
+def test_qeq_nonconvergence_reported():
+    import torch
+
+    calc = _calc("ffield.reax.HO.2015")
+    s, x = workloads.water_cluster()
+    eng = calc.engine
+    with pytest.raises(ValueError, match="did not converge"):
+        eng.evaluate(torch.as_tensor(x, device=eng.device), torch.as_tensor(eng.type_ids(s), device=eng.device),
+                     max_iter=1)
+    ok = eng.evaluate(torch.as_tensor(x, device=eng.device), torch.as_tensor(eng.type_ids(s), device=eng.device))
+    assert ok["qeq_iterations"] > 1
+
+
+def test_buffer_hints_grow_and_match_fresh_context():
+    """A context sized by a small system must still give exact results for a denser one."""
+    import torch
+
+    from xreac_amd.engine import Engine
+
+    ff = ForceField.bundled("ffield.reax.HO.2015")
+    used = Engine(ff)
+    s1, x1 = cases.water_cases()["distorted_dimer"]
+    used.evaluate(torch.as_tensor(x1, device=used.device), torch.as_tensor(used.type_ids(s1), device=used.device))
+    s2, x2, cell2 = workloads.water_box(side=6, seed=4)
+    args = (torch.as_tensor(x2, device=used.device), torch.as_tensor(used.type_ids(s2), device=used.device))
+    grown = used.evaluate(*args, cell=cell2, pbc=(True,) * 3)
+    fresh_engine = Engine(ff)
+    fresh = fresh_engine.evaluate(*args, cell=cell2, pbc=(True,) * 3)
+    assert grown["num_edges"] == fresh["num_edges"] and grown["num_bonds"] == fresh["num_bonds"]
+    np.testing.assert_array_equal(grown["components"], fresh["components"])
+    np.testing.assert_array_equal(grown["forces"].cpu().numpy(), fresh["forces"].cpu().numpy())
+    again = used.evaluate(*args, cell=cell2, pbc=(True,) * 3)  # now from hints, no read-back
+    np.testing.assert_array_equal(again["forces"].cpu().numpy(), fresh["forces"].cpu().numpy())
 
 
 def test_deterministic_repeat():

@@ -18,7 +18,7 @@ __device__ inline const double* PPm(const FFDev& ff, int ti, int tj) { return ff
 __global__ void k_bond_geom(Atoms at, Bonds bd, CellDev cell, FFDev ff, const int* __restrict__ nbr_j,
                             const int* __restrict__ nbr_s, unsigned* __restrict__ err) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= bd.nb) return;
+  if (b >= *bd.nbp) return;
   const int e = bd.edge[b], i = bd.bi[b];
   const int j = nbr_j[e], code = nbr_s[e];
   double v[3];
@@ -51,7 +51,7 @@ __global__ void k_bond_geom(Atoms at, Bonds bd, CellDev cell, FFDev ff, const in
 
 __global__ void k_bond_reverse(Bonds bd, unsigned* __restrict__ err) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= bd.nb) return;
+  if (b >= *bd.nbp) return;
   const int i = bd.bi[b], j = bd.bj[b], want = xr_neg_shift(bd.code[b]);
   int found = -1;
   for (int s = bd.ptr[j]; s < bd.ptr[j + 1]; ++s) {
@@ -78,7 +78,7 @@ __global__ void k_atom_totalp(Atoms at, Bonds bd) {
 // Corrected bond orders (energy.py:150-165).
 __global__ void k_bond_correct(Atoms at, Bonds bd, FFDev ff) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= bd.nb) return;
+  if (b >= *bd.nbp) return;
   const double* g = ff.gen;
   const int i = bd.bi[b], j = bd.bj[b];
   const int ti = at.type[i], tj = at.type[j];
@@ -327,7 +327,7 @@ __global__ void k_atom_backward(Atoms at, Bonds bd, FFDev ff) {
 // Through thresholds and corrections to the uncorrected quantities.
 __global__ void k_bond_backward1(Atoms at, Bonds bd, FFDev ff) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= bd.nb) return;
+  if (b >= *bd.nbp) return;
   const double* g = ff.gen;
   const int i = bd.bi[b], j = bd.bj[b];
   const int ti = at.type[i], tj = at.type[j];
@@ -391,7 +391,7 @@ __global__ void k_atom_backward2(Atoms at, Bonds bd) {
 
 __global__ void k_bond_backward2(Atoms at, Bonds bd) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= bd.nb) return;
+  if (b >= *bd.nbp) return;
   const double a_bop = bd.a_bop[b] + at.a_totp[bd.bi[b]];
   const double gr = a_bop * bd.dsig[b] + (a_bop + bd.a_pim[b]) * bd.dpi[b] + (a_bop + bd.a_ppm[b]) * bd.dpp[b];
   const double s = gr / bd.r[b];
@@ -427,12 +427,12 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 
 hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* nbr_j,
                            const int* nbr_s, EnergyAcc acc, unsigned* err, hipStream_t s) {
-  if (bd.nb > 0) {
-    hipLaunchKernelGGL(k_bond_geom, grid(bd.nb), dim3(BLK), 0, s, at, bd, cell, ff, nbr_j, nbr_s, err);
-    hipLaunchKernelGGL(k_bond_reverse, grid(bd.nb), dim3(BLK), 0, s, bd, err);
+  if (bd.cap > 0) {
+    hipLaunchKernelGGL(k_bond_geom, grid(bd.cap), dim3(BLK), 0, s, at, bd, cell, ff, nbr_j, nbr_s, err);
+    hipLaunchKernelGGL(k_bond_reverse, grid(bd.cap), dim3(BLK), 0, s, bd, err);
   }
   hipLaunchKernelGGL(k_atom_totalp, grid(at.n), dim3(BLK), 0, s, at, bd);
-  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_correct, grid(bd.nb), dim3(BLK), 0, s, at, bd, ff);
+  if (bd.cap > 0) hipLaunchKernelGGL(k_bond_correct, grid(bd.cap), dim3(BLK), 0, s, at, bd, ff);
   hipLaunchKernelGGL(k_atom_forward, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
   hipLaunchKernelGGL(k_atom_forward2, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
   hipLaunchKernelGGL(k_triple, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
@@ -442,9 +442,9 @@ hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, CellDev cell, FFDev
 
 hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, double* forces, hipStream_t s) {
   hipLaunchKernelGGL(k_atom_backward, grid(at.n), dim3(BLK), 0, s, at, bd, ff);
-  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_backward1, grid(bd.nb), dim3(BLK), 0, s, at, bd, ff);
+  if (bd.cap > 0) hipLaunchKernelGGL(k_bond_backward1, grid(bd.cap), dim3(BLK), 0, s, at, bd, ff);
   hipLaunchKernelGGL(k_atom_backward2, grid(at.n), dim3(BLK), 0, s, at, bd);
-  if (bd.nb > 0) hipLaunchKernelGGL(k_bond_backward2, grid(bd.nb), dim3(BLK), 0, s, at, bd);
+  if (bd.cap > 0) hipLaunchKernelGGL(k_bond_backward2, grid(bd.cap), dim3(BLK), 0, s, at, bd);
   hipLaunchKernelGGL(k_force, grid(at.n), dim3(BLK), 0, s, at, bd, forces);
   XR_LAUNCH_CHECK();
   return hipSuccess;

@@ -26,6 +26,7 @@ enum XrErr : unsigned {
   XR_ERR_COLLINEAR = 1u << 6,
   XR_ERR_REVERSE = 1u << 7,
   XR_ERR_NONFINITE = 1u << 8,
+  XR_ERR_CAPACITY = 1u << 9,  // a buffer sized from the previous call was too small
 };
 
 // ------------------------------------------------------- parameter layouts

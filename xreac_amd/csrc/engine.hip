@@ -84,6 +84,8 @@ struct xreac_ctx {
   Buffer atom_buf, bin_buf, edge_buf, bond_buf;
   XrProfiler prof;
   int last_cg_iters = 0;
+  double m_per_atom = 0.0;  // edges and bonds per atom of the last evaluation
+  double b_per_atom = 0.0;
   unsigned* err = nullptr;       // device
   unsigned* err_host = nullptr;  // pinned
   double* comps = nullptr;       // device [NCOMP]
@@ -147,6 +149,40 @@ void invert3(const double* a, double* inv) {
   inv[6] = (a[3] * a[7] - a[4] * a[6]) / det;
   inv[7] = (a[1] * a[6] - a[0] * a[7]) / det;
   inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+// Zero up to eight byte ranges (multiples of 4 bytes) in one launch.
+struct ZeroRanges {
+  unsigned* ptr[8];
+  long long words[8];
+  int count;
+};
+
+__global__ void k_zero_ranges(ZeroRanges z) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (int r = 0; r < z.count; ++r) {
+    unsigned* p = z.ptr[r];
+    for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < z.words[r]; w += stride) p[w] = 0u;
+  }
+}
+
+// Record total = ptr[n] and clamp every pointer to `cap`, so later kernels
+// stay inside buffers sized for `cap` entries even when the total exceeds it.
+__global__ void k_clamp_ptr(int* __restrict__ ptr, int n, int cap, int* __restrict__ total,
+                            unsigned* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int v = ptr[i];
+  if (i == n) {
+    *total = v;
+    if (v > cap) atomicOr(err, (unsigned)XR_ERR_CAPACITY);
+  }
+  if (v > cap) ptr[i] = cap;
+}
+
+hipError_t clamp_ptr(int* ptr, int n, int cap, int* total, unsigned* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_clamp_ptr, dim3((n + 1 + 255) / 256), dim3(256), 0, s, ptr, n, cap, total, err);
+  return hipGetLastError();
 }
 
 template <class T>
@@ -226,10 +262,17 @@ void xreac_ctx_destroy(xreac_ctx* c) {
 
 const char* xreac_error_message(const xreac_ctx* c) { return c ? c->message.c_str() : "null context"; }
 
-int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
-                   xreac_result* out) {
-  if (!ctx || !ffh || !sys || !opt || !out) return XREAC_INVALID;
-  ctx->message.clear();
+}  // extern "C"
+
+namespace {
+
+// One attempt.  With exact == false, edge and bond buffers are sized from the
+// previous evaluation's per-atom counts and no mid-pipeline read-back happens;
+// if a count exceeds its buffer, *overflow is set and the caller repeats the
+// evaluation with exact == true (sizes read back before allocation).
+int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
+                  xreac_result* out, bool exact, bool* overflow) {
+  *overflow = false;
   const int n = sys->n;
   if (n <= 0 || !sys->positions || !sys->types || !out->forces || !out->charges)
     return fail(ctx, XREAC_INVALID, "A nonempty system with device positions, types, forces and charges is required");
@@ -278,8 +321,8 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   carve_size<int>(off, n);               // bond_count
   carve_size<int>(off, n + 1);           // bond_ptr
   carve_size<int>(off, scan_ints);       // scan scratch
-  carve_size<int>(off, n);               // binned
-  carve_size<double2>(off, 6 * (size_t)n);  // CG vectors
+  carve_size<char>(off, xr_bin_record_bytes() * n);  // bin-ordered atom records
+  carve_size<double2>(off, 7 * (size_t)n);  // CG vectors: x, r, Ap (pairs), two (z, p) records
   carve_size<double>(off, n);            // inv_diag
   carve_size<double>(off, xr_cg_partials(n));
   carve_size<char>(off, xr_cg_state_bytes());  // cg scalars
@@ -297,8 +340,8 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   int* bond_count = cv.take<int>(n);
   int* bond_ptr = cv.take<int>(n + 1);
   int* scan_tmp = cv.take<int>(scan_ints);
-  int* binned = cv.take<int>(n);
-  double2* cgv = cv.take<double2>(6 * (size_t)n);
+  void* binned = cv.take<char>(xr_bin_record_bytes() * n);
+  double2* cgv = cv.take<double2>(7 * (size_t)n);
   double* inv_diag = cv.take<double>(n);
   double* cg_part = cv.take<double>(xr_cg_partials(n));
   void* cg_state = cv.take<char>(xr_cg_state_bytes());
@@ -347,7 +390,7 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   at.q = out->charges;
   EnergyAcc acc{epart, cap};
 
-  HIPCHK(hipMemsetAsync(ctx->err, 0, sizeof(unsigned), s), "memset");
+  int* dev_counts = reinterpret_cast<int*>(ctx->err) + 1;  // [0] total edges, [1] total bonds
 
   // -------------------------------------------------------- binning grid
   BinGeom g{};
@@ -402,34 +445,54 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   int* bin_start = cb.take<int>(nbins + 1);
   int* bin_fill = cb.take<int>(nbins + 1);
   int* bin_scan = cb.take<int>(xr_scan_scratch_ints((int)nbins));
-  HIPCHK(hipMemsetAsync(bin_count, 0, sizeof(int) * (nbins + 1), s), "memset");
-  HIPCHK(hipMemsetAsync(bin_fill, 0, sizeof(int) * (nbins + 1), s), "memset");
+  {
+    ZeroRanges z{};
+    auto add = [&](void* p, size_t bytes) {
+      z.ptr[z.count] = static_cast<unsigned*>(p);
+      z.words[z.count] = (long long)(bytes / 4);
+      ++z.count;
+    };
+    add(ctx->err, 4 * sizeof(unsigned));
+    add(bin_count, sizeof(int) * (nbins + 1));
+    add(bin_fill, sizeof(int) * (nbins + 1));
+    add(epart, sizeof(double) * NCH * cap);
+    add(at.F_hb, sizeof(double) * 3 * (size_t)n);
+    hipLaunchKernelGGL(k_zero_ranges, dim3(256), dim3(256), 0, s, z);
+    HIPCHK(hipGetLastError(), "zero");
+  }
   HIPCHK(xr_bin_atoms(n, sys->positions, cell, g, home, bin, bin_count, s), "bin");
   HIPCHK(xr_exclusive_scan(bin_count, (int)nbins, bin_start, bin_scan, s), "bin scan");
-  HIPCHK(xr_bin_scatter(n, bin, bin_start, bin_fill, binned, s), "bin scatter");
+  HIPCHK(xr_bin_scatter(n, sys->positions, home, bin, bin_start, bin_fill, binned, ctx->err, s), "bin scatter");
 
   // --------------------------------------------------------- neighbor rows
   HIPCHK(xr_neighbor_count(n, sys->positions, cell, g, home, bin, bin_start, binned, rc, 2048, counts, ctx->err, s),
          "neighbor count");
   HIPCHK(xr_exclusive_scan(counts, n, row_ptr, scan_tmp, s), "neighbor scan");
-  int m_host = 0;
-  HIPCHK(hipMemcpyAsync(&m_host, row_ptr + n, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-  HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, sizeof(unsigned), hipMemcpyDeviceToHost, s), "copy");
-  HIPCHK(hipStreamSynchronize(s), "neighbor sync");
-  if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
-  const int64_t m = m_host;
+  const double slack = 1.10;
+  int64_t m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
+  exact = exact || m_cap == 0;
+  if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
+  HIPCHK(clamp_ptr(row_ptr, n, exact ? INT32_MAX : (int)m_cap, dev_counts, ctx->err, s), "clamp");
+  if (exact) {
+    int m_host = 0;
+    HIPCHK(hipMemcpyAsync(&m_host, dev_counts, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+    HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, sizeof(unsigned), hipMemcpyDeviceToHost, s), "copy");
+    HIPCHK(hipStreamSynchronize(s), "neighbor sync");
+    if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
+    m_cap = m_host;
+  }
 
   off = 0;
-  carve_size<int>(off, m);
-  carve_size<int>(off, m);
-  carve_size<double>(off, m);
-  carve_size<unsigned char>(off, m);
+  carve_size<int>(off, m_cap);
+  carve_size<int>(off, m_cap);
+  carve_size<double>(off, m_cap);
+  carve_size<unsigned char>(off, m_cap);
   HIPCHK(ctx->edge_buf.ensure(off + 4096), "edge workspace");
   Carver ce{static_cast<char*>(ctx->edge_buf.ptr)};
-  int* nbr_j = ce.take<int>(m);
-  int* nbr_s = ce.take<int>(m);
-  double* hval = ce.take<double>(m);
-  unsigned char* bflag = ce.take<unsigned char>(m);
+  int* nbr_j = ce.take<int>(m_cap);
+  int* nbr_s = ce.take<int>(m_cap);
+  double* hval = ce.take<double>(m_cap);
+  unsigned char* bflag = ce.take<unsigned char>(m_cap);
   HIPCHK(xr_neighbor_fill(n, sys->positions, cell, g, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s,
                           ctx->err, s),
          "neighbor fill");
@@ -439,10 +502,19 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
                       bond_count, s),
          "edge pass");
   HIPCHK(xr_exclusive_scan(bond_count, n, bond_ptr, scan_tmp, s), "bond scan");
-  int nb_host = 0;
-  HIPCHK(hipMemcpyAsync(&nb_host, bond_ptr + n, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-  HIPCHK(hipStreamSynchronize(s), "bond sync");
-  const int64_t nbond = nb_host;
+  int64_t b_cap = 0;
+  if (!exact) {
+    b_cap = (int64_t)std::ceil(ctx->b_per_atom * n * 1.25) + 1024;
+    if (b_cap > m_cap) b_cap = m_cap;
+  }
+  HIPCHK(clamp_ptr(bond_ptr, n, exact ? INT32_MAX : (int)b_cap, dev_counts + 1, ctx->err, s), "clamp");
+  if (exact) {
+    int nb_host = 0;
+    HIPCHK(hipMemcpyAsync(&nb_host, dev_counts + 1, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+    HIPCHK(hipStreamSynchronize(s), "bond sync");
+    b_cap = nb_host;
+  }
+  const int64_t nbond = b_cap;
 
   off = 0;
   for (int k = 0; k < 5; ++k) carve_size<int>(off, nbond);  // edge, bi, bj, code, rev
@@ -452,7 +524,8 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   HIPCHK(ctx->bond_buf.ensure(off + 4096), "bond workspace");
   Carver cbd{static_cast<char*>(ctx->bond_buf.ptr)};
   Bonds bd{};
-  bd.nb = (int)nbond;
+  bd.cap = (int)nbond;
+  bd.nbp = dev_counts + 1;
   bd.ptr = bond_ptr;
   int* b_edge = cbd.take<int>(nbond);
   int* b_i = cbd.take<int>(nbond);
@@ -498,7 +571,7 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   bd.c_i = bn();
   bd.c_j = bn();
   if (bs > bs_end) return fail(ctx, XREAC_INVALID, "internal error: per-bond workspace layout");
-  if (nbond > 0) HIPCHK(xr_bond_fill(n, row_ptr, bflag, bond_ptr, b_edge, b_i, s), "bond fill");
+  if (nbond > 0) HIPCHK(xr_bond_fill(n, row_ptr, bflag, bond_ptr, (int)nbond, b_edge, b_i, s), "bond fill");
 
   // ------------------------------------------------------------------ QEq
   CgArgs cg{};
@@ -510,30 +583,38 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   cg.hval = hval;
   cg.X = cgv;
   cg.R = cgv + n;
-  cg.Z = cgv + 2 * (size_t)n;
-  cg.P0 = cgv + 3 * (size_t)n;
-  cg.P1 = cgv + 4 * (size_t)n;
-  cg.AP = cgv + 5 * (size_t)n;
+  cg.AP = cgv + 2 * (size_t)n;
+  cg.ZP0 = reinterpret_cast<double4*>(cgv + 3 * (size_t)n);
+  cg.ZP1 = reinterpret_cast<double4*>(cgv + 5 * (size_t)n);
   cg.inv_diag = inv_diag;
   cg.partials = cg_part;
   cg.state = cg_state;
   cg.tol = opt->qeq_tolerance > 0 ? opt->qeq_tolerance : 1e-10;
   cg.max_iter = opt->qeq_max_iterations > 0 ? opt->qeq_max_iterations : 2000;
   // Read back convergence first near the previous solve's iteration count.
-  cg.first_check = ctx->last_cg_iters > 0 ? std::max(4, ctx->last_cg_iters - 1) : 16;
+  cg.first_check = ctx->last_cg_iters > 0 ? ctx->last_cg_iters + 2 : 16;
+  cg.abort_flag = ctx->err;
   cg.check_every = 4;
   cg.total_charge = sys->total_charge;
   cg.q = out->charges;
   cg.mu = mu;
   int iters = 0;
   HIPCHK(xr_cg_solve(cg, s, &iters), "QEq");
+  if (iters == -2) {
+    // an earlier kernel flagged an input error; report it
+    HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, sizeof(unsigned), hipMemcpyDeviceToHost, s), "copy");
+    HIPCHK(hipStreamSynchronize(s), "error sync");
+    if (ctx->err_host[0] & XR_ERR_CAPACITY) {
+      *overflow = true;
+      return XREAC_OK;
+    }
+    return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
+  }
   if (iters < 0) return fail(ctx, XREAC_INVALID, "QEq conjugate gradient did not converge; check the geometry");
   out->qeq_iterations = iters;
   ctx->last_cg_iters = iters;
 
   // ----------------------------------------------------- energy and forces
-  HIPCHK(hipMemsetAsync(epart, 0, sizeof(double) * NCH * cap, s), "memset");
-  HIPCHK(hipMemsetAsync(at.F_hb, 0, sizeof(double) * 3 * (size_t)n, s), "memset");
   HIPCHK(xr_nonbonded(at, cell, ff, row_ptr, nbr_j, nbr_s, acc, s), "nonbonded");
   {
     hipError_t e = hipSuccess;
@@ -556,14 +637,22 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, 64 * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
   HIPCHK(hipStreamSynchronize(s), "final sync");
   ctx->prof.resolve();
+  const int* host_counts = reinterpret_cast<const int*>(ctx->err_host) + 1;
+  const int64_t m = host_counts[0], nb_total = host_counts[1];
+  if (m > m_cap || nb_total > nbond || (ctx->err_host[0] & XR_ERR_CAPACITY)) {
+    *overflow = true;
+    return XREAC_OK;
+  }
   if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
   std::memcpy(out->components, host_comps, sizeof(double) * NCOMP);
   out->num_edges = m;
-  out->num_bonds = nbond;
+  out->num_bonds = nb_total;
+  ctx->m_per_atom = (double)m / n;
+  ctx->b_per_atom = (double)nb_total / n;
 
   ctx->n = n;
   ctx->m = m;
-  ctx->nb = nbond;
+  ctx->nb = nb_total;
   ctx->row_ptr = row_ptr;
   ctx->nbr_j = nbr_j;
   ctx->nbr_s = nbr_s;
@@ -573,6 +662,21 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   ctx->bond_code = bd.code;
   ctx->bond_BO = bd.BO;
   return XREAC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
+                   xreac_result* out) {
+  if (!ctx || !ffh || !sys || !opt || !out) return XREAC_INVALID;
+  ctx->message.clear();
+  bool overflow = false;
+  int rc = evaluate_once(ctx, ffh, sys, opt, out, false, &overflow);
+  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ffh, sys, opt, out, true, &overflow);
+  if (rc == XREAC_OK && overflow) return fail(ctx, XREAC_DEVICE, "internal error: buffer sizing");
+  return rc;
 }
 
 int xreac_profile_enable(xreac_ctx* ctx, int enable) {

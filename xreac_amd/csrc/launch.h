@@ -18,10 +18,12 @@ hipError_t xr_exclusive_scan(const int* in, int n, int* out, int* scratch, hipSt
 hipError_t xr_coord_bounds(int n, const double* x, CellDev cell, double* bounds6, hipStream_t s);
 hipError_t xr_bin_atoms(int n, const double* x, CellDev cell, BinGeom g, int* home, int* bin, int* bin_count,
                         hipStream_t s);
-hipError_t xr_bin_scatter(int n, const int* bin, const int* bin_start, int* bin_fill, int* binned, hipStream_t s);
+size_t xr_bin_record_bytes();
+hipError_t xr_bin_scatter(int n, const double* x, const int* home, const int* bin, const int* bin_start,
+                          int* bin_fill, void* recs, unsigned* err, hipStream_t s);
 hipError_t xr_neighbor_count(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
-                             const int* bin_start, const int* binned, double cutoff, int row_cap, int* counts,
+                             const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
                              unsigned* err, hipStream_t s);
 hipError_t xr_neighbor_fill(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
-                            const int* bin_start, const int* binned, double cutoff, const int* row_ptr, int* nbr_j,
+                            const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
                             int* nbr_s, unsigned* err, hipStream_t s);

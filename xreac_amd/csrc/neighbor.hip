@@ -25,6 +25,14 @@ __device__ inline void frac_of(const CellDev& c, const double* x, int i, double 
 
 __device__ inline int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 
+// Bin-ordered copy of an atom: exact coordinates, index and its home image
+// (three 10-bit fields offset by 512), 32 bytes.
+struct BinRec {
+  double x, y, z;
+  int atom;
+  int home;
+};
+
 __global__ void k_coord_bounds(int n, const double* __restrict__ x, CellDev cell, double* __restrict__ out) {
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -82,12 +90,21 @@ __global__ void k_bin_atoms(int n, const double* __restrict__ x, CellDev cell, B
   atomicAdd(&bin_count[id], 1);
 }
 
-__global__ void k_bin_scatter(int n, const int* __restrict__ bin, const int* __restrict__ bin_start,
-                              int* __restrict__ bin_fill, int* __restrict__ binned) {
+__global__ void k_bin_scatter(int n, const double* __restrict__ x, const int* __restrict__ home,
+                              const int* __restrict__ bin, const int* __restrict__ bin_start,
+                              int* __restrict__ bin_fill, BinRec* __restrict__ recs, unsigned* __restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = bin[i];
-  binned[bin_start[b] + atomicAdd(&bin_fill[b], 1)] = i;
+  const int h0 = home[3 * i], h1 = home[3 * i + 1], h2 = home[3 * i + 2];
+  if (h0 < -511 || h0 > 511 || h1 < -511 || h1 > 511 || h2 < -511 || h2 > 511) atomicOr(err, (unsigned)XR_ERR_SHIFT_RANGE);
+  BinRec r;
+  r.x = x[3 * i];
+  r.y = x[3 * i + 1];
+  r.z = x[3 * i + 2];
+  r.atom = i;
+  r.home = xr_pack_shift(h0, h1, h2);
+  recs[bin_start[b] + atomicAdd(&bin_fill[b], 1)] = r;
 }
 
 // Visit every candidate image (j, T) of atom i in the surrounding bins; calls
@@ -99,7 +116,7 @@ __global__ void k_bin_scatter(int n, const int* __restrict__ bin, const int* __r
 template <class Fn>
 __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, const CellDev& cell, const BinGeom& g,
                                          const int* __restrict__ home, const int* __restrict__ bin,
-                                         const int* __restrict__ bin_start, const int* __restrict__ binned,
+                                         const int* __restrict__ bin_start, const BinRec* __restrict__ recs,
                                          double rc2, int t, int nthreads, unsigned& err, Fn fn) {
   const int id = bin[i];
   const int bz = id % g.nb[2], by = (id / g.nb[2]) % g.nb[1], bx = id / (g.nb[2] * g.nb[1]);
@@ -142,23 +159,35 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
         const int cz1 = last - t2 * g.nb[2];
         const int s = bin_start[col + cz0], e = bin_start[col + cz1 + 1];
         const int sz0 = g.pbc[2] ? t2 + hi2 : 0;
-        for (int p = s + t; p < e; p += nthreads) {
-          const int j = binned[p];
-          const int sx = g.pbc[0] ? sx0 - home[3 * j] : 0;
-          const int sy = g.pbc[1] ? sy0 - home[3 * j + 1] : 0;
-          const int sz = g.pbc[2] ? sz0 - home[3 * j + 2] : 0;
-          if (sx < -511 || sx > 511 || sy < -511 || sy > 511 || sz < -511 || sz > 511) {
-            err |= XR_ERR_SHIFT_RANGE;
-            continue;
+        // Every lane takes part in every round (hit = false past the end),
+        // so fn may use wave-wide ballots.
+        for (int base = s; base < e; base += nthreads) {
+          const int p = base + t;
+          bool hit = false;
+          int j = 0, code = 0;
+          double r2 = 0.0;
+          if (p < e) {
+            const BinRec rec = recs[p];
+            j = rec.atom;
+            int hj0, hj1, hj2;
+            xr_unpack_shift(rec.home, hj0, hj1, hj2);
+            const int sx = g.pbc[0] ? sx0 - hj0 : 0;
+            const int sy = g.pbc[1] ? sy0 - hj1 : 0;
+            const int sz = g.pbc[2] ? sz0 - hj2 : 0;
+            if (sx < -511 || sx > 511 || sy < -511 || sy > 511 || sz < -511 || sz > 511) {
+              err |= XR_ERR_SHIFT_RANGE;
+            } else {
+              code = xr_pack_shift(sx, sy, sz);
+              double off[3];
+              xr_offset(cell, code, off);
+              const double v0 = (rec.x - xi0) + off[0];
+              const double v1 = (rec.y - xi1) + off[1];
+              const double v2 = (rec.z - xi2) + off[2];
+              r2 = (v0 * v0 + v1 * v1) + v2 * v2;
+              hit = r2 <= rc2 && !(j == i && sx == 0 && sy == 0 && sz == 0);
+            }
           }
-          const int code = xr_pack_shift(sx, sy, sz);
-          double off[3];
-          xr_offset(cell, code, off);
-          const double v0 = (x[3 * j] - xi0) + off[0];
-          const double v1 = (x[3 * j + 1] - xi1) + off[1];
-          const double v2 = (x[3 * j + 2] - xi2) + off[2];
-          const double r2 = (v0 * v0 + v1 * v1) + v2 * v2;
-          if (r2 <= rc2 && !(j == i && sx == 0 && sy == 0 && sz == 0)) fn(j, code, r2);
+          fn(hit, j, code, r2);
         }
         zz = last + 1;
       }
@@ -168,7 +197,7 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
 
 __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
                                                    const int* __restrict__ home, const int* __restrict__ bin,
-                                                   const int* __restrict__ bin_start, const int* __restrict__ binned,
+                                                   const int* __restrict__ bin_start, const BinRec* __restrict__ binned,
                                                    double rc2, int row_cap, int* __restrict__ counts,
                                                    unsigned* __restrict__ err_out) {
   const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -177,9 +206,11 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
   unsigned err = 0;
   int cnt = 0;
   for_each_neighbor(i, x, cell, g, home, bin, bin_start, binned, rc2, lane, 64, err,
-                    [&](int j, int code, double r2) {
-                      ++cnt;
-                      if (j != i && r2 < 1e-12) err |= XR_ERR_COINCIDENT;
+                    [&](bool hit, int j, int, double r2) {
+                      if (hit) {
+                        ++cnt;
+                        if (j != i && r2 < 1e-12) err |= XR_ERR_COINCIDENT;
+                      }
                     });
   cnt = xr_wave_sum_int(cnt);
 #pragma unroll
@@ -191,50 +222,109 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
   }
 }
 
-__global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
-                                                  const int* __restrict__ home, const int* __restrict__ bin,
-                                                  const int* __restrict__ bin_start, const int* __restrict__ binned,
-                                                  double rc2, const int* __restrict__ row_ptr, int* __restrict__ nbr_j,
-                                                  int* __restrict__ nbr_s) {
-  __shared__ unsigned long long keys[ROW_CAP];
-  __shared__ int cursor;
-  const int i = blockIdx.x;
-  if (i >= n) return;
-  const int begin = row_ptr[i], count = row_ptr[i + 1] - begin;
-  if (count > ROW_CAP) return;  // reported by the count pass
-  if (threadIdx.x == 0) cursor = 0;
-  __syncthreads();
-  unsigned err = 0;
-  for_each_neighbor(i, x, cell, g, home, bin, bin_start, binned, rc2, threadIdx.x, blockDim.x, err,
-                    [&](int j, int code, double) {
-                      const int p = atomicAdd(&cursor, 1);
-                      if (p < ROW_CAP) keys[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
-                    });
-  __syncthreads();
-  int size = 1;
-  while (size < count) size <<= 1;
-  for (int t = count + threadIdx.x; t < size; t += blockDim.x) keys[t] = ~0ull;
-  __syncthreads();
-  for (int k = 2; k <= size; k <<= 1) {
-    for (int jj = k >> 1; jj > 0; jj >>= 1) {
-      for (int t = threadIdx.x; t < size; t += blockDim.x) {
-        const int partner = t ^ jj;
-        if (partner > t) {
-          const unsigned long long a = keys[t], b = keys[partner];
-          const bool up = (t & k) == 0;
-          if ((a > b) == up) {
-            keys[t] = b;
-            keys[partner] = a;
+// Sort the first `count` keys of s (lane-major, K keys per lane) ascending
+// with a bitonic network held in registers: partners closer than K sit in the
+// same lane, farther ones are exchanged with __shfl_xor.  One wave.
+template <int K>
+__device__ inline void wave_sort(unsigned long long* s, int count) {
+  constexpr int N = 64 * K;
+  const int lane = threadIdx.x & 63;
+  unsigned long long v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int idx = lane * K + k;
+    v[k] = idx < count ? s[idx] : ~0ull;
+  }
+#pragma unroll
+  for (int size = 2; size <= N; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= K) {
+        const int ls = stride / K;
+        const bool lower = (lane & ls) == 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const bool up = ((lane * K + k) & size) == 0;
+          const unsigned long long o = __shfl_xor(v[k], ls, 64);
+          const bool keep_min = lower == up;
+          v[k] = keep_min ? (v[k] < o ? v[k] : o) : (v[k] > o ? v[k] : o);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int q = k ^ stride;
+          if (q > k) {
+            const bool up = ((lane * K + k) & size) == 0;
+            const unsigned long long a = v[k], b = v[q];
+            const bool swap = (a > b) == up;
+            v[k] = swap ? b : a;
+            v[q] = swap ? a : b;
           }
         }
       }
-      __syncthreads();
     }
   }
-  for (int t = threadIdx.x; t < count; t += blockDim.x) {
-    const unsigned long long key = keys[t];
-    nbr_j[begin + t] = (int)(key >> 32);
-    nbr_s[begin + t] = (int)(key & 0xffffffffu);
+#pragma unroll
+  for (int k = 0; k < K; ++k) s[lane * K + k] = v[k];
+}
+
+__global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
+                                                 const int* __restrict__ home, const int* __restrict__ bin,
+                                                 const int* __restrict__ bin_start, const BinRec* __restrict__ binned,
+                                                 double rc2, const int* __restrict__ row_ptr, int* __restrict__ nbr_j,
+                                                 int* __restrict__ nbr_s) {
+  __shared__ unsigned long long keys[ROW_CAP];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int begin = row_ptr[i], count = row_ptr[i + 1] - begin;
+  if (count > ROW_CAP) return;  // reported by the count pass
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  unsigned err = 0;
+  int cursor = 0;  // identical in every lane
+  for_each_neighbor(i, x, cell, g, home, bin, bin_start, binned, rc2, lane, 64, err,
+                    [&](bool hit, int j, int code, double) {
+                      const unsigned long long m = __ballot(hit);
+                      if (hit) {
+                        const int p = cursor + __popcll(m & below);
+                        if (p < ROW_CAP) keys[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
+                      }
+                      cursor += __popcll(m);
+                    });
+  __syncthreads();
+  if (count <= 128) {
+    wave_sort<2>(keys, count);
+  } else if (count <= 256) {
+    wave_sort<4>(keys, count);
+  } else if (count <= 512) {
+    wave_sort<8>(keys, count);
+  } else {
+    // Long rows: bitonic network in LDS.
+    int size = 1024;
+    while (size < count) size <<= 1;
+    for (int t2 = count + lane; t2 < size; t2 += 64) keys[t2] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= size; k <<= 1) {
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        for (int t2 = lane; t2 < size; t2 += 64) {
+          const int partner = t2 ^ jj;
+          if (partner > t2) {
+            const unsigned long long a = keys[t2], b = keys[partner];
+            if ((a > b) == ((t2 & k) == 0)) {
+              keys[t2] = b;
+              keys[partner] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  __syncthreads();
+  for (int t2 = lane; t2 < count; t2 += 64) {
+    const unsigned long long key = keys[t2];
+    nbr_j[begin + t2] = (int)(key >> 32);
+    nbr_s[begin + t2] = (int)(key & 0xffffffffu);
   }
 }
 
@@ -253,30 +343,32 @@ hipError_t xr_bin_atoms(int n, const double* x, CellDev cell, BinGeom g, int* ho
   return hipSuccess;
 }
 
-hipError_t xr_bin_scatter(int n, const int* bin, const int* bin_start, int* bin_fill, int* binned, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, bin, bin_start, bin_fill, binned);
+size_t xr_bin_record_bytes() { return sizeof(BinRec); }
+
+hipError_t xr_bin_scatter(int n, const double* x, const int* home, const int* bin, const int* bin_start,
+                          int* bin_fill, void* recs, unsigned* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, x, home, bin, bin_start, bin_fill,
+                     static_cast<BinRec*>(recs), err);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
 hipError_t xr_neighbor_count(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
-                             const int* bin_start, const int* binned, double cutoff, int row_cap, int* counts,
+                             const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
                              unsigned* err, hipStream_t s) {
   const int cap = row_cap < ROW_CAP ? row_cap : ROW_CAP;
-  XR_TIMED(XT_NBR_COUNT, s,
-           hipLaunchKernelGGL(k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start,
-                              binned, cutoff * cutoff, cap, counts, err));
+  XR_LAUNCH(XT_NBR_COUNT, k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start,
+            static_cast<const BinRec*>(binned), cutoff * cutoff, cap, counts, err);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
 hipError_t xr_neighbor_fill(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
-                            const int* bin_start, const int* binned, double cutoff, const int* row_ptr, int* nbr_j,
+                            const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
                             int* nbr_s, unsigned* err, hipStream_t s) {
   (void)err;
-  XR_TIMED(XT_NBR_FILL, s,
-           hipLaunchKernelGGL(k_nbr_fill, dim3(n), dim3(64), 0, s, n, x, cell, g, home, bin, bin_start, binned,
-                              cutoff * cutoff, row_ptr, nbr_j, nbr_s));
+  XR_LAUNCH(XT_NBR_FILL, k_nbr_fill, dim3(n), dim3(64), 0, s, n, x, cell, g, home, bin, bin_start,
+            static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

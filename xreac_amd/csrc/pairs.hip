@@ -125,8 +125,7 @@ __global__ void k_energy_reduce(EnergyAcc acc, double* __restrict__ comps) {
 hipError_t xr_nonbonded(const Atoms& at, CellDev cell, FFDev ff, const int* row_ptr, const int* nbr_j,
                         const int* nbr_s, EnergyAcc acc, hipStream_t s) {
   const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  XR_TIMED(XT_NONBONDED, s,
-           hipLaunchKernelGGL(k_nonbonded, grid, dim3(256), 0, s, at, cell, ff, row_ptr, nbr_j, nbr_s, acc));
+  XR_LAUNCH(XT_NONBONDED, k_nonbonded, grid, dim3(256), 0, s, at, cell, ff, row_ptr, nbr_j, nbr_s, acc);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

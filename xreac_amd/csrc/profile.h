@@ -4,6 +4,7 @@
 // pair of pooled events; elapsed times are summed per timer after the
 // evaluation's final synchronisation.  Disabled, XR_TIMED costs one branch.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <utility>
@@ -37,6 +38,25 @@ struct XrProfiler {
     pending.emplace_back(id, used);
     (void)hipEventRecord(pool[used], s);
     used += 2;
+  }
+  // Event pair for a launch that records its own timestamps (hipExtLaunchKernelGGL).
+  bool slot(int id, hipEvent_t* a, hipEvent_t* b) {
+    if (!on) return false;
+    if (used + 2 > pool.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+          on = false;
+          return false;
+        }
+        pool.push_back(e);
+      }
+    }
+    pending.emplace_back(id, used);
+    *a = pool[used];
+    *b = pool[used + 1];
+    used += 2;
+    return true;
   }
   void end(hipStream_t s) {
     if (!on || pending.empty()) return;
@@ -74,4 +94,17 @@ extern thread_local XrProfiler* xr_active_profiler;
     if (p__ && p__->on) p__->begin((id), (stream));                   \
     launch;                                                           \
     if (p__ && p__->on) p__->end(stream);                             \
+  } while (0)
+
+// Launch a kernel; when profiling, the dispatch itself records the start and
+// stop events (no separate event packets on the stream).
+#define XR_LAUNCH(id, kernel, grid, block, shmem, stream, ...)                                  \
+  do {                                                                                          \
+    XrProfiler* p__ = xr_active_profiler;                                                       \
+    hipEvent_t a__, b__;                                                                        \
+    if (p__ && p__->slot((id), &a__, &b__)) {                                                   \
+      hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, a__, b__, 0, __VA_ARGS__);       \
+    } else {                                                                                    \
+      hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                      \
+    }                                                                                           \
   } while (0)

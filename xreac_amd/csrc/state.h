@@ -15,7 +15,8 @@ struct EnergyAcc {
 };
 
 struct Bonds {
-  int nb;                  // active directed bonds
+  int cap;                 // allocated bond slots (launch size of per-bond kernels)
+  const int* nbp;          // device: number of active directed bonds (<= cap)
   const int* ptr;          // [n+1]
   const int* edge;         // [nb] canonical edge index
   const int* bi;           // [nb] row atom

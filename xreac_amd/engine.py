@@ -90,26 +90,41 @@ class Engine:
         idx = self.tables.index
         return np.array([idx[s] for s in symbols], dtype=np.int32)
 
+    def outputs(self, n, properties=True):
+        """Device output buffers for an n-atom evaluation (reusable across calls via ``out=``)."""
+        torch = self.torch
+        dev = self.device
+        out = dict(
+            forces=torch.empty((n, 3), dtype=torch.float64, device=dev),
+            charges=torch.empty(n, dtype=torch.float64, device=dev),
+            total_bond_orders=None,
+            lone_pairs=None,
+            bond_counts=None,
+        )
+        if properties:
+            out["total_bond_orders"] = torch.empty(n, dtype=torch.float64, device=dev)
+            out["lone_pairs"] = torch.empty(n, dtype=torch.float64, device=dev)
+            out["bond_counts"] = torch.empty(n, dtype=torch.int32, device=dev)
+        return out
+
     def evaluate(self, positions, types, cell=None, pbc=(False, False, False), total_charge=0.0, *, tol=1e-10,
-                 max_iter=2000, lammps_angles=True, properties=True):
+                 max_iter=2000, lammps_angles=True, properties=True, out=None):
         """Evaluate one system whose positions (n,3 float64) and types (n int32) are device tensors.
 
         Returns a dict of device tensors (forces, charges and optionally
         total_bond_orders, lone_pairs, bond_counts), the 14 components (host
-        numpy) and the edge/bond/iteration counts.
+        numpy) and the edge/bond/iteration counts.  ``out`` (from
+        ``outputs``) is filled in place instead of allocating new buffers.
         """
         torch = self.torch
         n = int(positions.shape[0])
         assert positions.dtype == torch.float64 and positions.is_contiguous() and positions.shape == (n, 3)
         assert types.dtype == torch.int32 and types.is_contiguous() and types.shape == (n,)
-        dev = positions.device
-        forces = torch.empty((n, 3), dtype=torch.float64, device=dev)
-        charges = torch.empty(n, dtype=torch.float64, device=dev)
-        tbo = lp = bc = None
-        if properties:
-            tbo = torch.empty(n, dtype=torch.float64, device=dev)
-            lp = torch.empty(n, dtype=torch.float64, device=dev)
-            bc = torch.empty(n, dtype=torch.int32, device=dev)
+        if out is None:
+            out = self.outputs(n, properties)
+        forces, charges = out["forces"], out["charges"]
+        tbo, lp, bc = out["total_bond_orders"], out["lone_pairs"], out["bond_counts"]
+        assert forces.shape == (n, 3) and charges.shape == (n,)
         cell_arr = None
         if cell is not None:
             cell_arr = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
@@ -128,8 +143,7 @@ class Engine:
         res.total_bond_orders = _ptr(tbo) if tbo is not None else None
         res.lone_pairs = _ptr(lp) if lp is not None else None
         res.bond_counts = _ptr(bc) if bc is not None else None
-        with torch.cuda.device(dev):
-            status = self.lib.xreac_evaluate(self._ctx, self._ff, C.byref(sysd), C.byref(opt), C.byref(res))
+        status = self.lib.xreac_evaluate(self._ctx, self._ff, C.byref(sysd), C.byref(opt), C.byref(res))
         self._check(status, self._ctx)
         return dict(
             forces=forces,
