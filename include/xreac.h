@@ -57,7 +57,8 @@ enum {
   XREAC_TIMER_BOND_FORWARD = 6,
   XREAC_TIMER_MANYBODY = 7,
   XREAC_TIMER_BACKWARD = 8,
-  XREAC_TIMERS = 9
+  XREAC_TIMERS = 9,
+  XREAC_TIMER_ALL = (1 << 9) - 1
 };
 
 typedef struct xreac_ff xreac_ff;
@@ -121,9 +122,10 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ff, const xreac_system* sys, 
                    xreac_result* out);
 
 /* Kernel timing with hipEvents on the context stream (no reference
- * counterpart; measurement only).  Enabled, every launch of a timed kernel is
- * bracketed by two events and its elapsed time summed per timer. */
-int xreac_profile_enable(xreac_ctx* ctx, int enable);
+ * counterpart; measurement only).  timer_mask bit k enables timer k (0: off);
+ * every launch of an enabled kernel records start/stop events with its
+ * dispatch and its elapsed time is summed per timer. */
+int xreac_profile_enable(xreac_ctx* ctx, int timer_mask);
 int xreac_profile_read(xreac_ctx* ctx, int timer, double* total_ms, int64_t* launches, int reset);
 
 /* Canonical neighbor list of the last evaluation (host arrays): row_ptr [n+1],

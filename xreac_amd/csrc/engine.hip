@@ -679,9 +679,10 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   return rc;
 }
 
-int xreac_profile_enable(xreac_ctx* ctx, int enable) {
+int xreac_profile_enable(xreac_ctx* ctx, int timer_mask) {
   if (!ctx) return XREAC_INVALID;
-  ctx->prof.on = enable != 0;
+  ctx->prof.mask = (unsigned)timer_mask & ((1u << XT_COUNT) - 1u);
+  ctx->prof.on = ctx->prof.mask != 0;
   return XREAC_OK;
 }
 

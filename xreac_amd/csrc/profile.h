@@ -17,14 +17,16 @@ enum XrTimer {
 
 struct XrProfiler {
   bool on = false;
+  unsigned mask = 0;  // bit k: timer k is recorded
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   std::vector<std::pair<int, size_t>> pending;
   double total_ms[XT_COUNT] = {};
   long long launches[XT_COUNT] = {};
 
+  bool wants(int id) const { return on && ((mask >> id) & 1u); }
   void begin(int id, hipStream_t s) {
-    if (!on) return;
+    if (!wants(id)) return;
     if (used + 2 > pool.size()) {
       for (int k = 0; k < 64; ++k) {
         hipEvent_t e;
@@ -41,7 +43,7 @@ struct XrProfiler {
   }
   // Event pair for a launch that records its own timestamps (hipExtLaunchKernelGGL).
   bool slot(int id, hipEvent_t* a, hipEvent_t* b) {
-    if (!on) return false;
+    if (!wants(id)) return false;
     if (used + 2 > pool.size()) {
       for (int k = 0; k < 64; ++k) {
         hipEvent_t e;
@@ -91,9 +93,10 @@ extern thread_local XrProfiler* xr_active_profiler;
 #define XR_TIMED(id, stream, launch)                                  \
   do {                                                                \
     XrProfiler* p__ = xr_active_profiler;                             \
-    if (p__ && p__->on) p__->begin((id), (stream));                   \
+    const bool t__ = p__ && p__->wants(id);                           \
+    if (t__) p__->begin((id), (stream));                              \
     launch;                                                           \
-    if (p__ && p__->on) p__->end(stream);                             \
+    if (t__) p__->end(stream);                                        \
   } while (0)
 
 // Launch a kernel; when profiling, the dispatch itself records the start and

@@ -158,9 +158,14 @@ class Engine:
             n=n,
         )
 
-    def profile(self, enable=True):
-        """Enable or disable hipEvent timing of the library's kernels."""
-        self._check(self.lib.xreac_profile_enable(self._ctx, 1 if enable else 0), self._ctx)
+    def profile(self, enable=True, timers=None):
+        """Enable hipEvent timing of the named kernels (all when None), or disable it."""
+        mask = 0
+        if enable:
+            names = _lib.TIMERS if timers is None else timers
+            for name in names:
+                mask |= 1 << _lib.TIMERS.index(name)
+        self._check(self.lib.xreac_profile_enable(self._ctx, mask), self._ctx)
 
     def timers(self, reset=True):
         """{timer: (total_ms, launches)} accumulated since the last reset."""
