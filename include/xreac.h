@@ -89,10 +89,17 @@ typedef struct {
   double total_charge;       /* e */
 } xreac_system;
 
+enum {
+  XREAC_PRECISION_FP64 = 0,    /* FP64 CG throughout */
+  XREAC_PRECISION_MIXED = 1    /* FP32 A p sweeps, FP64 vectors and dots, FP64 true-residual restarts */
+};
+
 typedef struct {
   double qeq_tolerance;        /* relative residual of each CG right-hand side */
-  int32_t qeq_max_iterations;
+  int32_t qeq_max_iterations;  /* per system */
   int32_t lammps_angles;       /* 1: LAMMPS near-collinear angle derivative (fixed-charge mode) */
+  int32_t precision;           /* XREAC_PRECISION_* */
+  int32_t warm_start;          /* 1: start the CG from result->qeq_state (batched entry point) */
 } xreac_options;
 
 typedef struct {
@@ -107,6 +114,35 @@ typedef struct {
   int32_t qeq_iterations;      /* out */
 } xreac_result;
 
+/* A batch of independent systems packed atom-wise.  The reference evaluates
+ * one system per call; this is the same evaluation for many at once.
+ * System s owns atoms [atom_offsets[s], atom_offsets[s+1]) of every per-atom
+ * array; its results are bitwise those of evaluating it alone. */
+typedef struct {
+  int32_t num_systems;
+  const int32_t* atom_offsets;  /* host [num_systems + 1], atom_offsets[0] = 0, nondecreasing */
+  const double* positions;      /* device [N*3], N = atom_offsets[num_systems] */
+  const int32_t* types;         /* device [N] */
+  const double* cells;          /* host [num_systems*9] (rows = lattice vectors), or NULL: all nonperiodic */
+  const int32_t* pbc;           /* host [num_systems*3], or NULL: periodic along all three cell vectors */
+  const double* total_charges;  /* host [num_systems], or NULL: all neutral */
+} xreac_batch;
+
+typedef struct {
+  double* components;          /* host [num_systems][XREAC_COMPONENTS], kcal/mol */
+  int32_t* status;             /* host [num_systems]: XREAC_OK, or XREAC_INVALID (xreac_batch_error) */
+  int32_t* qeq_iterations;     /* host [num_systems] */
+  double* forces;              /* device [N*3] */
+  double* charges;             /* device [N] */
+  double* total_bond_orders;   /* device [N] or NULL */
+  double* lone_pairs;          /* device [N] or NULL */
+  int32_t* bond_counts;        /* device [N] or NULL */
+  double* qeq_state;           /* device [N*2] or NULL: (s, t) per atom; the starting guess when
+                                  options.warm_start is set; overwritten with the solution */
+  int64_t num_edges;           /* out: directed neighbor edges, all systems */
+  int64_t num_bonds;           /* out: directed active bonds, all systems */
+} xreac_batch_result;
+
 /* Force field. */
 int xreac_ff_create(const xreac_ff_tables* tables, xreac_ff** out);
 void xreac_ff_destroy(xreac_ff* ff);
@@ -120,6 +156,41 @@ const char* xreac_error_message(const xreac_ctx* ctx);
 /* Energy, QEq charges and fixed-charge forces (calculator.py:76-165). */
 int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ff, const xreac_system* sys, const xreac_options* opt,
                    xreac_result* out);
+
+/* The same evaluation over a batch (calculator.py:76-165 per system).  Returns
+ * XREAC_OK when the batch ran; each system then has its own status, and a
+ * failed system (e.g. coincident atoms) does not affect the others.  Other
+ * return codes mean nothing was evaluated. */
+int xreac_evaluate_batch(xreac_ctx* ctx, const xreac_ff* ff, const xreac_batch* batch, const xreac_options* opt,
+                         xreac_batch_result* out);
+
+/* Message of system s of the last batch ("" if it succeeded). */
+const char* xreac_batch_error(const xreac_ctx* ctx, int32_t system);
+
+/* Velocity Verlet over a packed batch, one step in two stages around a force
+ * evaluation (no reference counterpart: the reference has no dynamics).
+ * Units A, fs, amu (force-field masses), kcal/mol.  All pointers are device
+ * memory: offsets [num_systems + 1], types [N], dt_fs [num_systems] (0 keeps
+ * a system still), positions/velocities/forces [N*3].
+ *   stage 0: v += dt/2 F/m, then x += dt v   (forces of the current positions)
+ *   stage 1: v += dt/2 F/m                   (forces of the new positions);
+ *            kinetic [num_systems] (or NULL) receives sum m v^2 / 2 in kcal/mol. */
+int xreac_verlet(xreac_ctx* ctx, const xreac_ff* ff, int32_t num_systems, const int32_t* offsets,
+                 const int32_t* types, const double* dt_fs, double* positions, double* velocities,
+                 const double* forces, int32_t stage, double* kinetic);
+
+/* One FIRE iteration per system of a packed batch, as the reference's native
+ * relaxation loop (calculator.py:213-246): if max |F| <= force_tolerance the
+ * system is marked converged and not moved; otherwise velocities are mixed
+ * and kicked and positions moved (largest atom step capped at 0.1 A).
+ * Device memory: offsets [num_systems + 1], positions/velocities/forces [N*3],
+ * state [num_systems*4] = (dt, alpha, positive steps, iteration), start
+ * (0.02, 0.1, 0, 0) with zero velocities; status [num_systems]: 0 running,
+ * 1 converged, 2 iteration limit reached (systems with status != 0 are left
+ * untouched). */
+int xreac_fire_step(xreac_ctx* ctx, int32_t num_systems, const int32_t* offsets, double* positions,
+                    double* velocities, const double* forces, double* state, int32_t* status,
+                    double force_tolerance, int32_t max_iterations);
 
 /* Kernel timing with hipEvents on the context stream (no reference
  * counterpart; measurement only).  timer_mask bit k enables timer k (0: off);

@@ -12,6 +12,7 @@ import os
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libxreac_hip.so")
 
 XREAC_OK, XREAC_INVALID, XREAC_DEVICE, XREAC_NOMEM = 0, 1, 2, 3
+PRECISION_FP64, PRECISION_MIXED = 0, 1
 NCOMP = 14
 
 EXPORTED = (
@@ -21,6 +22,10 @@ EXPORTED = (
     "xreac_ctx_destroy",
     "xreac_error_message",
     "xreac_evaluate",
+    "xreac_evaluate_batch",
+    "xreac_batch_error",
+    "xreac_verlet",
+    "xreac_fire_step",
     "xreac_copy_neighbors",
     "xreac_copy_bonds",
     "xreac_profile_enable",
@@ -71,6 +76,36 @@ class Options(C.Structure):
         ("qeq_tolerance", C.c_double),
         ("qeq_max_iterations", C.c_int32),
         ("lammps_angles", C.c_int32),
+        ("precision", C.c_int32),
+        ("warm_start", C.c_int32),
+    ]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("num_systems", C.c_int32),
+        ("atom_offsets", C.c_void_p),
+        ("positions", C.c_void_p),
+        ("types", C.c_void_p),
+        ("cells", C.c_void_p),
+        ("pbc", C.c_void_p),
+        ("total_charges", C.c_void_p),
+    ]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("components", C.c_void_p),
+        ("status", C.c_void_p),
+        ("qeq_iterations", C.c_void_p),
+        ("forces", C.c_void_p),
+        ("charges", C.c_void_p),
+        ("total_bond_orders", C.c_void_p),
+        ("lone_pairs", C.c_void_p),
+        ("bond_counts", C.c_void_p),
+        ("qeq_state", C.c_void_p),
+        ("num_edges", C.c_int64),
+        ("num_bonds", C.c_int64),
     ]
 
 
@@ -116,6 +151,14 @@ def load():
     lib.xreac_error_message.restype = C.c_char_p
     lib.xreac_evaluate.argtypes = [vp, vp, C.POINTER(System), C.POINTER(Options), C.POINTER(Result)]
     lib.xreac_evaluate.restype = C.c_int
+    lib.xreac_evaluate_batch.argtypes = [vp, vp, C.POINTER(Batch), C.POINTER(Options), C.POINTER(BatchResult)]
+    lib.xreac_evaluate_batch.restype = C.c_int
+    lib.xreac_batch_error.argtypes = [vp, C.c_int32]
+    lib.xreac_batch_error.restype = C.c_char_p
+    lib.xreac_verlet.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, C.c_int32, vp]
+    lib.xreac_verlet.restype = C.c_int
+    lib.xreac_fire_step.argtypes = [vp, C.c_int32, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int32]
+    lib.xreac_fire_step.restype = C.c_int
     lib.xreac_copy_neighbors.argtypes = [vp, vp, vp, vp, C.c_int64]
     lib.xreac_copy_neighbors.restype = C.c_int
     lib.xreac_copy_bonds.argtypes = [vp, vp, vp, vp, vp, C.c_int64]

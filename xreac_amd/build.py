@@ -10,7 +10,16 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ("scan.hip", "neighbor.hip", "qeq.hip", "bondorder.hip", "manybody.hip", "pairs.hip", "engine.hip")
+SOURCES = (
+    "scan.hip",
+    "neighbor.hip",
+    "qeq.hip",
+    "bondorder.hip",
+    "manybody.hip",
+    "pairs.hip",
+    "integrate.hip",
+    "engine.hip",
+)
 FLAGS = (
     "--offload-arch=gfx950",
     "-O3",

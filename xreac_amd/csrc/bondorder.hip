@@ -15,14 +15,14 @@ __device__ inline const double* AP(const FFDev& ff, int t) { return ff.atom + t 
 __device__ inline const double* PPm(const FFDev& ff, int ti, int tj) { return ff.pair + (ti * ff.nt + tj) * PP_COUNT; }
 
 // ------------------------------------------------------------ bond geometry
-__global__ void k_bond_geom(Atoms at, Bonds bd, CellDev cell, FFDev ff, const int* __restrict__ nbr_j,
-                            const int* __restrict__ nbr_s, unsigned* __restrict__ err) {
+__global__ void k_bond_geom(Atoms at, Bonds bd, FFDev ff, const int* __restrict__ nbr_j,
+                            const int* __restrict__ nbr_s) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= *bd.nbp) return;
   const int e = bd.edge[b], i = bd.bi[b];
   const int j = nbr_j[e], code = nbr_s[e];
   double v[3];
-  const double r = xr_edge(at.x, cell, i, j, code, v);
+  const double r = xr_edge(at.x, xr_cell(at.S, i), i, j, code, v);
   const int ti = at.type[i], tj = at.type[j];
   BondRaw raw;
   xr_bond_raw(ff, ti, tj, r, raw);
@@ -45,11 +45,11 @@ __global__ void k_bond_geom(Atoms at, Bonds bd, CellDev cell, FFDev ff, const in
     if (!isfinite(p[PP_BOC3])) bad |= XR_ERR_BOC3;
     if (!isfinite(p[PP_BOC4])) bad |= XR_ERR_BOC4;
     if (!isfinite(p[PP_BOC5])) bad |= XR_ERR_BOC5;
-    if (bad) atomicOr(err, bad);
+    if (bad) xr_flag(at.S, i, bad);
   }
 }
 
-__global__ void k_bond_reverse(Bonds bd, unsigned* __restrict__ err) {
+__global__ void k_bond_reverse(Bonds bd, Systems S) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= *bd.nbp) return;
   const int i = bd.bi[b], j = bd.bj[b], want = xr_neg_shift(bd.code[b]);
@@ -61,7 +61,7 @@ __global__ void k_bond_reverse(Bonds bd, unsigned* __restrict__ err) {
     }
   }
   if (found < 0) {
-    atomicOr(err, (unsigned)XR_ERR_REVERSE);
+    xr_flag(S, i, XR_ERR_REVERSE);
     found = b;
   }
   bd.rev[b] = found;
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) k_atom_forward(Atoms at, Bonds bd, FFDev 
     at.bond_count[i] = cnt;
     at.A_dlp[i] = a_dlp;
   }
-  xr_block_partials<1>(elp, acc.partials + CH_LP * acc.cap, gridDim.x);
+  if (i < at.n) xr_energy(acc, CH_LP, i, elp[0]);
 }
 
 // Over/under-coordination, bond energy and the C2 correction; initialises
@@ -242,10 +242,11 @@ __global__ void __launch_bounds__(256) k_atom_forward2(Atoms at, Bonds bd, FFDev
     at.A_nlp[i] = 0.0;
     at.A_sbo2[i] = 0.0;
   }
-  double eb[1] = {e[0]}, ea[1] = {e[1]}, ec[1] = {e[2]};
-  xr_block_partials<1>(eb, acc.partials + CH_BOND * acc.cap, gridDim.x);
-  xr_block_partials<1>(ea, acc.partials + CH_ATOM * acc.cap, gridDim.x);
-  xr_block_partials<1>(ec, acc.partials + CH_C2 * acc.cap, gridDim.x);
+  if (i < at.n) {
+    xr_energy(acc, CH_BOND, i, e[0]);
+    xr_energy(acc, CH_ATOM, i, e[1]);
+    xr_energy(acc, CH_C2, i, e[2]);
+  }
 }
 
 // Triple-bond stabilisation on C-O half edges (energy.py:238-246).  Each end
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(256) k_triple(Atoms at, Bonds bd, FFDev ff, En
       at.A_total[i] += (A / Dn) * (-g[3] * ei);
     }
   }
-  xr_block_partials<1>(e, acc.partials + CH_TRIPLE * acc.cap, gridDim.x);
+  if (i < at.n) xr_energy(acc, CH_TRIPLE, i, e[0]);
 }
 
 // ----------------------------------------------------------------- adjoint
@@ -425,11 +426,11 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 
 }  // namespace
 
-hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* nbr_j,
-                           const int* nbr_s, EnergyAcc acc, unsigned* err, hipStream_t s) {
+hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, FFDev ff, const int* nbr_j, const int* nbr_s,
+                           EnergyAcc acc, hipStream_t s) {
   if (bd.cap > 0) {
-    hipLaunchKernelGGL(k_bond_geom, grid(bd.cap), dim3(BLK), 0, s, at, bd, cell, ff, nbr_j, nbr_s, err);
-    hipLaunchKernelGGL(k_bond_reverse, grid(bd.cap), dim3(BLK), 0, s, bd, err);
+    hipLaunchKernelGGL(k_bond_geom, grid(bd.cap), dim3(BLK), 0, s, at, bd, ff, nbr_j, nbr_s);
+    hipLaunchKernelGGL(k_bond_reverse, grid(bd.cap), dim3(BLK), 0, s, bd, at.S);
   }
   hipLaunchKernelGGL(k_atom_totalp, grid(at.n), dim3(BLK), 0, s, at, bd);
   if (bd.cap > 0) hipLaunchKernelGGL(k_bond_correct, grid(bd.cap), dim3(BLK), 0, s, at, bd, ff);

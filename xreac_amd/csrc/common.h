@@ -69,6 +69,40 @@ struct CellDev {
   int periodic;
 };
 
+// Cell-list grid of one system.
+struct BinGeom {
+  int nb[3];     // bins per direction
+  int reach[3];  // bins searched on each side
+  double lo[3];  // grid origin in fractional (periodic) or scaled units
+  double w[3];   // bin width in the same units
+  int pbc[3];
+};
+
+// One system of a batch.  Atoms of system s occupy [atom0, atom0 + n) of the
+// packed per-atom arrays; its bins occupy [bin0, bin0 + nb0*nb1*nb2) of the
+// global bin arrays; its CG blocks are [dir0, dir0 + ndir) and
+// [upd0, upd0 + nupd) of the two CG kernels' grids.
+struct SysDev {
+  CellDev cell;
+  BinGeom g;
+  int atom0, n;
+  int bin0;
+  int dir0, ndir;
+  int upd0, nupd;
+  double total_charge;
+};
+
+// Batch view shared by the kernels.
+struct Systems {
+  const SysDev* sys;  // [ns]
+  const int* asys;    // [N] system of each atom
+  unsigned* err;      // [ns] input-error bits per system
+  int ns;
+};
+
+__device__ inline const CellDev& xr_cell(const Systems& S, int i) { return S.sys[S.asys[i]].cell; }
+__device__ inline void xr_flag(const Systems& S, int i, unsigned bits) { atomicOr(&S.err[S.asys[i]], bits); }
+
 // ------------------------------------------------------------ shift packing
 __host__ __device__ inline int xr_pack_shift(int sx, int sy, int sz) {
   return ((sx + 512) << 20) | ((sy + 512) << 10) | (sz + 512);

@@ -1,13 +1,18 @@
-// C ABI and host orchestration of one evaluation (include/xreac.h).
+// C ABI and host orchestration of one batched evaluation (include/xreac.h).
 //
-// Pipeline on the context's stream (host synchronisation points marked *):
-//   bounds*  (only with nonperiodic directions)  -> bin, scatter
-//   neighbor count -> scan*  (M)  -> neighbor fill (canonical rows)
-//   edge pass (H values, active-bond flags) -> scan*  (B) -> bond compaction
-//   QEq CG (two RHS)*  -> nonbonded energy/forces
+// A call evaluates one or more independent systems packed atom-wise; a
+// single system is a batch of one.  Pipeline on the context's stream (host
+// synchronisation points marked *):
+//   system table upload; bounds* (only with nonperiodic directions) -> bin grids
+//   bin, scatter -> neighbor count -> scan -> neighbor fill (canonical rows)
+//   edge pass (H values, active-bond flags) -> scan -> bond compaction
+//   QEq CG (two RHS, per-system convergence)*  -> nonbonded energy/forces
 //   bond orders, lone pair, over/under, special terms
 //   angles, torsions, hydrogen bonds   (forward + adjoint deposits)
-//   bond-order adjoint -> forces -> energy reduction*
+//   bond-order adjoint -> forces -> per-system energy reduction*
+// Edge and bond buffers are sized from the previous call's per-atom counts;
+// an overflow is detected on the device and the call repeats with sizes read
+// back first (two more synchronisations).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -16,6 +21,7 @@
 #include <vector>
 
 #include "../../include/xreac.h"
+#include "integrate.h"
 #include "launch.h"
 #include "profile.h"
 #include "qeq.h"
@@ -81,14 +87,14 @@ struct xreac_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string message;
-  Buffer atom_buf, bin_buf, edge_buf, bond_buf;
+  std::vector<std::string> sys_messages;  // per system of the last batch
+  Buffer atom_buf, bin_buf, edge_buf, bond_buf, ctl_buf;
+  char* ctl_host = nullptr;  // pinned mirror of the control block
+  size_t ctl_host_bytes = 0;
   XrProfiler prof;
   int last_cg_iters = 0;
   double m_per_atom = 0.0;  // edges and bonds per atom of the last evaluation
   double b_per_atom = 0.0;
-  unsigned* err = nullptr;       // device
-  unsigned* err_host = nullptr;  // pinned
-  double* comps = nullptr;       // device [NCOMP]
   // last evaluation
   int n = 0;
   int64_t m = 0, nb = 0;
@@ -243,20 +249,13 @@ int xreac_ctx_create(int device, void* stream, xreac_ctx** out) {
   auto* c = new xreac_ctx();
   c->device = device;
   c->stream = static_cast<hipStream_t>(stream);
-  if (hipMalloc(&c->err, 64 * sizeof(double)) != hipSuccess ||
-      hipHostMalloc(&c->err_host, 64 * sizeof(double)) != hipSuccess) {
-    xreac_ctx_destroy(c);
-    return XREAC_NOMEM;
-  }
-  c->comps = reinterpret_cast<double*>(c->err) + 8;
   *out = c;
   return XREAC_OK;
 }
 
 void xreac_ctx_destroy(xreac_ctx* c) {
   if (!c) return;
-  if (c->err) (void)hipFree(c->err);
-  if (c->err_host) (void)hipHostFree(c->err_host);
+  if (c->ctl_host) (void)hipHostFree(c->ctl_host);
   delete c;
 }
 
@@ -266,16 +265,101 @@ const char* xreac_error_message(const xreac_ctx* c) { return c ? c->message.c_st
 
 namespace {
 
+// Internal form of xreac_batch / xreac_batch_result.
+struct BatchIn {
+  int ns;
+  const int32_t* offsets;  // host [ns + 1]
+  const double* x;         // device [N*3]
+  const int32_t* types;    // device [N]
+  const double* cells;     // host [ns*9] or null
+  const int32_t* pbc;      // host [ns*3] or null
+  const double* charges;   // host [ns] or null
+};
+
+struct BatchOut {
+  double* comps;     // host [ns*NCOMP]
+  int32_t* status;   // host [ns]
+  int32_t* iters;    // host [ns]
+  double* forces;    // device
+  double* charges;
+  double* tbo;
+  double* nlp;
+  int32_t* bcounts;
+  double* qeq_state;
+  int64_t num_edges = 0, num_bonds = 0;
+};
+
+template <class T>
+size_t carve_at(size_t& off, size_t count) {
+  off = (off + 255) & ~size_t(255);
+  const size_t at = off;
+  off += count * sizeof(T);
+  return at;
+}
+
+// Control block: small per-call state, mirrored in pinned host memory so
+// one copy brings every per-system result back.
+struct CtlLayout {
+  size_t flags, serr, comps, bounds, mu, cg, sys, total;
+};
+
+CtlLayout ctl_layout(int ns) {
+  CtlLayout c{};
+  size_t off = 0;
+  c.flags = carve_at<unsigned>(off, 16);  // [0] buffer-overflow bit, int [1] edges, int [2] bonds
+  c.serr = carve_at<unsigned>(off, ns);
+  c.comps = carve_at<double>(off, (size_t)ns * NCOMP);
+  c.bounds = carve_at<double>(off, (size_t)ns * 6);
+  c.mu = carve_at<double>(off, ns);
+  c.cg = carve_at<char>(off, xr_cg_state_bytes(ns));
+  c.sys = carve_at<SysDev>(off, ns);
+  c.total = (off + 255) & ~size_t(255);
+  return c;
+}
+
+__global__ void k_fill_systems(Systems S, int* __restrict__ asys) {
+  const SysDev& sd = S.sys[blockIdx.x];
+  for (int i = sd.atom0 + threadIdx.x; i < sd.atom0 + sd.n; i += blockDim.x) asys[i] = blockIdx.x;
+}
+
+// Bin grid of one system (rc/2-thick bins, searched two deep when periodic).
+void bin_geometry(SysDev& sd, const double* height, const double* bounds, double rcb) {
+  BinGeom& g = sd.g;
+  for (int k = 0; k < 3; ++k) {
+    g.pbc[k] = sd.cell.pbc[k];
+    if (sd.cell.pbc[k]) {
+      int nbk = (int)std::floor(2.0 * height[k] / rcb);
+      nbk = std::max(1, std::min(nbk, 128));
+      g.nb[k] = nbk;
+      g.w[k] = 1.0 / nbk;
+      g.lo[k] = 0.0;
+      const double h = height[k] / nbk;
+      g.reach[k] = h >= rcb ? 1 : (int)std::floor(rcb / h) + 1;
+    } else {
+      double w = 0.5 * rcb / height[k];
+      const double span = bounds[3 + k] - bounds[k];
+      long long nbk = (long long)std::floor(span / w) + 1;
+      if (nbk > 128) {
+        w = span / 127.0;
+        nbk = 128;
+      }
+      g.nb[k] = (int)nbk;
+      g.w[k] = w;
+      g.lo[k] = bounds[k];
+      g.reach[k] = (int)std::floor(rcb / (w * height[k])) + 1;
+    }
+  }
+}
+
 // One attempt.  With exact == false, edge and bond buffers are sized from the
 // previous evaluation's per-atom counts and no mid-pipeline read-back happens;
 // if a count exceeds its buffer, *overflow is set and the caller repeats the
 // evaluation with exact == true (sizes read back before allocation).
-int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
-                  xreac_result* out, bool exact, bool* overflow) {
+int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out,
+                  bool exact, bool* overflow) {
   *overflow = false;
-  const int n = sys->n;
-  if (n <= 0 || !sys->positions || !sys->types || !out->forces || !out->charges)
-    return fail(ctx, XREAC_INVALID, "A nonempty system with device positions, types, forces and charges is required");
+  const int ns = in.ns;
+  const int n = in.offsets[ns];
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   hipStream_t s = ctx->stream;
   const FFDev& ff = ffh->dev;
@@ -292,28 +376,63 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
   } active(&ctx->prof);
   const double rc = ffh->cutoff;
   const double bond_cut = std::min(5.0, rc);
+  const double rcb = rc * (1 + 1e-8) + 1e-12;
+  const bool mixed = opt->precision == XREAC_PRECISION_MIXED;
 
-  // ---------------------------------------------------------------- cell
-  CellDev cell{};
-  const bool periodic = sys->cell && (sys->pbc[0] || sys->pbc[1] || sys->pbc[2]);
-  if (periodic) {
-    std::memcpy(cell.L, sys->cell, sizeof(cell.L));
-  } else {
-    for (int k = 0; k < 9; ++k) cell.L[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  // ------------------------------------------------------- control block
+  const CtlLayout L = ctl_layout(ns);
+  HIPCHK(ctx->ctl_buf.ensure(L.total), "control block");
+  if (ctx->ctl_host_bytes < L.total) {
+    if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
+    ctx->ctl_host = nullptr;
+    ctx->ctl_host_bytes = 0;
+    const size_t want = L.total + L.total / 2;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&ctx->ctl_host), want), "pinned control block");
+    ctx->ctl_host_bytes = want;
   }
-  invert3(cell.L, cell.inv);
-  for (int k = 0; k < 3; ++k) cell.pbc[k] = periodic ? (sys->pbc[k] != 0) : 0;
-  cell.periodic = periodic ? 1 : 0;
-  double height[3];
-  for (int k = 0; k < 3; ++k) {
-    const double c0 = cell.inv[k], c1 = cell.inv[3 + k], c2 = cell.inv[6 + k];
-    height[k] = 1.0 / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  char* dc = static_cast<char*>(ctx->ctl_buf.ptr);
+  char* hc = ctx->ctl_host;
+  unsigned* d_flags = reinterpret_cast<unsigned*>(dc + L.flags);
+  SysDev* hsys = reinterpret_cast<SysDev*>(hc + L.sys);
+  std::vector<double> heights(3 * (size_t)ns);
+  int nd = 0, nu = 0;
+  bool need_bounds = false;
+  for (int q = 0; q < ns; ++q) {
+    SysDev sd;
+    std::memset(&sd, 0, sizeof(sd));
+    sd.atom0 = in.offsets[q];
+    sd.n = in.offsets[q + 1] - in.offsets[q];
+    const double* cp = in.cells ? in.cells + 9 * (size_t)q : nullptr;
+    int pb[3];
+    for (int k = 0; k < 3; ++k) pb[k] = cp ? (in.pbc ? (in.pbc[3 * q + k] != 0) : 1) : 0;
+    const bool periodic = cp && (pb[0] || pb[1] || pb[2]);
+    CellDev& cell = sd.cell;
+    if (periodic) {
+      std::memcpy(cell.L, cp, sizeof(cell.L));
+    } else {
+      for (int k = 0; k < 9; ++k) cell.L[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    }
+    invert3(cell.L, cell.inv);
+    for (int k = 0; k < 3; ++k) cell.pbc[k] = periodic ? pb[k] : 0;
+    cell.periodic = periodic ? 1 : 0;
+    for (int k = 0; k < 3; ++k) {
+      const double c0 = cell.inv[k], c1 = cell.inv[3 + k], c2 = cell.inv[6 + k];
+      heights[3 * q + k] = 1.0 / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+      need_bounds |= !cell.pbc[k];
+    }
+    xr_cg_blocks(sd.n, &sd.ndir, &sd.nupd);
+    sd.dir0 = nd;
+    sd.upd0 = nu;
+    nd += sd.ndir;
+    nu += sd.nupd;
+    sd.total_charge = in.charges ? in.charges[q] : 0.0;
+    hsys[q] = sd;
   }
 
   // ------------------------------------------------------ atom workspace
-  const int cap = (n + 3) / 4 + 1;  // energy partial slots per channel
   size_t off = 0;
   const size_t scan_ints = xr_scan_scratch_ints(std::max(n, 1 << 16));
+  carve_size<int>(off, n);               // system of each atom
   carve_size<int>(off, 3 * (size_t)n);   // home
   carve_size<int>(off, n);               // bin
   carve_size<int>(off, n);               // counts
@@ -323,16 +442,17 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
   carve_size<int>(off, scan_ints);       // scan scratch
   carve_size<char>(off, xr_bin_record_bytes() * n);  // bin-ordered atom records
   carve_size<double2>(off, 7 * (size_t)n);  // CG vectors: x, r, Ap (pairs), two (z, p) records
+  carve_size<float4>(off, 2 * (size_t)n);   // FP32 (z, p) records (mixed precision)
   carve_size<double>(off, n);            // inv_diag
-  carve_size<double>(off, xr_cg_partials(n));
-  carve_size<char>(off, xr_cg_state_bytes());  // cg scalars
-  carve_size<double>(off, 2);            // mu
+  carve_size<double>(off, n);            // eta
+  carve_size<double>(off, xr_cg_partials(nd, nu));
   carve_size<double>(off, (size_t)kAtomScalars * n);  // atom scalars
   carve_size<int>(off, n);               // bond counts (>0.3)
   carve_size<double>(off, 6 * (size_t)n);   // F_nb, F_hb
-  carve_size<double>(off, (size_t)NCH * cap);
+  carve_size<double>(off, (size_t)NCH * n);  // per-atom energy channels
   HIPCHK(ctx->atom_buf.ensure(off + 4096), "atom workspace");
   Carver cv{static_cast<char*>(ctx->atom_buf.ptr)};
+  int* asys = cv.take<int>(n);
   int* home = cv.take<int>(3 * (size_t)n);
   int* bin = cv.take<int>(n);
   int* counts = cv.take<int>(n);
@@ -342,20 +462,60 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
   int* scan_tmp = cv.take<int>(scan_ints);
   void* binned = cv.take<char>(xr_bin_record_bytes() * n);
   double2* cgv = cv.take<double2>(7 * (size_t)n);
+  float4* cgq = cv.take<float4>(2 * (size_t)n);
   double* inv_diag = cv.take<double>(n);
-  double* cg_part = cv.take<double>(xr_cg_partials(n));
-  void* cg_state = cv.take<char>(xr_cg_state_bytes());
-  double* mu = cv.take<double>(2);
+  double* eta = cv.take<double>(n);
+  double* cg_part = cv.take<double>(xr_cg_partials(nd, nu));
   double* scal = cv.take<double>((size_t)kAtomScalars * n);
   double* const scal_end = scal + (size_t)kAtomScalars * n;
   int* bcount03 = cv.take<int>(n);
   double* fbuf = cv.take<double>(6 * (size_t)n);
-  double* epart = cv.take<double>((size_t)NCH * cap);
+  double* energies = cv.take<double>((size_t)NCH * n);
+
+  Systems S{reinterpret_cast<const SysDev*>(dc + L.sys), asys, reinterpret_cast<unsigned*>(dc + L.serr), ns};
+
+  // ------------------------------------------------------- bin grids
+  if (need_bounds) {
+    HIPCHK(hipMemcpyAsync(dc + L.sys, hc + L.sys, sizeof(SysDev) * ns, hipMemcpyHostToDevice, s), "system table");
+    HIPCHK(xr_coord_bounds(S, in.x, reinterpret_cast<double*>(dc + L.bounds), s), "bounds");
+    HIPCHK(hipMemcpyAsync(hc + L.bounds, dc + L.bounds, sizeof(double) * 6 * ns, hipMemcpyDeviceToHost, s),
+           "bounds copy");
+    HIPCHK(hipStreamSynchronize(s), "bounds sync");
+  }
+  const double* hbounds = reinterpret_cast<const double*>(hc + L.bounds);
+  long long nbins = 0;
+  for (int q = 0; q < ns; ++q) {
+    SysDev& sd = hsys[q];
+    const bool all_pbc = sd.cell.pbc[0] && sd.cell.pbc[1] && sd.cell.pbc[2];
+    if (!all_pbc) {
+      for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(hbounds[6 * q + k]))
+          return fail(ctx, XREAC_INVALID, "positions must be a finite (N, 3) array in Angstrom");
+    }
+    bin_geometry(sd, &heights[3 * (size_t)q], hbounds + 6 * (size_t)q, rcb);
+    sd.bin0 = (int)nbins;
+    nbins += (long long)sd.g.nb[0] * sd.g.nb[1] * sd.g.nb[2];
+    if (nbins > INT32_MAX / 2) return fail(ctx, XREAC_INVALID, "too many cell-list bins in one batch");
+  }
+  HIPCHK(hipMemcpyAsync(dc + L.sys, hc + L.sys, sizeof(SysDev) * ns, hipMemcpyHostToDevice, s), "system table");
+
+  off = 0;
+  carve_size<int>(off, nbins + 1);
+  carve_size<int>(off, nbins + 1);
+  carve_size<int>(off, nbins + 1);
+  carve_size<int>(off, xr_scan_scratch_ints((int)nbins));
+  HIPCHK(ctx->bin_buf.ensure(off + 4096), "bin workspace");
+  Carver cb{static_cast<char*>(ctx->bin_buf.ptr)};
+  int* bin_count = cb.take<int>(nbins + 1);
+  int* bin_start = cb.take<int>(nbins + 1);
+  int* bin_fill = cb.take<int>(nbins + 1);
+  int* bin_scan = cb.take<int>(xr_scan_scratch_ints((int)nbins));
 
   Atoms at{};
   at.n = n;
-  at.x = sys->positions;
-  at.type = sys->types;
+  at.S = S;
+  at.x = in.x;
+  at.type = in.types;
   double* sp = scal;
   auto nxt = [&]() {
     double* p = sp;
@@ -387,64 +547,10 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
   at.bond_count = bcount03;
   at.F_nb = fbuf;
   at.F_hb = fbuf + 3 * (size_t)n;
-  at.q = out->charges;
-  EnergyAcc acc{epart, cap};
+  at.q = out.charges;
+  EnergyAcc acc{energies, n};
 
-  int* dev_counts = reinterpret_cast<int*>(ctx->err) + 1;  // [0] total edges, [1] total bonds
-
-  // -------------------------------------------------------- binning grid
-  BinGeom g{};
-  const double rcb = rc * (1 + 1e-8) + 1e-12;
-  bool need_bounds = false;
-  for (int k = 0; k < 3; ++k) need_bounds |= !cell.pbc[k];
-  double bounds[6] = {0, 0, 0, 0, 0, 0};
-  if (need_bounds) {
-    double* dev_bounds = reinterpret_cast<double*>(ctx->err) + 32;
-    HIPCHK(xr_coord_bounds(n, sys->positions, cell, dev_bounds, s), "bounds");
-    HIPCHK(hipMemcpyAsync(bounds, dev_bounds, sizeof(bounds), hipMemcpyDeviceToHost, s), "bounds copy");
-    HIPCHK(hipStreamSynchronize(s), "bounds sync");
-    for (int k = 0; k < 6; ++k)
-      if (!std::isfinite(bounds[k]))
-        return fail(ctx, XREAC_INVALID, "positions must be a finite (N, 3) array in Angstrom");
-  }
-  long long nbins = 1;
-  for (int k = 0; k < 3; ++k) {
-    g.pbc[k] = cell.pbc[k];
-    if (cell.pbc[k]) {
-      // Bins at least rc/2 thick along the face normal, searched two deep.
-      int nbk = (int)std::floor(2.0 * height[k] / rcb);
-      nbk = std::max(1, std::min(nbk, 128));
-      g.nb[k] = nbk;
-      g.w[k] = 1.0 / nbk;
-      g.lo[k] = 0.0;
-      const double h = height[k] / nbk;
-      g.reach[k] = h >= rcb ? 1 : (int)std::floor(rcb / h) + 1;
-    } else {
-      double w = 0.5 * rcb / height[k];
-      const double span = bounds[3 + k] - bounds[k];
-      long long nbk = (long long)std::floor(span / w) + 1;
-      if (nbk > 128) {
-        w = span / 127.0;
-        nbk = 128;
-      }
-      g.nb[k] = (int)nbk;
-      g.w[k] = w;
-      g.lo[k] = bounds[k];
-      g.reach[k] = (int)std::floor(rcb / (w * height[k])) + 1;
-    }
-    nbins *= g.nb[k];
-  }
-  off = 0;
-  carve_size<int>(off, nbins + 1);
-  carve_size<int>(off, nbins + 1);
-  carve_size<int>(off, nbins + 1);
-  carve_size<int>(off, xr_scan_scratch_ints((int)nbins));
-  HIPCHK(ctx->bin_buf.ensure(off + 4096), "bin workspace");
-  Carver cb{static_cast<char*>(ctx->bin_buf.ptr)};
-  int* bin_count = cb.take<int>(nbins + 1);
-  int* bin_start = cb.take<int>(nbins + 1);
-  int* bin_fill = cb.take<int>(nbins + 1);
-  int* bin_scan = cb.take<int>(xr_scan_scratch_ints((int)nbins));
+  int* dev_counts = reinterpret_cast<int*>(d_flags) + 1;  // [0] total edges, [1] total bonds
   {
     ZeroRanges z{};
     auto add = [&](void* p, size_t bytes) {
@@ -452,33 +558,34 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
       z.words[z.count] = (long long)(bytes / 4);
       ++z.count;
     };
-    add(ctx->err, 4 * sizeof(unsigned));
+    add(d_flags, 16 * sizeof(unsigned));
+    add(dc + L.serr, sizeof(unsigned) * ns);
+    add(dc + L.cg, xr_cg_state_bytes(ns) & ~size_t(3));
     add(bin_count, sizeof(int) * (nbins + 1));
     add(bin_fill, sizeof(int) * (nbins + 1));
-    add(epart, sizeof(double) * NCH * cap);
+    add(energies, sizeof(double) * NCH * (size_t)n);
     add(at.F_hb, sizeof(double) * 3 * (size_t)n);
     hipLaunchKernelGGL(k_zero_ranges, dim3(256), dim3(256), 0, s, z);
     HIPCHK(hipGetLastError(), "zero");
   }
-  HIPCHK(xr_bin_atoms(n, sys->positions, cell, g, home, bin, bin_count, s), "bin");
+  hipLaunchKernelGGL(k_fill_systems, dim3(ns), dim3(256), 0, s, S, asys);
+  HIPCHK(hipGetLastError(), "system index");
+  HIPCHK(xr_bin_atoms(n, in.x, S, home, bin, bin_count, s), "bin");
   HIPCHK(xr_exclusive_scan(bin_count, (int)nbins, bin_start, bin_scan, s), "bin scan");
-  HIPCHK(xr_bin_scatter(n, sys->positions, home, bin, bin_start, bin_fill, binned, ctx->err, s), "bin scatter");
+  HIPCHK(xr_bin_scatter(n, in.x, S, home, bin, bin_start, bin_fill, binned, s), "bin scatter");
 
   // --------------------------------------------------------- neighbor rows
-  HIPCHK(xr_neighbor_count(n, sys->positions, cell, g, home, bin, bin_start, binned, rc, 2048, counts, ctx->err, s),
-         "neighbor count");
+  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, 2048, counts, s), "neighbor count");
   HIPCHK(xr_exclusive_scan(counts, n, row_ptr, scan_tmp, s), "neighbor scan");
   const double slack = 1.10;
   int64_t m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
   exact = exact || m_cap == 0;
   if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
-  HIPCHK(clamp_ptr(row_ptr, n, exact ? INT32_MAX : (int)m_cap, dev_counts, ctx->err, s), "clamp");
+  HIPCHK(clamp_ptr(row_ptr, n, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags, s), "clamp");
   if (exact) {
     int m_host = 0;
     HIPCHK(hipMemcpyAsync(&m_host, dev_counts, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-    HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, sizeof(unsigned), hipMemcpyDeviceToHost, s), "copy");
     HIPCHK(hipStreamSynchronize(s), "neighbor sync");
-    if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
     m_cap = m_host;
   }
 
@@ -486,20 +593,20 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
   carve_size<int>(off, m_cap);
   carve_size<int>(off, m_cap);
   carve_size<double>(off, m_cap);
+  carve_size<float>(off, mixed ? m_cap : 0);
   carve_size<unsigned char>(off, m_cap);
   HIPCHK(ctx->edge_buf.ensure(off + 4096), "edge workspace");
   Carver ce{static_cast<char*>(ctx->edge_buf.ptr)};
   int* nbr_j = ce.take<int>(m_cap);
   int* nbr_s = ce.take<int>(m_cap);
   double* hval = ce.take<double>(m_cap);
+  float* hval32 = mixed ? ce.take<float>(m_cap) : nullptr;
   unsigned char* bflag = ce.take<unsigned char>(m_cap);
-  HIPCHK(xr_neighbor_fill(n, sys->positions, cell, g, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s,
-                          ctx->err, s),
-         "neighbor fill");
+  HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s, s), "neighbor fill");
 
   // ------------------------------------------------------- edges and bonds
-  HIPCHK(xr_edge_prep(n, sys->positions, sys->types, cell, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, bflag,
-                      bond_count, s),
+  HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, hval32, bflag, bond_count,
+                      s),
          "edge pass");
   HIPCHK(xr_exclusive_scan(bond_count, n, bond_ptr, scan_tmp, s), "bond scan");
   int64_t b_cap = 0;
@@ -507,7 +614,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
     b_cap = (int64_t)std::ceil(ctx->b_per_atom * n * 1.25) + 1024;
     if (b_cap > m_cap) b_cap = m_cap;
   }
-  HIPCHK(clamp_ptr(bond_ptr, n, exact ? INT32_MAX : (int)b_cap, dev_counts + 1, ctx->err, s), "clamp");
+  HIPCHK(clamp_ptr(bond_ptr, n, exact ? INT32_MAX : (int)b_cap, dev_counts + 1, d_flags, s), "clamp");
   if (exact) {
     int nb_host = 0;
     HIPCHK(hipMemcpyAsync(&nb_host, dev_counts + 1, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
@@ -575,80 +682,95 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
 
   // ------------------------------------------------------------------ QEq
   CgArgs cg{};
-  cg.n = n;
-  cg.type = sys->types;
+  cg.S = S;
+  cg.nd = nd;
+  cg.nu = nu;
+  cg.type = in.types;
   cg.ff = ff;
   cg.row_ptr = row_ptr;
   cg.nbr_j = nbr_j;
   cg.hval = hval;
+  cg.hval32 = hval32;
   cg.X = cgv;
   cg.R = cgv + n;
   cg.AP = cgv + 2 * (size_t)n;
   cg.ZP0 = reinterpret_cast<double4*>(cgv + 3 * (size_t)n);
   cg.ZP1 = reinterpret_cast<double4*>(cgv + 5 * (size_t)n);
+  cg.ZQ0 = cgq;
+  cg.ZQ1 = cgq + n;
   cg.inv_diag = inv_diag;
+  cg.eta = eta;
   cg.partials = cg_part;
-  cg.state = cg_state;
+  cg.state = dc + L.cg;
   cg.tol = opt->qeq_tolerance > 0 ? opt->qeq_tolerance : 1e-10;
   cg.max_iter = opt->qeq_max_iterations > 0 ? opt->qeq_max_iterations : 2000;
   // Read back convergence first near the previous solve's iteration count.
   cg.first_check = ctx->last_cg_iters > 0 ? ctx->last_cg_iters + 2 : 16;
-  cg.abort_flag = ctx->err;
   cg.check_every = 4;
-  cg.total_charge = sys->total_charge;
-  cg.q = out->charges;
-  cg.mu = mu;
-  int iters = 0;
-  HIPCHK(xr_cg_solve(cg, s, &iters), "QEq");
-  if (iters == -2) {
-    // an earlier kernel flagged an input error; report it
-    HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, sizeof(unsigned), hipMemcpyDeviceToHost, s), "copy");
-    HIPCHK(hipStreamSynchronize(s), "error sync");
-    if (ctx->err_host[0] & XR_ERR_CAPACITY) {
-      *overflow = true;
-      return XREAC_OK;
-    }
-    return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
-  }
-  if (iters < 0) return fail(ctx, XREAC_INVALID, "QEq conjugate gradient did not converge; check the geometry");
-  out->qeq_iterations = iters;
-  ctx->last_cg_iters = iters;
-
-  // ----------------------------------------------------- energy and forces
-  HIPCHK(xr_nonbonded(at, cell, ff, row_ptr, nbr_j, nbr_s, acc, s), "nonbonded");
-  {
-    hipError_t e = hipSuccess;
-    XR_TIMED(XT_BOND_FORWARD, s, e = xr_bond_forward(at, bd, cell, ff, nbr_j, nbr_s, acc, ctx->err, s));
-    HIPCHK(e, "bond orders");
-    XR_TIMED(XT_MANYBODY, s,
-             e = xr_manybody(at, bd, cell, ff, row_ptr, nbr_j, nbr_s, opt->lammps_angles, acc, ctx->err, s));
-    HIPCHK(e, "many-body");
-    XR_TIMED(XT_BACKWARD, s, e = xr_bond_backward(at, bd, ff, out->forces, s));
-    HIPCHK(e, "adjoint");
-  }
-  HIPCHK(xr_energy_reduce(acc, ctx->comps, s), "energy");
-  if (out->total_bond_orders)
-    HIPCHK(hipMemcpyAsync(out->total_bond_orders, at.total, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
-  if (out->lone_pairs)
-    HIPCHK(hipMemcpyAsync(out->lone_pairs, at.nlp, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
-  if (out->bond_counts)
-    HIPCHK(hipMemcpyAsync(out->bond_counts, at.bond_count, sizeof(int) * n, hipMemcpyDeviceToDevice, s), "copy");
-  double* host_comps = reinterpret_cast<double*>(ctx->err_host) + 8;
-  HIPCHK(hipMemcpyAsync(ctx->err_host, ctx->err, 64 * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
-  HIPCHK(hipStreamSynchronize(s), "final sync");
-  ctx->prof.resolve();
-  const int* host_counts = reinterpret_cast<const int*>(ctx->err_host) + 1;
-  const int64_t m = host_counts[0], nb_total = host_counts[1];
-  if (m > m_cap || nb_total > nbond || (ctx->err_host[0] & XR_ERR_CAPACITY)) {
+  cg.guess = (opt->warm_start && out.qeq_state) ? reinterpret_cast<const double2*>(out.qeq_state) : nullptr;
+  cg.keep = reinterpret_cast<double2*>(out.qeq_state);
+  cg.q = out.charges;
+  cg.mu = reinterpret_cast<double*>(dc + L.mu);
+  cg.abort_flag = d_flags;
+  CgOutcome cgo{};
+  HIPCHK(xr_cg_solve(cg, s, &cgo), "QEq");
+  if (cgo.aborted) {
     *overflow = true;
     return XREAC_OK;
   }
-  if (ctx->err_host[0]) return fail(ctx, XREAC_INVALID, error_text(ctx->err_host[0]));
-  std::memcpy(out->components, host_comps, sizeof(double) * NCOMP);
-  out->num_edges = m;
-  out->num_bonds = nb_total;
+
+  // ----------------------------------------------------- energy and forces
+  HIPCHK(xr_nonbonded(at, ff, row_ptr, nbr_j, nbr_s, acc, s), "nonbonded");
+  {
+    hipError_t e = hipSuccess;
+    XR_TIMED(XT_BOND_FORWARD, s, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s));
+    HIPCHK(e, "bond orders");
+    XR_TIMED(XT_MANYBODY, s, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, opt->lammps_angles, acc, s));
+    HIPCHK(e, "many-body");
+    XR_TIMED(XT_BACKWARD, s, e = xr_bond_backward(at, bd, ff, out.forces, s));
+    HIPCHK(e, "adjoint");
+  }
+  HIPCHK(xr_energy_reduce(S, acc, reinterpret_cast<double*>(dc + L.comps), s), "energy");
+  if (out.tbo)
+    HIPCHK(hipMemcpyAsync(out.tbo, at.total, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
+  if (out.nlp)
+    HIPCHK(hipMemcpyAsync(out.nlp, at.nlp, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
+  if (out.bcounts)
+    HIPCHK(hipMemcpyAsync(out.bcounts, at.bond_count, sizeof(int) * n, hipMemcpyDeviceToDevice, s), "copy");
+  HIPCHK(hipMemcpyAsync(hc, dc, L.sys, hipMemcpyDeviceToHost, s), "copy");
+  HIPCHK(hipStreamSynchronize(s), "final sync");
+  ctx->prof.resolve();
+  const unsigned* h_flags = reinterpret_cast<const unsigned*>(hc + L.flags);
+  const int* host_counts = reinterpret_cast<const int*>(h_flags) + 1;
+  const int64_t m = host_counts[0], nb_total = host_counts[1];
+  if (m > m_cap || nb_total > nbond || (h_flags[0] & XR_ERR_CAPACITY)) {
+    *overflow = true;
+    return XREAC_OK;
+  }
+  const unsigned* h_serr = reinterpret_cast<const unsigned*>(hc + L.serr);
+  const double* h_comps = reinterpret_cast<const double*>(hc + L.comps);
+  const CgSys* h_cg = xr_cg_systems(hc + L.cg);
+  ctx->sys_messages.assign(ns, std::string());
+  int max_iters = 0;
+  for (int q = 0; q < ns; ++q) {
+    out.iters[q] = h_cg[q].iters;
+    max_iters = std::max(max_iters, h_cg[q].iters);
+    std::memcpy(out.comps + (size_t)q * NCOMP, h_comps + (size_t)q * NCOMP, sizeof(double) * NCOMP);
+    if (h_serr[q]) {
+      out.status[q] = XREAC_INVALID;
+      ctx->sys_messages[q] = error_text(h_serr[q]);
+    } else if (h_cg[q].done != CG_DONE) {
+      out.status[q] = XREAC_INVALID;
+      ctx->sys_messages[q] = "QEq conjugate gradient did not converge; check the geometry";
+    } else {
+      out.status[q] = XREAC_OK;
+    }
+  }
+  out.num_edges = m;
+  out.num_bonds = nb_total;
   ctx->m_per_atom = (double)m / n;
   ctx->b_per_atom = (double)nb_total / n;
+  ctx->last_cg_iters = max_iters;
 
   ctx->n = n;
   ctx->m = m;
@@ -664,6 +786,14 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, 
   return XREAC_OK;
 }
 
+int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out) {
+  bool overflow = false;
+  int rc = evaluate_once(ctx, ffh, in, opt, out, false, &overflow);
+  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ffh, in, opt, out, true, &overflow);
+  if (rc == XREAC_OK && overflow) return fail(ctx, XREAC_DEVICE, "internal error: buffer sizing");
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -672,11 +802,93 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
                    xreac_result* out) {
   if (!ctx || !ffh || !sys || !opt || !out) return XREAC_INVALID;
   ctx->message.clear();
-  bool overflow = false;
-  int rc = evaluate_once(ctx, ffh, sys, opt, out, false, &overflow);
-  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ffh, sys, opt, out, true, &overflow);
-  if (rc == XREAC_OK && overflow) return fail(ctx, XREAC_DEVICE, "internal error: buffer sizing");
-  return rc;
+  if (sys->n <= 0 || !sys->positions || !sys->types || !out->forces || !out->charges)
+    return fail(ctx, XREAC_INVALID, "A nonempty system with device positions, types, forces and charges is required");
+  const int32_t offsets[2] = {0, sys->n};
+  BatchIn in{1, offsets, sys->positions, sys->types, sys->cell, sys->pbc, &sys->total_charge};
+  int32_t status = XREAC_OK, iters = 0;
+  BatchOut bo{};
+  bo.comps = out->components;
+  bo.status = &status;
+  bo.iters = &iters;
+  bo.forces = out->forces;
+  bo.charges = out->charges;
+  bo.tbo = out->total_bond_orders;
+  bo.nlp = out->lone_pairs;
+  bo.bcounts = out->bond_counts;
+  bo.qeq_state = nullptr;
+  xreac_options o = *opt;
+  o.warm_start = 0;
+  const int rc = run_batch(ctx, ffh, in, &o, bo);
+  if (rc != XREAC_OK) return rc;
+  out->num_edges = bo.num_edges;
+  out->num_bonds = bo.num_bonds;
+  out->qeq_iterations = iters;
+  if (status != XREAC_OK) return fail(ctx, status, ctx->sys_messages[0]);
+  return XREAC_OK;
+}
+
+int xreac_evaluate_batch(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_batch* b, const xreac_options* opt,
+                         xreac_batch_result* out) {
+  if (!ctx || !ffh || !b || !opt || !out) return XREAC_INVALID;
+  ctx->message.clear();
+  if (b->num_systems <= 0 || !b->atom_offsets || !b->positions || !b->types || !out->forces || !out->charges ||
+      !out->components || !out->status || !out->qeq_iterations)
+    return fail(ctx, XREAC_INVALID, "A nonempty batch with offsets, device positions, types, forces and charges "
+                                    "and host component/status/iteration arrays is required");
+  if (b->atom_offsets[0] != 0) return fail(ctx, XREAC_INVALID, "atom_offsets[0] must be 0");
+  for (int q = 0; q < b->num_systems; ++q)
+    if (b->atom_offsets[q + 1] <= b->atom_offsets[q])
+      return fail(ctx, XREAC_INVALID, "every system of a batch needs at least one atom");
+  if (opt->warm_start && !out->qeq_state) return fail(ctx, XREAC_INVALID, "warm_start needs result->qeq_state");
+  BatchIn in{b->num_systems, b->atom_offsets, b->positions, b->types, b->cells, b->pbc, b->total_charges};
+  BatchOut bo{};
+  bo.comps = out->components;
+  bo.status = out->status;
+  bo.iters = out->qeq_iterations;
+  bo.forces = out->forces;
+  bo.charges = out->charges;
+  bo.tbo = out->total_bond_orders;
+  bo.nlp = out->lone_pairs;
+  bo.bcounts = out->bond_counts;
+  bo.qeq_state = out->qeq_state;
+  const int rc = run_batch(ctx, ffh, in, opt, bo);
+  if (rc != XREAC_OK) return rc;
+  out->num_edges = bo.num_edges;
+  out->num_bonds = bo.num_bonds;
+  return XREAC_OK;
+}
+
+int xreac_verlet(xreac_ctx* ctx, const xreac_ff* ffh, int32_t ns, const int32_t* offsets, const int32_t* types,
+                 const double* dt_fs, double* positions, double* velocities, const double* forces, int32_t stage,
+                 double* kinetic) {
+  if (!ctx || !ffh || ns <= 0 || !offsets || !types || !dt_fs || !positions || !velocities || !forces ||
+      (stage != 0 && stage != 1))
+    return XREAC_INVALID;
+  ctx->message.clear();
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(xr_verlet(ffh->dev, ns, offsets, types, dt_fs, positions, velocities, forces, stage, kinetic, ctx->stream),
+         "verlet");
+  return XREAC_OK;
+}
+
+int xreac_fire_step(xreac_ctx* ctx, int32_t ns, const int32_t* offsets, double* positions, double* velocities,
+                    const double* forces, double* state, int32_t* status, double force_tolerance,
+                    int32_t max_iterations) {
+  if (!ctx || ns <= 0 || !offsets || !positions || !velocities || !forces || !state || !status ||
+      !(force_tolerance > 0) || max_iterations < 0)
+    return XREAC_INVALID;
+  ctx->message.clear();
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(xr_fire(ns, offsets, positions, velocities, forces, state, status, force_tolerance, max_iterations,
+                 ctx->stream),
+         "FIRE");
+  return XREAC_OK;
+}
+
+const char* xreac_batch_error(const xreac_ctx* ctx, int32_t system) {
+  if (!ctx || system < 0 || system >= (int32_t)ctx->sys_messages.size()) return "no such system";
+  return ctx->sys_messages[system].c_str();
 }
 
 int xreac_profile_enable(xreac_ctx* ctx, int timer_mask) {

@@ -139,10 +139,11 @@ __global__ void __launch_bounds__(256) k_angles(Atoms at, Bonds bd, FFDev ff, in
     at.A_total[j] += A_tot;
     at.A_sbo2[j] += A_sbo2;
   }
-  double ea[1] = {e[0]}, ep[1] = {e[1]}, ec[1] = {e[2]};
-  xr_block_partials<1>(ea, acc.partials + CH_ANGLE * acc.cap, gridDim.x);
-  xr_block_partials<1>(ep, acc.partials + CH_PEN * acc.cap, gridDim.x);
-  xr_block_partials<1>(ec, acc.partials + CH_COA * acc.cap, gridDim.x);
+  if (j < at.n) {
+    xr_energy(acc, CH_ANGLE, j, e[0]);
+    xr_energy(acc, CH_PEN, j, e[1]);
+    xr_energy(acc, CH_COA, j, e[2]);
+  }
 }
 
 // ---------------------------------------------------------------- torsions
@@ -256,8 +257,7 @@ __device__ inline void for_each_torsion(const Atoms& at, const Bonds& bd, const 
   }
 }
 
-__global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc,
-                                                  unsigned* __restrict__ err_out) {
+__global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   double e[2] = {0.0, 0.0};
   unsigned err = 0;
@@ -299,10 +299,11 @@ __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, 
     }
     at.A_db[x] += A_db;
   }
-  if (err) atomicOr(err_out, err);
-  double et[1] = {e[0]}, ec[1] = {e[1]};
-  xr_block_partials<1>(et, acc.partials + CH_TOR * acc.cap, gridDim.x);
-  xr_block_partials<1>(ec, acc.partials + CH_CONJ * acc.cap, gridDim.x);
+  if (x < at.n) {
+    if (err) xr_flag(at.S, x, err);
+    xr_energy(acc, CH_TOR, x, e[0]);
+    xr_energy(acc, CH_CONJ, x, e[1]);
+  }
 }
 
 // ---------------------------------------------------------------- H-bonds
@@ -332,13 +333,14 @@ __device__ inline void hbond_eval(const double* prm, double BOd, const double* v
 }
 
 // One wave per hydrogen: donor-side energy and derivatives.
-__global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, CellDev cell, FFDev ff,
+__global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
                                                   const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
                                                   const int* __restrict__ nbr_s, EnergyAcc acc) {
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double etot[1] = {0.0};
   if (j < at.n && AP(ff, at.type[j])[AP_HBOND] == 1.0) {
+    const CellDev& cell = xr_cell(at.S, j);
     const int nt = ff.nt, tj = at.type[j];
     const double hcut = fmin(7.5, ff.gen[12]);
     double F[3] = {0, 0, 0};
@@ -378,16 +380,17 @@ __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, CellDev ce
     }
 #pragma unroll
     for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
+    etot[0] = xr_wave_sum(etot[0]);
     if (lane == 0) {
 #pragma unroll
       for (int k2 = 0; k2 < 3; ++k2) at.F_hb[3 * j + k2] += F[k2];
+      xr_energy(acc, CH_HB, j, etot[0]);
     }
   }
-  xr_block_partials<1>(etot, acc.partials + CH_HB * acc.cap, gridDim.x);
 }
 
 // One wave per heavy atom: force from the acceptor-vector derivative.
-__global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, CellDev cell, FFDev ff,
+__global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, FFDev ff,
                                                      const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
                                                      const int* __restrict__ nbr_s) {
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -395,6 +398,7 @@ __global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, CellDev
   if (k >= at.n) return;
   const int tk = at.type[k];
   if (AP(ff, tk)[AP_HBOND] != 2.0) return;
+  const CellDev& cell = xr_cell(at.S, k);
   const int nt = ff.nt;
   const double hcut = fmin(7.5, ff.gen[12]);
   double F[3] = {0, 0, 0};
@@ -436,14 +440,13 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 
 }  // namespace
 
-hipError_t xr_manybody(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* row_ptr,
-                       const int* nbr_j, const int* nbr_s, int lammps_angles, EnergyAcc acc, unsigned* err,
-                       hipStream_t s) {
+hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
+                       const int* nbr_s, int lammps_angles, EnergyAcc acc, hipStream_t s) {
   hipLaunchKernelGGL(k_angles, grid(at.n), dim3(BLK), 0, s, at, bd, ff, lammps_angles, acc);
-  hipLaunchKernelGGL(k_torsions, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc, err);
+  hipLaunchKernelGGL(k_torsions, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, cell, ff, row_ptr, nbr_j, nbr_s, acc);
-  hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, cell, ff, row_ptr, nbr_j, nbr_s);
+  hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, acc);
+  hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

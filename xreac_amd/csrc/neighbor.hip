@@ -6,8 +6,10 @@
 // the caller's (possibly unwrapped) coordinates with the reference rounding.
 //
 // Bins live in fractional coordinates for periodic directions (each bin at
-// least rc thick measured along the face normal) and in scaled coordinates for
-// nonperiodic ones.  Rows are sorted in LDS by the 64-bit key
+// least rc/2 thick measured along the face normal) and in scaled coordinates
+// for nonperiodic ones.  In a batch every system has its own grid, placed at
+// its own offset in the global bin arrays; atoms only ever meet atoms of their
+// own system.  Rows are sorted in LDS by the 64-bit key
 // (j << 32) | packed(S), which is the reference's (i, j, Sx, Sy, Sz)
 // lexicographic order within row i (neighbors.py:65-98).
 #include "launch.h"
@@ -33,11 +35,15 @@ struct BinRec {
   int home;
 };
 
-__global__ void k_coord_bounds(int n, const double* __restrict__ x, CellDev cell, double* __restrict__ out) {
+// One block per system: fractional-coordinate bounds of its atoms.
+__global__ void __launch_bounds__(256) k_coord_bounds(Systems S, const double* __restrict__ x,
+                                                      double* __restrict__ out) {
+  const SysDev& sd = S.sys[blockIdx.x];
+  if (sd.cell.pbc[0] && sd.cell.pbc[1] && sd.cell.pbc[2]) return;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = sd.atom0 + threadIdx.x; i < sd.atom0 + sd.n; i += blockDim.x) {
     double f[3];
-    frac_of(cell, x, i, f);
+    frac_of(sd.cell, x, i, f);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       lo[k] = fmin(lo[k], f[k]);
@@ -51,25 +57,27 @@ __global__ void k_coord_bounds(int n, const double* __restrict__ x, CellDev cell
     red[3 + k][threadIdx.x] = hi[k];
   }
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + s]);
-        red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+        red[k][threadIdx.x] = fmin(red[k][threadIdx.x], red[k][threadIdx.x + st]);
+        red[3 + k][threadIdx.x] = fmax(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + st]);
       }
     }
     __syncthreads();
   }
-  if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 6) out[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_bin_atoms(int n, const double* __restrict__ x, CellDev cell, BinGeom g, int* __restrict__ home,
+__global__ void k_bin_atoms(int n, const double* __restrict__ x, Systems S, int* __restrict__ home,
                             int* __restrict__ bin, int* __restrict__ bin_count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const SysDev& sd = S.sys[S.asys[i]];
+  const BinGeom& g = sd.g;
   double f[3];
-  frac_of(cell, x, i, f);
+  frac_of(sd.cell, x, i, f);
   int b[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -85,19 +93,19 @@ __global__ void k_bin_atoms(int n, const double* __restrict__ x, CellDev cell, B
     int bk = (int)floor(u);
     b[k] = bk < 0 ? 0 : (bk >= g.nb[k] ? g.nb[k] - 1 : bk);
   }
-  const int id = (b[0] * g.nb[1] + b[1]) * g.nb[2] + b[2];
+  const int id = sd.bin0 + (b[0] * g.nb[1] + b[1]) * g.nb[2] + b[2];
   bin[i] = id;
   atomicAdd(&bin_count[id], 1);
 }
 
 __global__ void k_bin_scatter(int n, const double* __restrict__ x, const int* __restrict__ home,
                               const int* __restrict__ bin, const int* __restrict__ bin_start,
-                              int* __restrict__ bin_fill, BinRec* __restrict__ recs, unsigned* __restrict__ err) {
+                              int* __restrict__ bin_fill, BinRec* __restrict__ recs, Systems S) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int b = bin[i];
   const int h0 = home[3 * i], h1 = home[3 * i + 1], h2 = home[3 * i + 2];
-  if (h0 < -511 || h0 > 511 || h1 < -511 || h1 > 511 || h2 < -511 || h2 > 511) atomicOr(err, (unsigned)XR_ERR_SHIFT_RANGE);
+  if (h0 < -511 || h0 > 511 || h1 < -511 || h1 > 511 || h2 < -511 || h2 > 511) xr_flag(S, i, XR_ERR_SHIFT_RANGE);
   BinRec r;
   r.x = x[3 * i];
   r.y = x[3 * i + 1];
@@ -114,11 +122,13 @@ __global__ void k_bin_scatter(int n, const double* __restrict__ x, const int* __
 // where the periodic image shift changes); lanes stride over each segment
 // (lane = t, stride = nthreads).
 template <class Fn>
-__device__ inline void for_each_neighbor(int i, const double* __restrict__ x, const CellDev& cell, const BinGeom& g,
+__device__ inline void for_each_neighbor(int i, const double* __restrict__ x, const SysDev& sd,
                                          const int* __restrict__ home, const int* __restrict__ bin,
                                          const int* __restrict__ bin_start, const BinRec* __restrict__ recs,
                                          double rc2, int t, int nthreads, unsigned& err, Fn fn) {
-  const int id = bin[i];
+  const CellDev& cell = sd.cell;
+  const BinGeom& g = sd.g;
+  const int id = bin[i] - sd.bin0;
   const int bz = id % g.nb[2], by = (id / g.nb[2]) % g.nb[1], bx = id / (g.nb[2] * g.nb[1]);
   const int hi0 = home[3 * i], hi1 = home[3 * i + 1], hi2 = home[3 * i + 2];
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
@@ -143,7 +153,7 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
       } else if (c1 < 0 || c1 >= g.nb[1]) {
         continue;
       }
-      const int col = (c0 * g.nb[1] + c1) * g.nb[2];
+      const int col = sd.bin0 + (c0 * g.nb[1] + c1) * g.nb[2];
       const int sx0 = g.pbc[0] ? t0 + hi0 : 0;
       const int sy0 = g.pbc[1] ? t1 + hi1 : 0;
       int zz = zlo;
@@ -195,17 +205,17 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
   }
 }
 
-__global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
+__global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restrict__ x, Systems S,
                                                    const int* __restrict__ home, const int* __restrict__ bin,
                                                    const int* __restrict__ bin_start, const BinRec* __restrict__ binned,
-                                                   double rc2, int row_cap, int* __restrict__ counts,
-                                                   unsigned* __restrict__ err_out) {
+                                                   double rc2, int row_cap, int* __restrict__ counts) {
   const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
+  const int sys = __builtin_amdgcn_readfirstlane(S.asys[i]);
   unsigned err = 0;
   int cnt = 0;
-  for_each_neighbor(i, x, cell, g, home, bin, bin_start, binned, rc2, lane, 64, err,
+  for_each_neighbor(i, x, S.sys[sys], home, bin, bin_start, binned, rc2, lane, 64, err,
                     [&](bool hit, int j, int, double r2) {
                       if (hit) {
                         ++cnt;
@@ -218,7 +228,7 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
   if (lane == 0) {
     counts[i] = cnt;
     if (cnt > row_cap) err |= XR_ERR_ROW_TOO_LONG;
-    if (err) atomicOr(err_out, err);
+    if (err) atomicOr(&S.err[sys], err);
   }
 }
 
@@ -268,7 +278,7 @@ __device__ inline void wave_sort(unsigned long long* s, int count) {
   for (int k = 0; k < K; ++k) s[lane * K + k] = v[k];
 }
 
-__global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, CellDev cell, BinGeom g,
+__global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, Systems S,
                                                  const int* __restrict__ home, const int* __restrict__ bin,
                                                  const int* __restrict__ bin_start, const BinRec* __restrict__ binned,
                                                  double rc2, const int* __restrict__ row_ptr, int* __restrict__ nbr_j,
@@ -278,11 +288,20 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
   if (i >= n) return;
   const int lane = threadIdx.x & 63;
   const int begin = row_ptr[i], count = row_ptr[i + 1] - begin;
-  if (count > ROW_CAP) return;  // reported by the count pass
+  if (count > ROW_CAP) {
+    // Reported by the count pass.  The row still gets in-range entries (the
+    // atom itself, zero shift) so later kernels index valid memory; the
+    // system's results are discarded.
+    for (int t2 = lane; t2 < count; t2 += 64) {
+      nbr_j[begin + t2] = i;
+      nbr_s[begin + t2] = xr_pack_shift(0, 0, 0);
+    }
+    return;
+  }
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   unsigned err = 0;
   int cursor = 0;  // identical in every lane
-  for_each_neighbor(i, x, cell, g, home, bin, bin_start, binned, rc2, lane, 64, err,
+  for_each_neighbor(i, x, S.sys[S.asys[i]], home, bin, bin_start, binned, rc2, lane, 64, err,
                     [&](bool hit, int j, int code, double) {
                       const unsigned long long m = __ballot(hit);
                       if (hit) {
@@ -330,44 +349,43 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
 
 }  // namespace
 
-hipError_t xr_coord_bounds(int n, const double* x, CellDev cell, double* bounds6, hipStream_t s) {
-  hipLaunchKernelGGL(k_coord_bounds, dim3(1), dim3(256), 0, s, n, x, cell, bounds6);
+hipError_t xr_coord_bounds(const Systems& S, const double* x, double* bounds, hipStream_t s) {
+  hipLaunchKernelGGL(k_coord_bounds, dim3(S.ns), dim3(256), 0, s, S, x, bounds);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
-hipError_t xr_bin_atoms(int n, const double* x, CellDev cell, BinGeom g, int* home, int* bin, int* bin_count,
+hipError_t xr_bin_atoms(int n, const double* x, const Systems& S, int* home, int* bin, int* bin_count,
                         hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_atoms, dim3((n + 255) / 256), dim3(256), 0, s, n, x, cell, g, home, bin, bin_count);
+  hipLaunchKernelGGL(k_bin_atoms, dim3((n + 255) / 256), dim3(256), 0, s, n, x, S, home, bin, bin_count);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
 size_t xr_bin_record_bytes() { return sizeof(BinRec); }
 
-hipError_t xr_bin_scatter(int n, const double* x, const int* home, const int* bin, const int* bin_start,
-                          int* bin_fill, void* recs, unsigned* err, hipStream_t s) {
+hipError_t xr_bin_scatter(int n, const double* x, const Systems& S, const int* home, const int* bin,
+                          const int* bin_start, int* bin_fill, void* recs, hipStream_t s) {
   hipLaunchKernelGGL(k_bin_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, x, home, bin, bin_start, bin_fill,
-                     static_cast<BinRec*>(recs), err);
+                     static_cast<BinRec*>(recs), S);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
-hipError_t xr_neighbor_count(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
+hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int* home, const int* bin,
                              const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
-                             unsigned* err, hipStream_t s) {
+                             hipStream_t s) {
   const int cap = row_cap < ROW_CAP ? row_cap : ROW_CAP;
-  XR_LAUNCH(XT_NBR_COUNT, k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, cell, g, home, bin, bin_start,
-            static_cast<const BinRec*>(binned), cutoff * cutoff, cap, counts, err);
+  XR_LAUNCH(XT_NBR_COUNT, k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, S, home, bin, bin_start,
+            static_cast<const BinRec*>(binned), cutoff * cutoff, cap, counts);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
-hipError_t xr_neighbor_fill(int n, const double* x, CellDev cell, BinGeom g, const int* home, const int* bin,
+hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
-                            int* nbr_s, unsigned* err, hipStream_t s) {
-  (void)err;
-  XR_LAUNCH(XT_NBR_FILL, k_nbr_fill, dim3(n), dim3(64), 0, s, n, x, cell, g, home, bin, bin_start,
+                            int* nbr_s, hipStream_t s) {
+  XR_LAUNCH(XT_NBR_FILL, k_nbr_fill, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
             static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s);
   XR_LAUNCH_CHECK();
   return hipSuccess;

@@ -15,13 +15,13 @@ namespace {
 constexpr double C_ELE = 332.06371;
 constexpr double SELF_CONVERSION = 23.02;
 
-__global__ void __launch_bounds__(256) k_nonbonded(Atoms at, CellDev cell, FFDev ff, const int* __restrict__ row_ptr,
+__global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int* __restrict__ row_ptr,
                                                    const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
                                                    EnergyAcc acc) {
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  double e[3] = {0.0, 0.0, 0.0};  // vdw, coulomb, qeq
   if (i < at.n) {
+    const CellDev& cell = at.S.sys[__builtin_amdgcn_readfirstlane(at.S.asys[i])].cell;
     const double* g = ff.gen;
     const double rc = g[12];
     const double pw = g[28];
@@ -76,30 +76,34 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, CellDev cell, FFDev
     F2 = xr_wave_sum(F2);
     if (lane == 0) {
       const double* a = ff.atom + ti * AP_COUNT;
-      e[0] = 0.5 * ev_sum;
-      e[1] = 0.5 * C_ELE * ec_sum;
-      e[2] = SELF_CONVERSION * (a[AP_CHI] * qi + 0.5 * a[AP_ETA] * qi * qi);
+      xr_energy(acc, CH_VDW, i, 0.5 * ev_sum);
+      xr_energy(acc, CH_COUL, i, 0.5 * C_ELE * ec_sum);
+      xr_energy(acc, CH_QEQ, i, SELF_CONVERSION * (a[AP_CHI] * qi + 0.5 * a[AP_ETA] * qi * qi));
       at.F_nb[3 * i] = F0;
       at.F_nb[3 * i + 1] = F1;
       at.F_nb[3 * i + 2] = F2;
     }
   }
-  double ev[1] = {e[0]}, ec[1] = {e[1]}, eq[1] = {e[2]};
-  xr_block_partials<1>(ev, acc.partials + CH_VDW * acc.cap, gridDim.x);
-  xr_block_partials<1>(ec, acc.partials + CH_COUL * acc.cap, gridDim.x);
-  xr_block_partials<1>(eq, acc.partials + CH_QEQ * acc.cap, gridDim.x);
 }
 
-__global__ void k_energy_reduce(EnergyAcc acc, double* __restrict__ comps) {
-  // One wave per channel, fixed order within each.
+// One block per system: each channel summed over the system's atoms in a
+// fixed order (lane-strided, then a fixed tree), independent of where the
+// system sits in the batch.
+__global__ void __launch_bounds__(256) k_energy_reduce(Systems S, EnergyAcc acc, double* __restrict__ comps) {
+  const SysDev& sd = S.sys[blockIdx.x];
+  __shared__ double red[NCH][4];
   __shared__ double ch[NCH];
-  const int lane = threadIdx.x & 63, c = threadIdx.x >> 6;
-  if (c < NCH) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c = 0; c < NCH; ++c) {
+    const double* e = acc.e + (size_t)c * acc.n + sd.atom0;
     double s = 0.0;
-    for (int b = lane; b < acc.cap; b += 64) s += acc.partials[c * acc.cap + b];
+    for (int t = threadIdx.x; t < sd.n; t += blockDim.x) s += e[t];
     s = xr_wave_sum(s);
-    if (lane == 0) ch[c] = s;
+    if (lane == 0) red[c][w] = s;
   }
+  __syncthreads();
+  if (threadIdx.x < NCH) ch[threadIdx.x] = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) +
+                                           red[threadIdx.x][3];
   __syncthreads();
   if (threadIdx.x == 0) {
     double out[NCOMP];
@@ -116,22 +120,22 @@ __global__ void k_energy_reduce(EnergyAcc acc, double* __restrict__ comps) {
     out[C_VDW] = ch[CH_VDW];
     out[C_COULOMB] = ch[CH_COUL];
     out[C_QEQ] = ch[CH_QEQ];
-    for (int k = 0; k < NCOMP; ++k) comps[k] = out[k];
+    for (int k = 0; k < NCOMP; ++k) comps[(size_t)blockIdx.x * NCOMP + k] = out[k];
   }
 }
 
 }  // namespace
 
-hipError_t xr_nonbonded(const Atoms& at, CellDev cell, FFDev ff, const int* row_ptr, const int* nbr_j,
-                        const int* nbr_s, EnergyAcc acc, hipStream_t s) {
+hipError_t xr_nonbonded(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
+                        EnergyAcc acc, hipStream_t s) {
   const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  XR_LAUNCH(XT_NONBONDED, k_nonbonded, grid, dim3(256), 0, s, at, cell, ff, row_ptr, nbr_j, nbr_s, acc);
+  XR_LAUNCH(XT_NONBONDED, k_nonbonded, grid, dim3(256), 0, s, at, ff, row_ptr, nbr_j, nbr_s, acc);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
-hipError_t xr_energy_reduce(EnergyAcc acc, double* comps, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(64 * NCH), 0, s, acc, comps);
+hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* comps, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_reduce, dim3(S.ns), dim3(256), 0, s, S, acc, comps);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -2,33 +2,64 @@
 #pragma once
 #include "common.h"
 
+enum CgState { CG_RUNNING = 0, CG_DONE = 1, CG_RESTART = 2, CG_FAILED = 3, CG_STALLED = 4 };
+
+// Per-system solver state (device).
+struct CgSys {
+  int done;       // CgState
+  int iters;      // iterations executed by this system
+  int k0;         // global iteration index of the last (re)start
+  int restarts;
+  double rr0[2];  // true squared residuals at the last restart (mixed precision)
+};
+
+struct CgCounters {
+  int settled;  // systems whose state left CG_RUNNING
+  int restart;  // of those, systems waiting in CG_RESTART
+};
+
+// The state block holds the counters, then CgSys[ns] at byte offset 64.
+inline CgSys* xr_cg_systems(void* state) { return reinterpret_cast<CgSys*>(static_cast<char*>(state) + 64); }
+
 struct CgArgs {
-  int n;
+  Systems S;
+  int nd, nu;          // total direction / update blocks over all systems
   const int* type;
   FFDev ff;
   const int* row_ptr;
   const int* nbr_j;
   const double* hval;
+  const float* hval32;  // non-null: mixed precision (FP32 A p sweeps, FP64 restarts)
   double2 *X, *R, *AP;
   double4 *ZP0, *ZP1;  // (z_s, z_t, p_s, p_t), alternating by iteration
+  float4 *ZQ0, *ZQ1;   // FP32 copies gathered by the mixed-precision sweep
   double* inv_diag;
-  double* partials;  // xr_cg_partials(n) doubles
-  void* state;       // xr_cg_state_bytes() bytes
+  double* eta;
+  double* partials;  // xr_cg_partials(nd, nu) doubles
+  void* state;       // xr_cg_state_bytes(ns) bytes
   double tol;        // relative residual per right-hand side
-  int max_iter;
-  int first_check;   // iterations before the first convergence read-back
+  int max_iter;      // per system
+  int first_check;   // iterations before the first read-back
   int check_every;   // iterations between later read-backs
-  double total_charge;
-  double* q;         // out: charges
-  double* mu;        // out: 1 double
-  const unsigned* abort_flag;  // device word; nonzero stops the solve (input errors)
+  const double2* guess;  // optional starting (s, t) per atom
+  double2* keep;         // optional: receives the solution (s, t) per atom
+  double* q;             // out: charges
+  double* mu;            // out: one double per system
+  const unsigned* abort_flag;  // device word; nonzero stops the solve (buffer overflow)
 };
 
-hipError_t xr_edge_prep(int n, const double* x, const int* type, CellDev cell, FFDev ff, const int* row_ptr,
-                        const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, unsigned char* bflag,
-                        int* bond_count, hipStream_t s);
+struct CgOutcome {
+  bool aborted;    // stopped on abort_flag
+  bool exhausted;  // launch bound reached before every system settled
+  int sweeps;      // iterations launched
+};
+
+hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
+                        const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
+                        unsigned char* bflag, int* bond_count, hipStream_t s);
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int cap,
                         int* bond_edge, int* bond_i, hipStream_t s);
-size_t xr_cg_partials(int n);
-size_t xr_cg_state_bytes();
-hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, int* iterations);
+void xr_cg_blocks(int n, int* ndir, int* nupd);
+size_t xr_cg_partials(int nd, int nu);
+size_t xr_cg_state_bytes(int ns);
+hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out);

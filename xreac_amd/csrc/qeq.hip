@@ -13,6 +13,17 @@
 // One CG iteration is two kernels.  Each consumer block re-reduces the
 // producer's per-block partials in a fixed order, so no separate reduction
 // launches are needed and the result is deterministic.
+//
+// Batches: every system owns a fixed set of blocks in both CG kernels (sized
+// from its atom count only), reduces only its own partials and stops on its
+// own convergence test, so a system's charges are bitwise the same alone and
+// in any batch.
+//
+// Mixed precision: the A p sweep reads FP32 copies of H and of the gathered
+// (z, p) records while x, r, z, p and every dot product stay FP64.  When the
+// recurrence residual has dropped by the inner factor, the system is
+// restarted from its FP64 true residual r = b - A x (iterative refinement);
+// it is converged only when that true residual meets the tolerance.
 #include "launch.h"
 #include "qeq.h"
 #include "profile.h"
@@ -23,13 +34,15 @@ constexpr int ROWS_PER_BLOCK = 4;  // one wave per row, 256 threads
 
 // ---------------------------------------------------------------- edge pass
 __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restrict__ x, const int* __restrict__ type,
-                                                   CellDev cell, FFDev ff, const int* __restrict__ row_ptr,
+                                                   Systems S, FFDev ff, const int* __restrict__ row_ptr,
                                                    const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
                                                    double bond_cut, double* __restrict__ hval,
-                                                   unsigned char* __restrict__ bflag, int* __restrict__ bond_count) {
+                                                   float* __restrict__ hval32, unsigned char* __restrict__ bflag,
+                                                   int* __restrict__ bond_count) {
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
+  const CellDev& cell = S.sys[__builtin_amdgcn_readfirstlane(S.asys[i])].cell;
   const int ti = type[i];
   const double rc = ff.gen[12];
   const double cut = 0.01 * ff.gen[29];
@@ -42,6 +55,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
     double s, ds;
     xr_shield(r, rc, ff.pair[(ti * ff.nt + tj) * PP_COUNT + PP_GAMMA], s, ds);
     hval[e] = 14.4 * s;
+    if (hval32) hval32[e] = (float)(14.4 * s);
     unsigned char active = 0;
     if (r <= bond_cut) {
       BondRaw b;
@@ -83,20 +97,33 @@ __global__ void __launch_bounds__(256) k_bond_fill(int n, const int* __restrict_
 // --------------------------------------------------------------------- CG
 // Kernel boundaries order all cross-block communication: each kernel writes
 // per-block partial dot products, and every block of the next kernel reduces
-// them again in the same fixed order (redundant but cheap, and bitwise
-// reproducible).  Device-scope fences inside a kernel are avoided on purpose:
-// on a multi-XCD part they write back the whole L2.
-struct CgScalars {
-  int done, iters;
-};
-
-constexpr int UPD_BLOCK = 256;
+// its system's partials again in the same fixed order (redundant but cheap,
+// and bitwise reproducible).  Device-scope fences inside a kernel are avoided
+// on purpose: on a multi-XCD part they write back the whole L2.
+constexpr int UPD_BLOCK = 256;      // rows per update block
 constexpr int DIR_BLOCK = 1024;     // 16 waves, one row per wave at a time
-constexpr int DIR_MAX_BLOCKS = 256;  // 4096 waves in flight; few partial sums to reduce
+constexpr int DIR_MAX_BLOCKS = 256;  // per system: 4096 waves; few partial sums to reduce
+constexpr int INNER_MAX = 64;        // mixed precision: iterations between FP64 restarts
+constexpr double INNER_DROP2 = 1e-10;  // mixed precision: squared residual drop before a restart
+constexpr int MAX_RESTARTS = 12;
 
-// Fixed-order reduction of NV partial arrays ([k * nb + b]) by the whole block.
+// System owning global block b (first-block offsets ascend with the system index).
+__device__ inline int block_system(const Systems& S, int b, bool dir) {
+  int lo = 0, hi = S.ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int start = dir ? S.sys[mid].dir0 : S.sys[mid].upd0;
+    if (start <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Fixed-order reduction of NV partial arrays (part[k * stride + b0 + t],
+// t < cnt) by the whole block.
 template <int NV>
-__device__ inline void reduce_partials(const double* __restrict__ part, int nb, double (&out)[NV]) {
+__device__ inline void reduce_partials(const double* __restrict__ part, int stride, int b0, int cnt,
+                                       double (&out)[NV]) {
   __shared__ double red[NV][16];
   __shared__ double res[NV];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -104,7 +131,7 @@ __device__ inline void reduce_partials(const double* __restrict__ part, int nb, 
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
 #pragma unroll 4
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[k * nb + b];
+    for (int b = threadIdx.x; b < cnt; b += blockDim.x) s += part[k * stride + b0 + b];
     s = xr_wave_sum(s);
     if (lane == 0) red[k][wave] = s;
   }
@@ -120,23 +147,54 @@ __device__ inline void reduce_partials(const double* __restrict__ part, int nb, 
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(int n, const int* __restrict__ type, FFDev ff,
-                                                       double2* __restrict__ X, double2* __restrict__ R,
-                                                       double4* __restrict__ ZP, double* __restrict__ inv_diag,
+// Block partial of NV values into part[k * stride + blockIdx.x].
+template <int NV>
+__device__ inline void write_partials(double (&vals)[NV], double* __restrict__ part, int stride) {
+  xr_block_partials<NV>(vals, part, stride);
+}
+
+// Per right-hand side: has this system's residual reached its current target?
+// At a (re)start the residual is the true one and the target is the final
+// tolerance; in between, mixed precision aims for the inner drop.
+__device__ inline void rhs_converged(const double (&cur)[4], const double (&bb)[2], const CgSys& c, int j,
+                                     bool mixed, double tol2, bool (&conv)[2]) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    double target = tol2 * bb[r];
+    if (mixed && j > 0) target = fmax(target, INNER_DROP2 * c.rr0[r]);
+    conv[r] = cur[2 + r] <= target;
+  }
+}
+
+__device__ inline void settle(CgSys* c, CgCounters* cnt, int state) {
+  c->done = state;
+  atomicAdd(&cnt->settled, 1);
+  if (state == CG_RESTART) atomicAdd(&cnt->restart, 1);
+}
+
+__global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __restrict__ type, FFDev ff,
+                                                       const double2* __restrict__ guess, double2* __restrict__ X,
+                                                       double2* __restrict__ R, double4* __restrict__ ZP,
+                                                       float4* __restrict__ ZQ, double* __restrict__ inv_diag,
                                                        double* __restrict__ eta_out, double* __restrict__ part_b,
-                                                       double* __restrict__ part_bb, CgScalars* __restrict__ sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                       double* __restrict__ part_bb, int nu, CgSys* __restrict__ cg,
+                                                       CgCounters* __restrict__ cnt) {
+  const int sys = block_system(S, blockIdx.x, false);
+  const SysDev& sd = S.sys[sys];
+  const int local = blockIdx.x - sd.upd0;
+  const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
   double vals[4] = {0, 0, 0, 0};
   double bb[2] = {0, 0};
-  if (i < n) {
+  if (i < sd.atom0 + sd.n) {
     const double* a = ff.atom + type[i] * AP_COUNT;
     const double eta = a[AP_ETA];
     const double d = 1.0 / eta;
     const double2 r = make_double2(-a[AP_CHI], -1.0);
-    X[i] = make_double2(0.0, 0.0);
+    X[i] = guess ? guess[i] : make_double2(0.0, 0.0);
     R[i] = r;
     const double2 z = make_double2(r.x * d, r.y * d);
     ZP[i] = make_double4(z.x, z.y, 0.0, 0.0);  // (z_s, z_t, p_s, p_t), p = 0 before iteration 0
+    if (ZQ) ZQ[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
     inv_diag[i] = d;
     eta_out[i] = eta;
     vals[0] = r.x * z.x;
@@ -147,67 +205,189 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(int n, const int* __restr
     bb[1] = vals[3];
   }
   // slot 1 holds the residual state before iteration 0
-  xr_block_partials<4>(vals, part_b + 4 * gridDim.x, gridDim.x);
-  xr_block_partials<2>(bb, part_bb, gridDim.x);
-  if (i == 0) {
-    sc->done = 0;
-    sc->iters = -1;
+  write_partials<4>(vals, part_b + 4 * nu, nu);
+  write_partials<2>(bb, part_bb, nu);
+  if (local == 0 && threadIdx.x == 0) {
+    CgSys c{};
+    c.done = CG_RUNNING;
+    c.iters = 0;
+    c.k0 = 0;
+    c.restarts = 0;
+    cg[sys] = c;
+    if (S.err[sys]) {
+      // an input error was flagged upstream: nothing to solve
+      cg[sys].done = CG_FAILED;
+      atomicAdd(&cnt->settled, 1);
+    } else if (guess) {
+      // warm start: the first step computes the true residual of the guess
+      cg[sys].done = CG_RESTART;
+      atomicAdd(&cnt->settled, 1);
+      atomicAdd(&cnt->restart, 1);
+    }
   }
 }
 
-// Iteration k, first half: beta, p = z + beta p_old and A p (grid-stride rows,
-// one wave per row), partial p.Ap per block.  zp_in holds (z, p_old) of every
-// atom; the new p goes to the p half of zp_out.  Up to 512 edges of a row are
-// loaded before any gather so a row costs one memory latency.
-__global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(int n, int k, const int* __restrict__ row_ptr,
-                                                      const int* __restrict__ nbr_j, const double* __restrict__ hval,
-                                                      const double* __restrict__ eta, const double4* __restrict__ zp_in,
-                                                      double4* __restrict__ zp_out, double2* __restrict__ AP,
-                                                      const double* __restrict__ part_b,
-                                                      const double* __restrict__ part_bb, int nb,
-                                                      double* __restrict__ part_a, double tol2,
-                                                      CgScalars* __restrict__ sc) {
-  if (sc->done) return;
+// FP64 true residual r = b - A x for every system waiting in CG_RESTART, on
+// the update-block layout (16 waves, 256 rows per block).  Writes (z, p = 0)
+// into the records iteration k reads and the residual partials into the slot
+// iteration k reduces.  The state word itself is flipped by k_cg_resume, so
+// every block of a system sees the same state throughout this kernel.
+__global__ void __launch_bounds__(DIR_BLOCK) k_cg_restart(Systems S, int k, const int* __restrict__ type, FFDev ff,
+                                                          const int* __restrict__ row_ptr,
+                                                          const int* __restrict__ nbr_j,
+                                                          const double* __restrict__ hval,
+                                                          const double* __restrict__ eta,
+                                                          const double* __restrict__ inv_diag,
+                                                          const double2* __restrict__ X, double2* __restrict__ R,
+                                                          double4* __restrict__ ZP, float4* __restrict__ ZQ,
+                                                          double* __restrict__ part_b, int nu, CgSys* __restrict__ cg) {
+  const int sys = block_system(S, blockIdx.x, false);
+  if (cg[sys].done != CG_RESTART) return;
+  const SysDev& sd = S.sys[sys];
+  const int local = blockIdx.x - sd.upd0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row0 = sd.atom0 + local * UPD_BLOCK, row1 = min(row0 + UPD_BLOCK, sd.atom0 + sd.n);
+  double vals[4] = {0, 0, 0, 0};
+  for (int i = row0 + wave; i < row1; i += DIR_BLOCK / 64) {
+    double as = 0.0, at = 0.0;
+    for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
+      const double2 v = X[nbr_j[e]];
+      as += hval[e] * v.x;
+      at += hval[e] * v.y;
+    }
+    as = xr_wave_sum(as);
+    at = xr_wave_sum(at);
+    if (lane == 0) {
+      const double* a = ff.atom + type[i] * AP_COUNT;
+      const double2 xv = X[i];
+      const double d = eta[i];
+      const double2 r = make_double2(-a[AP_CHI] - (d * xv.x + as), -1.0 - (d * xv.y + at));
+      const double id = inv_diag[i];
+      const double2 z = make_double2(r.x * id, r.y * id);
+      R[i] = r;
+      ZP[i] = make_double4(z.x, z.y, 0.0, 0.0);
+      if (ZQ) ZQ[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
+      vals[0] += r.x * z.x;
+      vals[1] += r.y * z.y;
+      vals[2] += r.x * r.x;
+      vals[3] += r.y * r.y;
+    }
+  }
+  write_partials<4>(vals, part_b + ((k - 1) & 1) * 4 * nu, nu);
+  if (local == 0 && threadIdx.x == 0) {
+    cg[sys].k0 = k;
+    cg[sys].restarts += 1;
+  }
+}
+
+// Systems restarted by k_cg_restart run again; the counters drop them.
+__global__ void k_cg_resume(int ns, CgSys* __restrict__ cg, CgCounters* __restrict__ cnt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ns && cg[s].done == CG_RESTART) cg[s].done = CG_RUNNING;
+  if (s == 0) {
+    cnt->settled -= cnt->restart;
+    cnt->restart = 0;
+  }
+}
+
+// Iteration k, first half: beta, p = z + beta p_old and A p (rows of the
+// system strided over its blocks, one wave per row), partial p.Ap per block.
+// zp_in holds (z, p_old) of every atom; the new p goes to the p half of
+// zp_out.  Up to 512 edges of a row are loaded before any gather so a row
+// costs one memory latency.
+template <bool MIXED>
+__global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, const int* __restrict__ row_ptr,
+                                                            const int* __restrict__ nbr_j,
+                                                            const double* __restrict__ hval,
+                                                            const float* __restrict__ hval32,
+                                                            const double* __restrict__ eta,
+                                                            const double4* __restrict__ zp_in,
+                                                            double4* __restrict__ zp_out,
+                                                            const float4* __restrict__ zq_in,
+                                                            float4* __restrict__ zq_out, double2* __restrict__ AP,
+                                                            const double* __restrict__ part_b,
+                                                            const double* __restrict__ part_bb, int nu,
+                                                            double* __restrict__ part_a, int nd, double tol2,
+                                                            int max_iter, CgSys* __restrict__ cg,
+                                                            CgCounters* __restrict__ cnt) {
+  const int sys = block_system(S, blockIdx.x, true);
+  const CgSys c = cg[sys];
+  if (c.done != CG_RUNNING) return;
+  const SysDev& sd = S.sys[sys];
+  const int local = blockIdx.x - sd.dir0;
+  const int j = k - c.k0;
   double cur[4], bb[2];
-  reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nb, nb, cur);
-  reduce_partials<2>(part_bb, nb, bb);
-  const bool conv_s = cur[2] <= tol2 * bb[0];
-  const bool conv_t = cur[3] <= tol2 * bb[1];
-  if (conv_s && conv_t) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->done = 1;
-      sc->iters = k;
+  reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nu, nu, sd.upd0, sd.nupd, cur);
+  reduce_partials<2>(part_bb, nu, sd.upd0, sd.nupd, bb);
+  bool conv[2];
+  rhs_converged(cur, bb, c, j, MIXED, tol2, conv);
+  if ((conv[0] && conv[1]) || c.iters >= max_iter || (MIXED && j >= INNER_MAX)) {
+    if (local == 0 && threadIdx.x == 0) {
+      int state;
+      if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
+      else if (c.iters >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
+      else state = CG_RESTART;
+      settle(&cg[sys], cnt, state);
     }
     return;
   }
+  if (MIXED && j == 0 && local == 0 && threadIdx.x == 0) {
+    cg[sys].rr0[0] = cur[2];
+    cg[sys].rr0[1] = cur[3];
+  }
   double beta_s = 0.0, beta_t = 0.0;
-  if (k > 0) {
+  if (j > 0) {
     double prev[4];
-    reduce_partials<4>(part_b + (k & 1) * 4 * nb, nb, prev);
-    if (!conv_s && prev[0] != 0.0) beta_s = cur[0] / prev[0];
-    if (!conv_t && prev[1] != 0.0) beta_t = cur[1] / prev[1];
+    reduce_partials<4>(part_b + (k & 1) * 4 * nu, nu, sd.upd0, sd.nupd, prev);
+    if (!conv[0] && prev[0] != 0.0) beta_s = cur[0] / prev[0];
+    if (!conv[1] && prev[1] != 0.0) beta_t = cur[1] / prev[1];
   }
   const int lane = threadIdx.x & 63;
-  const int waves = gridDim.x * (blockDim.x >> 6);
+  const int waves = sd.ndir * (DIR_BLOCK >> 6);
+  const int end = sd.atom0 + sd.n;
   double pap_s = 0.0, pap_t = 0.0;
-  for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += waves) {
+  for (int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6); i < end; i += waves) {
     double acc_s = 0.0, acc_t = 0.0;
     const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
-    for (int base = e0; base < e1; base += 512) {
-      int jj[8];
-      double hh[8];
+    if (MIXED) {
+      const float bs = (float)beta_s, bt = (float)beta_t;
+      float fs = 0.0f, ft = 0.0f;
+      for (int base = e0; base < e1; base += 512) {
+        int jj[8];
+        float hh[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int e = base + u * 64 + lane;
-        const bool ok = e < e1;
-        jj[u] = ok ? nbr_j[e] : i;
-        hh[u] = ok ? hval[e] : 0.0;
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + u * 64 + lane;
+          const bool ok = e < e1;
+          jj[u] = ok ? nbr_j[e] : i;
+          hh[u] = ok ? hval32[e] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float4 v = zq_in[jj[u]];
+          fs += hh[u] * (v.x + bs * v.z);
+          ft += hh[u] * (v.y + bt * v.w);
+        }
       }
+      acc_s = (double)fs;
+      acc_t = (double)ft;
+    } else {
+      for (int base = e0; base < e1; base += 512) {
+        int jj[8];
+        double hh[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const double4 v = zp_in[jj[u]];
-        acc_s += hh[u] * (v.x + beta_s * v.z);
-        acc_t += hh[u] * (v.y + beta_t * v.w);
+        for (int u = 0; u < 8; ++u) {
+          const int e = base + u * 64 + lane;
+          const bool ok = e < e1;
+          jj[u] = ok ? nbr_j[e] : i;
+          hh[u] = ok ? hval[e] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const double4 v = zp_in[jj[u]];
+          acc_s += hh[u] * (v.x + beta_s * v.z);
+          acc_t += hh[u] * (v.y + beta_t * v.w);
+        }
       }
     }
     acc_s = xr_wave_sum(acc_s);
@@ -218,37 +398,43 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(int n, int k, const 
       const double d = eta[i];
       const double2 ap = make_double2(d * p.x + acc_s, d * p.y + acc_t);
       reinterpret_cast<double2*>(zp_out + i)[1] = p;
+      if (MIXED) reinterpret_cast<float2*>(zq_out + i)[1] = make_float2((float)p.x, (float)p.y);
       AP[i] = ap;
       pap_s += p.x * ap.x;
       pap_t += p.y * ap.y;
     }
   }
   double vals[2] = {pap_s, pap_t};
-  xr_block_partials<2>(vals, part_a, gridDim.x);
+  write_partials<2>(vals, part_a, nd);
 }
 
 // Iteration k, second half: alpha, x += alpha p, r -= alpha A p, z = r / diag
 // (into the z half of zp), partial r.z and r.r per block.
-__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(int n, int k, const double* __restrict__ inv_diag,
+template <bool MIXED>
+__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const double* __restrict__ inv_diag,
                                                          double2* __restrict__ X, double2* __restrict__ R,
-                                                         double4* __restrict__ zp, const double2* __restrict__ AP,
-                                                         const double* __restrict__ part_a, int na,
+                                                         double4* __restrict__ zp, float4* __restrict__ zq,
+                                                         const double2* __restrict__ AP,
+                                                         const double* __restrict__ part_a, int nd,
                                                          double* __restrict__ part_b,
-                                                         const double* __restrict__ part_bb, double tol2,
-                                                         const CgScalars* __restrict__ sc) {
-  if (sc->done) return;
-  const int nb = gridDim.x;
+                                                         const double* __restrict__ part_bb, int nu, double tol2,
+                                                         CgSys* __restrict__ cg) {
+  const int sys = block_system(S, blockIdx.x, false);
+  const CgSys c = cg[sys];
+  if (c.done != CG_RUNNING) return;
+  const SysDev& sd = S.sys[sys];
+  const int local = blockIdx.x - sd.upd0;
   double cur[4], pap[2], bb[2];
-  reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nb, nb, cur);
-  reduce_partials<2>(part_a, na, pap);
-  reduce_partials<2>(part_bb, nb, bb);
-  const bool conv_s = cur[2] <= tol2 * bb[0];
-  const bool conv_t = cur[3] <= tol2 * bb[1];
-  const double alpha_s = (!conv_s && pap[0] != 0.0) ? cur[0] / pap[0] : 0.0;
-  const double alpha_t = (!conv_t && pap[1] != 0.0) ? cur[1] / pap[1] : 0.0;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nu, nu, sd.upd0, sd.nupd, cur);
+  reduce_partials<2>(part_a, nd, sd.dir0, sd.ndir, pap);
+  reduce_partials<2>(part_bb, nu, sd.upd0, sd.nupd, bb);
+  bool conv[2];
+  rhs_converged(cur, bb, c, k - c.k0, MIXED, tol2, conv);
+  const double alpha_s = (!conv[0] && pap[0] != 0.0) ? cur[0] / pap[0] : 0.0;
+  const double alpha_t = (!conv[1] && pap[1] != 0.0) ? cur[1] / pap[1] : 0.0;
+  const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
   double vals[4] = {0, 0, 0, 0};
-  if (i < n) {
+  if (i < sd.atom0 + sd.n) {
     const double2 p = reinterpret_cast<const double2*>(zp + i)[1];
     const double2 ap = AP[i];
     double2 xv = X[i], r = R[i];
@@ -261,20 +447,25 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(int n, int k, const dou
     X[i] = xv;
     R[i] = r;
     reinterpret_cast<double2*>(zp + i)[0] = z;
+    if (MIXED) reinterpret_cast<float2*>(zq + i)[0] = make_float2((float)z.x, (float)z.y);
     vals[0] = r.x * z.x;
     vals[1] = r.y * z.y;
     vals[2] = r.x * r.x;
     vals[3] = r.y * r.y;
   }
-  xr_block_partials<4>(vals, part_b + (k & 1) * 4 * nb, nb);
+  write_partials<4>(vals, part_b + (k & 1) * 4 * nu, nu);
+  if (local == 0 && threadIdx.x == 0) cg[sys].iters = c.iters + 1;
 }
 
-// q = s + mu t with mu = (Q - sum s) / sum t.
-__global__ void __launch_bounds__(1024) k_qeq_charges(int n, const double2* __restrict__ X, double total_charge,
-                                                      double* __restrict__ q, double* __restrict__ mu_out) {
+// Per system: q = s + mu t with mu = (Q - sum s) / sum t; the (s, t) solution
+// is kept in `keep` (when given) as the next call's starting guess.
+__global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* __restrict__ X,
+                                                      double* __restrict__ q, double* __restrict__ mu_out,
+                                                      double2* __restrict__ keep) {
+  const SysDev& sd = S.sys[blockIdx.x];
   __shared__ double red[2][16];
   double ss = 0.0, st = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = sd.atom0 + threadIdx.x; i < sd.atom0 + sd.n; i += blockDim.x) {
     ss += X[i].x;
     st += X[i].y;
   }
@@ -293,22 +484,26 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(int n, const double2* __re
       a += red[0][k];
       b += red[1][k];
     }
-    mu_s = (total_charge - a) / b;
-    mu_out[0] = mu_s;
+    mu_s = (sd.total_charge - a) / b;
+    mu_out[blockIdx.x] = mu_s;
   }
   __syncthreads();
   const double mu = mu_s;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) q[i] = X[i].x + mu * X[i].y;
+  for (int i = sd.atom0 + threadIdx.x; i < sd.atom0 + sd.n; i += blockDim.x) {
+    const double2 v = X[i];
+    q[i] = v.x + mu * v.y;
+    if (keep) keep[i] = v;
+  }
 }
 
 }  // namespace
 
-hipError_t xr_edge_prep(int n, const double* x, const int* type, CellDev cell, FFDev ff, const int* row_ptr,
-                        const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, unsigned char* bflag,
-                        int* bond_count, hipStream_t s) {
+hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
+                        const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
+                        unsigned char* bflag, int* bond_count, hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
-  XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, cell, ff, row_ptr, nbr_j, nbr_s,
-            bond_cut, hval, bflag, bond_count);
+  XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr, nbr_j, nbr_s,
+            bond_cut, hval, hval32, bflag, bond_count);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
@@ -322,48 +517,74 @@ hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, c
   return hipSuccess;
 }
 
-size_t xr_cg_partials(int n) {
-  const int nb = (n + UPD_BLOCK - 1) / UPD_BLOCK;
-  return (size_t)(2 * 4 * nb + 2 * nb + 2 * DIR_MAX_BLOCKS + n) + 64;
+void xr_cg_blocks(int n, int* ndir, int* nupd) {
+  int d = (n + DIR_BLOCK / 64 - 1) / (DIR_BLOCK / 64);
+  *ndir = d < 1 ? 1 : (d > DIR_MAX_BLOCKS ? DIR_MAX_BLOCKS : d);
+  const int u = (n + UPD_BLOCK - 1) / UPD_BLOCK;
+  *nupd = u < 1 ? 1 : u;
 }
 
-size_t xr_cg_state_bytes() { return sizeof(CgScalars); }
+size_t xr_cg_partials(int nd, int nu) { return (size_t)(2 * 4 * nu + 2 * nu + 2 * nd) + 64; }
 
-hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, int* iterations) {
-  const int n = a.n;
-  const int nb = (n + UPD_BLOCK - 1) / UPD_BLOCK;
-  int na = (n + DIR_BLOCK / 64 - 1) / (DIR_BLOCK / 64);
-  if (na > DIR_MAX_BLOCKS) na = DIR_MAX_BLOCKS;
+size_t xr_cg_state_bytes(int ns) { return sizeof(CgSys) * (size_t)ns + sizeof(CgCounters) + 64; }
+
+hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
+  const int nu = a.nu, nd = a.nd, ns = a.S.ns;
   double* part_b = a.partials;
-  double* part_bb = part_b + 2 * 4 * nb;
-  double* part_a = part_bb + 2 * nb;
-  double* eta = part_a + 2 * DIR_MAX_BLOCKS;
-  CgScalars* sc = static_cast<CgScalars*>(a.state);
+  double* part_bb = part_b + 2 * 4 * nu;
+  double* part_a = part_bb + 2 * nu;
+  CgCounters* cnt = static_cast<CgCounters*>(a.state);
+  CgSys* cg = xr_cg_systems(a.state);
   const double tol2 = a.tol * a.tol;
-  hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(UPD_BLOCK), 0, s, n, a.type, a.ff, a.X, a.R, a.ZP0, a.inv_diag,
-                     eta, part_b, part_bb, sc);
+  const bool mixed = a.hval32 != nullptr;
+  float4* zq0 = mixed ? a.ZQ0 : nullptr;
+  float4* zq1 = mixed ? a.ZQ1 : nullptr;
+  // counters start at zero (cleared with the control block)
+  hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.guess, a.X, a.R, a.ZP0,
+                     zq0, a.inv_diag, a.eta, part_b, part_bb, nu, cg, cnt);
   XR_LAUNCH_CHECK();
   int k = 0;
-  int host_state[3] = {0, -1, 0};
-  bool first = true, stop = false;
-  while (!stop) {
+  // Bound on launches: every system settles within max_iter executed
+  // iterations plus one test per restart.
+  const long long k_limit =
+      (long long)a.max_iter + (long long)(MAX_RESTARTS + 2) * (INNER_MAX + a.check_every + 2) + a.first_check;
+  int host_state[3] = {0, 0, 0};
+  bool first = true;
+  out->aborted = false;
+  out->exhausted = false;
+  for (;;) {
+    // pending restarts (and the warm start) are served before iterating
+    if (host_state[1] > 0 || (first && a.guess)) {
+      hipLaunchKernelGGL(k_cg_restart, dim3(nu), dim3(DIR_BLOCK), 0, s, a.S, k, a.type, a.ff, a.row_ptr, a.nbr_j,
+                         a.hval, a.eta, a.inv_diag, a.X, a.R, (k & 1) ? a.ZP1 : a.ZP0, (k & 1) ? zq1 : zq0, part_b,
+                         nu, cg);
+      hipLaunchKernelGGL(k_cg_resume, dim3((ns + 255) / 256), dim3(256), 0, s, ns, cg, cnt);
+      XR_LAUNCH_CHECK();
+    }
     const int chunk = first ? a.first_check : a.check_every;
     first = false;
-    for (int c = 0; c < chunk; ++c) {
+    for (int c = 0; c < chunk && k < k_limit; ++c) {
       double4* zp_in = (k & 1) ? a.ZP1 : a.ZP0;
       double4* zp_out = (k & 1) ? a.ZP0 : a.ZP1;
-      XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction, dim3(na), dim3(DIR_BLOCK), 0, s, n, k, a.row_ptr, a.nbr_j, a.hval, eta,
-                zp_in, zp_out, a.AP, part_b, part_bb, nb, part_a, tol2, sc);
-      if (k == a.max_iter) {  // this launch only tests the final residual
-        stop = true;
-        break;
+      float4* zq_in = (k & 1) ? zq1 : zq0;
+      float4* zq_out = (k & 1) ? zq0 : zq1;
+      if (mixed) {
+        XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<true>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
+                  a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
+                  nd, tol2, a.max_iter, cg, cnt);
+        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
+                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
+      } else {
+        XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<false>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
+                  a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
+                  nd, tol2, a.max_iter, cg, cnt);
+        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
+                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
       }
-      XR_LAUNCH(XT_CG_UPDATE, k_cg_update, dim3(nb), dim3(UPD_BLOCK), 0, s, n, k, a.inv_diag, a.X, a.R, zp_out,
-                a.AP, part_a, na, part_b, part_bb, tol2, sc);
       ++k;
     }
     XR_LAUNCH_CHECK();
-    hipError_t e = hipMemcpyAsync(host_state, sc, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(host_state, cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
     if (a.abort_flag) {
       e = hipMemcpyAsync(host_state + 2, a.abort_flag, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -371,14 +592,19 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, int* iterations) {
     }
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    if (host_state[2]) {  // an input error was flagged upstream; stop iterating
-      *iterations = -2;
+    if (host_state[2]) {  // buffers overflowed upstream; the caller repeats the evaluation
+      out->aborted = true;
       return hipSuccess;
     }
-    if (host_state[0]) break;
+    if (host_state[0] >= ns && host_state[1] == 0) break;
+    if (k >= k_limit) {
+      out->exhausted = true;
+      break;
+    }
   }
-  *iterations = host_state[0] ? host_state[1] : -1;
-  hipLaunchKernelGGL(k_qeq_charges, dim3(1), dim3(1024), 0, s, n, a.X, a.total_charge, a.q, a.mu);
+  out->sweeps = k;
+  hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.X, a.q, a.mu, a.keep);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
+

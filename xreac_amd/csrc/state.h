@@ -10,9 +10,12 @@ enum Channel {
 };
 
 struct EnergyAcc {
-  double* partials;  // [NCH][cap]
-  int cap;           // partial slots per channel (zeroed before each evaluation)
+  double* e;  // [NCH][n] energy of each channel attributed to each atom (zeroed before each evaluation)
+  int n;
 };
+
+// Store atom i's share of a channel (one writer per channel and atom).
+__device__ inline void xr_energy(const EnergyAcc& a, int ch, int i, double v) { a.e[(size_t)ch * a.n + i] = v; }
 
 struct Bonds {
   int cap;                 // allocated bond slots (launch size of per-bond kernels)
@@ -40,6 +43,7 @@ struct Bonds {
 
 struct Atoms {
   int n;
+  Systems S;               // systems of the batch (cells, per-system error bits)
   const double* x;
   const int* type;
   double* totp;            // uncorrected total bond order
@@ -52,14 +56,14 @@ struct Atoms {
 };
 
 // bondorder.hip
-hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* nbr_j,
-                           const int* nbr_s, EnergyAcc acc, unsigned* err, hipStream_t s);
+hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, FFDev ff, const int* nbr_j, const int* nbr_s,
+                           EnergyAcc acc, hipStream_t s);
 hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, double* forces, hipStream_t s);
 // manybody.hip
-hipError_t xr_manybody(const Atoms& at, const Bonds& bd, CellDev cell, FFDev ff, const int* row_ptr,
-                       const int* nbr_j, const int* nbr_s, int lammps_angles, EnergyAcc acc, unsigned* err,
-                       hipStream_t s);
+hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
+                       const int* nbr_s, int lammps_angles, EnergyAcc acc, hipStream_t s);
 // pairs.hip
-hipError_t xr_nonbonded(const Atoms& at, CellDev cell, FFDev ff, const int* row_ptr, const int* nbr_j,
-                        const int* nbr_s, EnergyAcc acc, hipStream_t s);
-hipError_t xr_energy_reduce(EnergyAcc acc, double* comps, hipStream_t s);
+hipError_t xr_nonbonded(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
+                        EnergyAcc acc, hipStream_t s);
+// Per-system components [ns][NCOMP] from the per-atom channels.
+hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* comps, hipStream_t s);

@@ -35,6 +35,13 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data if isinstance(a, np.ndarray) else int(a.data_ptr()))
 
 
+def _precision(name):
+    try:
+        return {"fp64": _lib.PRECISION_FP64, "mixed": _lib.PRECISION_MIXED}[name]
+    except KeyError:
+        raise ValueError(f"precision must be 'fp64' or 'mixed', not {name!r}") from None
+
+
 class DeviceError(RuntimeError):
     pass
 
@@ -108,7 +115,7 @@ class Engine:
         return out
 
     def evaluate(self, positions, types, cell=None, pbc=(False, False, False), total_charge=0.0, *, tol=1e-10,
-                 max_iter=2000, lammps_angles=True, properties=True, out=None):
+                 max_iter=2000, lammps_angles=True, properties=True, out=None, precision="fp64"):
         """Evaluate one system whose positions (n,3 float64) and types (n int32) are device tensors.
 
         Returns a dict of device tensors (forces, charges and optionally
@@ -136,7 +143,7 @@ class Engine:
             (C.c_int32 * 3)(*[int(bool(v)) for v in pbc]),
             float(total_charge),
         )
-        opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0)
+        opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision), 0)
         res = _lib.Result()
         res.forces = _ptr(forces)
         res.charges = _ptr(charges)
@@ -157,6 +164,107 @@ class Engine:
             qeq_iterations=int(res.qeq_iterations),
             n=n,
         )
+
+    def batch_outputs(self, n, properties=False, qeq_state=False):
+        """Device output buffers for a batch of n atoms in total (reusable via ``out=``)."""
+        out = self.outputs(n, properties)
+        out["qeq_state"] = (
+            self.torch.zeros((n, 2), dtype=self.torch.float64, device=self.device) if qeq_state else None
+        )
+        return out
+
+    def evaluate_batch(self, positions, types, offsets, cells=None, pbc=None, total_charges=None, *, tol=1e-10,
+                       max_iter=2000, lammps_angles=True, properties=False, precision="fp64", warm_start=False,
+                       out=None):
+        """Evaluate independent systems packed atom-wise in one pipeline.
+
+        positions (N, 3) float64 and types (N,) int32 are device tensors;
+        ``offsets`` (ns + 1,) host integers delimit the systems; ``cells``
+        (ns, 3, 3) and ``pbc`` (ns, 3) are host arrays (None: all
+        nonperiodic); ``total_charges`` (ns,) or None.  ``out`` may carry a
+        (N, 2) ``qeq_state`` tensor: with ``warm_start`` it is the CG starting
+        guess, and it always receives the (s, t) solution.
+
+        Returns device forces/charges (and properties), host ``components``
+        (ns, 14), ``status`` (ns,), ``errors`` (message or None per system),
+        ``qeq_iterations`` (ns,) and edge/bond counts.  A failed system does
+        not affect the others.
+        """
+        torch = self.torch
+        offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        ns = len(offsets) - 1
+        n = int(offsets[-1])
+        assert ns >= 1 and positions.shape == (n, 3) and types.shape == (n,)
+        assert positions.dtype == torch.float64 and positions.is_contiguous()
+        assert types.dtype == torch.int32 and types.is_contiguous()
+        if out is None:
+            out = self.batch_outputs(n, properties, qeq_state=warm_start)
+        qs = out.get("qeq_state")
+        if warm_start and qs is None:
+            raise ValueError("warm_start needs out['qeq_state']")
+        keep = []
+        cells_arr = pbc_arr = q_arr = None
+        if cells is not None:
+            cells_arr = np.ascontiguousarray(cells, dtype=np.float64).reshape(ns, 9)
+            keep.append(cells_arr)
+            pbc_arr = np.ones((ns, 3), dtype=np.int32) if pbc is None else np.ascontiguousarray(pbc, dtype=np.int32)
+            keep.append(pbc_arr)
+        if total_charges is not None:
+            q_arr = np.ascontiguousarray(total_charges, dtype=np.float64).reshape(ns)
+        batch = _lib.Batch(
+            ns,
+            _ptr(offsets),
+            _ptr(positions),
+            _ptr(types),
+            _ptr(cells_arr) if cells_arr is not None else None,
+            _ptr(pbc_arr) if pbc_arr is not None else None,
+            _ptr(q_arr) if q_arr is not None else None,
+        )
+        comps = np.zeros((ns, _lib.NCOMP), dtype=np.float64)
+        status = np.zeros(ns, dtype=np.int32)
+        iters = np.zeros(ns, dtype=np.int32)
+        res = _lib.BatchResult()
+        res.components = _ptr(comps)
+        res.status = _ptr(status)
+        res.qeq_iterations = _ptr(iters)
+        res.forces = _ptr(out["forces"])
+        res.charges = _ptr(out["charges"])
+        for name in ("total_bond_orders", "lone_pairs", "bond_counts", "qeq_state"):
+            t = out.get(name)
+            setattr(res, name, _ptr(t) if t is not None else None)
+        opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision),
+                           1 if warm_start else 0)
+        rc = self.lib.xreac_evaluate_batch(self._ctx, self._ff, C.byref(batch), C.byref(opt), C.byref(res))
+        self._check(rc, self._ctx)
+        errors = [None if status[k] == _lib.XREAC_OK else self.lib.xreac_batch_error(self._ctx, k).decode()
+                  for k in range(ns)]
+        result = {k: out.get(k) for k in ("forces", "charges", "total_bond_orders", "lone_pairs", "bond_counts",
+                                          "qeq_state")}
+        result.update(components=comps, status=status, errors=errors, qeq_iterations=iters,
+                      num_edges=int(res.num_edges), num_bonds=int(res.num_bonds), offsets=offsets)
+        return result
+
+    def verlet(self, offsets, types, dt_fs, positions, velocities, forces, stage, kinetic=None):
+        """One velocity-Verlet stage on device tensors (offsets int32 [ns+1], dt_fs float64 [ns]).
+
+        stage 0 kicks by dt/2 and drifts by dt; stage 1 kicks by dt/2 with
+        the new forces and writes per-system kinetic energy (kcal/mol).
+        """
+        ns = int(offsets.shape[0]) - 1
+        for t in (offsets, types, dt_fs, positions, velocities, forces):
+            assert t.is_cuda and t.is_contiguous()
+        rc = self.lib.xreac_verlet(self._ctx, self._ff, ns, _ptr(offsets), _ptr(types), _ptr(dt_fs),
+                                   _ptr(positions), _ptr(velocities), _ptr(forces), int(stage),
+                                   _ptr(kinetic) if kinetic is not None else None)
+        self._check(rc, self._ctx)
+
+    def fire_step(self, offsets, positions, velocities, forces, state, status, force_tolerance, max_iterations):
+        """One FIRE iteration per system (device tensors; state float64 [ns, 4], status int32 [ns])."""
+        ns = int(offsets.shape[0]) - 1
+        rc = self.lib.xreac_fire_step(self._ctx, ns, _ptr(offsets), _ptr(positions), _ptr(velocities),
+                                      _ptr(forces), _ptr(state), _ptr(status), float(force_tolerance),
+                                      int(max_iterations))
+        self._check(rc, self._ctx)
 
     def profile(self, enable=True, timers=None):
         """Enable hipEvent timing of the named kernels (all when None), or disable it."""
