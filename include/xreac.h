@@ -100,6 +100,10 @@ typedef struct {
   int32_t lammps_angles;       /* 1: LAMMPS near-collinear angle derivative (fixed-charge mode) */
   int32_t precision;           /* XREAC_PRECISION_* */
   int32_t warm_start;          /* 1: start the CG from result->qeq_state (batched entry point) */
+  int32_t full_derivative;     /* 1: charge-response forces (derivative through QEq, one adjoint solve);
+                                  the plain angle derivative is then used, as calculator.py:79-86 does */
+  int32_t fixed_charges;       /* 1: no QEq; the charges array is read as the atomic charges (the
+                                  EnergyModel.components(x, fixed_charges) hook, energy.py:180-219) */
 } xreac_options;
 
 typedef struct {

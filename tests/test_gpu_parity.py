@@ -41,6 +41,12 @@ def _systems():
     out["water_cluster_192"] = ("ffield.reax.HO.2015", s, x, None, None, 0.0)
     s, x, cell = workloads.cho_mixture(counts=(("CH4", 12), ("O2", 12), ("H2O", 6), ("CH3OH", 4)), length=14.0, seed=7)
     out["cho_mixture_small"] = ("ffield.reax.CHO.2008", s, x, cell, [True] * 3, 0.0)
+    s, x = workloads.anatase_cell()
+    x = x + np.random.default_rng(4).normal(0, 0.02, x.shape)
+    out["tioh_anatase_bulk"] = ("ffield.reax.TiOH.Kim2013", s, x,
+                                np.diag([workloads.ANATASE_A, workloads.ANATASE_A, workloads.ANATASE_C]), [True] * 3, 0.0)
+    s, x, cell = workloads.anatase_water_slab(nx=2, ny=2, nz=1, waters=8, water_thickness=8.0, vacuum=6.0, seed=5)
+    out["tioh_slab_water"] = ("ffield.reax.TiOH.Kim2013", s, x, cell, [True] * 3, 0.0)
     # ~610 neighbors per atom within 10 A: exercises rows longer than 512
     rng = np.random.default_rng(3)
     grid = np.array(np.meshgrid(*[np.arange(10)] * 3, indexing="ij")).reshape(3, -1).T
@@ -163,3 +169,44 @@ def test_deterministic_repeat():
     assert a.energy == b.energy
     np.testing.assert_array_equal(a.forces, b.forces)
     np.testing.assert_array_equal(a.charges, b.charges)
+
+
+FULL = ["water_distorted_dimer", "water_cluster_192", "periodic_bulk_water_192", "charged_hydronium",
+        "charged_charged_water_4A", "small_water_4A", "cho_mixture_small", "zno_cluster20", "periodic_carbon_chain"]
+
+
+@pytest.mark.parametrize("name", FULL)
+def test_parity_full_derivative(name):
+    """Charge-response forces (calculator.py:79-86) against the oracle's autograd through the QEq solve."""
+    ffname, s, x, cell, pbc, q = SYSTEMS[name]
+    print(f"[full] {name}: n={len(s)}", flush=True)
+    calc = _calc(ffname)
+    gpu = calc.evaluate(s, x, cell=cell, pbc=pbc, total_charge=q, full_derivative=True)
+    ref = orc.evaluate(calc.force_field, s, x, cell=cell, pbc=pbc, total_charge=q, full_derivative=True)
+    assert gpu.full_derivative and gpu.force_convention == "charge_response"
+    assert abs(gpu.energy - ref["energy"]) <= E_REL * max(1.0, abs(ref["energy"]))
+    np.testing.assert_allclose(gpu.forces, ref["forces"], rtol=0, atol=F_ABS)
+    np.testing.assert_allclose(gpu.charges, ref["charges"], rtol=0, atol=Q_ABS)
+
+
+def test_full_derivative_is_the_energy_gradient():
+    """Charge-response forces against central differences of the QEq-equilibrated energy (h = 1e-5 A),
+    on the reference's ZnO dimer (tests/test_calculator.py:109-118)."""
+    calc = _calc("ffield.reax.ZnOH.2010")
+    s, x = cases.ZNO["zno"]
+    x = np.asarray(x, dtype=np.float64)
+    full = calc.evaluate(s, x, full_derivative=True)
+    fixed = calc.evaluate(s, x)
+    assert fixed.energy == full.energy
+    np.testing.assert_array_equal(fixed.charges, full.charges)
+    assert fixed.force_convention == "fixed_charge" and full.force_convention == "charge_response"
+    assert np.max(np.abs(full.forces - fixed.forces)) > 0.03
+    h = 1e-5
+    fd = np.zeros_like(x)
+    for a in range(len(s)):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[a, k] += h
+            xm[a, k] -= h
+            fd[a, k] = -(calc.evaluate(s, xp).energy - calc.evaluate(s, xm).energy) / (2 * h)
+    np.testing.assert_allclose(full.forces, fd, rtol=0, atol=2e-5)

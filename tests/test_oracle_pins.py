@@ -352,3 +352,69 @@ def test_independent_qeq(water_ff, name):
     expected = _independent_charges(water_ff, s, np.asarray(x), lattice, flags, q)
     np.testing.assert_allclose(r["charges"], expected, atol=1e-10)
     assert abs(r["charges"].sum() - q) < 1e-12
+
+
+# ------------------------------------------------------------ integrators
+def test_md_unit_constant_against_si():
+    """A/fs^2 per kcal/mol/A/amu, checked through the 300 K velocity scale of oxygen in SI units."""
+    from oracle import dynamics as od
+
+    kb_si, amu, kb_kcal = 1.380649e-23, 1.66053906660e-27, 0.0019872043
+    sigma_si = np.sqrt(kb_si * 300.0 / (15.999 * amu)) * 1e10 / 1e15  # A/fs
+    sigma = np.sqrt(kb_kcal * 300.0 * od.ACCEL / 15.999)
+    assert sigma == pytest.approx(sigma_si, rel=1e-6)
+
+
+def test_velocity_verlet_harmonic_oscillator():
+    from oracle import dynamics as od
+
+    k, m, x0 = 2.0, 4.0, 0.3
+    omega = np.sqrt(k * od.ACCEL / m)  # rad/fs
+
+    def harmonic(x):
+        return 0.5 * k * np.sum(x * x), -k * x
+
+    x = np.array([[x0, 0.0, 0.0]])
+    dt, steps = 0.05, 2000
+    xt, vt, pot, kin = od.velocity_verlet(harmonic, x, np.zeros_like(x), [m], dt, steps)
+    t = dt * steps
+    # velocity Verlet: phase error O(dt^2 omega^3 t), energy bounded, no drift
+    assert xt[0, 0] == pytest.approx(x0 * np.cos(omega * t), abs=1e-4)
+    assert vt[0, 0] == pytest.approx(-x0 * omega * np.sin(omega * t), abs=1e-6)
+    total = pot + kin
+    assert np.ptp(total) < 1e-4 * total[0]
+
+
+def test_fire_first_iterations_closed_form():
+    """Hand-derived first FIRE moves on E = k|x|^2/2 (calculator.py:225-245)."""
+    from oracle import dynamics as od
+
+    x0 = np.array([[1.0, 0.0, 0.0]])
+    # k = 1: it0 v = 0.02 F0, x1 = x0 + 0.0004 F0; it1 power > 0 and v is parallel to F,
+    # so the mixing keeps v, then v += 0.02 F1 and x2 = x1 + 0.0004 (F0 + F1).
+    _, _, _, traj = od.fire(lambda x: -x, x0, 1e-12, 2)
+    assert traj[1][0, 0] == pytest.approx(0.9996, abs=1e-15)
+    assert traj[2][0, 0] == pytest.approx(0.9996 - 0.0004 * 1.9996, abs=1e-15)
+    # k = 1000: the first step 0.4 A is capped to 0.1 A
+    _, _, _, traj = od.fire(lambda x: -1000.0 * x, x0, 1e-12, 1)
+    assert traj[1][0, 0] == pytest.approx(0.9, abs=1e-15)
+    # convergence is tested before moving
+    x, converged, it, traj = od.fire(lambda x: -x * 1e-9, x0, 1e-6, 5)
+    assert converged and it == 0 and len(traj) == 1
+
+
+def test_oracle_full_derivative_is_the_energy_gradient(zno_ff):
+    """Charge-response forces of the oracle against central differences of the equilibrated energy on the
+    reference's ZnO dimer (tests/test_calculator.py:109-118: gap to fixed-charge forces > 0.03)."""
+    s, x = cases.ZNO["zno"]
+    x = np.asarray(x, dtype=np.float64)
+    full = orc.evaluate(zno_ff, s, x, full_derivative=True)
+    fixed = orc.evaluate(zno_ff, s, x)
+    assert np.max(np.abs(full["forces"] - fixed["forces"])) > 0.03
+    h = 1e-5
+    for a, k in ((0, 0), (1, 0), (1, 1)):
+        xp, xm = x.copy(), x.copy()
+        xp[a, k] += h
+        xm[a, k] -= h
+        fd = -(orc.evaluate(zno_ff, s, xp)["energy"] - orc.evaluate(zno_ff, s, xm)["energy"]) / (2 * h)
+        assert full["forces"][a, k] == pytest.approx(fd, abs=1e-6)

@@ -78,6 +78,8 @@ class Options(C.Structure):
         ("lammps_angles", C.c_int32),
         ("precision", C.c_int32),
         ("warm_start", C.c_int32),
+        ("full_derivative", C.c_int32),
+        ("fixed_charges", C.c_int32),
     ]
 
 

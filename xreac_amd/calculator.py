@@ -84,13 +84,16 @@ class Calculator:
 
     def evaluate(self, symbols, positions, *, total_charge=0, cell=None, pbc=None, full_derivative=False,
                  neighbors=None):
-        """Energy, QEq charges and fixed-charge forces on the GPU (calculator.py:76-165)."""
+        """Energy, QEq charges and forces on the GPU (calculator.py:76-165).
+
+        Fixed-charge forces with the LAMMPS angle regularisation by default;
+        full_derivative=True gives charge-response forces (one adjoint QEq
+        solve) with the plain angle derivative, as the reference does.
+        """
         if not isinstance(full_derivative, bool):
             raise ValueError("full_derivative must be a boolean")
         symbols, x = validate_input(symbols, positions, total_charge, cell, pbc)
         self.force_field.validate_model(symbols)
-        if full_derivative:
-            raise NotImplementedError("charge-response forces (full_derivative=True) are not on the GPU path yet")
         if neighbors is not None:
             raise NotImplementedError("caller-supplied neighbor lists are not on the GPU path yet")
         boundary = Boundary(cell, pbc)
@@ -106,7 +109,8 @@ class Calculator:
             pbc=boundary.pbc,
             total_charge=total_charge,
             tol=self.qeq_tolerance,
-            lammps_angles=True,
+            lammps_angles=not full_derivative,
+            full_derivative=full_derivative,
         )
         comps = out["components"]
         forces = out["forces"].cpu().numpy()

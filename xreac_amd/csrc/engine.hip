@@ -300,7 +300,7 @@ size_t carve_at(size_t& off, size_t count) {
 // Control block: small per-call state, mirrored in pinned host memory so
 // one copy brings every per-system result back.
 struct CtlLayout {
-  size_t flags, serr, comps, bounds, mu, cg, sys, total;
+  size_t flags, serr, comps, bounds, mu, cg, cg2, sys, total;
 };
 
 CtlLayout ctl_layout(int ns) {
@@ -312,6 +312,7 @@ CtlLayout ctl_layout(int ns) {
   c.bounds = carve_at<double>(off, (size_t)ns * 6);
   c.mu = carve_at<double>(off, ns);
   c.cg = carve_at<char>(off, xr_cg_state_bytes(ns));
+  c.cg2 = carve_at<char>(off, xr_cg_state_bytes(ns));  // charge-response adjoint solve
   c.sys = carve_at<SysDev>(off, ns);
   c.total = (off + 255) & ~size_t(255);
   return c;
@@ -378,6 +379,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   const double bond_cut = std::min(5.0, rc);
   const double rcb = rc * (1 + 1e-8) + 1e-12;
   const bool mixed = opt->precision == XREAC_PRECISION_MIXED;
+  const bool given_q = opt->fixed_charges != 0;
+  const bool full = opt->full_derivative != 0 && !given_q;
+  const int lammps = full ? 0 : opt->lammps_angles;
 
   // ------------------------------------------------------- control block
   const CtlLayout L = ctl_layout(ns);
@@ -450,6 +454,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<int>(off, n);               // bond counts (>0.3)
   carve_size<double>(off, 6 * (size_t)n);   // F_nb, F_hb
   carve_size<double>(off, (size_t)NCH * n);  // per-atom energy channels
+  carve_size<double2>(off, full ? n : 0);    // charge-response right-hand sides (dE/dq, -1)
+  carve_size<double>(off, full ? n : 0);     // charge-response adjoint lambda
   HIPCHK(ctx->atom_buf.ensure(off + 4096), "atom workspace");
   Carver cv{static_cast<char*>(ctx->atom_buf.ptr)};
   int* asys = cv.take<int>(n);
@@ -471,6 +477,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   int* bcount03 = cv.take<int>(n);
   double* fbuf = cv.take<double>(6 * (size_t)n);
   double* energies = cv.take<double>((size_t)NCH * n);
+  double2* dEdq = full ? cv.take<double2>(n) : nullptr;
+  double* lam = full ? cv.take<double>(n) : nullptr;
 
   Systems S{reinterpret_cast<const SysDev*>(dc + L.sys), asys, reinterpret_cast<unsigned*>(dc + L.serr), ns};
 
@@ -561,6 +569,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     add(d_flags, 16 * sizeof(unsigned));
     add(dc + L.serr, sizeof(unsigned) * ns);
     add(dc + L.cg, xr_cg_state_bytes(ns) & ~size_t(3));
+    add(dc + L.cg2, xr_cg_state_bytes(ns) & ~size_t(3));
     add(bin_count, sizeof(int) * (nbins + 1));
     add(bin_fill, sizeof(int) * (nbins + 1));
     add(energies, sizeof(double) * NCH * (size_t)n);
@@ -712,23 +721,43 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.q = out.charges;
   cg.mu = reinterpret_cast<double*>(dc + L.mu);
   cg.abort_flag = d_flags;
-  CgOutcome cgo{};
-  HIPCHK(xr_cg_solve(cg, s, &cgo), "QEq");
-  if (cgo.aborted) {
-    *overflow = true;
-    return XREAC_OK;
+  if (!given_q) {
+    CgOutcome cgo{};
+    HIPCHK(xr_cg_solve(cg, s, &cgo), "QEq");
+    if (cgo.aborted) {
+      *overflow = true;
+      return XREAC_OK;
+    }
   }
 
   // ----------------------------------------------------- energy and forces
-  HIPCHK(xr_nonbonded(at, ff, row_ptr, nbr_j, nbr_s, acc, s), "nonbonded");
+  HIPCHK(xr_nonbonded(at, ff, row_ptr, nbr_j, nbr_s, acc, dEdq, s), "nonbonded");
   {
     hipError_t e = hipSuccess;
     XR_TIMED(XT_BOND_FORWARD, s, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s));
     HIPCHK(e, "bond orders");
-    XR_TIMED(XT_MANYBODY, s, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, opt->lammps_angles, acc, s));
+    XR_TIMED(XT_MANYBODY, s, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, lammps, acc, s));
     HIPCHK(e, "many-body");
     XR_TIMED(XT_BACKWARD, s, e = xr_bond_backward(at, bd, ff, out.forces, s));
     HIPCHK(e, "adjoint");
+  }
+  if (full) {
+    // Charge response: neutral adjoint KKT solve with right-hand sides (dE/dq, -1).
+    CgArgs adj = cg;
+    adj.rhs = dEdq;
+    adj.neutral = true;
+    adj.guess = nullptr;
+    adj.keep = nullptr;
+    adj.q = lam;
+    adj.mu = nullptr;
+    adj.state = dc + L.cg2;
+    CgOutcome o2{};
+    HIPCHK(xr_cg_solve(adj, s, &o2), "QEq adjoint");
+    if (o2.aborted) {
+      *overflow = true;
+      return XREAC_OK;
+    }
+    HIPCHK(xr_charge_response(at, ff, row_ptr, nbr_j, nbr_s, lam, out.forces, s), "charge response");
   }
   HIPCHK(xr_energy_reduce(S, acc, reinterpret_cast<double*>(dc + L.comps), s), "energy");
   if (out.tbo)
@@ -750,16 +779,17 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   const unsigned* h_serr = reinterpret_cast<const unsigned*>(hc + L.serr);
   const double* h_comps = reinterpret_cast<const double*>(hc + L.comps);
   const CgSys* h_cg = xr_cg_systems(hc + L.cg);
+  const CgSys* h_cg2 = xr_cg_systems(hc + L.cg2);
   ctx->sys_messages.assign(ns, std::string());
   int max_iters = 0;
   for (int q = 0; q < ns; ++q) {
-    out.iters[q] = h_cg[q].iters;
+    out.iters[q] = given_q ? 0 : h_cg[q].iters;
     max_iters = std::max(max_iters, h_cg[q].iters);
     std::memcpy(out.comps + (size_t)q * NCOMP, h_comps + (size_t)q * NCOMP, sizeof(double) * NCOMP);
     if (h_serr[q]) {
       out.status[q] = XREAC_INVALID;
       ctx->sys_messages[q] = error_text(h_serr[q]);
-    } else if (h_cg[q].done != CG_DONE) {
+    } else if ((!given_q && h_cg[q].done != CG_DONE) || (full && h_cg2[q].done != CG_DONE)) {
       out.status[q] = XREAC_INVALID;
       ctx->sys_messages[q] = "QEq conjugate gradient did not converge; check the geometry";
     } else {

@@ -15,9 +15,11 @@ namespace {
 constexpr double C_ELE = 332.06371;
 constexpr double SELF_CONVERSION = 23.02;
 
+// When dEdq is given, lane 0 also writes (dE/dq_i, -1): the right-hand sides
+// of the charge-response adjoint solve.
 __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int* __restrict__ row_ptr,
                                                    const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
-                                                   EnergyAcc acc) {
+                                                   EnergyAcc acc, double2* __restrict__ dEdq) {
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i < at.n) {
@@ -29,7 +31,7 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
     const double qi = at.q[i];
     const bool shielded = ff.vdw_type == 1 || ff.vdw_type == 3;
     const bool wall = ff.vdw_type == 2 || ff.vdw_type == 3;
-    double ev_sum = 0.0, ec_sum = 0.0, F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    double ev_sum = 0.0, ec_sum = 0.0, qs_sum = 0.0, F0 = 0.0, F1 = 0.0, F2 = 0.0;
     for (int ed = row_ptr[i] + lane; ed < row_ptr[i + 1]; ed += 64) {
       const int j = nbr_j[ed];
       const int tj = at.type[j];
@@ -61,9 +63,11 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
       }
       double s, ds;
       xr_shield(r, rc, p[PP_GAMMA], s, ds);
-      const double qq = qi * at.q[j];
+      const double qj = at.q[j];
+      const double qq = qi * qj;
       ev_sum += phi;
       ec_sum += qq * s;
+      qs_sum += qj * s;
       const double dtot = (dphi + C_ELE * qq * ds) / r;
       F0 += dtot * v[0];
       F1 += dtot * v[1];
@@ -71,6 +75,7 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
     }
     ev_sum = xr_wave_sum(ev_sum);
     ec_sum = xr_wave_sum(ec_sum);
+    if (dEdq) qs_sum = xr_wave_sum(qs_sum);
     F0 = xr_wave_sum(F0);
     F1 = xr_wave_sum(F1);
     F2 = xr_wave_sum(F2);
@@ -79,10 +84,52 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
       xr_energy(acc, CH_VDW, i, 0.5 * ev_sum);
       xr_energy(acc, CH_COUL, i, 0.5 * C_ELE * ec_sum);
       xr_energy(acc, CH_QEQ, i, SELF_CONVERSION * (a[AP_CHI] * qi + 0.5 * a[AP_ETA] * qi * qi));
+      // dE/dq_i = 332.06371 sum_e q_j s_e + 23.02 (chi_i + eta_i q_i)
+      if (dEdq) dEdq[i] = make_double2(C_ELE * qs_sum + SELF_CONVERSION * (a[AP_CHI] + a[AP_ETA] * qi), -1.0);
       at.F_nb[3 * i] = F0;
       at.F_nb[3 * i + 1] = F1;
       at.F_nb[3 * i + 2] = F2;
     }
+  }
+}
+
+// Charge-response force (full derivative through QEq).  With lambda the
+// neutral adjoint solution of [[H, 1], [1^T, 0]] [lambda; nu] = [dE/dq; 0],
+// dE/dx = dE/dx|_q - lambda^T (dH/dx) q, so every directed edge adds
+// -14.4 ds/dr (lambda_i q_j + lambda_j q_i) v / r to the row atom's force
+// (self-image edges cancel pairwise).  The reference obtains the same term by
+// differentiating through np.linalg.solve (src/xreac/energy.py:97-102).
+__global__ void __launch_bounds__(256) k_charge_response(Atoms at, FFDev ff, const int* __restrict__ row_ptr,
+                                                         const int* __restrict__ nbr_j,
+                                                         const int* __restrict__ nbr_s,
+                                                         const double* __restrict__ lam, double* __restrict__ F) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= at.n) return;
+  const CellDev& cell = at.S.sys[__builtin_amdgcn_readfirstlane(at.S.asys[i])].cell;
+  const double rc = ff.gen[12];
+  const int ti = at.type[i];
+  const double li = lam[i], qi = at.q[i];
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+  for (int ed = row_ptr[i] + lane; ed < row_ptr[i + 1]; ed += 64) {
+    const int j = nbr_j[ed];
+    const int tj = at.type[j];
+    double v[3];
+    const double r = xr_edge(at.x, cell, i, j, nbr_s[ed], v);
+    double s, ds;
+    xr_shield(r, rc, ff.pair[(ti * ff.nt + tj) * PP_COUNT + PP_GAMMA], s, ds);
+    const double c = 14.4 * ds * (li * at.q[j] + lam[j] * qi) / r;
+    F0 += c * v[0];
+    F1 += c * v[1];
+    F2 += c * v[2];
+  }
+  F0 = xr_wave_sum(F0);
+  F1 = xr_wave_sum(F1);
+  F2 = xr_wave_sum(F2);
+  if (lane == 0) {
+    F[3 * i] -= F0;
+    F[3 * i + 1] -= F1;
+    F[3 * i + 2] -= F2;
   }
 }
 
@@ -127,9 +174,17 @@ __global__ void __launch_bounds__(256) k_energy_reduce(Systems S, EnergyAcc acc,
 }  // namespace
 
 hipError_t xr_nonbonded(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
-                        EnergyAcc acc, hipStream_t s) {
+                        EnergyAcc acc, double2* dEdq, hipStream_t s) {
   const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  XR_LAUNCH(XT_NONBONDED, k_nonbonded, grid, dim3(256), 0, s, at, ff, row_ptr, nbr_j, nbr_s, acc);
+  XR_LAUNCH(XT_NONBONDED, k_nonbonded, grid, dim3(256), 0, s, at, ff, row_ptr, nbr_j, nbr_s, acc, dEdq);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_charge_response(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
+                              const double* lam, double* forces, hipStream_t s) {
+  const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
+  hipLaunchKernelGGL(k_charge_response, grid, dim3(256), 0, s, at, ff, row_ptr, nbr_j, nbr_s, lam, forces);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -2,20 +2,22 @@
 #pragma once
 #include "common.h"
 
-enum CgState { CG_RUNNING = 0, CG_DONE = 1, CG_RESTART = 2, CG_FAILED = 3, CG_STALLED = 4 };
+// CG_REFRESH: the system's next update launch replaces the recurrence residual
+// by the FP64 true residual b - A x and restarts the recurrence from it.
+enum CgState { CG_RUNNING = 0, CG_DONE = 1, CG_REFRESH = 2, CG_FAILED = 3, CG_STALLED = 4 };
 
 // Per-system solver state (device).
 struct CgSys {
   int done;       // CgState
   int iters;      // iterations executed by this system
-  int k0;         // global iteration index of the last (re)start
+  int k0;         // global iteration index of the last (re)start (set by the refresh)
   int restarts;
   double rr0[2];  // true squared residuals at the last restart (mixed precision)
 };
 
 struct CgCounters {
-  int settled;  // systems whose state left CG_RUNNING
-  int restart;  // of those, systems waiting in CG_RESTART
+  int settled;  // systems in a final state (done, failed, stalled)
+  int pad;
 };
 
 // The state block holds the counters, then CgSys[ns] at byte offset 64.
@@ -41,6 +43,8 @@ struct CgArgs {
   int max_iter;      // per system
   int first_check;   // iterations before the first read-back
   int check_every;   // iterations between later read-backs
+  const double2* rhs;    // optional right-hand sides per atom (default (-chi, -1))
+  bool neutral;          // normalise the result to zero sum instead of the systems' total charges
   const double2* guess;  // optional starting (s, t) per atom
   double2* keep;         // optional: receives the solution (s, t) per atom
   double* q;             // out: charges

@@ -21,9 +21,12 @@
 //
 // Mixed precision: the A p sweep reads FP32 copies of H and of the gathered
 // (z, p) records while x, r, z, p and every dot product stay FP64.  When the
-// recurrence residual has dropped by the inner factor, the system is
-// restarted from its FP64 true residual r = b - A x (iterative refinement);
-// it is converged only when that true residual meets the tolerance.
+// recurrence residual has dropped by the inner factor, the system's next
+// update launch replaces it by the FP64 true residual r = b - A x and the
+// recurrence restarts from there (iterative refinement); a system is
+// converged only when a true residual meets the tolerance.  A warm start
+// begins with the same refresh.  All of it runs on the device: the host only
+// polls how many systems have settled.
 #include "launch.h"
 #include "qeq.h"
 #include "profile.h"
@@ -168,11 +171,11 @@ __device__ inline void rhs_converged(const double (&cur)[4], const double (&bb)[
 
 __device__ inline void settle(CgSys* c, CgCounters* cnt, int state) {
   c->done = state;
-  atomicAdd(&cnt->settled, 1);
-  if (state == CG_RESTART) atomicAdd(&cnt->restart, 1);
+  if (state != CG_REFRESH) atomicAdd(&cnt->settled, 1);
 }
 
 __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __restrict__ type, FFDev ff,
+                                                       const double2* __restrict__ rhs,
                                                        const double2* __restrict__ guess, double2* __restrict__ X,
                                                        double2* __restrict__ R, double4* __restrict__ ZP,
                                                        float4* __restrict__ ZQ, double* __restrict__ inv_diag,
@@ -189,7 +192,7 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
     const double* a = ff.atom + type[i] * AP_COUNT;
     const double eta = a[AP_ETA];
     const double d = 1.0 / eta;
-    const double2 r = make_double2(-a[AP_CHI], -1.0);
+    const double2 r = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
     X[i] = guess ? guess[i] : make_double2(0.0, 0.0);
     R[i] = r;
     const double2 z = make_double2(r.x * d, r.y * d);
@@ -219,74 +222,10 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
       cg[sys].done = CG_FAILED;
       atomicAdd(&cnt->settled, 1);
     } else if (guess) {
-      // warm start: the first step computes the true residual of the guess
-      cg[sys].done = CG_RESTART;
-      atomicAdd(&cnt->settled, 1);
-      atomicAdd(&cnt->restart, 1);
+      // warm start: update launch 0 computes the true residual of the guess
+      cg[sys].done = CG_REFRESH;
+      cg[sys].k0 = -1;
     }
-  }
-}
-
-// FP64 true residual r = b - A x for every system waiting in CG_RESTART, on
-// the update-block layout (16 waves, 256 rows per block).  Writes (z, p = 0)
-// into the records iteration k reads and the residual partials into the slot
-// iteration k reduces.  The state word itself is flipped by k_cg_resume, so
-// every block of a system sees the same state throughout this kernel.
-__global__ void __launch_bounds__(DIR_BLOCK) k_cg_restart(Systems S, int k, const int* __restrict__ type, FFDev ff,
-                                                          const int* __restrict__ row_ptr,
-                                                          const int* __restrict__ nbr_j,
-                                                          const double* __restrict__ hval,
-                                                          const double* __restrict__ eta,
-                                                          const double* __restrict__ inv_diag,
-                                                          const double2* __restrict__ X, double2* __restrict__ R,
-                                                          double4* __restrict__ ZP, float4* __restrict__ ZQ,
-                                                          double* __restrict__ part_b, int nu, CgSys* __restrict__ cg) {
-  const int sys = block_system(S, blockIdx.x, false);
-  if (cg[sys].done != CG_RESTART) return;
-  const SysDev& sd = S.sys[sys];
-  const int local = blockIdx.x - sd.upd0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int row0 = sd.atom0 + local * UPD_BLOCK, row1 = min(row0 + UPD_BLOCK, sd.atom0 + sd.n);
-  double vals[4] = {0, 0, 0, 0};
-  for (int i = row0 + wave; i < row1; i += DIR_BLOCK / 64) {
-    double as = 0.0, at = 0.0;
-    for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
-      const double2 v = X[nbr_j[e]];
-      as += hval[e] * v.x;
-      at += hval[e] * v.y;
-    }
-    as = xr_wave_sum(as);
-    at = xr_wave_sum(at);
-    if (lane == 0) {
-      const double* a = ff.atom + type[i] * AP_COUNT;
-      const double2 xv = X[i];
-      const double d = eta[i];
-      const double2 r = make_double2(-a[AP_CHI] - (d * xv.x + as), -1.0 - (d * xv.y + at));
-      const double id = inv_diag[i];
-      const double2 z = make_double2(r.x * id, r.y * id);
-      R[i] = r;
-      ZP[i] = make_double4(z.x, z.y, 0.0, 0.0);
-      if (ZQ) ZQ[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
-      vals[0] += r.x * z.x;
-      vals[1] += r.y * z.y;
-      vals[2] += r.x * r.x;
-      vals[3] += r.y * r.y;
-    }
-  }
-  write_partials<4>(vals, part_b + ((k - 1) & 1) * 4 * nu, nu);
-  if (local == 0 && threadIdx.x == 0) {
-    cg[sys].k0 = k;
-    cg[sys].restarts += 1;
-  }
-}
-
-// Systems restarted by k_cg_restart run again; the counters drop them.
-__global__ void k_cg_resume(int ns, CgSys* __restrict__ cg, CgCounters* __restrict__ cnt) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < ns && cg[s].done == CG_RESTART) cg[s].done = CG_RUNNING;
-  if (s == 0) {
-    cnt->settled -= cnt->restart;
-    cnt->restart = 0;
   }
 }
 
@@ -312,7 +251,10 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
                                                             CgCounters* __restrict__ cnt) {
   const int sys = block_system(S, blockIdx.x, true);
   const CgSys c = cg[sys];
-  if (c.done != CG_RUNNING) return;
+  // A system refreshed by the previous update launch (k0 == k) runs again
+  // from j = 0; one still waiting for its refresh sits this launch out.
+  const bool resumed = c.done == CG_REFRESH && c.k0 == k;
+  if (c.done != CG_RUNNING && !resumed) return;
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.dir0;
   const int j = k - c.k0;
@@ -326,14 +268,17 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
       int state;
       if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
       else if (c.iters >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
-      else state = CG_RESTART;
+      else state = CG_REFRESH;
       settle(&cg[sys], cnt, state);
     }
     return;
   }
-  if (MIXED && j == 0 && local == 0 && threadIdx.x == 0) {
-    cg[sys].rr0[0] = cur[2];
-    cg[sys].rr0[1] = cur[3];
+  if (j == 0 && local == 0 && threadIdx.x == 0) {
+    if (MIXED) {
+      cg[sys].rr0[0] = cur[2];
+      cg[sys].rr0[1] = cur[3];
+    }
+    if (resumed) cg[sys].done = CG_RUNNING;
   }
   double beta_s = 0.0, beta_t = 0.0;
   if (j > 0) {
@@ -410,6 +355,45 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
 
 // Iteration k, second half: alpha, x += alpha p, r -= alpha A p, z = r / diag
 // (into the z half of zp), partial r.z and r.r per block.
+// Rows [row0, row1) of a system in CG_REFRESH: FP64 true residual
+// r = b - A x, z = r / diag, p = 0 (written as the records the next
+// iteration reads), and the residual partials of this block.
+template <bool MIXED>
+__device__ void refresh_rows(int row0, int row1, const int* __restrict__ type, const FFDev& ff,
+                             const double2* __restrict__ rhs, const int* __restrict__ row_ptr,
+                             const int* __restrict__ nbr_j, const double* __restrict__ hval,
+                             const double* __restrict__ eta, const double* __restrict__ inv_diag,
+                             const double2* __restrict__ X, double2* __restrict__ R, double4* __restrict__ zp,
+                             float4* __restrict__ zq, double (&vals)[4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int i = row0 + wave; i < row1; i += nw) {
+    double as = 0.0, at = 0.0;
+    for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
+      const double2 v = X[nbr_j[e]];
+      as += hval[e] * v.x;
+      at += hval[e] * v.y;
+    }
+    as = xr_wave_sum(as);
+    at = xr_wave_sum(at);
+    if (lane == 0) {
+      const double* a = ff.atom + type[i] * AP_COUNT;
+      const double2 b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
+      const double2 xv = X[i];
+      const double d = eta[i];
+      const double2 r = make_double2(b.x - (d * xv.x + as), b.y - (d * xv.y + at));
+      const double id = inv_diag[i];
+      const double2 z = make_double2(r.x * id, r.y * id);
+      R[i] = r;
+      zp[i] = make_double4(z.x, z.y, 0.0, 0.0);
+      if (MIXED) zq[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
+      vals[0] += r.x * z.x;
+      vals[1] += r.y * z.y;
+      vals[2] += r.x * r.x;
+      vals[3] += r.y * r.y;
+    }
+  }
+}
+
 template <bool MIXED>
 __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const double* __restrict__ inv_diag,
                                                          double2* __restrict__ X, double2* __restrict__ R,
@@ -418,12 +402,30 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
                                                          const double* __restrict__ part_a, int nd,
                                                          double* __restrict__ part_b,
                                                          const double* __restrict__ part_bb, int nu, double tol2,
-                                                         CgSys* __restrict__ cg) {
+                                                         CgSys* __restrict__ cg, const int* __restrict__ type,
+                                                         FFDev ff, const double2* __restrict__ rhs,
+                                                         const int* __restrict__ row_ptr,
+                                                         const int* __restrict__ nbr_j,
+                                                         const double* __restrict__ hval,
+                                                         const double* __restrict__ eta) {
   const int sys = block_system(S, blockIdx.x, false);
   const CgSys c = cg[sys];
-  if (c.done != CG_RUNNING) return;
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.upd0;
+  if (c.done == CG_REFRESH) {
+    // set by this iteration's direction launch (or by a warm start)
+    double vals[4] = {0, 0, 0, 0};
+    const int row0 = sd.atom0 + local * UPD_BLOCK;
+    const int row1 = min(row0 + UPD_BLOCK, sd.atom0 + sd.n);
+    refresh_rows<MIXED>(row0, row1, type, ff, rhs, row_ptr, nbr_j, hval, eta, inv_diag, X, R, zp, zq, vals);
+    write_partials<4>(vals, part_b + (k & 1) * 4 * nu, nu);
+    if (local == 0 && threadIdx.x == 0) {
+      cg[sys].k0 = k + 1;
+      cg[sys].restarts = c.restarts + 1;
+    }
+    return;
+  }
+  if (c.done != CG_RUNNING) return;
   double cur[4], pap[2], bb[2];
   reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nu, nu, sd.upd0, sd.nupd, cur);
   reduce_partials<2>(part_a, nd, sd.dir0, sd.ndir, pap);
@@ -461,7 +463,7 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
 // is kept in `keep` (when given) as the next call's starting guess.
 __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* __restrict__ X,
                                                       double* __restrict__ q, double* __restrict__ mu_out,
-                                                      double2* __restrict__ keep) {
+                                                      double2* __restrict__ keep, int neutral) {
   const SysDev& sd = S.sys[blockIdx.x];
   __shared__ double red[2][16];
   double ss = 0.0, st = 0.0;
@@ -484,8 +486,8 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
       a += red[0][k];
       b += red[1][k];
     }
-    mu_s = (sd.total_charge - a) / b;
-    mu_out[blockIdx.x] = mu_s;
+    mu_s = ((neutral ? 0.0 : sd.total_charge) - a) / b;
+    if (mu_out) mu_out[blockIdx.x] = mu_s;
   }
   __syncthreads();
   const double mu = mu_s;
@@ -540,27 +542,20 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   float4* zq0 = mixed ? a.ZQ0 : nullptr;
   float4* zq1 = mixed ? a.ZQ1 : nullptr;
   // counters start at zero (cleared with the control block)
-  hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.guess, a.X, a.R, a.ZP0,
+  hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.X, a.R, a.ZP0,
                      zq0, a.inv_diag, a.eta, part_b, part_bb, nu, cg, cnt);
   XR_LAUNCH_CHECK();
   int k = 0;
   // Bound on launches: every system settles within max_iter executed
   // iterations plus one test per restart.
-  const long long k_limit =
-      (long long)a.max_iter + (long long)(MAX_RESTARTS + 2) * (INNER_MAX + a.check_every + 2) + a.first_check;
+  // Every system settles within max_iter executed iterations plus one
+  // refresh launch per restart (and one for a warm start).
+  const long long k_limit = (long long)a.max_iter + 2LL * (MAX_RESTARTS + 3) + a.first_check + a.check_every;
   int host_state[3] = {0, 0, 0};
   bool first = true;
   out->aborted = false;
   out->exhausted = false;
   for (;;) {
-    // pending restarts (and the warm start) are served before iterating
-    if (host_state[1] > 0 || (first && a.guess)) {
-      hipLaunchKernelGGL(k_cg_restart, dim3(nu), dim3(DIR_BLOCK), 0, s, a.S, k, a.type, a.ff, a.row_ptr, a.nbr_j,
-                         a.hval, a.eta, a.inv_diag, a.X, a.R, (k & 1) ? a.ZP1 : a.ZP0, (k & 1) ? zq1 : zq0, part_b,
-                         nu, cg);
-      hipLaunchKernelGGL(k_cg_resume, dim3((ns + 255) / 256), dim3(256), 0, s, ns, cg, cnt);
-      XR_LAUNCH_CHECK();
-    }
     const int chunk = first ? a.first_check : a.check_every;
     first = false;
     for (int c = 0; c < chunk && k < k_limit; ++c) {
@@ -573,13 +568,15 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
                   nd, tol2, a.max_iter, cg, cnt);
         XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
-                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
+                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg, a.type, a.ff, a.rhs, a.row_ptr,
+                  a.nbr_j, a.hval, a.eta);
       } else {
         XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<false>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
                   nd, tol2, a.max_iter, cg, cnt);
         XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
-                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
+                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg, a.type, a.ff, a.rhs, a.row_ptr,
+                  a.nbr_j, a.hval, a.eta);
       }
       ++k;
     }
@@ -596,14 +593,14 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
       out->aborted = true;
       return hipSuccess;
     }
-    if (host_state[0] >= ns && host_state[1] == 0) break;
+    if (host_state[0] >= ns) break;
     if (k >= k_limit) {
       out->exhausted = true;
       break;
     }
   }
   out->sweeps = k;
-  hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.X, a.q, a.mu, a.keep);
+  hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.X, a.q, a.mu, a.keep, a.neutral ? 1 : 0);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -64,6 +64,9 @@ hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* ro
                        const int* nbr_s, int lammps_angles, EnergyAcc acc, hipStream_t s);
 // pairs.hip
 hipError_t xr_nonbonded(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
-                        EnergyAcc acc, hipStream_t s);
+                        EnergyAcc acc, double2* dEdq, hipStream_t s);
+// Adds the charge-response term (full derivative through QEq) to forces.
+hipError_t xr_charge_response(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
+                              const double* lam, double* forces, hipStream_t s);
 // Per-system components [ns][NCOMP] from the per-atom channels.
 hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* comps, hipStream_t s);

@@ -115,7 +115,8 @@ class Engine:
         return out
 
     def evaluate(self, positions, types, cell=None, pbc=(False, False, False), total_charge=0.0, *, tol=1e-10,
-                 max_iter=2000, lammps_angles=True, properties=True, out=None, precision="fp64"):
+                 max_iter=2000, lammps_angles=True, properties=True, out=None, precision="fp64",
+                 full_derivative=False, charges=None):
         """Evaluate one system whose positions (n,3 float64) and types (n int32) are device tensors.
 
         Returns a dict of device tensors (forces, charges and optionally
@@ -129,7 +130,11 @@ class Engine:
         assert types.dtype == torch.int32 and types.is_contiguous() and types.shape == (n,)
         if out is None:
             out = self.outputs(n, properties)
+        given = charges
         forces, charges = out["forces"], out["charges"]
+        if given is not None:
+            # evaluate at these charges instead of solving QEq
+            charges.copy_(self.torch.as_tensor(given, dtype=self.torch.float64, device=self.device).reshape(n))
         tbo, lp, bc = out["total_bond_orders"], out["lone_pairs"], out["bond_counts"]
         assert forces.shape == (n, 3) and charges.shape == (n,)
         cell_arr = None
@@ -143,7 +148,8 @@ class Engine:
             (C.c_int32 * 3)(*[int(bool(v)) for v in pbc]),
             float(total_charge),
         )
-        opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision), 0)
+        opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision), 0,
+                           1 if full_derivative else 0, 1 if given is not None else 0)
         res = _lib.Result()
         res.forces = _ptr(forces)
         res.charges = _ptr(charges)
@@ -175,7 +181,7 @@ class Engine:
 
     def evaluate_batch(self, positions, types, offsets, cells=None, pbc=None, total_charges=None, *, tol=1e-10,
                        max_iter=2000, lammps_angles=True, properties=False, precision="fp64", warm_start=False,
-                       out=None):
+                       full_derivative=False, out=None):
         """Evaluate independent systems packed atom-wise in one pipeline.
 
         positions (N, 3) float64 and types (N,) int32 are device tensors;
@@ -233,7 +239,7 @@ class Engine:
             t = out.get(name)
             setattr(res, name, _ptr(t) if t is not None else None)
         opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision),
-                           1 if warm_start else 0)
+                           1 if warm_start else 0, 1 if full_derivative else 0, 0)
         rc = self.lib.xreac_evaluate_batch(self._ctx, self._ff, C.byref(batch), C.byref(opt), C.byref(res))
         self._check(rc, self._ctx)
         errors = [None if status[k] == _lib.XREAC_OK else self.lib.xreac_batch_error(self._ctx, k).decode()
