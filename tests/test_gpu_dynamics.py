@@ -109,3 +109,18 @@ def test_fire_batch_matches_oracle():
             np.testing.assert_allclose(traj[k][off_h[q]:off_h[q + 1]], ref[k], rtol=0, atol=1e-9,
                                        err_msg=f"system {q} iteration {k}")
     assert (status.cpu().numpy() == 0).all()
+
+
+def test_calculator_relax_native_matches_oracle_fire():
+    """Calculator.relax(backend="native") (calculator.py:167-246) on the reference's ZnO dimer."""
+    from xreac_amd import Calculator
+
+    ff = ForceField.bundled("ffield.reax.ZnOH.2010")
+    calc = Calculator(ff, qeq_tolerance=1e-12)
+    s, x = cases.ZNO["zno"]
+    rel = calc.relax(s, x, backend="native")
+    assert rel.converged and rel.message == "Fixed-charge force tolerance reached"
+    xr, converged, it, _ = od.fire(lambda y: orc.evaluate(ff, s, y)["forces"], x, 1e-4, 500)
+    assert converged and rel.iterations == it
+    np.testing.assert_allclose(rel.positions, xr, rtol=0, atol=1e-7)
+    assert np.max(np.abs(rel.evaluation.forces)) <= 1e-4

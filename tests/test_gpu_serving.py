@@ -10,15 +10,21 @@ meet the FP64 oracle bars (plain FP64 QEq) or the mixed-precision targets
 import numpy as np
 import pytest
 
+from oracle import dynamics as od
 from oracle import reaxff as orc
 from xreac_amd import ForceField, workloads
 
 pytestmark = pytest.mark.gpu
 
 
-def _requests(count, seed, md_steps=4, max_atoms=160):
+def _requests(count, seed, md_steps=4, max_atoms=160, relax_every=0):
     plan = workloads.stream_plan(count, seed=seed, max_atoms=max_atoms, md_fraction=0.3, md_steps=md_steps)
-    return [workloads.build_request(p) for p in plan]
+    reqs = [workloads.build_request(p) for p in plan]
+    if relax_every:
+        for r in reqs[::relax_every]:
+            if r["md_steps"] == 0:
+                r["relax"] = True
+    return reqs
 
 
 def _serve(requests, **kw):
@@ -31,17 +37,25 @@ def _serve(requests, **kw):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_mix_equals_alone(precision):
-    reqs = _requests(24, seed=21)
+    reqs = _requests(24, seed=21, relax_every=5)
+    assert any(r.get("relax") for r in reqs)
     print(f"[serving] {len(reqs)} requests, {precision}", flush=True)
     # a small atom cap forces admission over several steps while others run
-    mixed, srv = _serve(reqs, precision=precision, max_batch_atoms=600)
+    kw = dict(precision=precision, relax_iterations=8)
+    mixed, srv = _serve(reqs, max_batch_atoms=600, **kw)
     assert srv.steps >= 6, srv.steps
     assert len(mixed) == len(reqs)
     for req in reqs:
-        alone, _ = _serve([req], precision=precision)
+        alone, _ = _serve([req], **kw)
         a, m = alone[req["rid"]], mixed[req["rid"]]
         assert m.error is None and a.error is None, (m.error, a.error)
-        assert m.evaluations == a.evaluations == req["md_steps"] + 1
+        assert m.evaluations == a.evaluations
+        if req.get("relax"):
+            assert 1 <= m.evaluations <= 9
+            np.testing.assert_array_equal(m.positions, a.positions)
+            assert m.converged == a.converged
+        else:
+            assert m.evaluations == req["md_steps"] + 1
         np.testing.assert_array_equal(m.forces, a.forces)
         np.testing.assert_array_equal(m.charges, a.charges)
         np.testing.assert_array_equal(m.components, a.components)
@@ -86,3 +100,23 @@ def test_nve_segment_stays_bounded():
         assert r.evaluations == 51
         assert np.all(np.isfinite(total))
         assert np.ptp(total) < 0.02 * r.kinetic.mean() + 0.1
+
+
+def test_served_relaxation_follows_oracle_fire():
+    reqs = [r for r in _requests(20, seed=13, max_atoms=40) if r["md_steps"] == 0][:2]
+    for r in reqs:
+        r["relax"] = True
+    res, _ = _serve(reqs, relax_iterations=10, tol=1e-12)
+    for req in reqs:
+        ff = ForceField.bundled(req["force_field"])
+
+        def forces(x):
+            return orc.evaluate(ff, req["symbols"], x, cell=req["cell"], pbc=req["pbc"],
+                                total_charge=req["total_charge"])["forces"]
+
+        x, converged, it, _ = od.fire(forces, req["positions"], 1e-4, 10)
+        r = res[req["rid"]]
+        print(f"[serving] FIRE rid={req['rid']} n={r.atoms} iterations={r.evaluations - 1} "
+              f"converged={r.converged}", flush=True)
+        assert r.converged == converged and r.evaluations == it + 1
+        np.testing.assert_allclose(r.positions, x, rtol=0, atol=1e-8)

@@ -152,35 +152,36 @@ class Calculator:
             raise ValueError("backend must be 'ase' or 'native'")
         if backend == "ase":
             raise ImportError("the ASE backend needs the ase package, which is not installed; use backend='native'")
-        velocity = np.zeros_like(x)
-        dt, dt_max, alpha = 0.02, 0.2, 0.1
-        positive = 0
-        kw = dict(total_charge=total_charge, cell=cell, pbc=pbc, full_derivative=False)
-        result = self.evaluate(symbols, x, **kw)
+        # FIRE on the device (xreac_fire_step restates calculator.py:225-245): positions, velocities and
+        # the QEq state stay in HBM; only the per-iteration status word comes back.
+        eng = self.engine
+        torch = eng.torch
+        boundary = Boundary(cell, pbc)
+        n = len(x)
+        xd = torch.as_tensor(x, device=eng.device).contiguous()
+        types = torch.as_tensor(eng.type_ids(symbols), device=eng.device)
+        velocity = torch.zeros_like(xd)
+        state = torch.tensor([[0.02, 0.1, 0.0, 0.0]], dtype=torch.float64, device=eng.device)
+        status = torch.zeros(1, dtype=torch.int32, device=eng.device)
+        offsets = np.array([0, n], dtype=np.int32)
+        off_dev = torch.as_tensor(offsets, device=eng.device)
+        periodic = boundary.cell is not None and boundary.periodic
+        cells = (np.asarray(boundary.cell, dtype=np.float64) if periodic else np.eye(3)).reshape(1, 3, 3)
+        pbcs = (np.asarray(boundary.pbc, dtype=np.int32) if periodic else np.zeros(3, dtype=np.int32)).reshape(1, 3)
+        out = eng.batch_outputs(n, qeq_state=True)
+        code = 0
         for it in range(max_iterations + 1):
-            force = result.forces
-            if np.max(np.abs(force)) <= force_tolerance:
-                return Relaxation(x, result, True, it, "Fixed-charge force tolerance reached")
-            if it == max_iterations:
+            res = eng.evaluate_batch(xd, types, offsets, cells, pbcs, [float(total_charge)], tol=self.qeq_tolerance,
+                                     warm_start=it > 0, out=out)
+            if res["errors"][0] is not None:
+                raise ValueError(res["errors"][0])
+            eng.fire_step(off_dev, xd, velocity, out["forces"], state, status, force_tolerance, max_iterations)
+            code = int(status.item())
+            if code:
                 break
-            power = float(np.sum(velocity * force))
-            if power > 0:
-                velocity = (1 - alpha) * velocity + alpha * np.linalg.norm(velocity) / np.linalg.norm(force) * force
-                positive += 1
-                if positive > 5:
-                    dt = min(dt * 1.1, dt_max)
-                    alpha *= 0.99
-            else:
-                velocity.fill(0.0)
-                positive = 0
-                alpha = 0.1
-                if it > 0:
-                    dt *= 0.5
-            velocity += dt * force
-            step = dt * velocity
-            largest = np.max(np.linalg.norm(step, axis=1))
-            if largest > 0.1:
-                step *= 0.1 / largest
-            x = x + step
-            result = self.evaluate(symbols, x, **kw)
-        return Relaxation(x, result, False, max_iterations, "Maximum relaxation iterations reached")
+        x = xd.cpu().numpy()
+        iterations = int(state[0, 3].item())
+        result = self.evaluate(symbols, x, total_charge=total_charge, cell=cell, pbc=pbc)
+        if code == 1:
+            return Relaxation(x, result, True, iterations, "Fixed-charge force tolerance reached")
+        return Relaxation(x, result, False, iterations, "Maximum relaxation iterations reached")

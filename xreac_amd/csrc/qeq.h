@@ -2,8 +2,9 @@
 #pragma once
 #include "common.h"
 
-// CG_REFRESH: the system's next update launch replaces the recurrence residual
-// by the FP64 true residual b - A x and restarts the recurrence from it.
+// CG_REFRESH: the system's next direction launch replaces the recurrence
+// residual by the FP64 true residual b - A x (and z, p = 0), the update
+// launch after it reduces the new residual, and the recurrence restarts.
 enum CgState { CG_RUNNING = 0, CG_DONE = 1, CG_REFRESH = 2, CG_FAILED = 3, CG_STALLED = 4 };
 
 // Per-system solver state (device).
@@ -12,6 +13,7 @@ struct CgSys {
   int iters;      // iterations executed by this system
   int k0;         // global iteration index of the last (re)start (set by the refresh)
   int restarts;
+  int kr;         // CG_REFRESH: launch whose direction kernel computes the true residual
   double rr0[2];  // true squared residuals at the last restart (mixed precision)
 };
 

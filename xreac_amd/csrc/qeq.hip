@@ -22,8 +22,10 @@
 // Mixed precision: the A p sweep reads FP32 copies of H and of the gathered
 // (z, p) records while x, r, z, p and every dot product stay FP64.  When the
 // recurrence residual has dropped by the inner factor, the system's next
-// update launch replaces it by the FP64 true residual r = b - A x and the
-// recurrence restarts from there (iterative refinement); a system is
+// direction launch replaces it by the FP64 true residual r = b - A x (one
+// FP64 sweep over its rows), the update launch after it reduces the new
+// residual, and the recurrence restarts from there (iterative refinement); a
+// system is
 // converged only when a true residual meets the tolerance.  A warm start
 // begins with the same refresh.  All of it runs on the device: the host only
 // polls how many systems have settled.
@@ -169,7 +171,8 @@ __device__ inline void rhs_converged(const double (&cur)[4], const double (&bb)[
   }
 }
 
-__device__ inline void settle(CgSys* c, CgCounters* cnt, int state) {
+__device__ inline void settle(CgSys* c, CgCounters* cnt, int state, int k) {
+  if (state == CG_REFRESH) c->kr = k + 1;  // the next direction launch refreshes
   c->done = state;
   if (state != CG_REFRESH) atomicAdd(&cnt->settled, 1);
 }
@@ -222,8 +225,9 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
       cg[sys].done = CG_FAILED;
       atomicAdd(&cnt->settled, 1);
     } else if (guess) {
-      // warm start: update launch 0 computes the true residual of the guess
+      // warm start: launch 0 computes the true residual of the guess
       cg[sys].done = CG_REFRESH;
+      cg[sys].kr = 0;
       cg[sys].k0 = -1;
     }
   }
@@ -248,15 +252,47 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
                                                             const double* __restrict__ part_bb, int nu,
                                                             double* __restrict__ part_a, int nd, double tol2,
                                                             int max_iter, CgSys* __restrict__ cg,
-                                                            CgCounters* __restrict__ cnt) {
+                                                            CgCounters* __restrict__ cnt, const int* __restrict__ type,
+                                                            FFDev ff, const double2* __restrict__ rhs,
+                                                            const double* __restrict__ inv_diag,
+                                                            const double2* __restrict__ X, double2* __restrict__ R) {
   const int sys = block_system(S, blockIdx.x, true);
   const CgSys c = cg[sys];
-  // A system refreshed by the previous update launch (k0 == k) runs again
-  // from j = 0; one still waiting for its refresh sits this launch out.
-  const bool resumed = c.done == CG_REFRESH && c.k0 == k;
-  if (c.done != CG_RUNNING && !resumed) return;
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.dir0;
+  const int lane = threadIdx.x & 63;
+  const int waves = sd.ndir * (DIR_BLOCK >> 6);
+  const int end = sd.atom0 + sd.n;
+  if (c.done == CG_REFRESH && c.kr == k) {
+    // Refresh: FP64 true residual r = b - A x, z = r / diag, p = 0 into the
+    // records the next launch reads.  The update launch reduces r.z and r.r.
+    for (int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6); i < end; i += waves) {
+      double as = 0.0, at = 0.0;
+      for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
+        const double2 v = X[nbr_j[e]];
+        as += hval[e] * v.x;
+        at += hval[e] * v.y;
+      }
+      as = xr_wave_sum(as);
+      at = xr_wave_sum(at);
+      if (lane == 0) {
+        const double* a = ff.atom + type[i] * AP_COUNT;
+        const double2 b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
+        const double2 xv = X[i];
+        const double d = eta[i];
+        const double2 r = make_double2(b.x - (d * xv.x + as), b.y - (d * xv.y + at));
+        const double id = inv_diag[i];
+        R[i] = r;
+        zp_out[i] = make_double4(r.x * id, r.y * id, 0.0, 0.0);
+        if (MIXED) zq_out[i] = make_float4((float)(r.x * id), (float)(r.y * id), 0.0f, 0.0f);
+      }
+    }
+    return;
+  }
+  // A system refreshed by the previous launch (k0 == k, set by its update
+  // kernel) runs again from j = 0; one in any other state sits this launch out.
+  const bool resumed = c.done == CG_REFRESH && c.k0 == k;
+  if (c.done != CG_RUNNING && !resumed) return;
   const int j = k - c.k0;
   double cur[4], bb[2];
   reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nu, nu, sd.upd0, sd.nupd, cur);
@@ -269,7 +305,7 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
       if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
       else if (c.iters >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
       else state = CG_REFRESH;
-      settle(&cg[sys], cnt, state);
+      settle(&cg[sys], cnt, state, k);
     }
     return;
   }
@@ -287,9 +323,6 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
     if (!conv[0] && prev[0] != 0.0) beta_s = cur[0] / prev[0];
     if (!conv[1] && prev[1] != 0.0) beta_t = cur[1] / prev[1];
   }
-  const int lane = threadIdx.x & 63;
-  const int waves = sd.ndir * (DIR_BLOCK >> 6);
-  const int end = sd.atom0 + sd.n;
   double pap_s = 0.0, pap_t = 0.0;
   for (int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6); i < end; i += waves) {
     double acc_s = 0.0, acc_t = 0.0;
@@ -355,45 +388,6 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
 
 // Iteration k, second half: alpha, x += alpha p, r -= alpha A p, z = r / diag
 // (into the z half of zp), partial r.z and r.r per block.
-// Rows [row0, row1) of a system in CG_REFRESH: FP64 true residual
-// r = b - A x, z = r / diag, p = 0 (written as the records the next
-// iteration reads), and the residual partials of this block.
-template <bool MIXED>
-__device__ void refresh_rows(int row0, int row1, const int* __restrict__ type, const FFDev& ff,
-                             const double2* __restrict__ rhs, const int* __restrict__ row_ptr,
-                             const int* __restrict__ nbr_j, const double* __restrict__ hval,
-                             const double* __restrict__ eta, const double* __restrict__ inv_diag,
-                             const double2* __restrict__ X, double2* __restrict__ R, double4* __restrict__ zp,
-                             float4* __restrict__ zq, double (&vals)[4]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int i = row0 + wave; i < row1; i += nw) {
-    double as = 0.0, at = 0.0;
-    for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
-      const double2 v = X[nbr_j[e]];
-      as += hval[e] * v.x;
-      at += hval[e] * v.y;
-    }
-    as = xr_wave_sum(as);
-    at = xr_wave_sum(at);
-    if (lane == 0) {
-      const double* a = ff.atom + type[i] * AP_COUNT;
-      const double2 b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
-      const double2 xv = X[i];
-      const double d = eta[i];
-      const double2 r = make_double2(b.x - (d * xv.x + as), b.y - (d * xv.y + at));
-      const double id = inv_diag[i];
-      const double2 z = make_double2(r.x * id, r.y * id);
-      R[i] = r;
-      zp[i] = make_double4(z.x, z.y, 0.0, 0.0);
-      if (MIXED) zq[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
-      vals[0] += r.x * z.x;
-      vals[1] += r.y * z.y;
-      vals[2] += r.x * r.x;
-      vals[3] += r.y * r.y;
-    }
-  }
-}
-
 template <bool MIXED>
 __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const double* __restrict__ inv_diag,
                                                          double2* __restrict__ X, double2* __restrict__ R,
@@ -402,22 +396,26 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
                                                          const double* __restrict__ part_a, int nd,
                                                          double* __restrict__ part_b,
                                                          const double* __restrict__ part_bb, int nu, double tol2,
-                                                         CgSys* __restrict__ cg, const int* __restrict__ type,
-                                                         FFDev ff, const double2* __restrict__ rhs,
-                                                         const int* __restrict__ row_ptr,
-                                                         const int* __restrict__ nbr_j,
-                                                         const double* __restrict__ hval,
-                                                         const double* __restrict__ eta) {
+                                                         CgSys* __restrict__ cg) {
   const int sys = block_system(S, blockIdx.x, false);
   const CgSys c = cg[sys];
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.upd0;
   if (c.done == CG_REFRESH) {
-    // set by this iteration's direction launch (or by a warm start)
+    // A refresh requested by this iteration's direction launch waits for the
+    // next one; a refresh computed by this iteration's direction launch is
+    // reduced here.
+    if (c.kr != k) return;
     double vals[4] = {0, 0, 0, 0};
-    const int row0 = sd.atom0 + local * UPD_BLOCK;
-    const int row1 = min(row0 + UPD_BLOCK, sd.atom0 + sd.n);
-    refresh_rows<MIXED>(row0, row1, type, ff, rhs, row_ptr, nbr_j, hval, eta, inv_diag, X, R, zp, zq, vals);
+    const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
+    if (i < sd.atom0 + sd.n) {
+      const double2 r = R[i];
+      const double2 z = reinterpret_cast<const double2*>(zp + i)[0];
+      vals[0] = r.x * z.x;
+      vals[1] = r.y * z.y;
+      vals[2] = r.x * r.x;
+      vals[3] = r.y * r.y;
+    }
     write_partials<4>(vals, part_b + (k & 1) * 4 * nu, nu);
     if (local == 0 && threadIdx.x == 0) {
       cg[sys].k0 = k + 1;
@@ -548,9 +546,9 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   int k = 0;
   // Bound on launches: every system settles within max_iter executed
   // iterations plus one test per restart.
-  // Every system settles within max_iter executed iterations plus one
-  // refresh launch per restart (and one for a warm start).
-  const long long k_limit = (long long)a.max_iter + 2LL * (MAX_RESTARTS + 3) + a.first_check + a.check_every;
+  // Every system settles within max_iter executed iterations plus two
+  // launches per refresh (the request and the FP64 sweep; one more for a warm start).
+  const long long k_limit = (long long)a.max_iter + 3LL * (MAX_RESTARTS + 3) + a.first_check + a.check_every;
   int host_state[3] = {0, 0, 0};
   bool first = true;
   out->aborted = false;
@@ -566,17 +564,15 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
       if (mixed) {
         XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<true>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
-                  nd, tol2, a.max_iter, cg, cnt);
+                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.inv_diag, a.X, a.R);
         XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
-                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg, a.type, a.ff, a.rhs, a.row_ptr,
-                  a.nbr_j, a.hval, a.eta);
+                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
       } else {
         XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<false>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
-                  nd, tol2, a.max_iter, cg, cnt);
+                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.inv_diag, a.X, a.R);
         XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
-                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg, a.type, a.ff, a.rhs, a.row_ptr,
-                  a.nbr_j, a.hval, a.eta);
+                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
       }
       ++k;
     }

@@ -1,7 +1,10 @@
 """Continuous-batching request server on one GPU.
 
-A request is one energy + forces + QEq evaluation, or an NVE segment of
-``md_steps`` velocity-Verlet steps (``md_steps + 1`` evaluations).  Requests
+A request is one energy + forces + QEq evaluation, an NVE segment of
+``md_steps`` velocity-Verlet steps (``md_steps + 1`` evaluations), or a FIRE
+relaxation (``relax=True``, the reference's native loop,
+src/xreac/calculator.py:213-246) that ends at the server's force tolerance or
+iteration limit.  Requests
 wait in a FIFO queue and are admitted at the start of every step while the
 estimated device footprint of the active set stays within the budget (a
 request larger than the budget is admitted alone).  Each step evaluates every
@@ -55,6 +58,7 @@ class Response:
     velocities: np.ndarray | None = None
     potential: np.ndarray | None = None  # per evaluation
     kinetic: np.ndarray | None = None
+    converged: bool | None = None  # relaxations
 
     @property
     def latency(self):
@@ -73,6 +77,8 @@ class _Job:
     types: object = None
     forces: object = None
     qstate: object = None
+    fire: object = None  # relaxations: FIRE state (1, 4) and status (1,) on the device
+    status: object = None
     evals: int = 0
     potential: list = field(default_factory=list)
     kinetic: list = field(default_factory=list)
@@ -89,7 +95,7 @@ class Server:
 
     def __init__(self, force_fields=("ffield.reax.HO.2015", "ffield.reax.CHO.2008"), device=0, *,
                  precision="fp64", warm_start=True, tol=1e-10, max_systems=None, max_batch_atoms=65536,
-                 budget_bytes=32 << 30, clock=time.perf_counter):
+                 budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, clock=time.perf_counter):
         import torch
 
         self.torch = torch
@@ -104,6 +110,8 @@ class Server:
         self.max_systems = max_systems
         self.max_batch_atoms = max_batch_atoms
         self.budget = budget_bytes
+        self.relax_tolerance = relax_tolerance
+        self.relax_iterations = relax_iterations
         self.clock = clock
         self.queue = deque()
         self.active = {k: [] for k in self.engines}
@@ -143,6 +151,10 @@ class Server:
                     masses = [ff.atoms[s]["mass"] for s in job.req["symbols"]]
                     v0 = maxwell_boltzmann(masses, job.req.get("temperature", 300.0), job.req.get("velocity_seed", 0))
                 job.v = torch.as_tensor(np.asarray(v0, dtype=np.float64), device=dev).contiguous()
+            elif job.req.get("relax"):
+                job.v = torch.zeros((job.n, 3), dtype=torch.float64, device=dev)
+                job.fire = torch.tensor([[0.02, 0.1, 0.0, 0.0]], dtype=torch.float64, device=dev)
+                job.status = torch.zeros(1, dtype=torch.int32, device=dev)
             self.active[job.req["force_field"]].append(job)
             self.used_bytes += job.bytes
             active += 1
@@ -169,8 +181,10 @@ class Server:
         T = torch.cat([j.types for j in jobs]).contiguous()
         zeros3 = lambda m: torch.zeros((m, 3), dtype=torch.float64, device=dev)  # noqa: E731
         md = [j.req.get("md_steps", 0) > 0 for j in jobs]
+        relax = [bool(j.req.get("relax")) for j in jobs]
         moving = [m and j.evals > 0 for m, j in zip(md, jobs)]
-        V = torch.cat([j.v if j.v is not None else zeros3(j.n) for j in jobs]).contiguous() if any(md) else None
+        V = (torch.cat([j.v if j.v is not None else zeros3(j.n) for j in jobs]).contiguous()
+             if any(md) or any(relax) else None)
         QS = torch.cat([j.qstate if j.qstate is not None else torch.zeros((j.n, 2), dtype=torch.float64, device=dev)
                         for j in jobs]).contiguous()
         cells = np.array([np.eye(3) if j.req.get("cell") is None else np.asarray(j.req["cell"], dtype=np.float64)
@@ -197,27 +211,44 @@ class Server:
             kin = torch.zeros(ns, dtype=torch.float64, device=dev)
             eng.verlet(off_dev, T, dt_dev, X, V, out["forces"], 1, kin)
             kin = kin.cpu().numpy()
+        fire_status = None
+        if any(relax):
+            if off_dev is None:
+                off_dev = torch.as_tensor(offsets, device=dev)
+            skip = torch.ones(1, dtype=torch.int32, device=dev)
+            dummy = torch.zeros((1, 4), dtype=torch.float64, device=dev)
+            FS = torch.cat([j.fire if r else dummy for j, r in zip(jobs, relax)]).contiguous()
+            ST = torch.cat([j.status if r else skip for j, r in zip(jobs, relax)]).contiguous()
+            eng.fire_step(off_dev, X, V, out["forces"], FS, ST, self.relax_tolerance, self.relax_iterations)
+            fire_status = ST.cpu().numpy()
         self.atom_evals += n
         comps = res["components"]
         done = []
         for k, j in enumerate(jobs):
             lo, hi = offsets[k], offsets[k + 1]
             j.x = X[lo:hi]
-            if V is not None and md[k]:
+            if V is not None and (md[k] or relax[k]):
                 j.v = V[lo:hi]
+            if relax[k]:
+                j.fire = FS[k:k + 1]
+                j.status = ST[k:k + 1]
             j.forces = out["forces"][lo:hi]
             j.qstate = QS[lo:hi]
             j.evals += 1
             j.potential.append(float(comps[k].sum()))
             if md[k]:
                 j.kinetic.append(float(kin[k]))
-            if res["errors"][k] is not None or j.evals > j.req.get("md_steps", 0):
+            if relax[k]:
+                finished_now = fire_status[k] != 0
+            else:
+                finished_now = j.evals > j.req.get("md_steps", 0)
+            if res["errors"][k] is not None or finished_now:
                 done.append((k, j))
         if not done:
             return
         F = out["forces"].cpu().numpy()
         Q = out["charges"].cpu().numpy()
-        Xh = X.cpu().numpy() if any(md[k] for k, _ in done) else None
+        Xh = X.cpu().numpy() if any(md[k] or relax[k] for k, _ in done) else None
         Vh = V.cpu().numpy() if V is not None and any(md[k] for k, _ in done) else None
         now = self.clock()
         finished = set()
@@ -230,6 +261,11 @@ class Server:
                 r.velocities = Vh[lo:hi].copy()
                 r.potential = np.array(j.potential)
                 r.kinetic = np.array(j.kinetic)
+            if relax[k]:
+                # FIRE leaves a converged (or exhausted) system where it was evaluated
+                r.positions = Xh[lo:hi].copy()
+                r.potential = np.array(j.potential)
+                r.converged = bool(fire_status[k] == 1)
             self.responses.append(r)
             self.used_bytes -= j.bytes
             finished.add(id(j))

@@ -38,12 +38,16 @@ def random_rotation(rng):
     )
 
 
-def water_cluster(molecules=64, radius=8.0, min_oo=2.7, jitter=0.02, seed=0):
+def water_cluster(molecules=64, radius=8.0, min_oo=2.7, jitter=0.02, seed=0, max_tries=10**7):
     """O sites rejection-sampled uniformly in a sphere with a minimum O-O distance."""
     rng = np.random.default_rng(seed)
     sites = np.empty((molecules, 3))
     count = 0
+    tries = 0
     while count < molecules:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError("could not place all water molecules; enlarge the radius")
         p = rng.uniform(-radius, radius, 3)
         if p @ p > radius * radius:
             continue
@@ -186,9 +190,11 @@ CHO_RATIO = (("CH4", 300), ("O2", 300), ("H2O", 150), ("CH3OH", 100))
 CHO_SIZES = {"CH4": 5, "O2": 2, "H2O": 3, "CH3OH": 6}
 
 
-def water_cluster_of(molecules, seed):
-    """Cluster at the density of the 64-molecule, 8 A cluster."""
-    return water_cluster(molecules=molecules, radius=8.0 * (molecules / 64.0) ** (1.0 / 3.0), seed=seed)
+def water_cluster_of(molecules, seed, min_oo=2.7):
+    """Cluster at the density of the 64-molecule, 8 A cluster (radius at least 1.5 min O-O, so that a
+    few-molecule cluster always has room for its sites)."""
+    radius = max(8.0 * (molecules / 64.0) ** (1.0 / 3.0), 1.5 * min_oo)
+    return water_cluster(molecules=molecules, radius=radius, min_oo=min_oo, seed=seed)
 
 
 def water_cell(molecules, density=0.997, jitter=0.1, seed=0):
