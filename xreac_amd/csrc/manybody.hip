@@ -220,89 +220,131 @@ __device__ inline bool torsion_eval(const Bonds& bd, const FFDev& ff, const doub
   return true;
 }
 
-// Walk the torsions around middle bond m = (j -> k) (half, BO > 0.001).
-// role 0: called by j's thread; role 1: called by k's thread with mk = rev(m).
-template <class Fn>
-__device__ inline void for_each_torsion(const Atoms& at, const Bonds& bd, const FFDev& ff, int m, unsigned& err,
-                                        Fn fn) {
-  const int j = bd.bi[m], k = bd.bj[m], mr = bd.rev[m];
+// Parameter key of torsion (a, m, c) around the half middle bond m = (j -> k),
+// or -1 when the reference's enumeration (energy.py:323-344) skips it: a is a
+// bond of j other than m, c a bond of k other than rev(m), all three above
+// the 0.001 cutoff with product above it, parameters present, and the 3-ring
+// i == l through the same image excluded.
+__device__ inline int torsion_key(const Atoms& at, const Bonds& bd, const FFDev& ff, int a, int m, int c, int tj,
+                                  int tk, int msx, int msy, int msz) {
+  if (a == m || c == bd.rev[m]) return -1;
+  const double ba = bd.BO[a], bc = bd.BO[c];
+  if (!(ba > THB_CUT) || !(bc > THB_CUT)) return -1;
+  const int i = bd.bj[a], l = bd.bj[c];
   const int nt = ff.nt;
-  const int tj = at.type[j], tk = at.type[k];
-  int msx, msy, msz;
-  xr_unpack_shift(bd.code[m], msx, msy, msz);
-  for (int a = bd.ptr[j]; a < bd.ptr[j + 1]; ++a) {
-    if (a == m || !(bd.BO[a] > THB_CUT)) continue;
-    const int i = bd.bj[a];
-    const int base = ((at.type[i] * nt + tj) * nt + tk) * nt;
-    int asx, asy, asz;
+  const int key = ((at.type[i] * nt + tj) * nt + tk) * nt + at.type[l];
+  if (!ff.tor_on[key]) return -1;
+  if (ba * bd.BO[m] * bc <= THB_CUT) return -1;
+  if (i == l) {
+    int asx, asy, asz, csx, csy, csz;
     xr_unpack_shift(bd.code[a], asx, asy, asz);
-    for (int c = bd.ptr[k]; c < bd.ptr[k + 1]; ++c) {
-      if (!(bd.BO[c] > THB_CUT)) continue;
-      const int l = bd.bj[c];
-      const int key = base + at.type[l];
-      if (!ff.tor_on[key]) continue;
-      if (c == mr || bd.BO[a] * bd.BO[m] * bd.BO[c] <= THB_CUT) continue;
-      if (i == l) {
-        int csx, csy, csz;
-        xr_unpack_shift(bd.code[c], csx, csy, csz);
-        if (asx == msx + csx && asy == msy + csy && asz == msz + csz) continue;
-      }
-      TorsionGrad o;
-      if (!torsion_eval(bd, ff, ff.tor + key * TOR_W, a, m, c, at.db[j], at.db[k], o)) {
-        err |= XR_ERR_COLLINEAR;
-        continue;
-      }
-      fn(a, c, o);
-    }
+    xr_unpack_shift(bd.code[c], csx, csy, csz);
+    if (asx == msx + csx && asy == msy + csy && asz == msz + csz) return -1;
   }
+  return key;
 }
 
+// One wave per atom x.  Every torsion contribution that lands on a bond of
+// x's own row goes to the lane owning that slot: as atom j of a half middle
+// bond m, x owns the left bonds a (lane of a loops over the right bonds c);
+// as atom k, x owns the right bonds c (lane of c loops over the left bonds
+// a).  The middle-bond terms of the j role and the energies are
+// wave-reduced.  Each slot has one writer and the order is fixed, so the
+// result is deterministic; rows longer than 64 bonds are taken in chunks.
 __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  double e[2] = {0.0, 0.0};
+  const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (x >= at.n) return;
+  const int p0 = bd.ptr[x], p1 = bd.ptr[x + 1];
+  double e_tor = 0.0, e_conj = 0.0, a_db = 0.0;
   unsigned err = 0;
-  if (x < at.n) {
-    double A_db = 0.0;
-    for (int mb = bd.ptr[x]; mb < bd.ptr[x + 1]; ++mb) {
+  for (int chunk = p0; chunk < p1; chunk += 64) {
+    const int mine = chunk + lane;
+    const bool owns = mine < p1;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, abo = 0.0;
+    for (int mb = p0; mb < p1; ++mb) {
       if (!(bd.BO[mb] > THB_CUT)) continue;
       const int rv = bd.rev[mb];
       const bool half = bd.edge[mb] < bd.edge[rv];
+      const int m = half ? mb : rv;
+      const int j = bd.bi[m], k = bd.bj[m];
+      const int tj = at.type[j], tk = at.type[k];
+      int msx, msy, msz;
+      xr_unpack_shift(bd.code[m], msx, msy, msz);
+      const double dbj = at.db[j], dbk = at.db[k];
       if (half) {
-        // x is the j end: owns left bonds a, middle m and db_j.
-        double gm[3] = {0, 0, 0}, aBm = 0.0, aPIm = 0.0;
-        for_each_torsion(at, bd, ff, mb, err, [&](int a, int c, const TorsionGrad& o) {
-          e[0] += o.E_tor;
-          e[1] += o.E_conj;
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            bd.G[3 * a + k] += o.Gu[k];
-            gm[k] += o.Gv[k];
+        // x = j: this lane's left bond a = mine, right bonds c of k
+        double gm0 = 0.0, gm1 = 0.0, gm2 = 0.0, abm = 0.0, apim = 0.0;
+        if (owns) {
+          for (int c = bd.ptr[k]; c < bd.ptr[k + 1]; ++c) {
+            const int key = torsion_key(at, bd, ff, mine, m, c, tj, tk, msx, msy, msz);
+            if (key < 0) continue;
+            TorsionGrad o;
+            if (!torsion_eval(bd, ff, ff.tor + key * TOR_W, mine, m, c, dbj, dbk, o)) {
+              err |= XR_ERR_COLLINEAR;
+              continue;
+            }
+            e_tor += o.E_tor;
+            e_conj += o.E_conj;
+            g0 += o.Gu[0];
+            g1 += o.Gu[1];
+            g2 += o.Gu[2];
+            abo += o.dB1;
+            gm0 += o.Gv[0];
+            gm1 += o.Gv[1];
+            gm2 += o.Gv[2];
+            abm += o.dB2;
+            apim += o.dPI;
+            a_db += o.dDS;
           }
-          bd.A_BO[a] += o.dB1;
-          aBm += o.dB2;
-          aPIm += o.dPI;
-          A_db += o.dDS;
-        });
-#pragma unroll
-        for (int k = 0; k < 3; ++k) bd.G[3 * mb + k] += gm[k];
-        bd.A_BO[mb] += aBm;
-        bd.A_PI[mb] += aPIm;
-      } else {
-        // x is the k end of the half bond rv: owns right bonds c and db_k.
-        for_each_torsion(at, bd, ff, rv, err, [&](int a, int c, const TorsionGrad& o) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) bd.G[3 * c + k] += o.Gw[k];
-          bd.A_BO[c] += o.dB3;
-          A_db += o.dDS;
-        });
+        }
+        gm0 = xr_wave_sum(gm0);
+        gm1 = xr_wave_sum(gm1);
+        gm2 = xr_wave_sum(gm2);
+        abm = xr_wave_sum(abm);
+        apim = xr_wave_sum(apim);
+        if (lane == 0) {
+          bd.G[3 * m] += gm0;
+          bd.G[3 * m + 1] += gm1;
+          bd.G[3 * m + 2] += gm2;
+          bd.A_BO[m] += abm;
+          bd.A_PI[m] += apim;
+        }
+      } else if (owns) {
+        // x = k: this lane's right bond c = mine, left bonds a of j
+        for (int a = bd.ptr[j]; a < bd.ptr[j + 1]; ++a) {
+          const int key = torsion_key(at, bd, ff, a, m, mine, tj, tk, msx, msy, msz);
+          if (key < 0) continue;
+          TorsionGrad o;
+          if (!torsion_eval(bd, ff, ff.tor + key * TOR_W, a, m, mine, dbj, dbk, o)) {
+            err |= XR_ERR_COLLINEAR;
+            continue;
+          }
+          g0 += o.Gw[0];
+          g1 += o.Gw[1];
+          g2 += o.Gw[2];
+          abo += o.dB3;
+          a_db += o.dDS;
+        }
       }
     }
-    at.A_db[x] += A_db;
+    if (owns) {
+      bd.G[3 * mine] += g0;
+      bd.G[3 * mine + 1] += g1;
+      bd.G[3 * mine + 2] += g2;
+      bd.A_BO[mine] += abo;
+    }
   }
-  if (x < at.n) {
+  e_tor = xr_wave_sum(e_tor);
+  e_conj = xr_wave_sum(e_conj);
+  a_db = xr_wave_sum(a_db);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) err |= __shfl_down(err, off, 64);
+  if (lane == 0) {
+    at.A_db[x] += a_db;
     if (err) xr_flag(at.S, x, err);
-    xr_energy(acc, CH_TOR, x, e[0]);
-    xr_energy(acc, CH_CONJ, x, e[1]);
+    xr_energy(acc, CH_TOR, x, e_tor);
+    xr_energy(acc, CH_CONJ, x, e_conj);
   }
 }
 
@@ -443,8 +485,8 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
                        const int* nbr_s, int lammps_angles, EnergyAcc acc, hipStream_t s) {
   hipLaunchKernelGGL(k_angles, grid(at.n), dim3(BLK), 0, s, at, bd, ff, lammps_angles, acc);
-  hipLaunchKernelGGL(k_torsions, grid(at.n), dim3(BLK), 0, s, at, bd, ff, acc);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
+  hipLaunchKernelGGL(k_torsions, wgrid, dim3(256), 0, s, at, bd, ff, acc);
   hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, acc);
   hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s);
   XR_LAUNCH_CHECK();
