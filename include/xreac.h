@@ -87,6 +87,15 @@ typedef struct {
   const double* cell;        /* host [9], lattice vectors as rows; NULL if nonperiodic */
   int32_t pbc[3];
   double total_charge;       /* e */
+  /* Optional caller-supplied full directed neighbor list (calculator.py:113-115,
+   * neighbors.py:101-131), device memory in canonical (i, j, Sx, Sy, Sz) order as
+   * CSR: neighbor_row_ptr [n+1], neighbor_j [M], neighbor_shifts [M*3]; NULL row
+   * pointer: the list is built on the device.  Edges beyond the cutoff (a skin)
+   * are allowed and contribute nothing. */
+  const int32_t* neighbor_row_ptr;
+  const int32_t* neighbor_j;
+  const int32_t* neighbor_shifts;
+  int64_t neighbor_count;
 } xreac_system;
 
 enum {
@@ -130,6 +139,12 @@ typedef struct {
   const double* cells;          /* host [num_systems*9] (rows = lattice vectors), or NULL: all nonperiodic */
   const int32_t* pbc;           /* host [num_systems*3], or NULL: periodic along all three cell vectors */
   const double* total_charges;  /* host [num_systems], or NULL: all neutral */
+  /* Optional supplied neighbor lists of all systems as one CSR over the packed
+   * atoms (global indices), as in xreac_system; NULL: built on the device. */
+  const int32_t* neighbor_row_ptr;
+  const int32_t* neighbor_j;
+  const int32_t* neighbor_shifts;
+  int64_t neighbor_count;
 } xreac_batch;
 
 typedef struct {

@@ -68,6 +68,10 @@ class System(C.Structure):
         ("cell", C.c_void_p),
         ("pbc", C.c_int32 * 3),
         ("total_charge", C.c_double),
+        ("neighbor_row_ptr", C.c_void_p),
+        ("neighbor_j", C.c_void_p),
+        ("neighbor_shifts", C.c_void_p),
+        ("neighbor_count", C.c_int64),
     ]
 
 
@@ -92,6 +96,10 @@ class Batch(C.Structure):
         ("cells", C.c_void_p),
         ("pbc", C.c_void_p),
         ("total_charges", C.c_void_p),
+        ("neighbor_row_ptr", C.c_void_p),
+        ("neighbor_j", C.c_void_p),
+        ("neighbor_shifts", C.c_void_p),
+        ("neighbor_count", C.c_int64),
     ]
 
 

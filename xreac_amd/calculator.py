@@ -14,7 +14,7 @@ from numbers import Real
 import numpy as np
 
 from .engine import COMPONENTS, Engine
-from .geometry import Boundary, validate_expansion_limit
+from .geometry import Boundary, canonical_neighbors, validate_expansion_limit
 
 
 @dataclass(frozen=True)
@@ -94,10 +94,15 @@ class Calculator:
             raise ValueError("full_derivative must be a boolean")
         symbols, x = validate_input(symbols, positions, total_charge, cell, pbc)
         self.force_field.validate_model(symbols)
-        if neighbors is not None:
-            raise NotImplementedError("caller-supplied neighbor lists are not on the GPU path yet")
         boundary = Boundary(cell, pbc)
-        repetitions = boundary.repetitions(self.force_field.general[12], len(x), self.max_expanded_atoms)
+        csr = None
+        if neighbors is not None:
+            # the caller's full directed list (calculator.py:113-115): validated and put in canonical order
+            csr = canonical_neighbors(neighbors, len(x), cell, pbc)
+            repetitions, backend = (1, 1, 1), "provided"
+        else:
+            repetitions = boundary.repetitions(self.force_field.general[12], len(x), self.max_expanded_atoms)
+            backend = "replicated"
         eng = self.engine
         torch = eng.torch
         types = torch.as_tensor(eng.type_ids(symbols), device=eng.device)
@@ -111,6 +116,7 @@ class Calculator:
             tol=self.qeq_tolerance,
             lammps_angles=not full_derivative,
             full_derivative=full_derivative,
+            neighbors=csr,
         )
         comps = out["components"]
         forces = out["forces"].cpu().numpy()
@@ -137,7 +143,7 @@ class Calculator:
             bond_counts=out["bond_counts"].cpu().numpy().astype(int),
             dipole=np.sum((x - center) * charges[:, None], axis=0),
             cell_repetitions=repetitions,
-            neighbor_backend="replicated",
+            neighbor_backend=backend,
         )
 
     def relax(self, symbols, positions, *, force_tolerance=1e-4, max_iterations=500, total_charge=0, cell=None,

@@ -27,6 +27,7 @@ enum XrErr : unsigned {
   XR_ERR_REVERSE = 1u << 7,
   XR_ERR_NONFINITE = 1u << 8,
   XR_ERR_CAPACITY = 1u << 9,  // a buffer sized from the previous call was too small
+  XR_ERR_NEIGHBORS = 1u << 10,  // a supplied neighbor entry points outside its system or its shift is too large
 };
 
 // ------------------------------------------------------- parameter layouts

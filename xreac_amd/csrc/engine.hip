@@ -140,6 +140,8 @@ std::string error_text(unsigned bits) {
   if (bits & XR_ERR_COLLINEAR)
     return "Collinear atoms in an active torsion: the dihedral derivative is undefined; perturb the geometry";
   if (bits & XR_ERR_REVERSE) return "Internal error: bond without a reverse edge";
+  if (bits & XR_ERR_NEIGHBORS)
+    return "Neighbor atom indices are out of range, or an image shift exceeds 511 cells";
   return "Non-finite energy, charges, or forces; check the geometry";
 }
 
@@ -274,6 +276,10 @@ struct BatchIn {
   const double* cells;     // host [ns*9] or null
   const int32_t* pbc;      // host [ns*3] or null
   const double* charges;   // host [ns] or null
+  const int32_t* nbr_row_ptr = nullptr;  // optional supplied CSR (device)
+  const int32_t* nbr_j = nullptr;
+  const int32_t* nbr_shifts = nullptr;
+  int64_t nbr_count = 0;
 };
 
 struct BatchOut {
@@ -584,18 +590,30 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   HIPCHK(xr_bin_scatter(n, in.x, S, home, bin, bin_start, bin_fill, binned, s), "bin scatter");
 
   // --------------------------------------------------------- neighbor rows
-  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, 2048, counts, s), "neighbor count");
-  HIPCHK(xr_exclusive_scan(counts, n, row_ptr, scan_tmp, s), "neighbor scan");
-  const double slack = 1.10;
-  int64_t m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
-  exact = exact || m_cap == 0;
-  if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
-  HIPCHK(clamp_ptr(row_ptr, n, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags, s), "clamp");
-  if (exact) {
-    int m_host = 0;
-    HIPCHK(hipMemcpyAsync(&m_host, dev_counts, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
-    HIPCHK(hipStreamSynchronize(s), "neighbor sync");
-    m_cap = m_host;
+  // A supplied list still gets the count pass: it checks every pair for
+  // coincident atoms, as validate_input does whatever list is supplied.
+  const bool supplied = in.nbr_row_ptr != nullptr;
+  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, supplied ? -1 : 2048, counts, s),
+         "neighbor count");
+  int64_t m_cap = 0;
+  if (supplied) {
+    m_cap = in.nbr_count;
+    HIPCHK(hipMemcpyAsync(row_ptr, in.nbr_row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToDevice, s), "copy");
+    // clamped to the supplied length, so a malformed pointer array cannot reach past the list
+    HIPCHK(clamp_ptr(row_ptr, n, (int)m_cap, dev_counts, d_flags, s), "clamp");
+  } else {
+    HIPCHK(xr_exclusive_scan(counts, n, row_ptr, scan_tmp, s), "neighbor scan");
+    const double slack = 1.10;
+    m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
+    exact = exact || m_cap == 0;
+    if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
+    HIPCHK(clamp_ptr(row_ptr, n, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags, s), "clamp");
+    if (exact) {
+      int m_host = 0;
+      HIPCHK(hipMemcpyAsync(&m_host, dev_counts, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+      HIPCHK(hipStreamSynchronize(s), "neighbor sync");
+      m_cap = m_host;
+    }
   }
 
   off = 0;
@@ -611,7 +629,12 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   double* hval = ce.take<double>(m_cap);
   float* hval32 = mixed ? ce.take<float>(m_cap) : nullptr;
   unsigned char* bflag = ce.take<unsigned char>(m_cap);
-  HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s, s), "neighbor fill");
+  if (supplied) {
+    HIPCHK(xr_pack_supplied(n, S, row_ptr, in.nbr_j, in.nbr_shifts, nbr_j, nbr_s, s), "supplied neighbors");
+  } else {
+    HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s, s),
+           "neighbor fill");
+  }
 
   // ------------------------------------------------------- edges and bonds
   HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, hval32, bflag, bond_count,
@@ -772,6 +795,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   const unsigned* h_flags = reinterpret_cast<const unsigned*>(hc + L.flags);
   const int* host_counts = reinterpret_cast<const int*>(h_flags) + 1;
   const int64_t m = host_counts[0], nb_total = host_counts[1];
+  if (supplied && m != in.nbr_count)
+    return fail(ctx, XREAC_INVALID, "neighbor_row_ptr does not match neighbor_count");
   if (m > m_cap || nb_total > nbond || (h_flags[0] & XR_ERR_CAPACITY)) {
     *overflow = true;
     return XREAC_OK;
@@ -836,6 +861,12 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
     return fail(ctx, XREAC_INVALID, "A nonempty system with device positions, types, forces and charges is required");
   const int32_t offsets[2] = {0, sys->n};
   BatchIn in{1, offsets, sys->positions, sys->types, sys->cell, sys->pbc, &sys->total_charge};
+  in.nbr_row_ptr = sys->neighbor_row_ptr;
+  in.nbr_j = sys->neighbor_j;
+  in.nbr_shifts = sys->neighbor_shifts;
+  in.nbr_count = sys->neighbor_count;
+  if (in.nbr_row_ptr && (!in.nbr_j || !in.nbr_shifts || in.nbr_count < 0 || in.nbr_count > INT32_MAX - 1))
+    return fail(ctx, XREAC_INVALID, "a supplied neighbor list needs row pointers, j, shifts and a count");
   int32_t status = XREAC_OK, iters = 0;
   BatchOut bo{};
   bo.comps = out->components;
@@ -872,6 +903,12 @@ int xreac_evaluate_batch(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_batch*
       return fail(ctx, XREAC_INVALID, "every system of a batch needs at least one atom");
   if (opt->warm_start && !out->qeq_state) return fail(ctx, XREAC_INVALID, "warm_start needs result->qeq_state");
   BatchIn in{b->num_systems, b->atom_offsets, b->positions, b->types, b->cells, b->pbc, b->total_charges};
+  in.nbr_row_ptr = b->neighbor_row_ptr;
+  in.nbr_j = b->neighbor_j;
+  in.nbr_shifts = b->neighbor_shifts;
+  in.nbr_count = b->neighbor_count;
+  if (in.nbr_row_ptr && (!in.nbr_j || !in.nbr_shifts || in.nbr_count < 0 || in.nbr_count > INT32_MAX - 1))
+    return fail(ctx, XREAC_INVALID, "a supplied neighbor list needs row pointers, j, shifts and a count");
   BatchOut bo{};
   bo.comps = out->components;
   bo.status = out->status;

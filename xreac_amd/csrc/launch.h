@@ -20,3 +20,5 @@ hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
                             int* nbr_s, hipStream_t s);
+hipError_t xr_pack_supplied(int n, const Systems& S, const int* row_ptr, const int* in_j, const int* in_s,
+                            int* nbr_j, int* nbr_s, hipStream_t s);

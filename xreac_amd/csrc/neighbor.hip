@@ -347,6 +347,31 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
   }
 }
 
+// Supplied list: copy j and pack the shifts into the device codes, one wave
+// per row.  An entry whose atom lies outside the row's system or whose shift
+// does not fit 10 bits is replaced by the row atom with zero shift (so later
+// kernels stay in bounds) and flags the system.
+__global__ void __launch_bounds__(256) k_pack_supplied(int n, Systems S, const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ in_j,
+                                                       const int* __restrict__ in_s, int* __restrict__ nbr_j,
+                                                       int* __restrict__ nbr_s) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const SysDev& sd = S.sys[S.asys[i]];
+  bool bad = false;
+  for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
+    int j = in_j[e];
+    const int sx = in_s[3 * e], sy = in_s[3 * e + 1], sz = in_s[3 * e + 2];
+    const bool ok = j >= sd.atom0 && j < sd.atom0 + sd.n && sx >= -511 && sx <= 511 && sy >= -511 && sy <= 511 &&
+                    sz >= -511 && sz <= 511;
+    nbr_j[e] = ok ? j : i;
+    nbr_s[e] = ok ? xr_pack_shift(sx, sy, sz) : xr_pack_shift(0, 0, 0);
+    bad |= !ok;
+  }
+  if (__ballot(bad) && lane == 0) xr_flag(S, i, XR_ERR_NEIGHBORS);
+}
+
 }  // namespace
 
 hipError_t xr_coord_bounds(const Systems& S, const double* x, double* bounds, hipStream_t s) {
@@ -375,7 +400,8 @@ hipError_t xr_bin_scatter(int n, const double* x, const Systems& S, const int* h
 hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int* home, const int* bin,
                              const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
                              hipStream_t s) {
-  const int cap = row_cap < ROW_CAP ? row_cap : ROW_CAP;
+  // row_cap < 0: count only (coincidence and image checks), no row-length limit
+  const int cap = row_cap < 0 ? 0x7fffffff : (row_cap < ROW_CAP ? row_cap : ROW_CAP);
   XR_LAUNCH(XT_NBR_COUNT, k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, S, home, bin, bin_start,
             static_cast<const BinRec*>(binned), cutoff * cutoff, cap, counts);
   XR_LAUNCH_CHECK();
@@ -387,6 +413,13 @@ hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int*
                             int* nbr_s, hipStream_t s) {
   XR_LAUNCH(XT_NBR_FILL, k_nbr_fill, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
             static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_pack_supplied(int n, const Systems& S, const int* row_ptr, const int* in_j, const int* in_s,
+                            int* nbr_j, int* nbr_s, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_supplied, dim3((n + 3) / 4), dim3(256), 0, s, n, S, row_ptr, in_j, in_s, nbr_j, nbr_s);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

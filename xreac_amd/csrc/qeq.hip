@@ -29,6 +29,8 @@
 // converged only when a true residual meets the tolerance.  A warm start
 // begins with the same refresh.  All of it runs on the device: the host only
 // polls how many systems have settled.
+#include <cstdlib>
+
 #include "launch.h"
 #include "qeq.h"
 #include "profile.h"
@@ -124,31 +126,92 @@ __device__ inline int block_system(const Systems& S, int b, bool dir) {
   return lo;
 }
 
-// Fixed-order reduction of NV partial arrays (part[k * stride + b0 + t],
-// t < cnt) by the whole block.
-template <int NV>
-__device__ inline void reduce_partials(const double* __restrict__ part, int stride, int b0, int cnt,
-                                       double (&out)[NV]) {
-  __shared__ double red[NV][16];
-  __shared__ double res[NV];
+// The direction kernel's prologue in one pass: the current and previous
+// residual dots (4 each, slot arrays) and |b|^2 (2) of one system.  Each value
+// is summed in a fixed order (strided per thread, then the waves in order),
+// with one barrier sequence for all ten.
+__device__ inline void reduce_direction(const double* __restrict__ cur_p, const double* __restrict__ prev_p,
+                                        const double* __restrict__ bb_p, int stride, int b0, int cnt,
+                                        double (&cur)[4], double (&prev)[4], double (&bb)[2]) {
+  __shared__ double red[10][16];
+  __shared__ double res[10];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double v[10];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-#pragma unroll 4
-    for (int b = threadIdx.x; b < cnt; b += blockDim.x) s += part[k * stride + b0 + b];
-    s = xr_wave_sum(s);
-    if (lane == 0) red[k][wave] = s;
+  for (int q = 0; q < 10; ++q) v[q] = 0.0;
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      v[q] += cur_p[q * stride + b0 + t];
+      v[4 + q] += prev_p[q * stride + b0 + t];
+    }
+    v[8] += bb_p[b0 + t];
+    v[9] += bb_p[stride + b0 + t];
+  }
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    const double w = xr_wave_sum(v[q]);
+    if (lane == 0) red[q][wave] = w;
   }
   __syncthreads();
-  if (threadIdx.x < NV) {
-    double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += red[threadIdx.x][w];
-    res[threadIdx.x] = s;
+  if (threadIdx.x < 10) {
+    double x = 0.0;
+    for (int w = 0; w < nw; ++w) x += red[threadIdx.x][w];
+    res[threadIdx.x] = x;
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < NV; ++k) out[k] = res[k];
+  for (int q = 0; q < 4; ++q) {
+    cur[q] = res[q];
+    prev[q] = res[4 + q];
+  }
+  bb[0] = res[8];
+  bb[1] = res[9];
+  __syncthreads();
+}
+
+// The update kernel's prologue in one pass: residual dots (4) and |b|^2 (2)
+// over the system's update blocks, p.Ap (2) over its direction blocks.
+__device__ inline void reduce_update(const double* __restrict__ cur_p, const double* __restrict__ bb_p, int su,
+                                     int u0, int nu_cnt, const double* __restrict__ pap_p, int sd, int d0, int nd_cnt,
+                                     double (&cur)[4], double (&bb)[2], double (&pap)[2]) {
+  __shared__ double red[8][16];
+  __shared__ double res[8];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = 0.0;
+  const int top = nu_cnt > nd_cnt ? nu_cnt : nd_cnt;
+  for (int t = threadIdx.x; t < top; t += blockDim.x) {
+    if (t < nu_cnt) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] += cur_p[q * su + u0 + t];
+      v[4] += bb_p[u0 + t];
+      v[5] += bb_p[su + u0 + t];
+    }
+    if (t < nd_cnt) {
+      v[6] += pap_p[d0 + t];
+      v[7] += pap_p[sd + d0 + t];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double w = xr_wave_sum(v[q]);
+    if (lane == 0) red[q][wave] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double x = 0.0;
+    for (int w = 0; w < nw; ++w) x += red[threadIdx.x][w];
+    res[threadIdx.x] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) cur[q] = res[q];
+  bb[0] = res[4];
+  bb[1] = res[5];
+  pap[0] = res[6];
+  pap[1] = res[7];
   __syncthreads();
 }
 
@@ -294,9 +357,9 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
   const bool resumed = c.done == CG_REFRESH && c.k0 == k;
   if (c.done != CG_RUNNING && !resumed) return;
   const int j = k - c.k0;
-  double cur[4], bb[2];
-  reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nu, nu, sd.upd0, sd.nupd, cur);
-  reduce_partials<2>(part_bb, nu, sd.upd0, sd.nupd, bb);
+  double cur[4], prev[4], bb[2];
+  reduce_direction(part_b + ((k - 1) & 1) * 4 * nu, part_b + (k & 1) * 4 * nu, part_bb, nu, sd.upd0, sd.nupd, cur,
+                   prev, bb);
   bool conv[2];
   rhs_converged(cur, bb, c, j, MIXED, tol2, conv);
   if ((conv[0] && conv[1]) || c.iters >= max_iter || (MIXED && j >= INNER_MAX)) {
@@ -318,8 +381,6 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
   }
   double beta_s = 0.0, beta_t = 0.0;
   if (j > 0) {
-    double prev[4];
-    reduce_partials<4>(part_b + (k & 1) * 4 * nu, nu, sd.upd0, sd.nupd, prev);
     if (!conv[0] && prev[0] != 0.0) beta_s = cur[0] / prev[0];
     if (!conv[1] && prev[1] != 0.0) beta_t = cur[1] / prev[1];
   }
@@ -425,9 +486,8 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
   }
   if (c.done != CG_RUNNING) return;
   double cur[4], pap[2], bb[2];
-  reduce_partials<4>(part_b + ((k - 1) & 1) * 4 * nu, nu, sd.upd0, sd.nupd, cur);
-  reduce_partials<2>(part_a, nd, sd.dir0, sd.ndir, pap);
-  reduce_partials<2>(part_bb, nu, sd.upd0, sd.nupd, bb);
+  reduce_update(part_b + ((k - 1) & 1) * 4 * nu, part_bb, nu, sd.upd0, sd.nupd, part_a, nd, sd.dir0, sd.ndir, cur,
+                bb, pap);
   bool conv[2];
   rhs_converged(cur, bb, c, k - c.k0, MIXED, tol2, conv);
   const double alpha_s = (!conv[0] && pap[0] != 0.0) ? cur[0] / pap[0] : 0.0;
@@ -517,9 +577,17 @@ hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, c
   return hipSuccess;
 }
 
+// Direction blocks per system: one wave per row up to a cap (XREAC_CG_MAX_BLOCKS
+// overrides the cap for tuning; it must be the same for every call that is
+// compared bitwise, since a system's blocks depend only on its size and this cap).
 void xr_cg_blocks(int n, int* ndir, int* nupd) {
+  static const int cap = [] {
+    const char* v = std::getenv("XREAC_CG_MAX_BLOCKS");
+    const int c = v ? std::atoi(v) : DIR_MAX_BLOCKS;
+    return c > 0 ? c : DIR_MAX_BLOCKS;
+  }();
   int d = (n + DIR_BLOCK / 64 - 1) / (DIR_BLOCK / 64);
-  *ndir = d < 1 ? 1 : (d > DIR_MAX_BLOCKS ? DIR_MAX_BLOCKS : d);
+  *ndir = d < 1 ? 1 : (d > cap ? cap : d);
   const int u = (n + UPD_BLOCK - 1) / UPD_BLOCK;
   *nupd = u < 1 ? 1 : u;
 }

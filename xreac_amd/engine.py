@@ -116,13 +116,15 @@ class Engine:
 
     def evaluate(self, positions, types, cell=None, pbc=(False, False, False), total_charge=0.0, *, tol=1e-10,
                  max_iter=2000, lammps_angles=True, properties=True, out=None, precision="fp64",
-                 full_derivative=False, charges=None):
+                 full_derivative=False, charges=None, neighbors=None):
         """Evaluate one system whose positions (n,3 float64) and types (n int32) are device tensors.
 
         Returns a dict of device tensors (forces, charges and optionally
         total_bond_orders, lone_pairs, bond_counts), the 14 components (host
         numpy) and the edge/bond/iteration counts.  ``out`` (from
         ``outputs``) is filled in place instead of allocating new buffers.
+        ``neighbors`` = canonical CSR ``(row_ptr, j, S)`` (host or device
+        int32, see ``geometry.canonical_neighbors``) replaces the device build.
         """
         torch = self.torch
         n = int(positions.shape[0])
@@ -140,6 +142,7 @@ class Engine:
         cell_arr = None
         if cell is not None:
             cell_arr = np.ascontiguousarray(cell, dtype=np.float64).reshape(3, 3)
+        nbr = self._device_csr(neighbors, n) if neighbors is not None else None
         sysd = _lib.System(
             n,
             _ptr(positions),
@@ -147,6 +150,7 @@ class Engine:
             C.c_void_p(cell_arr.ctypes.data) if cell_arr is not None else None,
             (C.c_int32 * 3)(*[int(bool(v)) for v in pbc]),
             float(total_charge),
+            *(self._csr_fields(nbr)),
         )
         opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision), 0,
                            1 if full_derivative else 0, 1 if given is not None else 0)
@@ -171,6 +175,22 @@ class Engine:
             n=n,
         )
 
+    def _device_csr(self, neighbors, n):
+        """Device int32 copies of a canonical CSR (row_ptr [n+1], j [M], S [M, 3])."""
+        torch = self.torch
+        row_ptr, j, S = neighbors
+        t = [torch.as_tensor(np.asarray(a.cpu() if hasattr(a, "cpu") else a, dtype=np.int32), device=self.device)
+             .contiguous() for a in (row_ptr, j, np.reshape(S, (-1, 3)))]
+        if t[0].shape[0] != n + 1 or t[2].shape[0] != t[1].shape[0]:
+            raise ValueError("neighbors must be a CSR (row_ptr [n + 1], j [M], S [M, 3])")
+        return t
+
+    @staticmethod
+    def _csr_fields(nbr):
+        if nbr is None:
+            return None, None, None, 0
+        return _ptr(nbr[0]), _ptr(nbr[1]), _ptr(nbr[2]), int(nbr[1].shape[0])
+
     def batch_outputs(self, n, properties=False, qeq_state=False):
         """Device output buffers for a batch of n atoms in total (reusable via ``out=``)."""
         out = self.outputs(n, properties)
@@ -181,7 +201,7 @@ class Engine:
 
     def evaluate_batch(self, positions, types, offsets, cells=None, pbc=None, total_charges=None, *, tol=1e-10,
                        max_iter=2000, lammps_angles=True, properties=False, precision="fp64", warm_start=False,
-                       full_derivative=False, out=None):
+                       full_derivative=False, out=None, neighbors=None):
         """Evaluate independent systems packed atom-wise in one pipeline.
 
         positions (N, 3) float64 and types (N,) int32 are device tensors;
@@ -217,6 +237,7 @@ class Engine:
             keep.append(pbc_arr)
         if total_charges is not None:
             q_arr = np.ascontiguousarray(total_charges, dtype=np.float64).reshape(ns)
+        nbr = self._device_csr(neighbors, n) if neighbors is not None else None
         batch = _lib.Batch(
             ns,
             _ptr(offsets),
@@ -225,6 +246,7 @@ class Engine:
             _ptr(cells_arr) if cells_arr is not None else None,
             _ptr(pbc_arr) if pbc_arr is not None else None,
             _ptr(q_arr) if q_arr is not None else None,
+            *(self._csr_fields(nbr)),
         )
         comps = np.zeros((ns, _lib.NCOMP), dtype=np.float64)
         status = np.zeros(ns, dtype=np.int32)

@@ -63,3 +63,48 @@ class Boundary:
             )
         reps[self.pbc] = need.astype(int)
         return tuple(int(v) for v in reps)
+
+
+SHIFT_LIMIT = 511  # image shifts are packed as three 10-bit fields on the device
+
+
+def canonical_neighbors(neighbors, atoms, cell=None, pbc=None):
+    """Validate a caller's full directed list ``(i, j, S)`` and return it as canonical CSR.
+
+    Same checks, order and messages as the reference's ``Neighbors``
+    (src/xreac/neighbors.py:101-131, duplicates and both directions from
+    ``_edge_order``, neighbors.py:63-98).  Returns ``row_ptr`` int32 (atoms + 1),
+    ``j`` int32 (M,) and ``S`` int32 (M, 3), rows in (i, j, Sx, Sy, Sz) order.
+    """
+    boundary = Boundary(cell, pbc)
+    if not isinstance(neighbors, (tuple, list)) or len(neighbors) != 3:
+        raise ValueError("neighbors must be a tuple (i, j, S)")
+    i, j, shifts = (np.asarray(v) for v in neighbors)
+    if i.ndim != 1 or j.shape != i.shape or shifts.shape != (len(i), 3):
+        raise ValueError("neighbors require i and j with shape (M,) and S with shape (M, 3)")
+    if any(v.dtype.kind not in "iu" for v in (i, j, shifts)):
+        raise ValueError("Neighbor indices and lattice shifts must be integers")
+    if np.any(i < 0) or np.any(j < 0) or np.any(i >= atoms) or np.any(j >= atoms):
+        raise ValueError("Neighbor atom indices are out of range")
+    if np.any(shifts[:, ~boundary.pbc] != 0):
+        raise ValueError("Neighbor shifts must be zero in nonperiodic directions")
+    if np.any((i == j) & np.all(shifts == 0, axis=1)):
+        raise ValueError("Zero-shift self neighbors are not allowed")
+    i = i.astype(np.int64)
+    j = j.astype(np.int64)
+    s = shifts.astype(np.int64)
+    if len(i) and np.max(np.abs(s)) > SHIFT_LIMIT:
+        raise ValueError(f"Neighbor shifts beyond {SHIFT_LIMIT} cells are not supported on the device")
+    order = np.lexsort((s[:, 2], s[:, 1], s[:, 0], j, i))
+    i, j, s = i[order], j[order], s[order]
+    forward = np.column_stack((i, j, s))
+    if len(i) > 1 and np.any(np.all(forward[1:] == forward[:-1], axis=1)):
+        raise ValueError("Duplicate neighbor edges are not allowed")
+    # every (i, j, S) needs its (j, i, -S): the reversed set, sorted, must be the same list
+    backward = np.column_stack((j, i, -s))
+    back = backward[np.lexsort((backward[:, 4], backward[:, 3], backward[:, 2], backward[:, 1], backward[:, 0]))]
+    if not np.array_equal(forward, back):
+        raise ValueError("neighbors must include both directions: (i, j, S) and (j, i, -S)")
+    row_ptr = np.zeros(atoms + 1, dtype=np.int32)
+    np.cumsum(np.bincount(i, minlength=atoms), out=row_ptr[1:])
+    return row_ptr, j.astype(np.int32), s.astype(np.int32)
