@@ -30,6 +30,7 @@
 // begins with the same refresh.  All of it runs on the device: the host only
 // polls how many systems have settled.
 #include <cstdlib>
+#include <type_traits>
 
 #include "launch.h"
 #include "qeq.h"
@@ -296,11 +297,54 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
   }
 }
 
+// Eight H entries per lane of one row (coalesced over the wave): columns and
+// values, padded with the row atom and zero.
+template <class HV>
+struct RowBatch {
+  int j[8];
+  HV h[8];
+};
+
+template <class HV>
+__device__ inline void load_batch(const int* __restrict__ nbr_j, const HV* __restrict__ hv, int base, int e1, int i,
+                                  int lane, RowBatch<HV>& b) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int e = base + u * 64 + lane;
+    const bool ok = e < e1;
+    b.j[u] = ok ? nbr_j[e] : i;
+    b.h[u] = ok ? hv[e] : HV(0);
+  }
+}
+
+// sum_e h_e p_j for one batch, p_j = z_j + beta p_old_j from the gathered records
+__device__ inline void gather_batch(const RowBatch<double>& b, const double4* __restrict__ zp, double bs, double bt,
+                                    double& as, double& at) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const double4 v = zp[b.j[u]];
+    as += b.h[u] * (v.x + bs * v.z);
+    at += b.h[u] * (v.y + bt * v.w);
+  }
+}
+
+__device__ inline void gather_batch(const RowBatch<float>& b, const float4* __restrict__ zq, float bs, float bt,
+                                    float& as, float& at) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const float4 v = zq[b.j[u]];
+    as += b.h[u] * (v.x + bs * v.z);
+    at += b.h[u] * (v.y + bt * v.w);
+  }
+}
+
 // Iteration k, first half: beta, p = z + beta p_old and A p (rows of the
 // system strided over its blocks, one wave per row), partial p.Ap per block.
 // zp_in holds (z, p_old) of every atom; the new p goes to the p half of
-// zp_out.  Up to 512 edges of a row are loaded before any gather so a row
-// costs one memory latency.
+// zp_out.  Rows are software-pipelined: each wave loads the first 512 H
+// entries of its next row before gathering for the current one, and those of
+// its first row before the prologue reduction, so the H stream overlaps the
+// gathers.
 template <bool MIXED>
 __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, const int* __restrict__ row_ptr,
                                                             const int* __restrict__ nbr_j,
@@ -357,6 +401,18 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
   const bool resumed = c.done == CG_REFRESH && c.k0 == k;
   if (c.done != CG_RUNNING && !resumed) return;
   const int j = k - c.k0;
+  using HV = typename std::conditional<MIXED, float, double>::type;
+  const HV* hv;
+  if constexpr (MIXED) hv = hval32;
+  else hv = hval;
+  int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6);
+  int e0 = 0, e1 = 0;
+  RowBatch<HV> cur_b;
+  if (i < end) {
+    e0 = row_ptr[i];
+    e1 = row_ptr[i + 1];
+    load_batch(nbr_j, hv, e0, e1, i, lane, cur_b);
+  }
   double cur[4], prev[4], bb[2];
   reduce_direction(part_b + ((k - 1) & 1) * 4 * nu, part_b + (k & 1) * 4 * nu, part_bb, nu, sd.upd0, sd.nupd, cur,
                    prev, bb);
@@ -385,48 +441,35 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
     if (!conv[1] && prev[1] != 0.0) beta_t = cur[1] / prev[1];
   }
   double pap_s = 0.0, pap_t = 0.0;
-  for (int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6); i < end; i += waves) {
-    double acc_s = 0.0, acc_t = 0.0;
-    const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
-    if (MIXED) {
+  while (i < end) {
+    const int inext = i + waves;
+    int n0 = 0, n1 = 0;
+    RowBatch<HV> next_b;
+    if (inext < end) {
+      n0 = row_ptr[inext];
+      n1 = row_ptr[inext + 1];
+      load_batch(nbr_j, hv, n0, n1, inext, lane, next_b);
+    }
+    double acc_s, acc_t;
+    if constexpr (MIXED) {
       const float bs = (float)beta_s, bt = (float)beta_t;
       float fs = 0.0f, ft = 0.0f;
-      for (int base = e0; base < e1; base += 512) {
-        int jj[8];
-        float hh[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = base + u * 64 + lane;
-          const bool ok = e < e1;
-          jj[u] = ok ? nbr_j[e] : i;
-          hh[u] = ok ? hval32[e] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const float4 v = zq_in[jj[u]];
-          fs += hh[u] * (v.x + bs * v.z);
-          ft += hh[u] * (v.y + bt * v.w);
-        }
+      gather_batch(cur_b, zq_in, bs, bt, fs, ft);
+      for (int base = e0 + 512; base < e1; base += 512) {
+        RowBatch<HV> more;
+        load_batch(nbr_j, hv, base, e1, i, lane, more);
+        gather_batch(more, zq_in, bs, bt, fs, ft);
       }
       acc_s = (double)fs;
       acc_t = (double)ft;
     } else {
-      for (int base = e0; base < e1; base += 512) {
-        int jj[8];
-        double hh[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int e = base + u * 64 + lane;
-          const bool ok = e < e1;
-          jj[u] = ok ? nbr_j[e] : i;
-          hh[u] = ok ? hval[e] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const double4 v = zp_in[jj[u]];
-          acc_s += hh[u] * (v.x + beta_s * v.z);
-          acc_t += hh[u] * (v.y + beta_t * v.w);
-        }
+      acc_s = 0.0;
+      acc_t = 0.0;
+      gather_batch(cur_b, zp_in, beta_s, beta_t, acc_s, acc_t);
+      for (int base = e0 + 512; base < e1; base += 512) {
+        RowBatch<HV> more;
+        load_batch(nbr_j, hv, base, e1, i, lane, more);
+        gather_batch(more, zp_in, beta_s, beta_t, acc_s, acc_t);
       }
     }
     acc_s = xr_wave_sum(acc_s);
@@ -442,6 +485,10 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
       pap_s += p.x * ap.x;
       pap_t += p.y * ap.y;
     }
+    i = inext;
+    e0 = n0;
+    e1 = n1;
+    cur_b = next_b;
   }
   double vals[2] = {pap_s, pap_t};
   write_partials<2>(vals, part_a, nd);
