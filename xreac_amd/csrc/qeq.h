@@ -37,7 +37,9 @@ struct CgArgs {
   double2 *X, *R, *AP;
   double4 *ZP0, *ZP1;  // (z_s, z_t, p_s, p_t), alternating by iteration
   float4 *ZQ0, *ZQ1;   // FP32 copies gathered by the mixed-precision sweep
-  double* inv_diag;
+  const unsigned char* join;  // per atom: strong bonds to the previous three atoms (edge pass)
+  double4* minv;              // per atom: its row of the block-Jacobi inverse
+  int2* binfo;                // per atom: (local row of its block start, block size)
   double* eta;
   double* partials;  // xr_cg_partials(nd, nu) doubles
   void* state;       // xr_cg_state_bytes(ns) bytes
@@ -62,7 +64,7 @@ struct CgOutcome {
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, hipStream_t s);
+                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s);
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int cap,
                         int* bond_edge, int* bond_i, hipStream_t s);
 void xr_cg_blocks(int n, int* ndir, int* nupd);

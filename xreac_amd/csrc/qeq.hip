@@ -39,6 +39,8 @@
 namespace {
 
 constexpr int ROWS_PER_BLOCK = 4;  // one wave per row, 256 threads
+constexpr double BLOCK_BOND = 0.3;  // raw bond order joining consecutive atoms into one CG block
+constexpr int BLOCK_MAX = 4;        // atoms per CG preconditioner block
 
 // ---------------------------------------------------------------- edge pass
 __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restrict__ x, const int* __restrict__ type,
@@ -46,7 +48,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
                                                    const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
                                                    double bond_cut, double* __restrict__ hval,
                                                    float* __restrict__ hval32, unsigned char* __restrict__ bflag,
-                                                   int* __restrict__ bond_count) {
+                                                   int* __restrict__ bond_count, unsigned char* __restrict__ join) {
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
@@ -55,6 +57,8 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   const double rc = ff.gen[12];
   const double cut = 0.01 * ff.gen[29];
   int cnt = 0;
+  unsigned jm = 0;  // bit d-1: strong same-image bond to atom i - d (d = 1..3), for the CG blocks
+  const int zero = xr_pack_shift(0, 0, 0);
   for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
     const int j = nbr_j[e];
     const int tj = type[j];
@@ -68,13 +72,20 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
     if (r <= bond_cut) {
       BondRaw b;
       xr_bond_raw(ff, ti, tj, r, b);
-      active = ((b.sig + b.pi) + b.pp) >= cut;
+      const double raw = (b.sig + b.pi) + b.pp;
+      active = raw >= cut;
+      if (raw >= BLOCK_BOND && j < i && j >= i - 3 && nbr_s[e] == zero) jm |= 1u << (i - j - 1);
     }
     bflag[e] = active;
     cnt += active;
   }
   cnt = xr_wave_sum_int(cnt);
-  if (lane == 0) bond_count[i] = cnt;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) jm |= __shfl_down(jm, off, 64);
+  if (lane == 0) {
+    bond_count[i] = cnt;
+    join[i] = (unsigned char)jm;
+  }
 }
 
 // Compact active edges per row, preserving canonical order.
@@ -241,36 +252,143 @@ __device__ inline void settle(CgSys* c, CgCounters* cnt, int state, int k) {
   if (state != CG_REFRESH) atomicAdd(&cnt->settled, 1);
 }
 
+// Block-Jacobi preconditioner.  Consecutive atoms joined by a strong
+// same-image bond (raw order >= 0.3 to one of the previous three atoms of the
+// block) form blocks of at most four atoms, e.g. one water molecule; blocks
+// never cross an update block, so they depend only on the system's own atoms.
+// M^-1 is the inverse of the blocks of H on its diagonal (image terms
+// included).  Every update block holds whole preconditioner blocks, so
+// applying M^-1 needs only the block's residuals, exchanged through LDS.
+struct BlockRows {
+  int first[UPD_BLOCK];  // local row of each row's block start
+  int size[UPD_BLOCK];
+};
+
+// Thread 0 assigns the rows [0, rows) of an update block to blocks.
+__device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, int rows, BlockRows& br) {
+  if (threadIdx.x == 0) {
+    int start = 0;
+    for (int t = 0; t < rows; ++t) {
+      bool joins = false;
+      if (t > 0 && t - start < BLOCK_MAX) {
+        for (int d = 1; d <= 3; ++d)
+          if (((join_sh[t] >> (d - 1)) & 1u) && t - d >= start) joins = true;
+      }
+      if (!joins) start = t;
+      br.first[t] = start;
+    }
+    for (int t = 0; t < rows;) {
+      int e = t + 1;
+      while (e < rows && br.first[e] == t) ++e;
+      for (int u = t; u < e; ++u) br.size[u] = e - t;
+      t = e;
+    }
+  }
+  __syncthreads();
+}
+
+// In-place inverse of a small SPD matrix (Gauss-Jordan, no pivoting needed).
+__device__ inline void invert_small(double (&a)[BLOCK_MAX][BLOCK_MAX], int m) {
+  double inv[BLOCK_MAX][BLOCK_MAX];
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) inv[r][c] = r == c ? 1.0 : 0.0;
+  for (int p = 0; p < m; ++p) {
+    const double d = 1.0 / a[p][p];
+    for (int c = 0; c < m; ++c) {
+      a[p][c] *= d;
+      inv[p][c] *= d;
+    }
+    for (int r = 0; r < m; ++r) {
+      if (r == p) continue;
+      const double f = a[r][p];
+      for (int c = 0; c < m; ++c) {
+        a[r][c] -= f * a[p][c];
+        inv[r][c] -= f * inv[p][c];
+      }
+    }
+  }
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) a[r][c] = inv[r][c];
+}
+
 __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __restrict__ type, FFDev ff,
                                                        const double2* __restrict__ rhs,
                                                        const double2* __restrict__ guess, double2* __restrict__ X,
                                                        double2* __restrict__ R, double4* __restrict__ ZP,
-                                                       float4* __restrict__ ZQ, double* __restrict__ inv_diag,
+                                                       float4* __restrict__ ZQ, const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ nbr_j, const double* __restrict__ hval,
+                                                       const unsigned char* __restrict__ join,
+                                                       double4* __restrict__ minv, int2* __restrict__ binfo,
                                                        double* __restrict__ eta_out, double* __restrict__ part_b,
                                                        double* __restrict__ part_bb, int nu, CgSys* __restrict__ cg,
                                                        CgCounters* __restrict__ cnt) {
   const int sys = block_system(S, blockIdx.x, false);
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.upd0;
-  const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
+  const int row0 = sd.atom0 + local * UPD_BLOCK;
+  const int rows = min(UPD_BLOCK, sd.atom0 + sd.n - row0);
+  const int t = threadIdx.x;
+  const int i = row0 + t;
+  const bool mine = t < rows;
+  __shared__ unsigned char join_sh[UPD_BLOCK];
+  __shared__ BlockRows br;
+  __shared__ double hb[UPD_BLOCK][BLOCK_MAX];
+  __shared__ double2 bsh[UPD_BLOCK];
+  join_sh[t] = mine ? join[i] : 0;
+  __syncthreads();
+  assign_blocks(join_sh, rows, br);
+  double eta = 0.0;
+  double2 b = make_double2(0.0, 0.0);
+  if (mine) {
+    const double* a = ff.atom + type[i] * AP_COUNT;
+    eta = a[AP_ETA];
+    b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
+    bsh[t] = b;
+    // row t of its block of H: eta on the diagonal plus every edge (all images) into the block
+    const int f = br.first[t], m = br.size[t];
+    for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = (c == t - f) ? eta : 0.0;
+    const int lo = row0 + f, hi = lo + m - 1;
+    int e = row_ptr[i], e1 = row_ptr[i + 1];
+    int l = e, h = e1;  // first edge with j >= lo
+    while (l < h) {
+      const int mid = (l + h) >> 1;
+      if (nbr_j[mid] < lo) l = mid + 1;
+      else h = mid;
+    }
+    for (e = l; e < e1 && nbr_j[e] <= hi; ++e) hb[t][nbr_j[e] - lo] += hval[e];
+  }
+  __syncthreads();
+  __shared__ double inv_sh[UPD_BLOCK][BLOCK_MAX];
+  if (mine && br.first[t] == t) {
+    const int m = br.size[t];
+    double a[BLOCK_MAX][BLOCK_MAX];
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < m; ++c) a[r][c] = hb[t + r][c];
+    invert_small(a, m);
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < BLOCK_MAX; ++c) inv_sh[t + r][c] = c < m ? a[r][c] : 0.0;
+  }
+  __syncthreads();
   double vals[4] = {0, 0, 0, 0};
   double bb[2] = {0, 0};
-  if (i < sd.atom0 + sd.n) {
-    const double* a = ff.atom + type[i] * AP_COUNT;
-    const double eta = a[AP_ETA];
-    const double d = 1.0 / eta;
-    const double2 r = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
+  if (mine) {
+    const int f = br.first[t], m = br.size[t];
+    double2 z = make_double2(0.0, 0.0);
+    for (int c = 0; c < m; ++c) {
+      z.x += inv_sh[t][c] * bsh[f + c].x;
+      z.y += inv_sh[t][c] * bsh[f + c].y;
+    }
+    minv[i] = make_double4(inv_sh[t][0], inv_sh[t][1], inv_sh[t][2], inv_sh[t][3]);
+    binfo[i] = make_int2(f, m);
     X[i] = guess ? guess[i] : make_double2(0.0, 0.0);
-    R[i] = r;
-    const double2 z = make_double2(r.x * d, r.y * d);
+    R[i] = b;
     ZP[i] = make_double4(z.x, z.y, 0.0, 0.0);  // (z_s, z_t, p_s, p_t), p = 0 before iteration 0
     if (ZQ) ZQ[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
-    inv_diag[i] = d;
     eta_out[i] = eta;
-    vals[0] = r.x * z.x;
-    vals[1] = r.y * z.y;
-    vals[2] = r.x * r.x;
-    vals[3] = r.y * r.y;
+    vals[0] = b.x * z.x;
+    vals[1] = b.y * z.y;
+    vals[2] = b.x * b.x;
+    vals[3] = b.y * b.y;
     bb[0] = vals[2];
     bb[1] = vals[3];
   }
@@ -295,6 +413,19 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
       cg[sys].k0 = -1;
     }
   }
+}
+
+// z = M^-1 r for one row of an update block, from the block's residuals in LDS
+// (bi = local row of the block start, block size).
+__device__ inline double2 apply_block(const double4& mi, const int2& bi, const double2* __restrict__ rsh) {
+  const int f = bi.x;
+  const double w[4] = {mi.x, mi.y, mi.z, mi.w};
+  double2 z = make_double2(0.0, 0.0);
+  for (int c = 0; c < bi.y; ++c) {
+    z.x += w[c] * rsh[f + c].x;
+    z.y += w[c] * rsh[f + c].y;
+  }
+  return z;
 }
 
 // Eight H entries per lane of one row (coalesced over the wave): columns and
@@ -361,7 +492,6 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
                                                             int max_iter, CgSys* __restrict__ cg,
                                                             CgCounters* __restrict__ cnt, const int* __restrict__ type,
                                                             FFDev ff, const double2* __restrict__ rhs,
-                                                            const double* __restrict__ inv_diag,
                                                             const double2* __restrict__ X, double2* __restrict__ R) {
   const int sys = block_system(S, blockIdx.x, true);
   const CgSys c = cg[sys];
@@ -371,8 +501,8 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
   const int waves = sd.ndir * (DIR_BLOCK >> 6);
   const int end = sd.atom0 + sd.n;
   if (c.done == CG_REFRESH && c.kr == k) {
-    // Refresh: FP64 true residual r = b - A x, z = r / diag, p = 0 into the
-    // records the next launch reads.  The update launch reduces r.z and r.r.
+    // Refresh: FP64 true residual r = b - A x.  The update launch applies the
+    // preconditioner, writes (z, p = 0) and reduces r.z and r.r.
     for (int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6); i < end; i += waves) {
       double as = 0.0, at = 0.0;
       for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
@@ -387,11 +517,7 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
         const double2 b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
         const double2 xv = X[i];
         const double d = eta[i];
-        const double2 r = make_double2(b.x - (d * xv.x + as), b.y - (d * xv.y + at));
-        const double id = inv_diag[i];
-        R[i] = r;
-        zp_out[i] = make_double4(r.x * id, r.y * id, 0.0, 0.0);
-        if (MIXED) zq_out[i] = make_float4((float)(r.x * id), (float)(r.y * id), 0.0f, 0.0f);
+        R[i] = make_double2(b.x - (d * xv.x + as), b.y - (d * xv.y + at));
       }
     }
     return;
@@ -494,10 +620,11 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
   write_partials<2>(vals, part_a, nd);
 }
 
-// Iteration k, second half: alpha, x += alpha p, r -= alpha A p, z = r / diag
+// Iteration k, second half: alpha, x += alpha p, r -= alpha A p, z = M^-1 r
 // (into the z half of zp), partial r.z and r.r per block.
 template <bool MIXED>
-__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const double* __restrict__ inv_diag,
+__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const double4* __restrict__ minv,
+                                                         const int2* __restrict__ binfo,
                                                          double2* __restrict__ X, double2* __restrict__ R,
                                                          double4* __restrict__ zp, float4* __restrict__ zq,
                                                          const double2* __restrict__ AP,
@@ -509,16 +636,22 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
   const CgSys c = cg[sys];
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.upd0;
+  __shared__ double2 rsh[UPD_BLOCK];
+  const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
+  const bool mine = i < sd.atom0 + sd.n;
   if (c.done == CG_REFRESH) {
     // A refresh requested by this iteration's direction launch waits for the
-    // next one; a refresh computed by this iteration's direction launch is
-    // reduced here.
+    // next one; a true residual computed by this iteration's direction launch
+    // is preconditioned and reduced here.
     if (c.kr != k) return;
     double vals[4] = {0, 0, 0, 0};
-    const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
-    if (i < sd.atom0 + sd.n) {
-      const double2 r = R[i];
-      const double2 z = reinterpret_cast<const double2*>(zp + i)[0];
+    const double2 r = mine ? R[i] : make_double2(0.0, 0.0);
+    rsh[threadIdx.x] = r;
+    __syncthreads();
+    if (mine) {
+      const double2 z = apply_block(minv[i], binfo[i], rsh);
+      zp[i] = make_double4(z.x, z.y, 0.0, 0.0);
+      if (MIXED) zq[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
       vals[0] = r.x * z.x;
       vals[1] = r.y * z.y;
       vals[2] = r.x * r.x;
@@ -539,20 +672,24 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
   rhs_converged(cur, bb, c, k - c.k0, MIXED, tol2, conv);
   const double alpha_s = (!conv[0] && pap[0] != 0.0) ? cur[0] / pap[0] : 0.0;
   const double alpha_t = (!conv[1] && pap[1] != 0.0) ? cur[1] / pap[1] : 0.0;
-  const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
   double vals[4] = {0, 0, 0, 0};
-  if (i < sd.atom0 + sd.n) {
+  double2 r = make_double2(0.0, 0.0);
+  if (mine) {
     const double2 p = reinterpret_cast<const double2*>(zp + i)[1];
     const double2 ap = AP[i];
-    double2 xv = X[i], r = R[i];
+    double2 xv = X[i];
+    r = R[i];
     xv.x += alpha_s * p.x;
     xv.y += alpha_t * p.y;
     r.x -= alpha_s * ap.x;
     r.y -= alpha_t * ap.y;
-    const double d = inv_diag[i];
-    const double2 z = make_double2(r.x * d, r.y * d);
     X[i] = xv;
     R[i] = r;
+  }
+  rsh[threadIdx.x] = r;
+  __syncthreads();
+  if (mine) {
+    const double2 z = apply_block(minv[i], binfo[i], rsh);
     reinterpret_cast<double2*>(zp + i)[0] = z;
     if (MIXED) reinterpret_cast<float2*>(zq + i)[0] = make_float2((float)z.x, (float)z.y);
     vals[0] = r.x * z.x;
@@ -607,10 +744,10 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, hipStream_t s) {
+                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
   XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr, nbr_j, nbr_s,
-            bond_cut, hval, hval32, bflag, bond_count);
+            bond_cut, hval, hval32, bflag, bond_count, join);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
@@ -656,7 +793,7 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   float4* zq1 = mixed ? a.ZQ1 : nullptr;
   // counters start at zero (cleared with the control block)
   hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.X, a.R, a.ZP0,
-                     zq0, a.inv_diag, a.eta, part_b, part_bb, nu, cg, cnt);
+                     zq0, a.row_ptr, a.nbr_j, a.hval, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt);
   XR_LAUNCH_CHECK();
   int k = 0;
   // Bound on launches: every system settles within max_iter executed
@@ -679,14 +816,14 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
       if (mixed) {
         XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<true>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
-                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.inv_diag, a.X, a.R);
-        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
+                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.X, a.R);
+        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.minv, a.binfo, a.X, a.R,
                   zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
       } else {
         XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<false>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
-                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.inv_diag, a.X, a.R);
-        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.inv_diag, a.X, a.R,
+                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.X, a.R);
+        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.minv, a.binfo, a.X, a.R,
                   zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
       }
       ++k;
