@@ -104,6 +104,13 @@ struct Systems {
 __device__ inline const CellDev& xr_cell(const Systems& S, int i) { return S.sys[S.asys[i]].cell; }
 __device__ inline void xr_flag(const Systems& S, int i, unsigned bits) { atomicOr(&S.err[S.asys[i]], bits); }
 
+// Per-edge flag byte written by the edge pass.
+enum EdgeFlag : unsigned char {
+  EF_BOND = 1,         // raw bond order at or above the cutoff: an active bond
+  EF_HB_DONOR = 2,     // row atom is an H-bond hydrogen, far atom an acceptor, r <= min(7.5, rc)
+  EF_HB_ACCEPTOR = 4,  // row atom is an acceptor, far atom an H-bond hydrogen, r <= min(7.5, rc)
+};
+
 // ------------------------------------------------------------ shift packing
 __host__ __device__ inline int xr_pack_shift(int sx, int sy, int sz) {
   return ((sx + 512) << 20) | ((sy + 512) << 10) | (sz + 512);

@@ -314,7 +314,7 @@ CtlLayout ctl_layout(int ns) {
   size_t off = 0;
   c.flags = carve_at<unsigned>(off, 16);  // [0] buffer-overflow bit, int [1] edges, int [2] bonds
   c.serr = carve_at<unsigned>(off, ns);
-  c.comps = carve_at<double>(off, (size_t)ns * NCOMP);
+  c.comps = carve_at<double>(off, (size_t)ns * NCH);  // channel sums; components on the host
   c.bounds = carve_at<double>(off, (size_t)ns * 6);
   c.mu = carve_at<double>(off, ns);
   c.cg = carve_at<char>(off, xr_cg_state_bytes(ns));
@@ -456,6 +456,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double4>(off, n);           // block-Jacobi rows
   carve_size<int2>(off, n);              // block placement
   carve_size<unsigned char>(off, n);     // block join flags
+  carve_size<double>(off, 7 * (size_t)n);  // H to atoms i-3..i+3
   carve_size<double>(off, n);            // eta
   carve_size<double>(off, xr_cg_partials(nd, nu));
   carve_size<double>(off, (size_t)kAtomScalars * n);  // atom scalars
@@ -480,6 +481,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   double4* minv = cv.take<double4>(n);
   int2* binfo = cv.take<int2>(n);
   unsigned char* join = cv.take<unsigned char>(n);
+  double* hnear = cv.take<double>(7 * (size_t)n);
   double* eta = cv.take<double>(n);
   double* cg_part = cv.take<double>(xr_cg_partials(nd, nu));
   double* scal = cv.take<double>((size_t)kAtomScalars * n);
@@ -642,7 +644,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
 
   // ------------------------------------------------------- edges and bonds
   HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, hval32, bflag, bond_count,
-                      join, s),
+                      join, hnear, s),
          "edge pass");
   HIPCHK(xr_exclusive_scan(bond_count, n, bond_ptr, scan_tmp, s), "bond scan");
   int64_t b_cap = 0;
@@ -735,6 +737,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.ZQ0 = cgq;
   cg.ZQ1 = cgq + n;
   cg.join = join;
+  cg.hnear = hnear;
   cg.minv = minv;
   cg.binfo = binfo;
   cg.eta = eta;
@@ -765,7 +768,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     hipError_t e = hipSuccess;
     XR_TIMED(XT_BOND_FORWARD, s, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s));
     HIPCHK(e, "bond orders");
-    XR_TIMED(XT_MANYBODY, s, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, lammps, acc, s));
+    XR_TIMED(XT_MANYBODY, s, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, bflag, lammps, acc, s));
     HIPCHK(e, "many-body");
     XR_TIMED(XT_BACKWARD, s, e = xr_bond_backward(at, bd, ff, out.forces, s));
     HIPCHK(e, "adjoint");
@@ -816,7 +819,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   for (int q = 0; q < ns; ++q) {
     out.iters[q] = given_q ? 0 : h_cg[q].iters;
     max_iters = std::max(max_iters, h_cg[q].iters);
-    std::memcpy(out.comps + (size_t)q * NCOMP, h_comps + (size_t)q * NCOMP, sizeof(double) * NCOMP);
+    xr_components(h_comps + (size_t)q * NCH, out.comps + (size_t)q * NCOMP);
     if (h_serr[q]) {
       out.status[q] = XREAC_INVALID;
       ctx->sys_messages[q] = error_text(h_serr[q]);

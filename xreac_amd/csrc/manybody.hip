@@ -30,190 +30,122 @@ __device__ inline void cross3(const double* a, const double* b, double* c) {
 }
 
 // ------------------------------------------------------------------ angles
-// Valence angle, penalty and angle conjugation of the pair (a, c), a < c,
-// around centre j (energy.py:278-318).  Returns false if the pair is skipped.
-struct AngleOut {
-  double e[3];                      // angle, penalty, conjugation
-  double gA, gC, a_ta, a_tc;        // d/dBO and d/dtotal of the far atoms
-  double Ga[3], Gc[3];              // d/d(bond vectors)
-  double A_db, A_d, A_tot, A_sbo2;  // centre adjoints
-};
-
-struct AngleCentre {
-  double db, d, totj, sbo2, p3, p5, Q, dQ, Cj, dCj, esb;
-  int tj;
-};
-
-__device__ inline AngleCentre angle_centre(const Atoms& at, const FFDev& ff, int j) {
-  const double* g = ff.gen;
-  AngleCentre c;
-  c.tj = at.type[j];
-  const double* aj = AP(ff, c.tj);
-  c.db = at.db[j];
-  c.d = at.d[j];
-  c.totj = at.total[j];
-  c.sbo2 = at.sbo2[j];
-  c.p3 = aj[AP_PVAL3];
-  c.p5 = aj[AP_PVAL5];
-  const double ex3 = exp(-g[20] * c.d), ex4 = exp(g[21] * c.d);
-  c.Q = (2 + ex3) / (1 + ex3 + ex4);
-  c.dQ = (-g[20] * ex3 * (1 + ex3 + ex4) - (2 + ex3) * (-g[20] * ex3 + g[21] * ex4)) /
-         ((1 + ex3 + ex4) * (1 + ex3 + ex4));
-  const double ecj = exp(g[2] * (c.totj - aj[AP_VALVAL]));
-  c.Cj = 1 / (1 + ecj);
-  c.dCj = -g[2] * ecj * c.Cj * c.Cj;
-  c.esb = exp(-g[17] * (2 - c.sbo2));
-  return c;
-}
-
-__device__ inline bool angle_pair(const Atoms& at, const Bonds& bd, const FFDev& ff, const AngleCentre& cj, int a,
-                                  int c, int lammps, AngleOut& o) {
-  const double* g = ff.gen;
-  const double Ba = bd.BO[a], Bc = bd.BO[c];
-  if (!(Ba > THB_CUT) || !(Bc > THB_CUT) || Ba * Bc <= THB_PRODUCT_CUT) return false;
-  const int ia = bd.bj[a], kc = bd.bj[c];
-  const int nt = ff.nt;
-  const int key = (at.type[ia] * nt + cj.tj) * nt + at.type[kc];
-  const int r0 = ff.ang_off[key], r1 = ff.ang_off[key + 1];
-  if (r0 == r1) return false;
-  const double* va = bd.vec + 3 * a;
-  const double* vc = bd.vec + 3 * c;
-  const double ra = bd.r[a], rc = bd.r[c];
-  const double cosv = dot3(va, vc) / (ra * rc);
-  double theta, dth;
-  if (lammps) {
-    const double cc = fmin(fmax(cosv, -1.0), 1.0);
-    theta = acos(cc);
-    dth = -1.0 / fmax(sin(theta), 1e-5);
-  } else {
-    const double lo = -1 + 1e-14, hi = 1 - 1e-14;
-    const double cc = fmin(fmax(cosv, lo), hi);
-    theta = acos(cc);
-    dth = (cc == lo || cc == hi) ? 0.0 : -1.0 / sqrt(1 - cc * cc);
-  }
-  const double bij = Ba - THB_CUT, bjk = Bc - THB_CUT;
-  const double tota = at.total[ia], totc = at.total[kc];
-  const double p3 = cj.p3, p5 = cj.p5, db = cj.db;
-  double dE_dcos = 0.0;
-  o.e[0] = o.e[1] = o.e[2] = 0.0;
-  o.gA = o.gC = o.a_ta = o.a_tc = 0.0;
-  o.A_db = o.A_d = o.A_tot = o.A_sbo2 = 0.0;
-  for (int row = r0; row < r1; ++row) {
-    const double* pr = ff.ang + row * ANG_W;
-    const double th00 = pr[0], v1 = pr[1], v2 = pr[2], coa1 = pr[3], v7 = pr[4], pen1 = pr[5], v4 = pr[6];
-    const double theta0 = (180 - th00 * (1 - cj.esb)) * DEG;
-    const double pa = pow(bij, v4), pc = pow(bjk, v4);
-    const double ea = exp(-p3 * pa), ec = exp(-p3 * pc);
-    const double Fa = 1 - ea, Fc = 1 - ec;
-    const double f7 = Fa * Fc;
-    const double ex6 = exp(g[14] * db), ex7 = exp(-v7 * db);
-    const double den8 = 1 + ex6 + ex7;
-    const double R = (2 + ex6) / den8;
-    const double f8 = p5 - (p5 - 1) * R;
-    const double dR = (g[14] * ex6 * den8 - (2 + ex6) * (g[14] * ex6 - v7 * ex7)) / (den8 * den8);
-    const double dt = theta0 - theta;
-    const double exang = exp(-v2 * (dt * dt));
-    const double sgn = v1 >= 0 ? 1 - exang : -exang;
-    o.e[0] += f7 * f8 * v1 * sgn;
-    const double dE_dth = -f7 * f8 * v1 * 2 * v2 * dt * exang;
-    dE_dcos += dE_dth * dth;
-    o.A_sbo2 += -dE_dth * th00 * g[17] * cj.esb * DEG;
-    const double dE_df7 = f8 * v1 * sgn, dE_df8 = f7 * v1 * sgn;
-    o.gA += dE_df7 * Fc * p3 * v4 * (bij > 0 ? pa / bij : 0.0) * ea;
-    o.gC += dE_df7 * Fa * p3 * v4 * (bjk > 0 ? pc / bjk : 0.0) * ec;
-    o.A_db += dE_df8 * (-(p5 - 1) * dR);
-    // penalty
-    const double Ebb = exp(-g[19] * ((bij - 2) * (bij - 2) + (bjk - 2) * (bjk - 2)));
-    const double Ep = pen1 * cj.Q * Ebb;
-    o.e[1] += Ep;
-    o.A_d += pen1 * cj.dQ * Ebb;
-    o.gA += Ep * (-2 * g[19] * (bij - 2));
-    o.gC += Ep * (-2 * g[19] * (bjk - 2));
-    // angle conjugation
-    const double E1 = exp(-g[38] * ((tota - bij) * (tota - bij) + (totc - bjk) * (totc - bjk)));
-    const double E2 = exp(-g[30] * ((bij - 1.5) * (bij - 1.5) + (bjk - 1.5) * (bjk - 1.5)));
-    const double Ec = coa1 * cj.Cj * E1 * E2;
-    o.e[2] += Ec;
-    o.A_tot += coa1 * cj.dCj * E1 * E2;
-    o.a_ta += Ec * (-2 * g[38] * (tota - bij));
-    o.a_tc += Ec * (-2 * g[38] * (totc - bjk));
-    o.gA += Ec * (2 * g[38] * (tota - bij) - 2 * g[30] * (bij - 1.5));
-    o.gC += Ec * (2 * g[38] * (totc - bjk) - 2 * g[30] * (bjk - 1.5));
-  }
-  const double inv = 1.0 / (ra * rc);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    o.Ga[k] = dE_dcos * (vc[k] * inv - cosv * va[k] / (ra * ra));
-    o.Gc[k] = dE_dcos * (va[k] * inv - cosv * vc[k] / (rc * rc));
-  }
-  return true;
-}
-
-// One wave per centre j.  Lane l owns bond slot l of j's row (chunks of 64):
-// it evaluates every pair its slot belongs to, in canonical (lower, higher)
-// order, and keeps its own bond's share; energies and centre adjoints are
-// taken once per pair (by the lower slot) and wave-reduced.  Each slot has one
-// writer and the order is fixed.
+// One thread per centre over its unordered bond pairs (a < c), in row order.
+// Centres have few bonds, so a wave per centre (each pair evaluated by the
+// lanes of both bonds) was measured slower on liquid water.
 __global__ void __launch_bounds__(256) k_angles(Atoms at, Bonds bd, FFDev ff, int lammps, EnergyAcc acc) {
-  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (j >= at.n) return;
-  const int b0 = bd.ptr[j], b1 = bd.ptr[j + 1];
-  double e0 = 0.0, e1 = 0.0, e2 = 0.0, A_db = 0.0, A_d = 0.0, A_tot = 0.0, A_sbo2 = 0.0;
-  if (b1 - b0 >= 2) {
-    const AngleCentre cj = angle_centre(at, ff, j);
-    for (int chunk = b0; chunk < b1; chunk += 64) {
-      const int mine = chunk + lane;
-      if (mine >= b1 || !(bd.BO[mine] > THB_CUT)) continue;
-      double g0 = 0.0, g1 = 0.0, g2 = 0.0, abo = 0.0, atf = 0.0;
-      for (int other = b0; other < b1; ++other) {
-        if (other == mine) continue;
-        const bool low = mine < other;
-        AngleOut o;
-        if (!angle_pair(at, bd, ff, cj, low ? mine : other, low ? other : mine, lammps, o)) continue;
-        if (low) {
-          g0 += o.Ga[0];
-          g1 += o.Ga[1];
-          g2 += o.Ga[2];
-          abo += o.gA;
-          atf += o.a_ta;
-          e0 += o.e[0];
-          e1 += o.e[1];
-          e2 += o.e[2];
-          A_db += o.A_db;
-          A_d += o.A_d;
-          A_tot += o.A_tot;
-          A_sbo2 += o.A_sbo2;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  double e[3] = {0.0, 0.0, 0.0};
+  if (j < at.n) {
+    const double* g = ff.gen;
+    const int tj = at.type[j];
+    const double* aj = AP(ff, tj);
+    const int nt = ff.nt;
+    const double db = at.db[j], d = at.d[j], totj = at.total[j], sbo2 = at.sbo2[j];
+    const double p3 = aj[AP_PVAL3], p5 = aj[AP_PVAL5];
+    // centre-only factors
+    const double ex3 = exp(-g[20] * d), ex4 = exp(g[21] * d);
+    const double Q = (2 + ex3) / (1 + ex3 + ex4);
+    const double dQ = (-g[20] * ex3 * (1 + ex3 + ex4) - (2 + ex3) * (-g[20] * ex3 + g[21] * ex4)) /
+                      ((1 + ex3 + ex4) * (1 + ex3 + ex4));
+    const double ecj = exp(g[2] * (totj - aj[AP_VALVAL]));
+    const double Cj = 1 / (1 + ecj);
+    const double dCj = -g[2] * ecj * Cj * Cj;
+    const double esb = exp(-g[17] * (2 - sbo2));
+    double A_db = 0.0, A_d = 0.0, A_tot = 0.0, A_sbo2 = 0.0;
+    const int b0 = bd.ptr[j], b1 = bd.ptr[j + 1];
+    for (int a = b0; a < b1; ++a) {
+      const double Ba = bd.BO[a];
+      if (!(Ba > THB_CUT)) continue;
+      for (int c = a + 1; c < b1; ++c) {
+        const double Bc = bd.BO[c];
+        if (!(Bc > THB_CUT)) continue;
+        if (Ba * Bc <= THB_PRODUCT_CUT) continue;
+        const int ia = bd.bj[a], kc = bd.bj[c];
+        const int key = (at.type[ia] * nt + tj) * nt + at.type[kc];
+        const int r0 = ff.ang_off[key], r1 = ff.ang_off[key + 1];
+        if (r0 == r1) continue;
+        const double* va = bd.vec + 3 * a;
+        const double* vc = bd.vec + 3 * c;
+        const double ra = bd.r[a], rc = bd.r[c];
+        const double cosv = dot3(va, vc) / (ra * rc);
+        double theta, dth;
+        if (lammps) {
+          const double cc = fmin(fmax(cosv, -1.0), 1.0);
+          theta = acos(cc);
+          dth = -1.0 / fmax(sin(theta), 1e-5);
         } else {
-          g0 += o.Gc[0];
-          g1 += o.Gc[1];
-          g2 += o.Gc[2];
-          abo += o.gC;
-          atf += o.a_tc;
+          const double lo = -1 + 1e-14, hi = 1 - 1e-14;
+          const double cc = fmin(fmax(cosv, lo), hi);
+          theta = acos(cc);
+          dth = (cc == lo || cc == hi) ? 0.0 : -1.0 / sqrt(1 - cc * cc);
+        }
+        const double bij = Ba - THB_CUT, bjk = Bc - THB_CUT;
+        const double tota = at.total[ia], totc = at.total[kc];
+        double gA = 0.0, gC = 0.0, dE_dcos = 0.0, a_ta = 0.0, a_tc = 0.0;
+        for (int row = r0; row < r1; ++row) {
+          const double* pr = ff.ang + row * ANG_W;
+          const double th00 = pr[0], v1 = pr[1], v2 = pr[2], coa1 = pr[3], v7 = pr[4], pen1 = pr[5], v4 = pr[6];
+          const double theta0 = (180 - th00 * (1 - esb)) * DEG;
+          const double pa = pow(bij, v4), pc = pow(bjk, v4);
+          const double ea = exp(-p3 * pa), ec = exp(-p3 * pc);
+          const double Fa = 1 - ea, Fc = 1 - ec;
+          const double f7 = Fa * Fc;
+          const double ex6 = exp(g[14] * db), ex7 = exp(-v7 * db);
+          const double den8 = 1 + ex6 + ex7;
+          const double R = (2 + ex6) / den8;
+          const double f8 = p5 - (p5 - 1) * R;
+          const double dR = (g[14] * ex6 * den8 - (2 + ex6) * (g[14] * ex6 - v7 * ex7)) / (den8 * den8);
+          const double dt = theta0 - theta;
+          const double exang = exp(-v2 * (dt * dt));
+          const double s = v1 >= 0 ? 1 - exang : -exang;
+          e[0] += f7 * f8 * v1 * s;
+          const double dE_dth = -f7 * f8 * v1 * 2 * v2 * dt * exang;
+          dE_dcos += dE_dth * dth;
+          A_sbo2 += -dE_dth * th00 * g[17] * esb * DEG;
+          const double dE_df7 = f8 * v1 * s, dE_df8 = f7 * v1 * s;
+          gA += dE_df7 * Fc * p3 * v4 * (bij > 0 ? pa / bij : 0.0) * ea;
+          gC += dE_df7 * Fa * p3 * v4 * (bjk > 0 ? pc / bjk : 0.0) * ec;
+          A_db += dE_df8 * (-(p5 - 1) * dR);
+          // penalty
+          const double Ebb = exp(-g[19] * ((bij - 2) * (bij - 2) + (bjk - 2) * (bjk - 2)));
+          const double Ep = pen1 * Q * Ebb;
+          e[1] += Ep;
+          A_d += pen1 * dQ * Ebb;
+          gA += Ep * (-2 * g[19] * (bij - 2));
+          gC += Ep * (-2 * g[19] * (bjk - 2));
+          // angle conjugation
+          const double E1 = exp(-g[38] * ((tota - bij) * (tota - bij) + (totc - bjk) * (totc - bjk)));
+          const double E2 = exp(-g[30] * ((bij - 1.5) * (bij - 1.5) + (bjk - 1.5) * (bjk - 1.5)));
+          const double Ec = coa1 * Cj * E1 * E2;
+          e[2] += Ec;
+          A_tot += coa1 * dCj * E1 * E2;
+          a_ta += Ec * (-2 * g[38] * (tota - bij));
+          a_tc += Ec * (-2 * g[38] * (totc - bjk));
+          gA += Ec * (2 * g[38] * (tota - bij) - 2 * g[30] * (bij - 1.5));
+          gC += Ec * (2 * g[38] * (totc - bjk) - 2 * g[30] * (bjk - 1.5));
+        }
+        bd.A_BO[a] += gA;
+        bd.A_BO[c] += gC;
+        bd.A_totfar[a] += a_ta;
+        bd.A_totfar[c] += a_tc;
+        const double inv = 1.0 / (ra * rc);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          bd.G[3 * a + k] += dE_dcos * (vc[k] * inv - cosv * va[k] / (ra * ra));
+          bd.G[3 * c + k] += dE_dcos * (va[k] * inv - cosv * vc[k] / (rc * rc));
         }
       }
-      bd.G[3 * mine] += g0;
-      bd.G[3 * mine + 1] += g1;
-      bd.G[3 * mine + 2] += g2;
-      bd.A_BO[mine] += abo;
-      bd.A_totfar[mine] += atf;
     }
-  }
-  e0 = xr_wave_sum(e0);
-  e1 = xr_wave_sum(e1);
-  e2 = xr_wave_sum(e2);
-  A_db = xr_wave_sum(A_db);
-  A_d = xr_wave_sum(A_d);
-  A_tot = xr_wave_sum(A_tot);
-  A_sbo2 = xr_wave_sum(A_sbo2);
-  if (lane == 0) {
     at.A_db[j] += A_db;
     at.A_d[j] += A_d;
     at.A_total[j] += A_tot;
     at.A_sbo2[j] += A_sbo2;
-    xr_energy(acc, CH_ANGLE, j, e0);
-    xr_energy(acc, CH_PEN, j, e1);
-    xr_energy(acc, CH_COA, j, e2);
+  }
+  if (j < at.n) {
+    xr_energy(acc, CH_ANGLE, j, e[0]);
+    xr_energy(acc, CH_PEN, j, e[1]);
+    xr_energy(acc, CH_COA, j, e[2]);
   }
 }
 
@@ -448,7 +380,8 @@ __device__ inline void hbond_eval(const double* prm, double BOd, const double* v
 // One wave per hydrogen: donor-side energy and derivatives.
 __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
                                                   const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                                  const int* __restrict__ nbr_s, EnergyAcc acc) {
+                                                  const int* __restrict__ nbr_s,
+                                                  const unsigned char* __restrict__ eflag, EnergyAcc acc) {
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   double etot[1] = {0.0};
@@ -466,9 +399,11 @@ __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
       const int kbase = (at.type[kd] * nt + tj) * nt;
       double Gd[3] = {0, 0, 0}, dBO = 0.0;
       for (int e = row_ptr[j] + lane; e < row_ptr[j + 1]; e += 64) {
+        // candidates (acceptor type, r <= min(7.5, rc)) were flagged by the edge pass
+        if (!(eflag[e] & EF_HB_DONOR) || e == ed) continue;
         const int k = nbr_j[e];
         const int tk = at.type[k];
-        if (e == ed || AP(ff, tk)[AP_HBOND] != 2.0 || !ff.hb_on[kbase + tk]) continue;
+        if (!ff.hb_on[kbase + tk]) continue;
         double va[3];
         const double ra = xr_edge(at.x, cell, j, k, nbr_s[e], va);
         if (!(ra <= hcut)) continue;
@@ -505,7 +440,8 @@ __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
 // One wave per heavy atom: force from the acceptor-vector derivative.
 __global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, FFDev ff,
                                                      const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                                     const int* __restrict__ nbr_s) {
+                                                     const int* __restrict__ nbr_s,
+                                                     const unsigned char* __restrict__ eflag) {
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (k >= at.n) return;
@@ -516,9 +452,9 @@ __global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, FFDev f
   const double hcut = fmin(7.5, ff.gen[12]);
   double F[3] = {0, 0, 0};
   for (int e = row_ptr[k] + lane; e < row_ptr[k + 1]; e += 64) {
+    if (!(eflag[e] & EF_HB_ACCEPTOR)) continue;  // H-bond hydrogen within min(7.5, rc), flagged by the edge pass
     const int j = nbr_j[e];
     const int tj = at.type[j];
-    if (AP(ff, tj)[AP_HBOND] != 1.0) continue;
     const int code = nbr_s[e];
     double vk[3];
     const double ra = xr_edge(at.x, cell, k, j, code, vk);
@@ -554,12 +490,13 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 }  // namespace
 
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
-                       const int* nbr_s, int lammps_angles, EnergyAcc acc, hipStream_t s) {
+                       const int* nbr_s, const unsigned char* eflag, int lammps_angles, EnergyAcc acc,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_angles, grid(at.n), dim3(BLK), 0, s, at, bd, ff, lammps_angles, acc);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  hipLaunchKernelGGL(k_angles, wgrid, dim3(256), 0, s, at, bd, ff, lammps_angles, acc);
   hipLaunchKernelGGL(k_torsions, wgrid, dim3(256), 0, s, at, bd, ff, acc);
-  hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, acc);
-  hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s);
+  hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag, acc);
+  hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

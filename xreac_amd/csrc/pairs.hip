@@ -133,42 +133,20 @@ __global__ void __launch_bounds__(256) k_charge_response(Atoms at, FFDev ff, con
   }
 }
 
-// One block per system: each channel summed over the system's atoms in a
-// fixed order (lane-strided, then a fixed tree), independent of where the
-// system sits in the batch.
-__global__ void __launch_bounds__(256) k_energy_reduce(Systems S, EnergyAcc acc, double* __restrict__ comps) {
+// One block per (system, channel): the channel summed over the system's atoms
+// in a fixed order (lane-strided, then a fixed tree), independent of where
+// the system sits in the batch.
+__global__ void __launch_bounds__(256) k_energy_reduce(Systems S, EnergyAcc acc, double* __restrict__ chan) {
   const SysDev& sd = S.sys[blockIdx.x];
-  __shared__ double red[NCH][4];
-  __shared__ double ch[NCH];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int c = 0; c < NCH; ++c) {
-    const double* e = acc.e + (size_t)c * acc.n + sd.atom0;
-    double s = 0.0;
-    for (int t = threadIdx.x; t < sd.n; t += blockDim.x) s += e[t];
-    s = xr_wave_sum(s);
-    if (lane == 0) red[c][w] = s;
-  }
+  const int c = blockIdx.y;
+  __shared__ double red[4];
+  const double* e = acc.e + (size_t)c * acc.n + sd.atom0;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < sd.n; t += blockDim.x) s += e[t];
+  s = xr_wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x < NCH) ch[threadIdx.x] = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) +
-                                           red[threadIdx.x][3];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double out[NCOMP];
-    for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
-    out[C_BOND] = ch[CH_BOND] + ch[CH_TRIPLE];
-    out[C_ATOM] = ch[CH_ATOM];
-    out[C_LONE] = ch[CH_LP] + ch[CH_C2];
-    out[C_ANGLE] = ch[CH_ANGLE];
-    out[C_PENALTY] = ch[CH_PEN];
-    out[C_COA] = ch[CH_COA];
-    out[C_HBOND] = ch[CH_HB];
-    out[C_TORSION] = ch[CH_TOR];
-    out[C_CONJ] = ch[CH_CONJ];
-    out[C_VDW] = ch[CH_VDW];
-    out[C_COULOMB] = ch[CH_COUL];
-    out[C_QEQ] = ch[CH_QEQ];
-    for (int k = 0; k < NCOMP; ++k) comps[(size_t)blockIdx.x * NCOMP + k] = out[k];
-  }
+  if (threadIdx.x == 0) chan[(size_t)blockIdx.x * NCH + c] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 }  // namespace
@@ -189,8 +167,8 @@ hipError_t xr_charge_response(const Atoms& at, FFDev ff, const int* row_ptr, con
   return hipSuccess;
 }
 
-hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* comps, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_reduce, dim3(S.ns), dim3(256), 0, s, S, acc, comps);
+hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* chan, hipStream_t s) {
+  hipLaunchKernelGGL(k_energy_reduce, dim3(S.ns, NCH), dim3(256), 0, s, S, acc, chan);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -38,6 +38,7 @@ struct CgArgs {
   double4 *ZP0, *ZP1;  // (z_s, z_t, p_s, p_t), alternating by iteration
   float4 *ZQ0, *ZQ1;   // FP32 copies gathered by the mixed-precision sweep
   const unsigned char* join;  // per atom: strong bonds to the previous three atoms (edge pass)
+  const double* hnear;        // per atom: H summed over images to atoms i-3..i+3 (edge pass)
   double4* minv;              // per atom: its row of the block-Jacobi inverse
   int2* binfo;                // per atom: (local row of its block start, block size)
   double* eta;
@@ -64,7 +65,8 @@ struct CgOutcome {
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s);
+                        unsigned char* bflag, int* bond_count, unsigned char* join, double* hnear,
+                        hipStream_t s);
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int cap,
                         int* bond_edge, int* bond_i, hipStream_t s);
 void xr_cg_blocks(int n, int* ndir, int* nupd);

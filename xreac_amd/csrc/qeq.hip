@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
                                                    const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
                                                    double bond_cut, double* __restrict__ hval,
                                                    float* __restrict__ hval32, unsigned char* __restrict__ bflag,
-                                                   int* __restrict__ bond_count, unsigned char* __restrict__ join) {
+                                                   int* __restrict__ bond_count, unsigned char* __restrict__ join,
+                                                   double* __restrict__ hnear) {
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
@@ -56,8 +57,11 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   const int ti = type[i];
   const double rc = ff.gen[12];
   const double cut = 0.01 * ff.gen[29];
+  const double hcut = fmin(7.5, rc);
+  const double hb_i = ff.atom[ti * AP_COUNT + AP_HBOND];
   int cnt = 0;
   unsigned jm = 0;  // bit d-1: strong same-image bond to atom i - d (d = 1..3), for the CG blocks
+  double near[7] = {0, 0, 0, 0, 0, 0, 0};  // sum of h over all images to atom i + d, d = -3..3
   const int zero = xr_pack_shift(0, 0, 0);
   for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
     const int j = nbr_j[e];
@@ -68,6 +72,12 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
     xr_shield(r, rc, ff.pair[(ti * ff.nt + tj) * PP_COUNT + PP_GAMMA], s, ds);
     hval[e] = 14.4 * s;
     if (hval32) hval32[e] = (float)(14.4 * s);
+    const int dj = j - i;
+    if (dj >= -3 && dj <= 3) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q)
+        if (q == dj + 3) near[q] += 14.4 * s;
+    }
     unsigned char active = 0;
     if (r <= bond_cut) {
       BondRaw b;
@@ -76,15 +86,25 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
       active = raw >= cut;
       if (raw >= BLOCK_BOND && j < i && j >= i - 3 && nbr_s[e] == zero) jm |= 1u << (i - j - 1);
     }
-    bflag[e] = active;
+    unsigned char flag = active ? EF_BOND : 0;
+    if (r <= hcut) {
+      const double hb_j = ff.atom[tj * AP_COUNT + AP_HBOND];
+      if (hb_i == 1.0 && hb_j == 2.0) flag |= EF_HB_DONOR;
+      if (hb_i == 2.0 && hb_j == 1.0) flag |= EF_HB_ACCEPTOR;
+    }
+    bflag[e] = flag;
     cnt += active;
   }
   cnt = xr_wave_sum_int(cnt);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) jm |= __shfl_down(jm, off, 64);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) near[q] = xr_wave_sum(near[q]);
   if (lane == 0) {
     bond_count[i] = cnt;
     join[i] = (unsigned char)jm;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) hnear[7 * (size_t)i + q] = near[q];
   }
 }
 
@@ -100,7 +120,7 @@ __global__ void __launch_bounds__(256) k_bond_fill(int n, const int* __restrict_
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int e0 = row_ptr[i]; e0 < row_ptr[i + 1]; e0 += 64) {
     const int e = e0 + lane;
-    const bool on = e < row_ptr[i + 1] && bflag[e];
+    const bool on = e < row_ptr[i + 1] && (bflag[e] & EF_BOND);
     const unsigned long long m = __ballot(on);
     if (on) {
       const int b = base + __popcll(m & below);
@@ -315,8 +335,7 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
                                                        const double2* __restrict__ rhs,
                                                        const double2* __restrict__ guess, double2* __restrict__ X,
                                                        double2* __restrict__ R, double4* __restrict__ ZP,
-                                                       float4* __restrict__ ZQ, const int* __restrict__ row_ptr,
-                                                       const int* __restrict__ nbr_j, const double* __restrict__ hval,
+                                                       float4* __restrict__ ZQ, const double* __restrict__ hnear,
                                                        const unsigned char* __restrict__ join,
                                                        double4* __restrict__ minv, int2* __restrict__ binfo,
                                                        double* __restrict__ eta_out, double* __restrict__ part_b,
@@ -344,18 +363,13 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
     eta = a[AP_ETA];
     b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
     bsh[t] = b;
-    // row t of its block of H: eta on the diagonal plus every edge (all images) into the block
+    // row t of its block of H: eta on the diagonal plus the edge-pass sums over all images to
+    // each block mate (block mates are at most three rows apart)
     const int f = br.first[t], m = br.size[t];
-    for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = (c == t - f) ? eta : 0.0;
-    const int lo = row0 + f, hi = lo + m - 1;
-    int e = row_ptr[i], e1 = row_ptr[i + 1];
-    int l = e, h = e1;  // first edge with j >= lo
-    while (l < h) {
-      const int mid = (l + h) >> 1;
-      if (nbr_j[mid] < lo) l = mid + 1;
-      else h = mid;
+    for (int c = 0; c < BLOCK_MAX; ++c) {
+      const int d = f + c - t;
+      hb[t][c] = c < m ? (d == 0 ? eta : 0.0) + hnear[7 * (size_t)i + d + 3] : 0.0;
     }
-    for (e = l; e < e1 && nbr_j[e] <= hi; ++e) hb[t][nbr_j[e] - lo] += hval[e];
   }
   __syncthreads();
   __shared__ double inv_sh[UPD_BLOCK][BLOCK_MAX];
@@ -744,10 +758,11 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s) {
+                        unsigned char* bflag, int* bond_count, unsigned char* join, double* hnear,
+                        hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
   XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr, nbr_j, nbr_s,
-            bond_cut, hval, hval32, bflag, bond_count, join);
+            bond_cut, hval, hval32, bflag, bond_count, join, hnear);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
@@ -793,7 +808,7 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   float4* zq1 = mixed ? a.ZQ1 : nullptr;
   // counters start at zero (cleared with the control block)
   hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.X, a.R, a.ZP0,
-                     zq0, a.row_ptr, a.nbr_j, a.hval, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt);
+                     zq0, a.hnear, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt);
   XR_LAUNCH_CHECK();
   int k = 0;
   // Bound on launches: every system settles within max_iter executed

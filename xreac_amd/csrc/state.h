@@ -61,12 +61,30 @@ hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, FFDev ff, const int
 hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, double* forces, hipStream_t s);
 // manybody.hip
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
-                       const int* nbr_s, int lammps_angles, EnergyAcc acc, hipStream_t s);
+                       const int* nbr_s, const unsigned char* eflag, int lammps_angles, EnergyAcc acc,
+                       hipStream_t s);
 // pairs.hip
 hipError_t xr_nonbonded(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
                         EnergyAcc acc, double2* dEdq, hipStream_t s);
 // Adds the charge-response term (full derivative through QEq) to forces.
 hipError_t xr_charge_response(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
                               const double* lam, double* forces, hipStream_t s);
-// Per-system components [ns][NCOMP] from the per-atom channels.
-hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* comps, hipStream_t s);
+// Per-system channel sums [ns][NCH] from the per-atom channels.
+hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* chan, hipStream_t s);
+
+// The reported components from one system's channel sums.
+inline void xr_components(const double* ch, double* out) {
+  for (int k = 0; k < NCOMP; ++k) out[k] = 0.0;
+  out[C_BOND] = ch[CH_BOND] + ch[CH_TRIPLE];
+  out[C_ATOM] = ch[CH_ATOM];
+  out[C_LONE] = ch[CH_LP] + ch[CH_C2];
+  out[C_ANGLE] = ch[CH_ANGLE];
+  out[C_PENALTY] = ch[CH_PEN];
+  out[C_COA] = ch[CH_COA];
+  out[C_HBOND] = ch[CH_HB];
+  out[C_TORSION] = ch[CH_TOR];
+  out[C_CONJ] = ch[CH_CONJ];
+  out[C_VDW] = ch[CH_VDW];
+  out[C_COULOMB] = ch[CH_COUL];
+  out[C_QEQ] = ch[CH_QEQ];
+}
