@@ -61,6 +61,7 @@ struct FFDev {
   const unsigned char* tor_on; // [nt^4]
   const double* hb;            // [nt^3][4] keyed (t_donor, t_H, t_acceptor)
   const unsigned char* hb_on;  // [nt^3]
+  const double* vdw_b;         // [nt*nt] (1/gamma_w)^p_vdw, the shielding offset of each pair
 };
 
 struct CellDev {
@@ -77,6 +78,8 @@ struct BinGeom {
   double lo[3];  // grid origin in fractional (periodic) or scaled units
   double w[3];   // bin width in the same units
   int pbc[3];
+  double thick[3];  // Cartesian distance between the faces of a bin along each plane normal
+  double kappa;     // bound on the largest eigenvalue of N N^T, N = unit plane normals (1: orthogonal)
 };
 
 // One system of a batch.  Atoms of system s occupy [atom0, atom0 + n) of the
@@ -164,6 +167,36 @@ __device__ inline int xr_wave_sum_int(int v) {
 }
 __device__ inline unsigned xr_lane() { return __lane_id(); }
 
+// Orders a wave's LDS writes before its later LDS reads by other lanes of the
+// same wave (no block barrier: waves of a block run independent loops).
+__device__ inline void xr_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave-private LDS queue of row entries: scans [e, end) 64 entries at a time
+// and appends, in row order, those for which keep(index) holds, until the row
+// ends or fewer than 64 free slots remain.  Advances e past what it scanned
+// and returns the number queued; q holds cap >= 64 ints.  Sparse selections
+// (H-bond candidates) are then evaluated one dense batch of 64 at a time.
+template <class Keep>
+__device__ inline int xr_queue_edges(int* q, int cap, int& e, int end, Keep keep) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int count = 0;
+  while (e < end && count <= cap - 64) {
+    const int idx = e + lane;
+    const bool take = idx < end && keep(idx);
+    const unsigned long long m = __ballot(take);
+    if (take) q[count + __popcll(m & below)] = idx;
+    count += __popcll(m);
+    e += 64;
+  }
+  xr_wave_sync();
+  return count;
+}
+
 // Block reduction of NV doubles into partials[k * nblocks + blockIdx.x].
 // Fixed tree order -> deterministic.  Block size must be a multiple of 64.
 template <int NV>
@@ -228,11 +261,12 @@ __device__ inline void xr_bond_raw(const FFDev& ff, int ti, int tj, double r, Bo
 __device__ inline void xr_taper(double r, double rc, double& t, double& dt) {
   if (r < rc) {
     const double u = r / rc;
-    const double u4 = pow(u, 4.0);
+    const double u2 = u * u;
+    const double u4 = u2 * u2;
     const double inner = -35 + u * (84 + u * (-70 + 20 * u));
     t = 1 + u4 * inner;
     // d/du [u^4 inner] = 4u^3 inner + u^4 (84 - 140u + 60u^2)
-    const double du = 4 * u * u * u * inner + u4 * (84 + u * (-140 + 60 * u));
+    const double du = 4 * (u2 * u) * inner + u4 * (84 + u * (-140 + 60 * u));
     dt = du / rc;
   } else {
     t = 0.0;
@@ -240,13 +274,19 @@ __device__ inline void xr_taper(double r, double rc, double& t, double& dt) {
   }
 }
 
-// Shielded Coulomb kernel shield = taper / (r^3 + gamma_ij)^(1/3) (energy.py:92).
+// Shielded Coulomb kernel shield = taper / (r^3 + gamma_ij)^(1/3) (energy.py:92),
+// from a taper value t and its derivative dt already at hand.
+__device__ inline void xr_shield_tapered(double r, double t, double dt, double gamma, double& s, double& ds) {
+  const double r2 = r * r;
+  const double base = r2 * r + gamma;
+  const double inv = 1.0 / cbrt(base);
+  s = t * inv;
+  // d/dr base^(1/3) = r^2 base^(1/3) / base
+  ds = (dt - t * (r2 / base)) * inv;
+}
+
 __device__ inline void xr_shield(double r, double rc, double gamma, double& s, double& ds) {
   double t, dt;
   xr_taper(r, rc, t, dt);
-  const double base = pow(r, 3.0) + gamma;
-  const double den = pow(base, 1.0 / 3.0);
-  s = t / den;
-  // d/dr den = (1/3) base^(-2/3) 3 r^2 = r^2 den / base
-  ds = dt / den - t * (r * r / base) / den;
+  xr_shield_tapered(r, t, dt, gamma, s, ds);
 }

@@ -231,6 +231,12 @@ int xreac_ff_create(const xreac_ff_tables* t, xreac_ff** out) {
   if (e == hipSuccess) e = upload(ff, t->torsion_on, nt * nt * nt * nt, &ff->dev.tor_on);
   if (e == hipSuccess) e = upload(ff, t->hbond, nt * nt * nt * HB_W, &ff->dev.hb);
   if (e == hipSuccess) e = upload(ff, t->hbond_on, nt * nt * nt, &ff->dev.hb_on);
+  if (e == hipSuccess) {
+    // per-pair constant of the shielded vdW distance (r^p + (1/gamma_w)^p)^(1/p)
+    std::vector<double> b(nt * nt);
+    for (size_t k = 0; k < nt * nt; ++k) b[k] = std::pow(1.0 / t->pair[k * PP_COUNT + PP_GAMMAW], t->general[28]);
+    e = upload(ff, b.data(), nt * nt, &ff->dev.vdw_b);
+  }
   if (e != hipSuccess) {
     xreac_ff_destroy(ff);
     return e == hipErrorOutOfMemory ? XREAC_NOMEM : XREAC_DEVICE;
@@ -355,7 +361,28 @@ void bin_geometry(SysDev& sd, const double* height, const double* bounds, double
       g.lo[k] = bounds[k];
       g.reach[k] = (int)std::floor(rcb / (w * height[k])) + 1;
     }
+    g.thick[k] = g.w[k] * height[k];
   }
+  // Gershgorin bound on the largest eigenvalue of the Gram matrix of the unit
+  // plane normals (columns of the inverse cell): a displacement v has
+  // sum_k (v . n_k)^2 <= kappa |v|^2, which lets the search prune bins by
+  // their summed squared slab distances for any cell shape.
+  double n[3][3];
+  for (int k = 0; k < 3; ++k) {
+    const double c0 = sd.cell.inv[k], c1 = sd.cell.inv[3 + k], c2 = sd.cell.inv[6 + k];
+    const double len = std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    n[k][0] = c0 / len;
+    n[k][1] = c1 / len;
+    n[k][2] = c2 / len;
+  }
+  double kappa = 1.0;
+  for (int k = 0; k < 3; ++k) {
+    double row = 0.0;
+    for (int l = 0; l < 3; ++l)
+      row += l == k ? 1.0 : std::fabs(n[k][0] * n[l][0] + n[k][1] * n[l][1] + n[k][2] * n[l][2]);
+    kappa = std::max(kappa, row);
+  }
+  g.kappa = kappa * (1.0 + 1e-12);
 }
 
 // One attempt.  With exact == false, edge and bond buffers are sized from the

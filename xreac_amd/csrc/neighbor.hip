@@ -132,12 +132,31 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
   const int bz = id % g.nb[2], by = (id / g.nb[2]) % g.nb[1], bx = id / (g.nb[2] * g.nb[1]);
   const int hi0 = home[3 * i], hi1 = home[3 * i + 1], hi2 = home[3 * i + 2];
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
-  int zlo = bz - g.reach[2], zhi = bz + g.reach[2];
-  if (!g.pbc[2]) {
-    zlo = zlo < 0 ? 0 : zlo;
-    zhi = zhi >= g.nb[2] ? g.nb[2] - 1 : zhi;
+  // Position of atom i inside its bin (bin units, clamped to [0, 1]), the
+  // same arithmetic as the binning pass.
+  double a[3];
+  {
+    double f[3];
+    frac_of(cell, x, i, f);
+    const int bb[3] = {bx, by, bz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double u = g.pbc[k] ? (f[k] - floor(f[k])) / g.w[k] : (f[k] - g.lo[k]) / g.w[k];
+      a[k] = fmin(1.0, fmax(0.0, u - bb[k]));
+    }
   }
+  // Lower bound on the distance from atom i to the slab of bins at offset o
+  // along plane normal k (a small slack absorbs rounding of the bin
+  // assignment).  A bin whose three bounds give sum d_k^2 > kappa rc^2 holds
+  // no neighbor.
+  const double lim2 = g.kappa * rc2;
+  auto gap = [&](int k, int o) {
+    const double d = o > 0 ? o - a[k] : (o < 0 ? a[k] - o - 1 : 0.0);
+    return fmax(0.0, d * g.thick[k] - 1e-8);
+  };
   for (int o0 = -g.reach[0]; o0 <= g.reach[0]; ++o0) {
+    const double d0 = gap(0, o0);
+    if (d0 * d0 > lim2) continue;
     int c0 = bx + o0, t0 = 0;
     if (g.pbc[0]) {
       t0 = floordiv(c0, g.nb[0]);
@@ -146,12 +165,24 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
       continue;
     }
     for (int o1 = -g.reach[1]; o1 <= g.reach[1]; ++o1) {
+      const double d1 = gap(1, o1);
+      const double rem2 = lim2 - d0 * d0 - d1 * d1;
+      if (rem2 < 0) continue;
       int c1 = by + o1, t1 = 0;
       if (g.pbc[1]) {
         t1 = floordiv(c1, g.nb[1]);
         c1 -= t1 * g.nb[1];
       } else if (c1 < 0 || c1 >= g.nb[1]) {
         continue;
+      }
+      // z offsets o with gap(2, o)^2 <= rem2
+      const double rz = fmin((sqrt(rem2) + 1e-8) / g.thick[2], 1e6);
+      const int olo = -(int)floor(rz + 1 - a[2]), ohi = (int)floor(rz + a[2]);
+      int zlo = bz + (olo > -g.reach[2] ? olo : -g.reach[2]);
+      int zhi = bz + (ohi < g.reach[2] ? ohi : g.reach[2]);
+      if (!g.pbc[2]) {
+        zlo = zlo < 0 ? 0 : zlo;
+        zhi = zhi >= g.nb[2] ? g.nb[2] - 1 : zhi;
       }
       const int col = sd.bin0 + (c0 * g.nb[1] + c1) * g.nb[2];
       const int sx0 = g.pbc[0] ? t0 + hi0 : 0;

@@ -27,6 +27,7 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
     const double* g = ff.gen;
     const double rc = g[12];
     const double pw = g[28];
+    const double inv_pw = 1.0 / pw;
     const int ti = at.type[i];
     const double qi = at.q[i];
     const bool shielded = ff.vdw_type == 1 || ff.vdw_type == 3;
@@ -35,7 +36,8 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
     for (int ed = row_ptr[i] + lane; ed < row_ptr[i + 1]; ed += 64) {
       const int j = nbr_j[ed];
       const int tj = at.type[j];
-      const double* p = ff.pair + (ti * ff.nt + tj) * PP_COUNT;
+      const int pij = ti * ff.nt + tj;
+      const double* p = ff.pair + pij * PP_COUNT;
       double v[3];
       const double r = xr_edge(at.x, cell, i, j, nbr_s[ed], v);
       double t, dt;
@@ -43,10 +45,11 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
       if (t == 0.0 && dt == 0.0) continue;
       double f13, df13;
       if (shielded) {
-        const double rp = pow(r, pw);
-        const double base = rp + pow(1 / p[PP_GAMMAW], pw);
-        f13 = pow(base, 1 / pw);
-        df13 = rp / r * f13 / base;
+        // f13 = (r^p + (1/gamma_w)^p)^(1/p); d f13/dr = r^(p-1) f13 / base
+        const double rp = exp(pw * log(r));
+        const double base = rp + ff.vdw_b[pij];
+        f13 = exp(log(base) * inv_pw);
+        df13 = (rp * f13) / (r * base);
       } else {
         f13 = r;
         df13 = 1.0;
@@ -62,7 +65,7 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
         dphi += dt * ecr + t * ecr * (-p[PP_ACORE] / p[PP_RCORE]);
       }
       double s, ds;
-      xr_shield(r, rc, p[PP_GAMMA], s, ds);
+      xr_shield_tapered(r, t, dt, p[PP_GAMMA], s, ds);
       const double qj = at.q[j];
       const double qq = qi * qj;
       ev_sum += phi;
