@@ -377,19 +377,29 @@ __device__ inline void hbond_eval(const double* prm, double BOd, const double* v
   }
 }
 
-// One wave per hydrogen: donor-side energy and derivatives.
+constexpr int HB_QUEUE = 256;  // LDS slots per wave for H-bond candidate edges
+
+// One wave per hydrogen: donor-side energy and derivatives.  The row's
+// candidate edges (acceptor type within min(7.5, rc), flagged by the edge
+// pass) are queued first, so each donor bond evaluates them in dense batches.
 __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
                                                   const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
                                                   const int* __restrict__ nbr_s,
                                                   const unsigned char* __restrict__ eflag, EnergyAcc acc) {
+  __shared__ int queue[4][HB_QUEUE];
   const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  double etot[1] = {0.0};
-  if (j < at.n && AP(ff, at.type[j])[AP_HBOND] == 1.0) {
-    const CellDev& cell = xr_cell(at.S, j);
-    const int nt = ff.nt, tj = at.type[j];
-    const double hcut = fmin(7.5, ff.gen[12]);
-    double F[3] = {0, 0, 0};
+  if (j >= at.n || AP(ff, at.type[j])[AP_HBOND] != 1.0) return;
+  int* q = queue[threadIdx.x >> 6];
+  const CellDev& cell = xr_cell(at.S, j);
+  const int nt = ff.nt, tj = at.type[j];
+  const double hcut = fmin(7.5, ff.gen[12]);
+  double etot = 0.0;
+  double F[3] = {0, 0, 0};
+  int e = row_ptr[j];
+  const int end = row_ptr[j + 1];
+  while (e < end) {
+    const int count = xr_queue_edges(q, HB_QUEUE, e, end, [&](int x) { return (eflag[x] & EF_HB_DONOR) != 0; });
     for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
       const int kd = bd.bj[d];
       if (AP(ff, at.type[kd])[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
@@ -398,18 +408,18 @@ __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
       const int ed = bd.edge[d];
       const int kbase = (at.type[kd] * nt + tj) * nt;
       double Gd[3] = {0, 0, 0}, dBO = 0.0;
-      for (int e = row_ptr[j] + lane; e < row_ptr[j + 1]; e += 64) {
-        // candidates (acceptor type, r <= min(7.5, rc)) were flagged by the edge pass
-        if (!(eflag[e] & EF_HB_DONOR) || e == ed) continue;
-        const int k = nbr_j[e];
+      for (int t = lane; t < count; t += 64) {
+        const int c = q[t];
+        if (c == ed) continue;
+        const int k = nbr_j[c];
         const int tk = at.type[k];
         if (!ff.hb_on[kbase + tk]) continue;
         double va[3];
-        const double ra = xr_edge(at.x, cell, j, k, nbr_s[e], va);
+        const double ra = xr_edge(at.x, cell, j, k, nbr_s[c], va);
         if (!(ra <= hcut)) continue;
         HbTerm o;
         hbond_eval(ff.hb + (kbase + tk) * HB_W, BOd, vd, rd, va, ra, o);
-        etot[0] += o.E;
+        etot += o.E;
         dBO += o.dBO;
 #pragma unroll
         for (int k2 = 0; k2 < 3; ++k2) {
@@ -426,53 +436,64 @@ __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
         for (int k2 = 0; k2 < 3; ++k2) bd.G[3 * d + k2] += Gd[k2];
       }
     }
+    xr_wave_sync();  // the queue is read before it is refilled
+  }
 #pragma unroll
-    for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
-    etot[0] = xr_wave_sum(etot[0]);
-    if (lane == 0) {
+  for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
+  etot = xr_wave_sum(etot);
+  if (lane == 0) {
 #pragma unroll
-      for (int k2 = 0; k2 < 3; ++k2) at.F_hb[3 * j + k2] += F[k2];
-      xr_energy(acc, CH_HB, j, etot[0]);
-    }
+    for (int k2 = 0; k2 < 3; ++k2) at.F_hb[3 * j + k2] += F[k2];
+    xr_energy(acc, CH_HB, j, etot);
   }
 }
 
-// One wave per heavy atom: force from the acceptor-vector derivative.
+// One wave per heavy atom: force from the acceptor-vector derivative.  The
+// H-bond hydrogens within min(7.5, rc) (flagged by the edge pass) are queued,
+// then each lane takes one hydrogen and walks its donor bonds.
 __global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, FFDev ff,
                                                      const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
                                                      const int* __restrict__ nbr_s,
                                                      const unsigned char* __restrict__ eflag) {
+  __shared__ int queue[4][HB_QUEUE];
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (k >= at.n) return;
   const int tk = at.type[k];
   if (AP(ff, tk)[AP_HBOND] != 2.0) return;
+  int* q = queue[threadIdx.x >> 6];
   const CellDev& cell = xr_cell(at.S, k);
   const int nt = ff.nt;
   const double hcut = fmin(7.5, ff.gen[12]);
   double F[3] = {0, 0, 0};
-  for (int e = row_ptr[k] + lane; e < row_ptr[k + 1]; e += 64) {
-    if (!(eflag[e] & EF_HB_ACCEPTOR)) continue;  // H-bond hydrogen within min(7.5, rc), flagged by the edge pass
-    const int j = nbr_j[e];
-    const int tj = at.type[j];
-    const int code = nbr_s[e];
-    double vk[3];
-    const double ra = xr_edge(at.x, cell, k, j, code, vk);
-    if (!(ra <= hcut)) continue;
-    const double va[3] = {-vk[0], -vk[1], -vk[2]};
-    const int back = xr_neg_shift(code);
-    for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
-      const int kd = bd.bj[d];
-      const int td = at.type[kd];
-      if (AP(ff, td)[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
-      if (kd == k && bd.code[d] == back) continue;  // the acceptor edge is the donor edge
-      const int key = (td * nt + tj) * nt + tk;
-      if (!ff.hb_on[key]) continue;
-      HbTerm o;
-      hbond_eval(ff.hb + key * HB_W, bd.BO[d], bd.vec + 3 * d, bd.r[d], va, ra, o);
+  int e = row_ptr[k];
+  const int end = row_ptr[k + 1];
+  while (e < end) {
+    const int count = xr_queue_edges(q, HB_QUEUE, e, end, [&](int x) { return (eflag[x] & EF_HB_ACCEPTOR) != 0; });
+    for (int t = lane; t < count; t += 64) {
+      const int c = q[t];
+      const int j = nbr_j[c];
+      const int tj = at.type[j];
+      const int code = nbr_s[c];
+      double vk[3];
+      const double ra = xr_edge(at.x, cell, k, j, code, vk);
+      if (!(ra <= hcut)) continue;
+      const double va[3] = {-vk[0], -vk[1], -vk[2]};
+      const int back = xr_neg_shift(code);
+      for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
+        const int kd = bd.bj[d];
+        const int td = at.type[kd];
+        if (AP(ff, td)[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
+        if (kd == k && bd.code[d] == back) continue;  // the acceptor edge is the donor edge
+        const int key = (td * nt + tj) * nt + tk;
+        if (!ff.hb_on[key]) continue;
+        HbTerm o;
+        hbond_eval(ff.hb + key * HB_W, bd.BO[d], bd.vec + 3 * d, bd.r[d], va, ra, o);
 #pragma unroll
-      for (int k2 = 0; k2 < 3; ++k2) F[k2] -= o.Ga[k2];
+        for (int k2 = 0; k2 < 3; ++k2) F[k2] -= o.Ga[k2];
+      }
     }
+    xr_wave_sync();  // the queue is read before it is refilled
   }
 #pragma unroll
   for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
