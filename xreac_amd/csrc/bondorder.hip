@@ -441,11 +441,16 @@ hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, FFDev ff, const int
   return hipSuccess;
 }
 
-hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, double* forces, hipStream_t s) {
+hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, hipStream_t s) {
   hipLaunchKernelGGL(k_atom_backward, grid(at.n), dim3(BLK), 0, s, at, bd, ff);
   if (bd.cap > 0) hipLaunchKernelGGL(k_bond_backward1, grid(bd.cap), dim3(BLK), 0, s, at, bd, ff);
   hipLaunchKernelGGL(k_atom_backward2, grid(at.n), dim3(BLK), 0, s, at, bd);
   if (bd.cap > 0) hipLaunchKernelGGL(k_bond_backward2, grid(bd.cap), dim3(BLK), 0, s, at, bd);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_force_assembly(const Atoms& at, const Bonds& bd, double* forces, hipStream_t s) {
   hipLaunchKernelGGL(k_force, grid(at.n), dim3(BLK), 0, s, at, bd, forces);
   XR_LAUNCH_CHECK();
   return hipSuccess;

@@ -160,6 +160,14 @@ __device__ inline double xr_wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, XR_WAVE);
   return v;  // valid in lane 0
 }
+// Sum over aligned groups of G lanes (G a power of two <= 64), valid in the
+// group's first lane; the same tree as xr_wave_sum restricted to the group.
+template <int G>
+__device__ inline double xr_group_sum(double v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_down(v, off, G);
+  return v;
+}
 __device__ inline int xr_wave_sum_int(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, XR_WAVE);

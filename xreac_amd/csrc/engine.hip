@@ -95,6 +95,14 @@ struct xreac_ctx {
   int last_cg_iters = 0;
   double m_per_atom = 0.0;  // edges and bonds per atom of the last evaluation
   double b_per_atom = 0.0;
+  int max_row = 0;  // longest neighbor row of the last built list
+  // Second stream (default priority) for the charge-independent bonded terms,
+  // which run while the QEq solve occupies the context's stream.
+  hipStream_t side = nullptr;
+  hipEvent_t bonds_ready = nullptr, side_done = nullptr;
+  // QEq read-back: pinned slots and their copy events
+  int* poll_host = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   // last evaluation
   int n = 0;
   int64_t m = 0, nb = 0;
@@ -263,6 +271,16 @@ int xreac_ctx_create(int device, void* stream, xreac_ctx** out) {
 
 void xreac_ctx_destroy(xreac_ctx* c) {
   if (!c) return;
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
+  if (c->bonds_ready) (void)hipEventDestroy(c->bonds_ready);
+  if (c->side_done) (void)hipEventDestroy(c->side_done);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (hipEvent_t ev : c->poll_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->poll_host) (void)hipHostFree(c->poll_host);
   if (c->ctl_host) (void)hipHostFree(c->ctl_host);
   delete c;
 }
@@ -597,7 +615,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   at.q = out.charges;
   EnergyAcc acc{energies, n};
 
-  int* dev_counts = reinterpret_cast<int*>(d_flags) + 1;  // [0] total edges, [1] total bonds
+  int* dev_counts = reinterpret_cast<int*>(d_flags) + 1;  // [0] total edges, [1] total bonds, [2] longest row
   {
     ZeroRanges z{};
     auto add = [&](void* p, size_t bytes) {
@@ -626,8 +644,10 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // A supplied list still gets the count pass: it checks every pair for
   // coincident atoms, as validate_input does whatever list is supplied.
   const bool supplied = in.nbr_row_ptr != nullptr;
-  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, supplied ? -1 : 2048, counts, s),
+  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, supplied ? -1 : 2048, counts,
+                           dev_counts + 2, s),
          "neighbor count");
+  int max_row = ctx->max_row;
   int64_t m_cap = 0;
   if (supplied) {
     m_cap = in.nbr_count;
@@ -642,10 +662,11 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
     HIPCHK(clamp_ptr(row_ptr, n, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags, s), "clamp");
     if (exact) {
-      int m_host = 0;
-      HIPCHK(hipMemcpyAsync(&m_host, dev_counts, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
+      int h[3] = {0, 0, 0};
+      HIPCHK(hipMemcpyAsync(h, dev_counts, sizeof(h), hipMemcpyDeviceToHost, s), "copy");
       HIPCHK(hipStreamSynchronize(s), "neighbor sync");
-      m_cap = m_host;
+      m_cap = h[0];
+      max_row = h[2];
     }
   }
 
@@ -665,7 +686,11 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   if (supplied) {
     HIPCHK(xr_pack_supplied(n, S, row_ptr, in.nbr_j, in.nbr_shifts, nbr_j, nbr_s, s), "supplied neighbors");
   } else {
-    HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s, s),
+    // The fill instance is sized from the longest row: known in the exact
+    // pass, else the last evaluation's plus a margin.  A longer row makes the
+    // instance raise the capacity bit and the evaluation is repeated exactly.
+    HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s,
+                            exact ? max_row : (max_row > 0 ? max_row + 64 : 2048), d_flags, s),
            "neighbor fill");
   }
 
@@ -745,6 +770,32 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   if (bs > bs_end) return fail(ctx, XREAC_INVALID, "internal error: per-bond workspace layout");
   if (nbond > 0) HIPCHK(xr_bond_fill(n, row_ptr, bflag, bond_ptr, (int)nbond, b_edge, b_i, s), "bond fill");
 
+  // ------------------------------------------- bonded terms on the side stream
+  // Bond orders, many-body terms and their adjoints depend on positions only;
+  // they run on a second stream while QEq runs here, and the force assembly
+  // waits for them.  Every return below joins the side stream first.
+  if (!ctx->side) {
+    HIPCHK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "side stream");
+    HIPCHK(hipEventCreateWithFlags(&ctx->bonds_ready, hipEventDisableTiming), "event");
+    HIPCHK(hipEventCreateWithFlags(&ctx->side_done, hipEventDisableTiming), "event");
+    for (hipEvent_t& ev : ctx->poll_ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&ctx->poll_host), 16 * sizeof(int)), "pinned poll slots");
+  }
+  hipStream_t s2 = ctx->side;
+  HIPCHK(hipEventRecord(ctx->bonds_ready, s), "event");
+  HIPCHK(hipStreamWaitEvent(s2, ctx->bonds_ready, 0), "event");
+  {
+    hipError_t e = hipSuccess;
+    XR_TIMED(XT_BOND_FORWARD, s2, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s2));
+    HIPCHK(e, "bond orders");
+    XR_TIMED(XT_MANYBODY, s2, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, bflag, lammps, acc, s2));
+    HIPCHK(e, "many-body");
+    XR_TIMED(XT_BACKWARD, s2, e = xr_bond_backward(at, bd, ff, s2));
+    HIPCHK(e, "adjoint");
+  }
+  HIPCHK(hipEventRecord(ctx->side_done, s2), "event");
+  auto join_side = [&]() { return hipStreamWaitEvent(s, ctx->side_done, 0); };
+
   // ------------------------------------------------------------------ QEq
   CgArgs cg{};
   cg.S = S;
@@ -774,15 +825,20 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.max_iter = opt->qeq_max_iterations > 0 ? opt->qeq_max_iterations : 2000;
   // Read back convergence first near the previous solve's iteration count.
   cg.first_check = ctx->last_cg_iters > 0 ? ctx->last_cg_iters + 2 : 16;
-  cg.check_every = 4;
+  cg.check_every = 2;
   cg.guess = (opt->warm_start && out.qeq_state) ? reinterpret_cast<const double2*>(out.qeq_state) : nullptr;
   cg.keep = reinterpret_cast<double2*>(out.qeq_state);
   cg.q = out.charges;
   cg.mu = reinterpret_cast<double*>(dc + L.mu);
   cg.abort_flag = d_flags;
+  cg.poll_host = ctx->poll_host;
+  cg.poll_ev[0] = ctx->poll_ev[0];
+  cg.poll_ev[1] = ctx->poll_ev[1];
   if (!given_q) {
     CgOutcome cgo{};
-    HIPCHK(xr_cg_solve(cg, s, &cgo), "QEq");
+    const hipError_t e = xr_cg_solve(cg, s, &cgo);
+    HIPCHK(join_side(), "event");
+    HIPCHK(e, "QEq");
     if (cgo.aborted) {
       *overflow = true;
       return XREAC_OK;
@@ -791,15 +847,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
 
   // ----------------------------------------------------- energy and forces
   HIPCHK(xr_nonbonded(at, ff, row_ptr, nbr_j, nbr_s, acc, dEdq, s), "nonbonded");
-  {
-    hipError_t e = hipSuccess;
-    XR_TIMED(XT_BOND_FORWARD, s, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s));
-    HIPCHK(e, "bond orders");
-    XR_TIMED(XT_MANYBODY, s, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, bflag, lammps, acc, s));
-    HIPCHK(e, "many-body");
-    XR_TIMED(XT_BACKWARD, s, e = xr_bond_backward(at, bd, ff, out.forces, s));
-    HIPCHK(e, "adjoint");
-  }
+  HIPCHK(join_side(), "event");
+  HIPCHK(xr_force_assembly(at, bd, out.forces, s), "forces");
   if (full) {
     // Charge response: neutral adjoint KKT solve with right-hand sides (dE/dq, -1).
     CgArgs adj = cg;
@@ -861,6 +910,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   out.num_bonds = nb_total;
   ctx->m_per_atom = (double)m / n;
   ctx->b_per_atom = (double)nb_total / n;
+  if (!supplied) ctx->max_row = host_counts[2];
   ctx->last_cg_iters = max_iters;
 
   ctx->n = n;

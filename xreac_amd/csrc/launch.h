@@ -16,9 +16,9 @@ hipError_t xr_bin_scatter(int n, const double* x, const Systems& S, const int* h
                           const int* bin_start, int* bin_fill, void* recs, hipStream_t s);
 hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int* home, const int* bin,
                              const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
-                             hipStream_t s);
+                             int* max_row, hipStream_t s);
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
-                            int* nbr_s, hipStream_t s);
+                            int* nbr_s, int max_row, unsigned* flags, hipStream_t s);
 hipError_t xr_pack_supplied(int n, const Systems& S, const int* row_ptr, const int* in_j, const int* in_s,
                             int* nbr_j, int* nbr_s, hipStream_t s);

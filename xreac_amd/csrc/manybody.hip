@@ -247,21 +247,29 @@ __device__ inline int torsion_key(const Atoms& at, const Bonds& bd, const FFDev&
   return key;
 }
 
-// One wave per atom x.  Every torsion contribution that lands on a bond of
-// x's own row goes to the lane owning that slot: as atom j of a half middle
-// bond m, x owns the left bonds a (lane of a loops over the right bonds c);
-// as atom k, x owns the right bonds c (lane of c loops over the left bonds
-// a).  The middle-bond terms of the j role and the energies are
-// wave-reduced.  Each slot has one writer and the order is fixed, so the
-// result is deterministic; rows longer than 64 bonds are taken in chunks.
+// One group of G lanes per atom x: G = TOR_GROUP for atoms with at most
+// TOR_GROUP bonds (several atoms per wave), G = 64 for the rest.  Every
+// torsion contribution that lands on a bond of x's own row goes to the lane
+// owning that slot: as atom j of a half middle bond m, x owns the left bonds
+// a (lane of a loops over the right bonds c); as atom k, x owns the right
+// bonds c (lane of c loops over the left bonds a).  The middle-bond terms of
+// the j role and the energies are reduced over the group.  Each slot has one
+// writer and the order is fixed, and which instance takes an atom depends
+// only on its own bond count, so the result is deterministic and the same
+// for a system alone or in a batch; rows longer than 64 bonds are taken in
+// chunks.
+constexpr int TOR_GROUP = 8;
+template <int G>
 __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc) {
-  const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
+  const int x = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / G) + ((threadIdx.x & 63) / G);
+  const int lane = threadIdx.x & (G - 1);
   if (x >= at.n) return;
   const int p0 = bd.ptr[x], p1 = bd.ptr[x + 1];
+  // atoms with at most TOR_GROUP bonds go to the 8-lane groups, the rest to whole waves
+  if (G < 64 ? p1 - p0 > G : p1 - p0 <= TOR_GROUP) return;
   double e_tor = 0.0, e_conj = 0.0, a_db = 0.0;
   unsigned err = 0;
-  for (int chunk = p0; chunk < p1; chunk += 64) {
+  for (int chunk = p0; chunk < p1; chunk += G) {
     const int mine = chunk + lane;
     const bool owns = mine < p1;
     double g0 = 0.0, g1 = 0.0, g2 = 0.0, abo = 0.0;
@@ -301,11 +309,11 @@ __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, 
             a_db += o.dDS;
           }
         }
-        gm0 = xr_wave_sum(gm0);
-        gm1 = xr_wave_sum(gm1);
-        gm2 = xr_wave_sum(gm2);
-        abm = xr_wave_sum(abm);
-        apim = xr_wave_sum(apim);
+        gm0 = xr_group_sum<G>(gm0);
+        gm1 = xr_group_sum<G>(gm1);
+        gm2 = xr_group_sum<G>(gm2);
+        abm = xr_group_sum<G>(abm);
+        apim = xr_group_sum<G>(apim);
         if (lane == 0) {
           bd.G[3 * m] += gm0;
           bd.G[3 * m + 1] += gm1;
@@ -338,11 +346,11 @@ __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, 
       bd.A_BO[mine] += abo;
     }
   }
-  e_tor = xr_wave_sum(e_tor);
-  e_conj = xr_wave_sum(e_conj);
-  a_db = xr_wave_sum(a_db);
+  e_tor = xr_group_sum<G>(e_tor);
+  e_conj = xr_group_sum<G>(e_conj);
+  a_db = xr_group_sum<G>(a_db);
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) err |= __shfl_down(err, off, 64);
+  for (int off = G / 2; off > 0; off >>= 1) err |= __shfl_down(err, off, G);
   if (lane == 0) {
     at.A_db[x] += a_db;
     if (err) xr_flag(at.S, x, err);
@@ -515,7 +523,10 @@ hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* ro
                        hipStream_t s) {
   hipLaunchKernelGGL(k_angles, grid(at.n), dim3(BLK), 0, s, at, bd, ff, lammps_angles, acc);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  hipLaunchKernelGGL(k_torsions, wgrid, dim3(256), 0, s, at, bd, ff, acc);
+  const int per_block = 4 * (64 / TOR_GROUP);
+  hipLaunchKernelGGL(k_torsions<TOR_GROUP>, dim3(at.n > 0 ? (at.n + per_block - 1) / per_block : 1), dim3(256), 0,
+                     s, at, bd, ff, acc);
+  hipLaunchKernelGGL(k_torsions<64>, wgrid, dim3(256), 0, s, at, bd, ff, acc);
   hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag, acc);
   hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag);
   XR_LAUNCH_CHECK();

@@ -115,22 +115,25 @@ __global__ void k_bin_scatter(int n, const double* __restrict__ x, const int* __
   recs[bin_start[b] + atomicAdd(&bin_fill[b], 1)] = r;
 }
 
-// Visit every candidate image (j, T) of atom i in the surrounding bins; calls
-// fn(j, packed_shift, r2) for those passing the exact distance test.  Bins
-// with consecutive z index are contiguous in `binned`, so for each (x, y)
-// offset the z-range is walked as one or a few contiguous segments (split only
-// where the periodic image shift changes); lanes stride over each segment
-// (lane = t, stride = nthreads).
+// Visit every candidate image (j, T) of atom i in the bins around it and call
+// fn(hit, j, packed_shift, r2) for each.  Bins that can hold a neighbor (the
+// slab-distance bound below) are taken 64 at a time, one per lane: each lane
+// reads its bin's record range, a wave prefix sum lays the ranges end to end,
+// and the lanes stride over the joined list, so one round of 64 candidates
+// spans several bins and one latency of range loads serves 64 bins.  Every
+// lane calls fn in every round (hit = false past the end), so fn may use
+// wave-wide ballots.
 template <class Fn>
 __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, const SysDev& sd,
                                          const int* __restrict__ home, const int* __restrict__ bin,
                                          const int* __restrict__ bin_start, const BinRec* __restrict__ recs,
-                                         double rc2, int t, int nthreads, unsigned& err, Fn fn) {
+                                         double rc2, unsigned& err, Fn fn) {
+  const int lane = threadIdx.x & 63;
   const CellDev& cell = sd.cell;
   const BinGeom& g = sd.g;
   const int id = bin[i] - sd.bin0;
-  const int bz = id % g.nb[2], by = (id / g.nb[2]) % g.nb[1], bx = id / (g.nb[2] * g.nb[1]);
-  const int hi0 = home[3 * i], hi1 = home[3 * i + 1], hi2 = home[3 * i + 2];
+  const int bb[3] = {id / (g.nb[2] * g.nb[1]), (id / g.nb[2]) % g.nb[1], id % g.nb[2]};
+  const int hi[3] = {home[3 * i], home[3 * i + 1], home[3 * i + 2]};
   const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
   // Position of atom i inside its bin (bin units, clamped to [0, 1]), the
   // same arithmetic as the binning pass.
@@ -138,100 +141,92 @@ __device__ inline void for_each_neighbor(int i, const double* __restrict__ x, co
   {
     double f[3];
     frac_of(cell, x, i, f);
-    const int bb[3] = {bx, by, bz};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const double u = g.pbc[k] ? (f[k] - floor(f[k])) / g.w[k] : (f[k] - g.lo[k]) / g.w[k];
       a[k] = fmin(1.0, fmax(0.0, u - bb[k]));
     }
   }
-  // Lower bound on the distance from atom i to the slab of bins at offset o
-  // along plane normal k (a small slack absorbs rounding of the bin
-  // assignment).  A bin whose three bounds give sum d_k^2 > kappa rc^2 holds
-  // no neighbor.
+  // Lower bound on the distance from atom i to the bins at offset o along
+  // plane normal k (a small slack absorbs rounding of the bin assignment).  A
+  // bin whose bounds give sum_k d_k^2 > kappa rc^2 holds no neighbor.
   const double lim2 = g.kappa * rc2;
-  auto gap = [&](int k, int o) {
-    const double d = o > 0 ? o - a[k] : (o < 0 ? a[k] - o - 1 : 0.0);
-    return fmax(0.0, d * g.thick[k] - 1e-8);
-  };
-  for (int o0 = -g.reach[0]; o0 <= g.reach[0]; ++o0) {
-    const double d0 = gap(0, o0);
-    if (d0 * d0 > lim2) continue;
-    int c0 = bx + o0, t0 = 0;
-    if (g.pbc[0]) {
-      t0 = floordiv(c0, g.nb[0]);
-      c0 -= t0 * g.nb[0];
-    } else if (c0 < 0 || c0 >= g.nb[0]) {
-      continue;
+  const int span1 = 2 * g.reach[1] + 1, span2 = 2 * g.reach[2] + 1;
+  const int nbins = (2 * g.reach[0] + 1) * span1 * span2;
+  for (int b0 = 0; b0 < nbins; b0 += 64) {
+    // this lane's bin: candidate range [start, start + len) and image base
+    const int q = b0 + lane;
+    int start = 0, len = 0;
+    int base[3] = {0, 0, 0};
+    if (q < nbins) {
+      const int o[3] = {q / (span1 * span2) - g.reach[0], (q / span2) % span1 - g.reach[1], q % span2 - g.reach[2]};
+      double d2 = 0.0;
+      int c[3];
+      bool ok = true;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double d = o[k] > 0 ? o[k] - a[k] : (o[k] < 0 ? a[k] - o[k] - 1 : 0.0);
+        const double gk = fmax(0.0, d * g.thick[k] - 1e-8);
+        d2 += gk * gk;
+        c[k] = bb[k] + o[k];
+        if (g.pbc[k]) {
+          const int t = floordiv(c[k], g.nb[k]);
+          c[k] -= t * g.nb[k];
+          base[k] = t + hi[k];
+        } else if (c[k] < 0 || c[k] >= g.nb[k]) {
+          ok = false;
+        }
+      }
+      if (ok && d2 <= lim2) {
+        const int cell_id = sd.bin0 + (c[0] * g.nb[1] + c[1]) * g.nb[2] + c[2];
+        start = bin_start[cell_id];
+        len = bin_start[cell_id + 1] - start;
+      }
     }
-    for (int o1 = -g.reach[1]; o1 <= g.reach[1]; ++o1) {
-      const double d1 = gap(1, o1);
-      const double rem2 = lim2 - d0 * d0 - d1 * d1;
-      if (rem2 < 0) continue;
-      int c1 = by + o1, t1 = 0;
-      if (g.pbc[1]) {
-        t1 = floordiv(c1, g.nb[1]);
-        c1 -= t1 * g.nb[1];
-      } else if (c1 < 0 || c1 >= g.nb[1]) {
-        continue;
-      }
-      // z offsets o with gap(2, o)^2 <= rem2
-      const double rz = fmin((sqrt(rem2) + 1e-8) / g.thick[2], 1e6);
-      const int olo = -(int)floor(rz + 1 - a[2]), ohi = (int)floor(rz + a[2]);
-      int zlo = bz + (olo > -g.reach[2] ? olo : -g.reach[2]);
-      int zhi = bz + (ohi < g.reach[2] ? ohi : g.reach[2]);
-      if (!g.pbc[2]) {
-        zlo = zlo < 0 ? 0 : zlo;
-        zhi = zhi >= g.nb[2] ? g.nb[2] - 1 : zhi;
-      }
-      const int col = sd.bin0 + (c0 * g.nb[1] + c1) * g.nb[2];
-      const int sx0 = g.pbc[0] ? t0 + hi0 : 0;
-      const int sy0 = g.pbc[1] ? t1 + hi1 : 0;
-      int zz = zlo;
-      while (zz <= zhi) {
-        // one segment: consecutive z bins sharing the same image shift t2
-        const int t2 = g.pbc[2] ? floordiv(zz, g.nb[2]) : 0;
-        const int cz0 = zz - t2 * g.nb[2];
-        int last = zhi;
-        if (g.pbc[2]) {
-          const int seg_end = (t2 + 1) * g.nb[2] - 1;
-          last = last < seg_end ? last : seg_end;
+    // inclusive prefix of the range lengths over the lanes
+    int incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const int count = __shfl(incl, 63, 64);
+    const int excl = incl - len;
+    for (int r0 = 0; r0 < count; r0 += 64) {
+      const int f = r0 + lane;
+      // owner lane: the number of lanes whose ranges end at or before f
+      int sl = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(incl, sl + step - 1, 64) <= f) sl += step;
+      sl = sl < 63 ? sl : 63;
+      const int p = __shfl(start, sl, 64) + (f - __shfl(excl, sl, 64));
+      const int s0 = __shfl(base[0], sl, 64), s1 = __shfl(base[1], sl, 64), s2 = __shfl(base[2], sl, 64);
+      bool hit = false;
+      int j = 0, code = 0;
+      double r2 = 0.0;
+      if (f < count) {
+        const BinRec rec = recs[p];
+        j = rec.atom;
+        int hj0, hj1, hj2;
+        xr_unpack_shift(rec.home, hj0, hj1, hj2);
+        const int sx = g.pbc[0] ? s0 - hj0 : 0;
+        const int sy = g.pbc[1] ? s1 - hj1 : 0;
+        const int sz = g.pbc[2] ? s2 - hj2 : 0;
+        if (sx < -511 || sx > 511 || sy < -511 || sy > 511 || sz < -511 || sz > 511) {
+          err |= XR_ERR_SHIFT_RANGE;
+        } else {
+          code = xr_pack_shift(sx, sy, sz);
+          double off[3];
+          xr_offset(cell, code, off);
+          const double v0 = (rec.x - xi0) + off[0];
+          const double v1 = (rec.y - xi1) + off[1];
+          const double v2 = (rec.z - xi2) + off[2];
+          r2 = (v0 * v0 + v1 * v1) + v2 * v2;
+          hit = r2 <= rc2 && !(j == i && sx == 0 && sy == 0 && sz == 0);
         }
-        const int cz1 = last - t2 * g.nb[2];
-        const int s = bin_start[col + cz0], e = bin_start[col + cz1 + 1];
-        const int sz0 = g.pbc[2] ? t2 + hi2 : 0;
-        // Every lane takes part in every round (hit = false past the end),
-        // so fn may use wave-wide ballots.
-        for (int base = s; base < e; base += nthreads) {
-          const int p = base + t;
-          bool hit = false;
-          int j = 0, code = 0;
-          double r2 = 0.0;
-          if (p < e) {
-            const BinRec rec = recs[p];
-            j = rec.atom;
-            int hj0, hj1, hj2;
-            xr_unpack_shift(rec.home, hj0, hj1, hj2);
-            const int sx = g.pbc[0] ? sx0 - hj0 : 0;
-            const int sy = g.pbc[1] ? sy0 - hj1 : 0;
-            const int sz = g.pbc[2] ? sz0 - hj2 : 0;
-            if (sx < -511 || sx > 511 || sy < -511 || sy > 511 || sz < -511 || sz > 511) {
-              err |= XR_ERR_SHIFT_RANGE;
-            } else {
-              code = xr_pack_shift(sx, sy, sz);
-              double off[3];
-              xr_offset(cell, code, off);
-              const double v0 = (rec.x - xi0) + off[0];
-              const double v1 = (rec.y - xi1) + off[1];
-              const double v2 = (rec.z - xi2) + off[2];
-              r2 = (v0 * v0 + v1 * v1) + v2 * v2;
-              hit = r2 <= rc2 && !(j == i && sx == 0 && sy == 0 && sz == 0);
-            }
-          }
-          fn(hit, j, code, r2);
-        }
-        zz = last + 1;
       }
+      fn(hit, j, code, r2);
     }
   }
 }
@@ -246,7 +241,7 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
   const int sys = __builtin_amdgcn_readfirstlane(S.asys[i]);
   unsigned err = 0;
   int cnt = 0;
-  for_each_neighbor(i, x, S.sys[sys], home, bin, bin_start, binned, rc2, lane, 64, err,
+  for_each_neighbor(i, x, S.sys[sys], home, bin, bin_start, binned, rc2, err,
                     [&](bool hit, int j, int, double r2) {
                       if (hit) {
                         ++cnt;
@@ -260,6 +255,24 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
     counts[i] = cnt;
     if (cnt > row_cap) err |= XR_ERR_ROW_TOO_LONG;
     if (err) atomicOr(&S.err[sys], err);
+  }
+}
+
+// Longest row of the list (one block; the fill instance is chosen from it).
+__global__ void __launch_bounds__(1024) k_row_max(int n, const int* __restrict__ counts, int* __restrict__ out) {
+  __shared__ int red[16];
+  int m = 0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) m = counts[t] > m ? counts[t] : m;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_down(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] > m ? red[w] : m;
+    *out = m;
   }
 }
 
@@ -309,35 +322,42 @@ __device__ inline void wave_sort(unsigned long long* s, int count) {
   for (int k = 0; k < K; ++k) s[lane * K + k] = v[k];
 }
 
+// Rows are staged in LDS of CAP keys: the 512-key (4 KB) and 1024-key (8 KB)
+// instances sort in registers and leave room for many waves per CU; a launch
+// that meets a row longer than its CAP raises the capacity bit, and the
+// evaluation is repeated with the instance the longest row needs.
+template <int CAP>
 __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, Systems S,
                                                  const int* __restrict__ home, const int* __restrict__ bin,
                                                  const int* __restrict__ bin_start, const BinRec* __restrict__ binned,
                                                  double rc2, const int* __restrict__ row_ptr, int* __restrict__ nbr_j,
-                                                 int* __restrict__ nbr_s) {
-  __shared__ unsigned long long keys[ROW_CAP];
+                                                 int* __restrict__ nbr_s, unsigned* __restrict__ flags) {
+  __shared__ unsigned long long keys[CAP];
   const int i = blockIdx.x;
   if (i >= n) return;
   const int lane = threadIdx.x & 63;
   const int begin = row_ptr[i], count = row_ptr[i + 1] - begin;
-  if (count > ROW_CAP) {
-    // Reported by the count pass.  The row still gets in-range entries (the
-    // atom itself, zero shift) so later kernels index valid memory; the
-    // system's results are discarded.
+  if (count > CAP) {
+    // Over ROW_CAP: reported by the count pass and the system's results are
+    // discarded; over CAP only: the evaluation is repeated with the long-row
+    // instance.  Either way the row gets in-range entries (the atom itself,
+    // zero shift) so later kernels index valid memory.
     for (int t2 = lane; t2 < count; t2 += 64) {
       nbr_j[begin + t2] = i;
       nbr_s[begin + t2] = xr_pack_shift(0, 0, 0);
     }
+    if (count <= ROW_CAP && lane == 0) atomicOr(flags, (unsigned)XR_ERR_CAPACITY);
     return;
   }
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   unsigned err = 0;
   int cursor = 0;  // identical in every lane
-  for_each_neighbor(i, x, S.sys[S.asys[i]], home, bin, bin_start, binned, rc2, lane, 64, err,
+  for_each_neighbor(i, x, S.sys[S.asys[i]], home, bin, bin_start, binned, rc2, err,
                     [&](bool hit, int j, int code, double) {
                       const unsigned long long m = __ballot(hit);
                       if (hit) {
                         const int p = cursor + __popcll(m & below);
-                        if (p < ROW_CAP) keys[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
+                        if (p < CAP) keys[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
                       }
                       cursor += __popcll(m);
                     });
@@ -348,7 +368,9 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
     wave_sort<4>(keys, count);
   } else if (count <= 512) {
     wave_sort<8>(keys, count);
-  } else {
+  } else if constexpr (CAP == 1024) {
+    wave_sort<16>(keys, count);
+  } else if constexpr (CAP > 1024) {
     // Long rows: bitonic network in LDS.
     int size = 1024;
     while (size < count) size <<= 1;
@@ -430,20 +452,30 @@ hipError_t xr_bin_scatter(int n, const double* x, const Systems& S, const int* h
 
 hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int* home, const int* bin,
                              const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
-                             hipStream_t s) {
+                             int* max_row, hipStream_t s) {
   // row_cap < 0: count only (coincidence and image checks), no row-length limit
   const int cap = row_cap < 0 ? 0x7fffffff : (row_cap < ROW_CAP ? row_cap : ROW_CAP);
   XR_LAUNCH(XT_NBR_COUNT, k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, S, home, bin, bin_start,
             static_cast<const BinRec*>(binned), cutoff * cutoff, cap, counts);
+  XR_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_row_max, dim3(1), dim3(1024), 0, s, n, counts, max_row);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
 
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
-                            int* nbr_s, hipStream_t s) {
-  XR_LAUNCH(XT_NBR_FILL, k_nbr_fill, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
-            static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s);
+                            int* nbr_s, int max_row, unsigned* flags, hipStream_t s) {
+  if (max_row <= 512) {
+    XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<512>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
+              static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s, flags);
+  } else if (max_row <= 1024) {
+    XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<1024>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
+              static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s, flags);
+  } else {
+    XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<ROW_CAP>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
+              static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s, flags);
+  }
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -19,7 +19,7 @@ struct CgSys {
 
 struct CgCounters {
   int settled;  // systems in a final state (done, failed, stalled)
-  int pad;
+  int abort;    // copy of the upstream overflow word, taken at init
 };
 
 // The state block holds the counters, then CgSys[ns] at byte offset 64.
@@ -55,6 +55,8 @@ struct CgArgs {
   double* q;             // out: charges
   double* mu;            // out: one double per system
   const unsigned* abort_flag;  // device word; nonzero stops the solve (buffer overflow)
+  int* poll_host;              // pinned host memory, two slots of two ints (settled, abort)
+  hipEvent_t poll_ev[2];       // completion of each slot's copy
 };
 
 struct CgOutcome {

@@ -142,6 +142,9 @@ __global__ void __launch_bounds__(256) k_bond_fill(int n, const int* __restrict_
 constexpr int UPD_BLOCK = 256;      // rows per update block
 constexpr int DIR_BLOCK = 1024;     // 16 waves, one row per wave at a time
 constexpr int DIR_MAX_BLOCKS = 256;  // per system: 4096 waves; few partial sums to reduce
+// Rows per wave a direction block aims for: a block's prologue and partials
+// cost the same whatever its rows, so small systems get few, full blocks.
+constexpr int DIR_ROWS_PER_WAVE = 3;
 constexpr int INNER_MAX = 64;        // mixed precision: iterations between FP64 restarts
 constexpr double INNER_DROP2 = 1e-10;  // mixed precision: squared residual drop before a restart
 constexpr int MAX_RESTARTS = 12;
@@ -158,20 +161,42 @@ __device__ inline int block_system(const Systems& S, int b, bool dir) {
   return lo;
 }
 
-// The direction kernel's prologue in one pass: the current and previous
-// residual dots (4 each, slot arrays) and |b|^2 (2) of one system.  Each value
-// is summed in a fixed order (strided per thread, then the waves in order),
-// with one barrier sequence for all ten.
+// Sums of NV series of cnt block partials in a fixed order: thread-strided
+// loads over the whole block (one latency), the wave tree in the waves that
+// hold data, then those waves in order.  One barrier; the order depends only
+// on cnt and the block size, so only on the system.
+template <int NV, class Load>
+__device__ inline void block_sums(int cnt, Load load, double (&out)[NV]) {
+  __shared__ double red[NV][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int used = (cnt + 63) >> 6 < (int)(blockDim.x >> 6) ? (cnt + 63) >> 6 : (int)(blockDim.x >> 6);
+  if (wave < used) {
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) load(t, v);
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double w = xr_wave_sum(v[q]);
+      if (lane == 0) red[q][wave] = w;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double x = 0.0;
+    for (int w = 0; w < used; ++w) x += red[q][w];
+    out[q] = x;
+  }
+}
+
+// The direction kernel's prologue: current and previous residual dots (4 + 4)
+// and |b|^2 (2) over the system's update-block partials.
 __device__ inline void reduce_direction(const double* __restrict__ cur_p, const double* __restrict__ prev_p,
                                         const double* __restrict__ bb_p, int stride, int b0, int cnt,
                                         double (&cur)[4], double (&prev)[4], double (&bb)[2]) {
-  __shared__ double red[10][16];
-  __shared__ double res[10];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  double v[10];
-#pragma unroll
-  for (int q = 0; q < 10; ++q) v[q] = 0.0;
-  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+  double res[10];
+  block_sums<10>(cnt, [&](int t, double (&v)[10]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       v[q] += cur_p[q * stride + b0 + t];
@@ -179,19 +204,7 @@ __device__ inline void reduce_direction(const double* __restrict__ cur_p, const 
     }
     v[8] += bb_p[b0 + t];
     v[9] += bb_p[stride + b0 + t];
-  }
-#pragma unroll
-  for (int q = 0; q < 10; ++q) {
-    const double w = xr_wave_sum(v[q]);
-    if (lane == 0) red[q][wave] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x < 10) {
-    double x = 0.0;
-    for (int w = 0; w < nw; ++w) x += red[threadIdx.x][w];
-    res[threadIdx.x] = x;
-  }
-  __syncthreads();
+  }, res);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     cur[q] = res[q];
@@ -199,22 +212,15 @@ __device__ inline void reduce_direction(const double* __restrict__ cur_p, const 
   }
   bb[0] = res[8];
   bb[1] = res[9];
-  __syncthreads();
 }
 
-// The update kernel's prologue in one pass: residual dots (4) and |b|^2 (2)
-// over the system's update blocks, p.Ap (2) over its direction blocks.
+// The update kernel's prologue: residual dots (4) and |b|^2 (2) over the
+// system's update blocks, p.Ap (2) over its direction blocks.
 __device__ inline void reduce_update(const double* __restrict__ cur_p, const double* __restrict__ bb_p, int su,
                                      int u0, int nu_cnt, const double* __restrict__ pap_p, int sd, int d0, int nd_cnt,
                                      double (&cur)[4], double (&bb)[2], double (&pap)[2]) {
-  __shared__ double red[8][16];
-  __shared__ double res[8];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  double v[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = 0.0;
-  const int top = nu_cnt > nd_cnt ? nu_cnt : nd_cnt;
-  for (int t = threadIdx.x; t < top; t += blockDim.x) {
+  double res[8];
+  block_sums<8>(nu_cnt > nd_cnt ? nu_cnt : nd_cnt, [&](int t, double (&v)[8]) {
     if (t < nu_cnt) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] += cur_p[q * su + u0 + t];
@@ -225,26 +231,13 @@ __device__ inline void reduce_update(const double* __restrict__ cur_p, const dou
       v[6] += pap_p[d0 + t];
       v[7] += pap_p[sd + d0 + t];
     }
-  }
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const double w = xr_wave_sum(v[q]);
-    if (lane == 0) red[q][wave] = w;
-  }
-  __syncthreads();
-  if (threadIdx.x < 8) {
-    double x = 0.0;
-    for (int w = 0; w < nw; ++w) x += red[threadIdx.x][w];
-    res[threadIdx.x] = x;
-  }
-  __syncthreads();
+  }, res);
 #pragma unroll
   for (int q = 0; q < 4; ++q) cur[q] = res[q];
   bb[0] = res[4];
   bb[1] = res[5];
   pap[0] = res[6];
   pap[1] = res[7];
-  __syncthreads();
 }
 
 // Block partial of NV values into part[k * stride + blockIdx.x].
@@ -284,51 +277,69 @@ struct BlockRows {
   int size[UPD_BLOCK];
 };
 
-// Thread 0 assigns the rows [0, rows) of an update block to blocks.
+// Assigns the rows [0, rows) of an update block to blocks, as one pass in row
+// order would: row t joins the current block when it is fewer than BLOCK_MAX
+// rows past the block's start and bonded to an earlier row of it.  A row with
+// no strong bond to the three rows before it always starts a block, so each
+// such row's thread walks on to the next one and all walks run in parallel.
 __device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, int rows, BlockRows& br) {
-  if (threadIdx.x == 0) {
-    int start = 0;
-    for (int t = 0; t < rows; ++t) {
+  const int t = threadIdx.x;
+  auto starts = [&](int r) { return r == 0 || (join_sh[r] & 7u) == 0; };
+  if (t < rows && starts(t)) {
+    int start = t;
+    br.first[t] = t;
+    for (int r = t + 1; r < rows && !starts(r); ++r) {
       bool joins = false;
-      if (t > 0 && t - start < BLOCK_MAX) {
+      if (r - start < BLOCK_MAX) {
         for (int d = 1; d <= 3; ++d)
-          if (((join_sh[t] >> (d - 1)) & 1u) && t - d >= start) joins = true;
+          if (((join_sh[r] >> (d - 1)) & 1u) && r - d >= start) joins = true;
       }
-      if (!joins) start = t;
-      br.first[t] = start;
+      if (!joins) start = r;
+      br.first[r] = start;
     }
-    for (int t = 0; t < rows;) {
-      int e = t + 1;
-      while (e < rows && br.first[e] == t) ++e;
-      for (int u = t; u < e; ++u) br.size[u] = e - t;
-      t = e;
-    }
+  }
+  __syncthreads();
+  if (t < rows) {
+    const int f = br.first[t];
+    int e = f + 1;
+    while (e < rows && br.first[e] == f) ++e;
+    br.size[t] = e - f;
   }
   __syncthreads();
 }
 
 // In-place inverse of a small SPD matrix (Gauss-Jordan, no pivoting needed).
-__device__ inline void invert_small(double (&a)[BLOCK_MAX][BLOCK_MAX], int m) {
+__device__ inline void invert_small(double (&a)[BLOCK_MAX][BLOCK_MAX]) {
+  // Gauss-Jordan without pivoting (H blocks are symmetric positive definite);
+  // fixed trip counts keep the matrices in registers
   double inv[BLOCK_MAX][BLOCK_MAX];
-  for (int r = 0; r < m; ++r)
-    for (int c = 0; c < m; ++c) inv[r][c] = r == c ? 1.0 : 0.0;
-  for (int p = 0; p < m; ++p) {
+#pragma unroll
+  for (int r = 0; r < BLOCK_MAX; ++r)
+#pragma unroll
+    for (int c = 0; c < BLOCK_MAX; ++c) inv[r][c] = r == c ? 1.0 : 0.0;
+#pragma unroll
+  for (int p = 0; p < BLOCK_MAX; ++p) {
     const double d = 1.0 / a[p][p];
-    for (int c = 0; c < m; ++c) {
+#pragma unroll
+    for (int c = 0; c < BLOCK_MAX; ++c) {
       a[p][c] *= d;
       inv[p][c] *= d;
     }
-    for (int r = 0; r < m; ++r) {
+#pragma unroll
+    for (int r = 0; r < BLOCK_MAX; ++r) {
       if (r == p) continue;
       const double f = a[r][p];
-      for (int c = 0; c < m; ++c) {
+#pragma unroll
+      for (int c = 0; c < BLOCK_MAX; ++c) {
         a[r][c] -= f * a[p][c];
         inv[r][c] -= f * inv[p][c];
       }
     }
   }
-  for (int r = 0; r < m; ++r)
-    for (int c = 0; c < m; ++c) a[r][c] = inv[r][c];
+#pragma unroll
+  for (int r = 0; r < BLOCK_MAX; ++r)
+#pragma unroll
+    for (int c = 0; c < BLOCK_MAX; ++c) a[r][c] = inv[r][c];
 }
 
 __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __restrict__ type, FFDev ff,
@@ -340,7 +351,8 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
                                                        double4* __restrict__ minv, int2* __restrict__ binfo,
                                                        double* __restrict__ eta_out, double* __restrict__ part_b,
                                                        double* __restrict__ part_bb, int nu, CgSys* __restrict__ cg,
-                                                       CgCounters* __restrict__ cnt) {
+                                                       CgCounters* __restrict__ cnt,
+                                                       const unsigned* __restrict__ abort_flag) {
   const int sys = block_system(S, blockIdx.x, false);
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.upd0;
@@ -374,13 +386,24 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
   __syncthreads();
   __shared__ double inv_sh[UPD_BLOCK][BLOCK_MAX];
   if (mine && br.first[t] == t) {
+    // the m x m block padded to BLOCK_MAX with the identity: the padding
+    // never mixes with the block, whose inverse comes out unchanged
     const int m = br.size[t];
     double a[BLOCK_MAX][BLOCK_MAX];
-    for (int r = 0; r < m; ++r)
-      for (int c = 0; c < m; ++c) a[r][c] = hb[t + r][c];
-    invert_small(a, m);
-    for (int r = 0; r < m; ++r)
-      for (int c = 0; c < BLOCK_MAX; ++c) inv_sh[t + r][c] = c < m ? a[r][c] : 0.0;
+#pragma unroll
+    for (int r = 0; r < BLOCK_MAX; ++r) {
+      const int row = t + r < UPD_BLOCK ? t + r : UPD_BLOCK - 1;
+#pragma unroll
+      for (int c = 0; c < BLOCK_MAX; ++c) a[r][c] = (r < m && c < m) ? hb[row][c] : (r == c ? 1.0 : 0.0);
+    }
+    invert_small(a);
+#pragma unroll
+    for (int r = 0; r < BLOCK_MAX; ++r) {
+      if (r < m) {
+#pragma unroll
+        for (int c = 0; c < BLOCK_MAX; ++c) inv_sh[t + r][c] = c < m ? a[r][c] : 0.0;
+      }
+    }
   }
   __syncthreads();
   double vals[4] = {0, 0, 0, 0};
@@ -409,6 +432,7 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
   // slot 1 holds the residual state before iteration 0
   write_partials<4>(vals, part_b + 4 * nu, nu);
   write_partials<2>(bb, part_bb, nu);
+  if (blockIdx.x == 0 && threadIdx.x == 0) cnt->abort = abort_flag ? (int)(*abort_flag != 0) : 0;
   if (local == 0 && threadIdx.x == 0) {
     CgSys c{};
     c.done = CG_RUNNING;
@@ -483,6 +507,36 @@ __device__ inline void gather_batch(const RowBatch<float>& b, const float4* __re
   }
 }
 
+// The same sums with z and p_old kept apart (a = sum h z, sum h p_old per
+// right-hand side), for the row gathered before beta is known.
+__device__ inline void gather_split(const RowBatch<double>& b, const double4* __restrict__ zp, double (&a)[4]) {
+  // two groups of four 32-B gathers: all eight in flight at once would not
+  // fit the 128 registers of a 1024-thread block next to the prefetched row
+#pragma unroll
+  for (int g = 0; g < 8; g += 4) {
+#pragma unroll
+    for (int u = g; u < g + 4; ++u) {
+      const double4 v = zp[b.j[u]];
+      a[0] += b.h[u] * v.x;
+      a[1] += b.h[u] * v.y;
+      a[2] += b.h[u] * v.z;
+      a[3] += b.h[u] * v.w;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+__device__ inline void gather_split(const RowBatch<float>& b, const float4* __restrict__ zq, float (&a)[4]) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const float4 v = zq[b.j[u]];
+    a[0] += b.h[u] * v.x;
+    a[1] += b.h[u] * v.y;
+    a[2] += b.h[u] * v.z;
+    a[3] += b.h[u] * v.w;
+  }
+}
+
 // Iteration k, first half: beta, p = z + beta p_old and A p (rows of the
 // system strided over its blocks, one wave per row), partial p.Ap per block.
 // zp_in holds (z, p_old) of every atom; the new p goes to the p half of
@@ -553,6 +607,39 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
     e1 = row_ptr[i + 1];
     load_batch(nbr_j, hv, e0, e1, i, lane, cur_b);
   }
+  // The wave's first row is gathered before beta is known (sum h z and
+  // sum h p_old apart), so the prologue reduction overlaps it; the next row's
+  // first 512 entries are already on their way.
+  int inext = i + waves, n0 = 0, n1 = 0;
+  RowBatch<HV> next_b;
+  if (inext < end) {
+    n0 = row_ptr[inext];
+    n1 = row_ptr[inext + 1];
+    load_batch(nbr_j, hv, n0, n1, inext, lane, next_b);
+  }
+  double first[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < end) {
+    if constexpr (MIXED) {
+      float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      gather_split(cur_b, zq_in, f);
+      for (int base = e0 + 512; base < e1; base += 512) {
+        RowBatch<HV> more;
+        load_batch(nbr_j, hv, base, e1, i, lane, more);
+        gather_split(more, zq_in, f);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) first[q] = (double)f[q];
+    } else {
+      gather_split(cur_b, zp_in, first);
+      for (int base = e0 + 512; base < e1; base += 512) {
+        RowBatch<HV> more;
+        load_batch(nbr_j, hv, base, e1, i, lane, more);
+        gather_split(more, zp_in, first);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) first[q] = xr_wave_sum(first[q]);
   double cur[4], prev[4], bb[2];
   reduce_direction(part_b + ((k - 1) & 1) * 4 * nu, part_b + (k & 1) * 4 * nu, part_bb, nu, sd.upd0, sd.nupd, cur,
                    prev, bb);
@@ -581,10 +668,27 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
     if (!conv[1] && prev[1] != 0.0) beta_t = cur[1] / prev[1];
   }
   double pap_s = 0.0, pap_t = 0.0;
+  // p = z + beta p_old and A p = eta p + sum h p_j for row i from its sums
+  auto finish_row = [&](int row, double acc_s, double acc_t) {
+    const double4 v = zp_in[row];
+    const double2 pr = make_double2(v.x + beta_s * v.z, v.y + beta_t * v.w);
+    const double d = eta[row];
+    const double2 ap = make_double2(d * pr.x + acc_s, d * pr.y + acc_t);
+    reinterpret_cast<double2*>(zp_out + row)[1] = pr;
+    if (MIXED) reinterpret_cast<float2*>(zq_out + row)[1] = make_float2((float)pr.x, (float)pr.y);
+    AP[row] = ap;
+    pap_s += pr.x * ap.x;
+    pap_t += pr.y * ap.y;
+  };
+  if (i < end && lane == 0) finish_row(i, first[0] + beta_s * first[2], first[1] + beta_t * first[3]);
+  i = inext;
+  e0 = n0;
+  e1 = n1;
+  cur_b = next_b;
   while (i < end) {
-    const int inext = i + waves;
-    int n0 = 0, n1 = 0;
-    RowBatch<HV> next_b;
+    inext = i + waves;
+    n0 = 0;
+    n1 = 0;
     if (inext < end) {
       n0 = row_ptr[inext];
       n1 = row_ptr[inext + 1];
@@ -614,17 +718,7 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
     }
     acc_s = xr_wave_sum(acc_s);
     acc_t = xr_wave_sum(acc_t);
-    if (lane == 0) {
-      const double4 v = zp_in[i];
-      const double2 p = make_double2(v.x + beta_s * v.z, v.y + beta_t * v.w);
-      const double d = eta[i];
-      const double2 ap = make_double2(d * p.x + acc_s, d * p.y + acc_t);
-      reinterpret_cast<double2*>(zp_out + i)[1] = p;
-      if (MIXED) reinterpret_cast<float2*>(zq_out + i)[1] = make_float2((float)p.x, (float)p.y);
-      AP[i] = ap;
-      pap_s += p.x * ap.x;
-      pap_t += p.y * ap.y;
-    }
+    if (lane == 0) finish_row(i, acc_s, acc_t);
     i = inext;
     e0 = n0;
     e1 = n1;
@@ -776,7 +870,7 @@ hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, c
   return hipSuccess;
 }
 
-// Direction blocks per system: one wave per row up to a cap (XREAC_CG_MAX_BLOCKS
+// Direction blocks per system: about DIR_ROWS_PER_WAVE rows per wave up to a cap (XREAC_CG_MAX_BLOCKS
 // overrides the cap for tuning; it must be the same for every call that is
 // compared bitwise, since a system's blocks depend only on its size and this cap).
 void xr_cg_blocks(int n, int* ndir, int* nupd) {
@@ -785,7 +879,8 @@ void xr_cg_blocks(int n, int* ndir, int* nupd) {
     const int c = v ? std::atoi(v) : DIR_MAX_BLOCKS;
     return c > 0 ? c : DIR_MAX_BLOCKS;
   }();
-  int d = (n + DIR_BLOCK / 64 - 1) / (DIR_BLOCK / 64);
+  const int rows_per_block = DIR_ROWS_PER_WAVE * (DIR_BLOCK / 64);
+  int d = (n + rows_per_block - 1) / rows_per_block;
   *ndir = d < 1 ? 1 : (d > cap ? cap : d);
   const int u = (n + UPD_BLOCK - 1) / UPD_BLOCK;
   *nupd = u < 1 ? 1 : u;
@@ -808,22 +903,16 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   float4* zq1 = mixed ? a.ZQ1 : nullptr;
   // counters start at zero (cleared with the control block)
   hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.X, a.R, a.ZP0,
-                     zq0, a.hnear, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt);
+                     zq0, a.hnear, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt, a.abort_flag);
   XR_LAUNCH_CHECK();
   int k = 0;
-  // Bound on launches: every system settles within max_iter executed
-  // iterations plus one test per restart.
   // Every system settles within max_iter executed iterations plus two
   // launches per refresh (the request and the FP64 sweep; one more for a warm start).
   const long long k_limit = (long long)a.max_iter + 3LL * (MAX_RESTARTS + 3) + a.first_check + a.check_every;
-  int host_state[3] = {0, 0, 0};
-  bool first = true;
   out->aborted = false;
   out->exhausted = false;
-  for (;;) {
-    const int chunk = first ? a.first_check : a.check_every;
-    first = false;
-    for (int c = 0; c < chunk && k < k_limit; ++c) {
+  auto enqueue = [&](int iters) {
+    for (int c = 0; c < iters && k < k_limit; ++c) {
       double4* zp_in = (k & 1) ? a.ZP1 : a.ZP0;
       double4* zp_out = (k & 1) ? a.ZP0 : a.ZP1;
       float4* zq_in = (k & 1) ? zq1 : zq0;
@@ -843,21 +932,50 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
       }
       ++k;
     }
-    XR_LAUNCH_CHECK();
-    hipError_t e = hipMemcpyAsync(host_state, cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
+    return hipGetLastError();
+  };
+  // Read-back of (settled, abort) into a pinned slot behind each chunk.  The
+  // first chunk is sized from the previous solve and waited for at once; if
+  // systems are still running, later chunks are issued one ahead of the slot
+  // being waited for, so the device keeps working while the host reads.
+  // Launches that run after every system has settled find nothing to do.
+  auto post = [&](int slot) {
+    hipError_t e = hipMemcpyAsync(a.poll_host + 2 * slot, cnt, 2 * sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(a.poll_ev[slot], s);
+    return e;
+  };
+  bool pending[2] = {false, false};
+  auto issue = [&](int slot, int iters) {
+    if (k >= k_limit) return hipSuccess;
+    hipError_t e = enqueue(iters);
+    if (e == hipSuccess) e = post(slot);
+    if (e == hipSuccess) pending[slot] = true;
+    return e;
+  };
+  hipError_t e = issue(0, a.first_check);
+  if (e != hipSuccess) return e;
+  int cur = 0;
+  bool pipelined = false;
+  for (;;) {
+    e = hipEventSynchronize(a.poll_ev[cur]);
     if (e != hipSuccess) return e;
-    if (a.abort_flag) {
-      e = hipMemcpyAsync(host_state + 2, a.abort_flag, sizeof(int), hipMemcpyDeviceToHost, s);
-      if (e != hipSuccess) return e;
-    }
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-    if (host_state[2]) {  // buffers overflowed upstream; the caller repeats the evaluation
+    pending[cur] = false;
+    const int* st = a.poll_host + 2 * cur;
+    if (st[1]) {  // buffers overflowed upstream; the caller repeats the evaluation
       out->aborted = true;
       return hipSuccess;
     }
-    if (host_state[0] >= ns) break;
-    if (k >= k_limit) {
+    if (st[0] >= ns) break;
+    if (!pipelined) {  // from here on one chunk runs while the host reads the other's slot
+      pipelined = true;
+      e = issue(1 - cur, a.check_every);
+      if (e != hipSuccess) return e;
+    }
+    e = issue(cur, a.check_every);
+    if (e != hipSuccess) return e;
+    if (pending[1 - cur]) {
+      cur = 1 - cur;
+    } else if (!pending[cur]) {
       out->exhausted = true;
       break;
     }

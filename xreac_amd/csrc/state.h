@@ -58,7 +58,9 @@ struct Atoms {
 // bondorder.hip
 hipError_t xr_bond_forward(const Atoms& at, const Bonds& bd, FFDev ff, const int* nbr_j, const int* nbr_s,
                            EnergyAcc acc, hipStream_t s);
-hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, double* forces, hipStream_t s);
+hipError_t xr_bond_backward(const Atoms& at, const Bonds& bd, FFDev ff, hipStream_t s);
+// Final per-atom forces from the bond gradients and the nonbonded and H-bond forces.
+hipError_t xr_force_assembly(const Atoms& at, const Bonds& bd, double* forces, hipStream_t s);
 // manybody.hip
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
                        const int* nbr_s, const unsigned char* eflag, int lammps_angles, EnergyAcc acc,
