@@ -65,6 +65,31 @@ def test_mix_equals_alone(precision):
             np.testing.assert_array_equal(m.potential, a.potential)
 
 
+def test_admission_respects_atom_cap():
+    """All requests queued at once: every evaluated batch stays within max_batch_atoms (a request larger
+    than the cap is admitted alone)."""
+    from xreac_amd.serving import Server
+
+    reqs = _requests(40, seed=17, max_atoms=160)
+    cap = 400
+    srv = Server(max_batch_atoms=cap)
+    sizes = []
+    for eng in srv.engines.values():
+        run = eng.evaluate_batch
+
+        def timed(*a, _run=run, **k):
+            sizes.append(int(a[2][-1]))
+            return _run(*a, **k)
+
+        eng.evaluate_batch = timed
+    out = srv.run(reqs)
+    assert len(out) == len(reqs)
+    biggest = max(len(r["symbols"]) for r in reqs)
+    print(f"[serving] {len(sizes)} batches, largest {max(sizes)} atoms (cap {cap})", flush=True)
+    assert max(sizes) <= max(cap, biggest)
+    assert len(sizes) > len(srv.engines)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_served_evaluations_match_oracle(precision):
     reqs = [r for r in _requests(40, seed=5, max_atoms=90) if r["md_steps"] == 0][:8]

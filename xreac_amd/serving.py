@@ -72,16 +72,97 @@ class _Job:
     periodic: bool
     bytes: int
     arrival: float
-    x: object = None
-    v: object = None
-    types: object = None
-    forces: object = None
-    qstate: object = None
-    fire: object = None  # relaxations: FIRE state (1, 4) and status (1,) on the device
-    status: object = None
     evals: int = 0
     potential: list = field(default_factory=list)
     kinetic: list = field(default_factory=list)
+
+
+class _Group:
+    """Device state of the active requests of one force field, packed in job order.
+
+    Positions, velocities, previous forces, QEq state and FIRE state live in
+    one set of tensors, so a step costs a fixed number of tensor operations
+    whatever the number of requests: arrivals are appended with one copy per
+    array, finished requests dropped with one gather.
+    """
+
+    def __init__(self, torch, eng):
+        self.torch, self.eng, self.dev = torch, eng, eng.device
+        self.jobs = []
+        self.sizes = np.zeros(0, dtype=np.int64)
+        self.md = np.zeros(0, dtype=bool)
+        self.relax = np.zeros(0, dtype=bool)
+        self.dt = np.zeros(0, dtype=np.float64)
+        self.cells = np.zeros((0, 3, 3), dtype=np.float64)
+        self.pbcs = np.zeros((0, 3), dtype=np.int32)
+        self.charges = np.zeros(0, dtype=np.float64)
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.X = torch.zeros((0, 3), **f64)
+        self.V = torch.zeros((0, 3), **f64)
+        self.F = torch.zeros((0, 3), **f64)
+        self.QS = torch.zeros((0, 2), **f64)
+        self.T = torch.zeros(0, dtype=torch.int32, device=self.dev)
+        self.FS = torch.zeros((0, 4), **f64)
+        self.ST = torch.zeros(0, dtype=torch.int32, device=self.dev)
+
+    def append(self, jobs, ff):
+        """Pack newly admitted jobs behind the active ones (one host-to-device copy per array)."""
+        torch = self.torch
+        x, t, v, fs, st = [], [], [], [], []
+        for job in jobs:
+            req = job.req
+            x.append(np.asarray(req["positions"], dtype=np.float64).reshape(job.n, 3))
+            t.append(self.eng.type_ids(req["symbols"]))
+            if req.get("md_steps", 0) > 0:
+                v0 = req.get("velocities")
+                if v0 is None:
+                    masses = [ff.atoms[s]["mass"] for s in req["symbols"]]
+                    v0 = maxwell_boltzmann(masses, req.get("temperature", 300.0), req.get("velocity_seed", 0))
+                v.append(np.asarray(v0, dtype=np.float64).reshape(job.n, 3))
+            else:
+                v.append(np.zeros((job.n, 3)))
+            relax = bool(req.get("relax"))
+            fs.append([0.02, 0.1, 0.0, 0.0] if relax else [0.0, 0.0, 0.0, 0.0])
+            st.append(0 if relax else 1)  # status 1: not relaxing, FIRE leaves the system alone
+        m = sum(job.n for job in jobs)
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.X = torch.cat([self.X, torch.as_tensor(np.concatenate(x), **f64)])
+        self.V = torch.cat([self.V, torch.as_tensor(np.concatenate(v), **f64)])
+        self.T = torch.cat([self.T, torch.as_tensor(np.concatenate(t), dtype=torch.int32, device=self.dev)])
+        self.F = torch.cat([self.F, torch.zeros((m, 3), **f64)])
+        self.QS = torch.cat([self.QS, torch.zeros((m, 2), **f64)])
+        self.FS = torch.cat([self.FS, torch.as_tensor(np.array(fs, dtype=np.float64), **f64)])
+        self.ST = torch.cat([self.ST, torch.as_tensor(np.array(st, dtype=np.int32), device=self.dev)])
+        self.jobs.extend(jobs)
+        self.sizes = np.concatenate([self.sizes, [job.n for job in jobs]])
+        self.md = np.concatenate([self.md, [job.req.get("md_steps", 0) > 0 for job in jobs]])
+        self.relax = np.concatenate([self.relax, [bool(job.req.get("relax")) for job in jobs]])
+        self.dt = np.concatenate([self.dt, [job.req.get("dt", 0.25) for job in jobs]])
+        self.cells = np.concatenate([self.cells, [np.eye(3) if job.req.get("cell") is None else
+                                                  np.asarray(job.req["cell"], dtype=np.float64).reshape(3, 3)
+                                                  for job in jobs]])
+        self.pbcs = np.concatenate([self.pbcs, np.array([[0, 0, 0] if job.req.get("cell") is None else
+                                                         [int(bool(p)) for p in job.req["pbc"]] for job in jobs],
+                                                        dtype=np.int32).reshape(-1, 3)])
+        self.charges = np.concatenate([self.charges, [job.req.get("total_charge", 0.0) for job in jobs]])
+
+    def drop(self, done):
+        """Remove the jobs flagged in ``done`` (bool per job) with one gather per array."""
+        torch = self.torch
+        keep = ~done
+        atom_keep = np.repeat(keep, self.sizes)
+        ia = torch.as_tensor(np.flatnonzero(atom_keep), device=self.dev)
+        isys = torch.as_tensor(np.flatnonzero(keep), device=self.dev)
+        self.X = self.X.index_select(0, ia)
+        self.V = self.V.index_select(0, ia)
+        self.F = self.F.index_select(0, ia)
+        self.QS = self.QS.index_select(0, ia)
+        self.T = self.T.index_select(0, ia)
+        self.FS = self.FS.index_select(0, isys)
+        self.ST = self.ST.index_select(0, isys)
+        self.jobs = [j for j, k in zip(self.jobs, keep) if k]
+        for name in ("sizes", "md", "relax", "dt", "cells", "pbcs", "charges"):
+            setattr(self, name, getattr(self, name)[keep])
 
 
 class Server:
@@ -114,7 +195,7 @@ class Server:
         self.relax_iterations = relax_iterations
         self.clock = clock
         self.queue = deque()
-        self.active = {k: [] for k in self.engines}
+        self.active = {k: _Group(torch, eng) for k, eng in self.engines.items()}
         self.used_bytes = 0
         self.responses = []
         self.atom_evals = 0
@@ -130,9 +211,9 @@ class Server:
         self.queue.append(job)
 
     def _admit(self):
-        torch = self.torch
-        active = sum(len(v) for v in self.active.values())
-        atoms = sum(j.n for v in self.active.values() for j in v)
+        active = sum(len(g.jobs) for g in self.active.values())
+        atoms = sum(int(g.sizes.sum()) for g in self.active.values())
+        admitted = {}
         while self.queue:
             job = self.queue[0]
             if self.max_systems is not None and active >= self.max_systems:
@@ -140,140 +221,90 @@ class Server:
             if active and (self.used_bytes + job.bytes > self.budget or atoms + job.n > self.max_batch_atoms):
                 break
             self.queue.popleft()
-            eng = self.engines[job.req["force_field"]]
-            dev = eng.device
-            job.x = torch.as_tensor(np.asarray(job.req["positions"], dtype=np.float64), device=dev).contiguous()
-            job.types = torch.as_tensor(eng.type_ids(job.req["symbols"]), device=dev)
-            if job.req.get("md_steps", 0) > 0:
-                v0 = job.req.get("velocities")
-                if v0 is None:
-                    ff = self._ff[job.req["force_field"]]
-                    masses = [ff.atoms[s]["mass"] for s in job.req["symbols"]]
-                    v0 = maxwell_boltzmann(masses, job.req.get("temperature", 300.0), job.req.get("velocity_seed", 0))
-                job.v = torch.as_tensor(np.asarray(v0, dtype=np.float64), device=dev).contiguous()
-            elif job.req.get("relax"):
-                job.v = torch.zeros((job.n, 3), dtype=torch.float64, device=dev)
-                job.fire = torch.tensor([[0.02, 0.1, 0.0, 0.0]], dtype=torch.float64, device=dev)
-                job.status = torch.zeros(1, dtype=torch.int32, device=dev)
-            self.active[job.req["force_field"]].append(job)
+            admitted.setdefault(job.req["force_field"], []).append(job)
             self.used_bytes += job.bytes
             active += 1
             atoms += job.n
+        for name, jobs in admitted.items():
+            self.active[name].append(jobs, self._ff[name])
 
-    # ------------------------------------------------------------------ step
     def step(self):
         """Admit what fits, then evaluate every active request once."""
         self._admit()
-        for key, jobs in self.active.items():
-            if jobs:
-                self._run_group(self.engines[key], jobs)
+        for group in self.active.values():
+            if group.jobs:
+                self._run_group(group)
         self.steps += 1
 
-    def _run_group(self, eng, jobs):
+    def _run_group(self, g):
         torch = self.torch
-        dev = eng.device
-        ns = len(jobs)
-        sizes = [j.n for j in jobs]
+        eng, dev = g.eng, g.dev
+        ns = len(g.jobs)
         offsets = np.zeros(ns + 1, dtype=np.int32)
-        np.cumsum(sizes, out=offsets[1:])
+        np.cumsum(g.sizes, out=offsets[1:])
         n = int(offsets[-1])
-        X = torch.cat([j.x for j in jobs]).contiguous()
-        T = torch.cat([j.types for j in jobs]).contiguous()
-        zeros3 = lambda m: torch.zeros((m, 3), dtype=torch.float64, device=dev)  # noqa: E731
-        md = [j.req.get("md_steps", 0) > 0 for j in jobs]
-        relax = [bool(j.req.get("relax")) for j in jobs]
-        moving = [m and j.evals > 0 for m, j in zip(md, jobs)]
-        V = (torch.cat([j.v if j.v is not None else zeros3(j.n) for j in jobs]).contiguous()
-             if any(md) or any(relax) else None)
-        QS = torch.cat([j.qstate if j.qstate is not None else torch.zeros((j.n, 2), dtype=torch.float64, device=dev)
-                        for j in jobs]).contiguous()
-        cells = np.array([np.eye(3) if j.req.get("cell") is None else np.asarray(j.req["cell"], dtype=np.float64)
-                          .reshape(3, 3) for j in jobs])
-        pbcs = np.array([[0, 0, 0] if j.req.get("cell") is None else [int(bool(p)) for p in j.req["pbc"]]
-                         for j in jobs], dtype=np.int32)
-        charges = np.array([j.req.get("total_charge", 0.0) for j in jobs], dtype=np.float64)
-        off_dev = dt_dev = kin = None
-        if any(moving):
-            off_dev = torch.as_tensor(offsets, device=dev)
-            dt_dev = torch.as_tensor([j.req.get("dt", 0.25) if mv else 0.0 for j, mv in zip(jobs, moving)],
-                                     dtype=torch.float64, device=dev)
-            F_prev = torch.cat([j.forces if j.forces is not None else zeros3(j.n) for j in jobs]).contiguous()
-            eng.verlet(off_dev, T, dt_dev, X, V, F_prev, 0)
+        evals = np.fromiter((j.evals for j in g.jobs), dtype=np.int64, count=ns)
+        moving = g.md & (evals > 0)
+        off_dev = torch.as_tensor(offsets, device=dev) if (g.md.any() or g.relax.any()) else None
+        dt_dev = torch.as_tensor(np.where(moving, g.dt, 0.0), dtype=torch.float64, device=dev) if g.md.any() else None
+        if moving.any():
+            eng.verlet(off_dev, g.T, dt_dev, g.X, g.V, g.F, 0)
         out = dict(forces=torch.empty((n, 3), dtype=torch.float64, device=dev),
                    charges=torch.empty(n, dtype=torch.float64, device=dev),
-                   total_bond_orders=None, lone_pairs=None, bond_counts=None, qeq_state=QS)
-        res = eng.evaluate_batch(X, T, offsets, cells, pbcs, charges, tol=self.tol, precision=self.precision,
-                                 warm_start=self.warm_start, out=out)
-        if any(md):
-            if off_dev is None:
-                off_dev = torch.as_tensor(offsets, device=dev)
-                dt_dev = torch.zeros(ns, dtype=torch.float64, device=dev)
-            kin = torch.zeros(ns, dtype=torch.float64, device=dev)
-            eng.verlet(off_dev, T, dt_dev, X, V, out["forces"], 1, kin)
-            kin = kin.cpu().numpy()
+                   total_bond_orders=None, lone_pairs=None, bond_counts=None, qeq_state=g.QS)
+        res = eng.evaluate_batch(g.X, g.T, offsets, g.cells, g.pbcs, g.charges, tol=self.tol,
+                                 precision=self.precision, warm_start=self.warm_start, out=out)
+        kin = None
+        if g.md.any():
+            kin_t = torch.zeros(ns, dtype=torch.float64, device=dev)
+            eng.verlet(off_dev, g.T, dt_dev, g.X, g.V, out["forces"], 1, kin_t)
+            kin = kin_t.cpu().numpy()
         fire_status = None
-        if any(relax):
-            if off_dev is None:
-                off_dev = torch.as_tensor(offsets, device=dev)
-            skip = torch.ones(1, dtype=torch.int32, device=dev)
-            dummy = torch.zeros((1, 4), dtype=torch.float64, device=dev)
-            FS = torch.cat([j.fire if r else dummy for j, r in zip(jobs, relax)]).contiguous()
-            ST = torch.cat([j.status if r else skip for j, r in zip(jobs, relax)]).contiguous()
-            eng.fire_step(off_dev, X, V, out["forces"], FS, ST, self.relax_tolerance, self.relax_iterations)
-            fire_status = ST.cpu().numpy()
+        if g.relax.any():
+            eng.fire_step(off_dev, g.X, g.V, out["forces"], g.FS, g.ST, self.relax_tolerance, self.relax_iterations)
+            fire_status = g.ST.cpu().numpy()
         self.atom_evals += n
+        g.F = out["forces"]
         comps = res["components"]
-        done = []
-        for k, j in enumerate(jobs):
-            lo, hi = offsets[k], offsets[k + 1]
-            j.x = X[lo:hi]
-            if V is not None and (md[k] or relax[k]):
-                j.v = V[lo:hi]
-            if relax[k]:
-                j.fire = FS[k:k + 1]
-                j.status = ST[k:k + 1]
-            j.forces = out["forces"][lo:hi]
-            j.qstate = QS[lo:hi]
+        pot = comps.sum(axis=1)
+        errors = res["errors"]
+        done = np.zeros(ns, dtype=bool)
+        for k, j in enumerate(g.jobs):
             j.evals += 1
-            j.potential.append(float(comps[k].sum()))
-            if md[k]:
+            j.potential.append(float(pot[k]))
+            if g.md[k]:
                 j.kinetic.append(float(kin[k]))
-            if relax[k]:
-                finished_now = fire_status[k] != 0
-            else:
-                finished_now = j.evals > j.req.get("md_steps", 0)
-            if res["errors"][k] is not None or finished_now:
-                done.append((k, j))
-        if not done:
+            finished = fire_status[k] != 0 if g.relax[k] else j.evals > j.req.get("md_steps", 0)
+            done[k] = errors[k] is not None or finished
+        if not done.any():
             return
         F = out["forces"].cpu().numpy()
         Q = out["charges"].cpu().numpy()
-        Xh = X.cpu().numpy() if any(md[k] or relax[k] for k, _ in done) else None
-        Vh = V.cpu().numpy() if V is not None and any(md[k] for k, _ in done) else None
+        Xh = g.X.cpu().numpy() if (g.md | g.relax)[done].any() else None
+        Vh = g.V.cpu().numpy() if g.md[done].any() else None
         now = self.clock()
-        finished = set()
-        for k, j in done:
+        for k in np.flatnonzero(done):
+            j = g.jobs[k]
             lo, hi = offsets[k], offsets[k + 1]
-            r = Response(j.req["rid"], j.n, j.evals, float(comps[k].sum()), comps[k].copy(), F[lo:hi].copy(),
-                         Q[lo:hi].copy(), j.arrival, now, res["errors"][k])
-            if md[k]:
+            r = Response(j.req["rid"], j.n, j.evals, float(pot[k]), comps[k].copy(), F[lo:hi].copy(),
+                         Q[lo:hi].copy(), j.arrival, now, errors[k])
+            if g.md[k]:
                 r.positions = Xh[lo:hi].copy()
                 r.velocities = Vh[lo:hi].copy()
                 r.potential = np.array(j.potential)
                 r.kinetic = np.array(j.kinetic)
-            if relax[k]:
+            if g.relax[k]:
                 # FIRE leaves a converged (or exhausted) system where it was evaluated
                 r.positions = Xh[lo:hi].copy()
                 r.potential = np.array(j.potential)
                 r.converged = bool(fire_status[k] == 1)
             self.responses.append(r)
             self.used_bytes -= j.bytes
-            finished.add(id(j))
-        jobs[:] = [j for j in jobs if id(j) not in finished]
+        g.drop(done)
 
     # ------------------------------------------------------------------- run
     def idle(self):
-        return not self.queue and not any(self.active.values())
+        return not self.queue and not any(g.jobs for g in self.active.values())
 
     def run(self, requests, arrivals=None):
         """Serve ``requests`` arriving at ``arrivals`` (seconds from now; all at once if None).
