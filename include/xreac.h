@@ -211,11 +211,15 @@ int xreac_fire_step(xreac_ctx* ctx, int32_t num_systems, const int32_t* offsets,
                     double* velocities, const double* forces, double* state, int32_t* status,
                     double force_tolerance, int32_t max_iterations);
 
-/* Kernel timing with hipEvents on the context stream (no reference
+/* Kernel timing with hipEvents on the launching stream (no reference
  * counterpart; measurement only).  timer_mask bit k enables timer k (0: off);
- * every launch of an enabled kernel records start/stop events with its
- * dispatch and its elapsed time is summed per timer. */
+ * each sampled launch of an enabled kernel records start/stop events with
+ * its dispatch and its elapsed time is summed per timer. */
 int xreac_profile_enable(xreac_ctx* ctx, int timer_mask);
+/* Record events on one dispatch in `every` of each enabled timer (default 1),
+ * so a timed run keeps the event overhead small.  No reference counterpart
+ * (measurement only). */
+int xreac_profile_stride(xreac_ctx* ctx, int every);
 int xreac_profile_read(xreac_ctx* ctx, int timer, double* total_ms, int64_t* launches, int reset);
 
 /* Canonical neighbor list of the last evaluation (host arrays): row_ptr [n+1],

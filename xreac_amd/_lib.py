@@ -29,6 +29,7 @@ EXPORTED = (
     "xreac_copy_neighbors",
     "xreac_copy_bonds",
     "xreac_profile_enable",
+    "xreac_profile_stride",
     "xreac_profile_read",
 )
 TIMERS = (
@@ -175,6 +176,8 @@ def load():
     lib.xreac_copy_bonds.restype = C.c_int
     lib.xreac_profile_enable.argtypes = [vp, C.c_int]
     lib.xreac_profile_enable.restype = C.c_int
+    lib.xreac_profile_stride.argtypes = [vp, C.c_int]
+    lib.xreac_profile_stride.restype = C.c_int
     lib.xreac_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
     lib.xreac_profile_read.restype = C.c_int
     _LIB = lib

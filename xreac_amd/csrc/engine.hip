@@ -1051,6 +1051,13 @@ int xreac_profile_enable(xreac_ctx* ctx, int timer_mask) {
   return XREAC_OK;
 }
 
+int xreac_profile_stride(xreac_ctx* ctx, int every) {
+  if (!ctx || every < 1) return XREAC_INVALID;
+  ctx->prof.every = every;
+  for (long long& c : ctx->prof.seen) c = 0;
+  return XREAC_OK;
+}
+
 int xreac_profile_read(xreac_ctx* ctx, int timer, double* total_ms, int64_t* launches, int reset) {
   if (!ctx || timer < 0 || timer >= XT_COUNT || !total_ms || !launches) return XREAC_INVALID;
   *total_ms = ctx->prof.total_ms[timer];

@@ -18,6 +18,8 @@ enum XrTimer {
 struct XrProfiler {
   bool on = false;
   unsigned mask = 0;  // bit k: timer k is recorded
+  int every = 1;      // record one dispatch in `every` of each timer
+  long long seen[XT_COUNT] = {};
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   std::vector<std::pair<int, size_t>> pending;
@@ -25,14 +27,17 @@ struct XrProfiler {
   long long launches[XT_COUNT] = {};
 
   bool wants(int id) const { return on && ((mask >> id) & 1u); }
-  void begin(int id, hipStream_t s) {
-    if (!wants(id)) return;
+  // Whether this dispatch of timer id is recorded (one in `every`).
+  bool sample(int id) { return wants(id) && (seen[id]++ % every) == 0; }
+  // Start event of a timed group; returns false (and records nothing) when not sampled.
+  bool begin(int id, hipStream_t s) {
+    if (!sample(id)) return false;
     if (used + 2 > pool.size()) {
       for (int k = 0; k < 64; ++k) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) {
           on = false;
-          return;
+          return false;
         }
         pool.push_back(e);
       }
@@ -40,10 +45,11 @@ struct XrProfiler {
     pending.emplace_back(id, used);
     (void)hipEventRecord(pool[used], s);
     used += 2;
+    return true;
   }
   // Event pair for a launch that records its own timestamps (hipExtLaunchKernelGGL).
   bool slot(int id, hipEvent_t* a, hipEvent_t* b) {
-    if (!wants(id)) return false;
+    if (!sample(id)) return false;
     if (used + 2 > pool.size()) {
       for (int k = 0; k < 64; ++k) {
         hipEvent_t e;
@@ -80,6 +86,7 @@ struct XrProfiler {
     for (int k = 0; k < XT_COUNT; ++k) {
       total_ms[k] = 0.0;
       launches[k] = 0;
+      seen[k] = 0;
     }
   }
   ~XrProfiler() {
@@ -93,8 +100,7 @@ extern thread_local XrProfiler* xr_active_profiler;
 #define XR_TIMED(id, stream, launch)                                  \
   do {                                                                \
     XrProfiler* p__ = xr_active_profiler;                             \
-    const bool t__ = p__ && p__->wants(id);                           \
-    if (t__) p__->begin((id), (stream));                              \
+    const bool t__ = p__ && p__->begin((id), (stream));               \
     launch;                                                           \
     if (t__) p__->end(stream);                                        \
   } while (0)

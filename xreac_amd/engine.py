@@ -294,13 +294,14 @@ class Engine:
                                       int(max_iterations))
         self._check(rc, self._ctx)
 
-    def profile(self, enable=True, timers=None):
-        """Enable hipEvent timing of the named kernels (all when None), or disable it."""
+    def profile(self, enable=True, timers=None, every=1):
+        """Enable hipEvent timing of the named kernels (all when None) on one dispatch in ``every``, or disable it."""
         mask = 0
         if enable:
             names = _lib.TIMERS if timers is None else timers
             for name in names:
                 mask |= 1 << _lib.TIMERS.index(name)
+        self._check(self.lib.xreac_profile_stride(self._ctx, int(every)), self._ctx)
         self._check(self.lib.xreac_profile_enable(self._ctx, mask), self._ctx)
 
     def timers(self, reset=True):
