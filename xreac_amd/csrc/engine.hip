@@ -498,10 +498,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<char>(off, xr_bin_record_bytes() * n);  // bin-ordered atom records
   carve_size<double2>(off, 7 * (size_t)n);  // CG vectors: x, r, Ap (pairs), two (z, p) records
   carve_size<float4>(off, 2 * (size_t)n);   // FP32 (z, p) records (mixed precision)
-  carve_size<double4>(off, n);           // block-Jacobi rows
+  carve_size<double>(off, xr_cg_block_doubles(n));  // block-Jacobi inverse rows
   carve_size<int2>(off, n);              // block placement
   carve_size<unsigned char>(off, n);     // block join flags
-  carve_size<double>(off, 7 * (size_t)n);  // H to atoms i-3..i+3
   carve_size<double>(off, n);            // eta
   carve_size<double>(off, xr_cg_partials(nd, nu));
   carve_size<double>(off, (size_t)kAtomScalars * n);  // atom scalars
@@ -523,10 +522,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   void* binned = cv.take<char>(xr_bin_record_bytes() * n);
   double2* cgv = cv.take<double2>(7 * (size_t)n);
   float4* cgq = cv.take<float4>(2 * (size_t)n);
-  double4* minv = cv.take<double4>(n);
+  double* minv = cv.take<double>(xr_cg_block_doubles(n));
   int2* binfo = cv.take<int2>(n);
   unsigned char* join = cv.take<unsigned char>(n);
-  double* hnear = cv.take<double>(7 * (size_t)n);
   double* eta = cv.take<double>(n);
   double* cg_part = cv.take<double>(xr_cg_partials(nd, nu));
   double* scal = cv.take<double>((size_t)kAtomScalars * n);
@@ -696,7 +694,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
 
   // ------------------------------------------------------- edges and bonds
   HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, hval32, bflag, bond_count,
-                      join, hnear, s),
+                      join, s),
          "edge pass");
   HIPCHK(xr_exclusive_scan(bond_count, n, bond_ptr, scan_tmp, s), "bond scan");
   int64_t b_cap = 0;
@@ -815,8 +813,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.ZQ0 = cgq;
   cg.ZQ1 = cgq + n;
   cg.join = join;
-  cg.hnear = hnear;
   cg.minv = minv;
+  cg.n = n;
   cg.binfo = binfo;
   cg.eta = eta;
   cg.partials = cg_part;

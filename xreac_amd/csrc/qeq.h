@@ -38,8 +38,8 @@ struct CgArgs {
   double4 *ZP0, *ZP1;  // (z_s, z_t, p_s, p_t), alternating by iteration
   float4 *ZQ0, *ZQ1;   // FP32 copies gathered by the mixed-precision sweep
   const unsigned char* join;  // per atom: strong bonds to the previous three atoms (edge pass)
-  const double* hnear;        // per atom: H summed over images to atoms i-3..i+3 (edge pass)
-  double4* minv;              // per atom: its row of the block-Jacobi inverse
+  int n;                      // atoms over all systems
+  double* minv;               // block-Jacobi inverse rows, column-major: entry c of row i at minv[c * n + i]
   int2* binfo;                // per atom: (local row of its block start, block size)
   double* eta;
   double* partials;  // xr_cg_partials(nd, nu) doubles
@@ -67,11 +67,11 @@ struct CgOutcome {
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, unsigned char* join, double* hnear,
-                        hipStream_t s);
+                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s);
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int cap,
                         int* bond_edge, int* bond_i, hipStream_t s);
 void xr_cg_blocks(int n, int* ndir, int* nupd);
 size_t xr_cg_partials(int nd, int nu);
 size_t xr_cg_state_bytes(int ns);
+size_t xr_cg_block_doubles(int n);  // doubles of the block-Jacobi inverse rows of n atoms
 hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out);

@@ -40,7 +40,8 @@ namespace {
 
 constexpr int ROWS_PER_BLOCK = 4;  // one wave per row, 256 threads
 constexpr double BLOCK_BOND = 0.3;  // raw bond order joining consecutive atoms into one CG block
-constexpr int BLOCK_MAX = 4;        // atoms per CG preconditioner block
+constexpr int RUN_MAX = 16;         // atoms per bonded run (molecule) of the CG preconditioner
+constexpr int BLOCK_MAX = 16;       // atoms per CG preconditioner block (whole runs packed together)
 
 // ---------------------------------------------------------------- edge pass
 __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restrict__ x, const int* __restrict__ type,
@@ -48,8 +49,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
                                                    const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
                                                    double bond_cut, double* __restrict__ hval,
                                                    float* __restrict__ hval32, unsigned char* __restrict__ bflag,
-                                                   int* __restrict__ bond_count, unsigned char* __restrict__ join,
-                                                   double* __restrict__ hnear) {
+                                                   int* __restrict__ bond_count, unsigned char* __restrict__ join) {
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
@@ -61,7 +61,6 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   const double hb_i = ff.atom[ti * AP_COUNT + AP_HBOND];
   int cnt = 0;
   unsigned jm = 0;  // bit d-1: strong same-image bond to atom i - d (d = 1..3), for the CG blocks
-  double near[7] = {0, 0, 0, 0, 0, 0, 0};  // sum of h over all images to atom i + d, d = -3..3
   const int zero = xr_pack_shift(0, 0, 0);
   for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
     const int j = nbr_j[e];
@@ -72,12 +71,6 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
     xr_shield(r, rc, ff.pair[(ti * ff.nt + tj) * PP_COUNT + PP_GAMMA], s, ds);
     hval[e] = 14.4 * s;
     if (hval32) hval32[e] = (float)(14.4 * s);
-    const int dj = j - i;
-    if (dj >= -3 && dj <= 3) {
-#pragma unroll
-      for (int q = 0; q < 7; ++q)
-        if (q == dj + 3) near[q] += 14.4 * s;
-    }
     unsigned char active = 0;
     if (r <= bond_cut) {
       BondRaw b;
@@ -98,13 +91,9 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   cnt = xr_wave_sum_int(cnt);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) jm |= __shfl_down(jm, off, 64);
-#pragma unroll
-  for (int q = 0; q < 7; ++q) near[q] = xr_wave_sum(near[q]);
   if (lane == 0) {
     bond_count[i] = cnt;
     join[i] = (unsigned char)jm;
-#pragma unroll
-    for (int q = 0; q < 7; ++q) hnear[7 * (size_t)i + q] = near[q];
   }
 }
 
@@ -277,12 +266,17 @@ struct BlockRows {
   int size[UPD_BLOCK];
 };
 
-// Assigns the rows [0, rows) of an update block to blocks, as one pass in row
-// order would: row t joins the current block when it is fewer than BLOCK_MAX
-// rows past the block's start and bonded to an earlier row of it.  A row with
-// no strong bond to the three rows before it always starts a block, so each
-// such row's thread walks on to the next one and all walks run in parallel.
+// Preconditioner blocks of an update block's rows [0, rows).  First the
+// bonded runs, as one pass in row order would find them: row t joins the
+// current run when it is fewer than RUN_MAX rows past the run's start and
+// bonded to an earlier row of it.  A row with no strong bond to the three
+// rows before it always starts a run, so each such row's thread walks on to
+// the next one and all walks run in parallel.  Then whole consecutive runs
+// are packed greedily into blocks of at most BLOCK_MAX rows (thread 0, over
+// the run starts held as a bit mask); a block row finds its block start and
+// size from the mask of block starts.
 __device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, int rows, BlockRows& br) {
+  __shared__ unsigned long long run_mask[UPD_BLOCK / 64], blk_mask[UPD_BLOCK / 64];
   const int t = threadIdx.x;
   auto starts = [&](int r) { return r == 0 || (join_sh[r] & 7u) == 0; };
   if (t < rows && starts(t)) {
@@ -290,7 +284,7 @@ __device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, 
     br.first[t] = t;
     for (int r = t + 1; r < rows && !starts(r); ++r) {
       bool joins = false;
-      if (r - start < BLOCK_MAX) {
+      if (r - start < RUN_MAX) {
         for (int d = 1; d <= 3; ++d)
           if (((join_sh[r] >> (d - 1)) & 1u) && r - d >= start) joins = true;
       }
@@ -299,60 +293,71 @@ __device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, 
     }
   }
   __syncthreads();
+  const unsigned long long m = __ballot(t < rows && br.first[t] == t);
+  if ((t & 63) == 0) run_mask[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    // Runs start at s_0 = 0 < s_1 < ...; run k ends where run k+1 starts (or
+    // at rows).  A run that would take the current block past BLOCK_MAX rows
+    // opens a new block (runs are at most RUN_MAX <= BLOCK_MAX rows).
+    blk_mask[0] = 1ull;
+    for (int w = 1; w < UPD_BLOCK / 64; ++w) blk_mask[w] = 0ull;
+    int bstart = 0, run = 0;
+    for (int w = 0; w < UPD_BLOCK / 64; ++w) {
+      unsigned long long bits = run_mask[w];
+      while (bits) {
+        const int next = w * 64 + __ffsll((long long)bits) - 1;
+        bits &= bits - 1;
+        if (next == 0) continue;
+        if (next - bstart > BLOCK_MAX) {
+          bstart = run;
+          blk_mask[run >> 6] |= 1ull << (run & 63);
+        }
+        run = next;
+      }
+    }
+    if (rows - bstart > BLOCK_MAX) blk_mask[run >> 6] |= 1ull << (run & 63);
+  }
+  __syncthreads();
   if (t < rows) {
-    const int f = br.first[t];
-    int e = f + 1;
-    while (e < rows && br.first[e] == f) ++e;
-    br.size[t] = e - f;
+    // block start: the highest block-start bit at or below t; end: the next one above t (or rows)
+    int f = 0;
+    for (int w = t >> 6; w >= 0; --w) {
+      unsigned long long bits = blk_mask[w];
+      if (w == (t >> 6)) bits &= (t & 63) == 63 ? ~0ull : ((1ull << ((t & 63) + 1)) - 1);
+      if (bits) {
+        f = w * 64 + 63 - __clzll((long long)bits);
+        break;
+      }
+    }
+    int e = rows;
+    for (int w = t >> 6; w < UPD_BLOCK / 64; ++w) {
+      unsigned long long bits = blk_mask[w];
+      if (w == (t >> 6)) bits &= (t & 63) == 63 ? 0ull : ~((1ull << ((t & 63) + 1)) - 1);
+      if (bits) {
+        e = w * 64 + __ffsll((long long)bits) - 1;
+        break;
+      }
+    }
+    br.first[t] = f;
+    br.size[t] = (e < rows ? e : rows) - f;
   }
   __syncthreads();
 }
 
-// In-place inverse of a small SPD matrix (Gauss-Jordan, no pivoting needed).
-__device__ inline void invert_small(double (&a)[BLOCK_MAX][BLOCK_MAX]) {
-  // Gauss-Jordan without pivoting (H blocks are symmetric positive definite);
-  // fixed trip counts keep the matrices in registers
-  double inv[BLOCK_MAX][BLOCK_MAX];
-#pragma unroll
-  for (int r = 0; r < BLOCK_MAX; ++r)
-#pragma unroll
-    for (int c = 0; c < BLOCK_MAX; ++c) inv[r][c] = r == c ? 1.0 : 0.0;
-#pragma unroll
-  for (int p = 0; p < BLOCK_MAX; ++p) {
-    const double d = 1.0 / a[p][p];
-#pragma unroll
-    for (int c = 0; c < BLOCK_MAX; ++c) {
-      a[p][c] *= d;
-      inv[p][c] *= d;
-    }
-#pragma unroll
-    for (int r = 0; r < BLOCK_MAX; ++r) {
-      if (r == p) continue;
-      const double f = a[r][p];
-#pragma unroll
-      for (int c = 0; c < BLOCK_MAX; ++c) {
-        a[r][c] -= f * a[p][c];
-        inv[r][c] -= f * inv[p][c];
-      }
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < BLOCK_MAX; ++r)
-#pragma unroll
-    for (int c = 0; c < BLOCK_MAX; ++c) a[r][c] = inv[r][c];
-}
 
 __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __restrict__ type, FFDev ff,
                                                        const double2* __restrict__ rhs,
                                                        const double2* __restrict__ guess, double2* __restrict__ X,
                                                        double2* __restrict__ R, double4* __restrict__ ZP,
-                                                       float4* __restrict__ ZQ, const double* __restrict__ hnear,
+                                                       float4* __restrict__ ZQ, const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ nbr_j, const double* __restrict__ hval,
                                                        const unsigned char* __restrict__ join,
-                                                       double4* __restrict__ minv, int2* __restrict__ binfo,
+                                                       double* __restrict__ minv, int2* __restrict__ binfo,
                                                        double* __restrict__ eta_out, double* __restrict__ part_b,
                                                        double* __restrict__ part_bb, int nu, CgSys* __restrict__ cg,
                                                        CgCounters* __restrict__ cnt,
-                                                       const unsigned* __restrict__ abort_flag) {
+                                                       const unsigned* __restrict__ abort_flag, int n_all) {
   const int sys = block_system(S, blockIdx.x, false);
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.upd0;
@@ -363,59 +368,73 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
   const bool mine = t < rows;
   __shared__ unsigned char join_sh[UPD_BLOCK];
   __shared__ BlockRows br;
-  __shared__ double hb[UPD_BLOCK][BLOCK_MAX];
+  __shared__ double hb[UPD_BLOCK][BLOCK_MAX];  // each row's row of its H block, inverted in place
   __shared__ double2 bsh[UPD_BLOCK];
   join_sh[t] = mine ? join[i] : 0;
   __syncthreads();
   assign_blocks(join_sh, rows, br);
   double eta = 0.0;
   double2 b = make_double2(0.0, 0.0);
+  const int f = mine ? br.first[t] : t, m = mine ? br.size[t] : 0;
   if (mine) {
     const double* a = ff.atom + type[i] * AP_COUNT;
     eta = a[AP_ETA];
     b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
     bsh[t] = b;
-    // row t of its block of H: eta on the diagonal plus the edge-pass sums over all images to
-    // each block mate (block mates are at most three rows apart)
-    const int f = br.first[t], m = br.size[t];
-    for (int c = 0; c < BLOCK_MAX; ++c) {
-      const int d = f + c - t;
-      hb[t][c] = c < m ? (d == 0 ? eta : 0.0) + hnear[7 * (size_t)i + d + 3] : 0.0;
+    // row t of its block of H: eta on the diagonal plus h summed over all
+    // images to each block mate, from the row's neighbors in [g0, g0 + m)
+#pragma unroll
+    for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = c == t - f ? eta : 0.0;
+    const int g0 = row0 + f;
+    int lo = row_ptr[i], hi = row_ptr[i + 1];
+    const int e_end = hi;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (nbr_j[mid] < g0) lo = mid + 1;
+      else hi = mid;
+    }
+    for (int e = lo; e < e_end; ++e) {
+      const int jj = nbr_j[e];
+      if (jj >= g0 + m) break;
+      hb[t][jj - g0] += hval[e];
     }
   }
   __syncthreads();
-  __shared__ double inv_sh[UPD_BLOCK][BLOCK_MAX];
-  if (mine && br.first[t] == t) {
-    // the m x m block padded to BLOCK_MAX with the identity: the padding
-    // never mixes with the block, whose inverse comes out unchanged
-    const int m = br.size[t];
-    double a[BLOCK_MAX][BLOCK_MAX];
+  // In-place Gauss-Jordan inversion of every block, one thread per block row
+  // (no pivoting: the blocks of H are symmetric positive definite).  Each
+  // step reads the old pivot row before any row of the block is rewritten.
+  for (int p = 0; p < BLOCK_MAX; ++p) {
+    const bool act = mine && p < m;
+    double prow[BLOCK_MAX];
+    double pp = 1.0, fr = 0.0;
+    if (act) {
 #pragma unroll
-    for (int r = 0; r < BLOCK_MAX; ++r) {
-      const int row = t + r < UPD_BLOCK ? t + r : UPD_BLOCK - 1;
-#pragma unroll
-      for (int c = 0; c < BLOCK_MAX; ++c) a[r][c] = (r < m && c < m) ? hb[row][c] : (r == c ? 1.0 : 0.0);
+      for (int c = 0; c < BLOCK_MAX; ++c) prow[c] = hb[f + p][c];
+      pp = hb[f + p][p];
+      fr = hb[t][p];
     }
-    invert_small(a);
+    __syncthreads();
+    if (act) {
+      const double piv = 1.0 / pp;
+      if (t == f + p) {
 #pragma unroll
-    for (int r = 0; r < BLOCK_MAX; ++r) {
-      if (r < m) {
+        for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = (c == p ? 1.0 : prow[c]) * piv;
+      } else {
 #pragma unroll
-        for (int c = 0; c < BLOCK_MAX; ++c) inv_sh[t + r][c] = c < m ? a[r][c] : 0.0;
+        for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = (c == p ? 0.0 : hb[t][c]) - fr * ((c == p ? 1.0 : prow[c]) * piv);
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
   double vals[4] = {0, 0, 0, 0};
   double bb[2] = {0, 0};
   if (mine) {
-    const int f = br.first[t], m = br.size[t];
     double2 z = make_double2(0.0, 0.0);
     for (int c = 0; c < m; ++c) {
-      z.x += inv_sh[t][c] * bsh[f + c].x;
-      z.y += inv_sh[t][c] * bsh[f + c].y;
+      z.x += hb[t][c] * bsh[f + c].x;
+      z.y += hb[t][c] * bsh[f + c].y;
+      minv[(size_t)c * n_all + i] = hb[t][c];
     }
-    minv[i] = make_double4(inv_sh[t][0], inv_sh[t][1], inv_sh[t][2], inv_sh[t][3]);
     binfo[i] = make_int2(f, m);
     X[i] = guess ? guess[i] : make_double2(0.0, 0.0);
     R[i] = b;
@@ -455,13 +474,31 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
 
 // z = M^-1 r for one row of an update block, from the block's residuals in LDS
 // (bi = local row of the block start, block size).
-__device__ inline double2 apply_block(const double4& mi, const int2& bi, const double2* __restrict__ rsh) {
-  const int f = bi.x;
-  const double w[4] = {mi.x, mi.y, mi.z, mi.w};
+// A row's block-Jacobi inverse row (column-major storage: entry c of row i at
+// minv[c * stride + i]), loaded into registers in one unrolled batch so the
+// loads are in flight together.
+struct MinvRow {
+  double w[BLOCK_MAX];
+  int2 bi;
+};
+
+__device__ inline void load_minv(const double* __restrict__ minv, int stride, int i, bool mine,
+                                 const int2* __restrict__ binfo, MinvRow& m) {
+  m.bi = mine ? binfo[i] : make_int2(0, 0);
+#pragma unroll
+  for (int c = 0; c < BLOCK_MAX; ++c) m.w[c] = c < m.bi.y ? minv[(size_t)c * stride + i] : 0.0;
+}
+
+// z = M^-1 r for one row from the block's residuals in LDS
+__device__ inline double2 apply_block(const MinvRow& m, const double2* __restrict__ rsh) {
   double2 z = make_double2(0.0, 0.0);
-  for (int c = 0; c < bi.y; ++c) {
-    z.x += w[c] * rsh[f + c].x;
-    z.y += w[c] * rsh[f + c].y;
+#pragma unroll
+  for (int c = 0; c < BLOCK_MAX; ++c) {
+    if (c < m.bi.y) {
+      const double2 r = rsh[m.bi.x + c];
+      z.x += m.w[c] * r.x;
+      z.y += m.w[c] * r.y;
+    }
   }
   return z;
 }
@@ -731,7 +768,7 @@ __global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, co
 // Iteration k, second half: alpha, x += alpha p, r -= alpha A p, z = M^-1 r
 // (into the z half of zp), partial r.z and r.r per block.
 template <bool MIXED>
-__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const double4* __restrict__ minv,
+__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, int n_all, const double* __restrict__ minv,
                                                          const int2* __restrict__ binfo,
                                                          double2* __restrict__ X, double2* __restrict__ R,
                                                          double4* __restrict__ zp, float4* __restrict__ zq,
@@ -747,6 +784,9 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
   __shared__ double2 rsh[UPD_BLOCK];
   const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
   const bool mine = i < sd.atom0 + sd.n;
+  if (c.done != CG_RUNNING && !(c.done == CG_REFRESH && c.kr == k)) return;
+  MinvRow mrow;
+  load_minv(minv, n_all, i, mine, binfo, mrow);
   if (c.done == CG_REFRESH) {
     // A refresh requested by this iteration's direction launch waits for the
     // next one; a true residual computed by this iteration's direction launch
@@ -757,7 +797,7 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
     rsh[threadIdx.x] = r;
     __syncthreads();
     if (mine) {
-      const double2 z = apply_block(minv[i], binfo[i], rsh);
+      const double2 z = apply_block(mrow, rsh);
       zp[i] = make_double4(z.x, z.y, 0.0, 0.0);
       if (MIXED) zq[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
       vals[0] = r.x * z.x;
@@ -797,7 +837,7 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, const
   rsh[threadIdx.x] = r;
   __syncthreads();
   if (mine) {
-    const double2 z = apply_block(minv[i], binfo[i], rsh);
+    const double2 z = apply_block(mrow, rsh);
     reinterpret_cast<double2*>(zp + i)[0] = z;
     if (MIXED) reinterpret_cast<float2*>(zq + i)[0] = make_float2((float)z.x, (float)z.y);
     vals[0] = r.x * z.x;
@@ -852,11 +892,10 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, unsigned char* join, double* hnear,
-                        hipStream_t s) {
+                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
   XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr, nbr_j, nbr_s,
-            bond_cut, hval, hval32, bflag, bond_count, join, hnear);
+            bond_cut, hval, hval32, bflag, bond_count, join);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
@@ -888,6 +927,8 @@ void xr_cg_blocks(int n, int* ndir, int* nupd) {
 
 size_t xr_cg_partials(int nd, int nu) { return (size_t)(2 * 4 * nu + 2 * nu + 2 * nd) + 64; }
 
+size_t xr_cg_block_doubles(int n) { return (size_t)n * BLOCK_MAX; }
+
 size_t xr_cg_state_bytes(int ns) { return sizeof(CgSys) * (size_t)ns + sizeof(CgCounters) + 64; }
 
 hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
@@ -903,7 +944,8 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   float4* zq1 = mixed ? a.ZQ1 : nullptr;
   // counters start at zero (cleared with the control block)
   hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.X, a.R, a.ZP0,
-                     zq0, a.hnear, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt, a.abort_flag);
+                     zq0, a.row_ptr, a.nbr_j, a.hval, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt,
+                     a.abort_flag, a.n);
   XR_LAUNCH_CHECK();
   int k = 0;
   // Every system settles within max_iter executed iterations plus two
@@ -921,13 +963,13 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
         XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<true>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
                   nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.X, a.R);
-        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.minv, a.binfo, a.X, a.R,
+        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.n, a.minv, a.binfo, a.X, a.R,
                   zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
       } else {
         XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<false>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
                   a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
                   nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.X, a.R);
-        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.minv, a.binfo, a.X, a.R,
+        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.n, a.minv, a.binfo, a.X, a.R,
                   zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
       }
       ++k;

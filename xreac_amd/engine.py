@@ -47,7 +47,7 @@ class DeviceError(RuntimeError):
 
 
 class Engine:
-    def __init__(self, force_field, device=0):
+    def __init__(self, force_field, device=0, stream=None):
         import torch
 
         if not torch.cuda.is_available():
@@ -66,10 +66,12 @@ class Engine:
         )
         self._ff = C.c_void_p()
         self._check(self.lib.xreac_ff_create(C.byref(tab), C.byref(self._ff)), None)
-        stream = torch.cuda.current_stream(self.device).cuda_stream
+        # the context launches on `stream` (a torch.cuda.Stream; default: the current stream)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
         self._ctx = C.c_void_p()
         with torch.cuda.device(self.device):
-            self._check(self.lib.xreac_ctx_create(device, C.c_void_p(stream), C.byref(self._ctx)), None)
+            self._check(self.lib.xreac_ctx_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(self._ctx)),
+                        None)
 
     def close(self):
         if getattr(self, "_ctx", None):

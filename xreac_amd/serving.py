@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -184,7 +185,9 @@ class Server:
         for name in force_fields:
             ff = ForceField.bundled(name)
             self._ff[name] = ff
-            self.engines[name] = Engine(ff, device)
+            # one stream per force field (default priority): the groups' steps overlap on the device
+            self.engines[name] = Engine(ff, device, stream=torch.cuda.Stream(device))
+        self._pool = ThreadPoolExecutor(max_workers=max(1, len(self.engines)))
         self.precision = precision
         self.warm_start = warm_start
         self.tol = tol
@@ -226,17 +229,36 @@ class Server:
             active += 1
             atoms += job.n
         for name, jobs in admitted.items():
-            self.active[name].append(jobs, self._ff[name])
+            group = self.active[name]
+            with self.torch.cuda.stream(group.eng.stream):  # the group's tensors live on its stream
+                group.append(jobs, self._ff[name])
 
     def step(self):
-        """Admit what fits, then evaluate every active request once."""
+        """Admit what fits, then evaluate every active request once.
+
+        The force fields' groups run on their own streams from a thread each
+        (the library releases the interpreter lock while it works), so one
+        group's host work overlaps the other's kernels; results are merged in
+        a fixed order.
+        """
         self._admit()
-        for group in self.active.values():
-            if group.jobs:
-                self._run_group(group)
+        groups = [g for g in self.active.values() if g.jobs]
+        if len(groups) > 1:
+            results = list(self._pool.map(self._run_group, groups))
+        else:
+            results = [self._run_group(g) for g in groups]
+        for evals, responses, freed in results:
+            self.atom_evals += evals
+            self.responses.extend(responses)
+            self.used_bytes -= freed
         self.steps += 1
 
     def _run_group(self, g):
+        """One step of a group on its engine's stream; returns (atom evaluations, finished responses, bytes freed)."""
+        with self.torch.cuda.stream(g.eng.stream):
+            return self._step_group(g)
+
+    def _step_group(self, g):
         torch = self.torch
         eng, dev = g.eng, g.dev
         ns = len(g.jobs)
@@ -263,7 +285,6 @@ class Server:
         if g.relax.any():
             eng.fire_step(off_dev, g.X, g.V, out["forces"], g.FS, g.ST, self.relax_tolerance, self.relax_iterations)
             fire_status = g.ST.cpu().numpy()
-        self.atom_evals += n
         g.F = out["forces"]
         comps = res["components"]
         pot = comps.sum(axis=1)
@@ -277,12 +298,13 @@ class Server:
             finished = fire_status[k] != 0 if g.relax[k] else j.evals > j.req.get("md_steps", 0)
             done[k] = errors[k] is not None or finished
         if not done.any():
-            return
+            return n, [], 0
         F = out["forces"].cpu().numpy()
         Q = out["charges"].cpu().numpy()
         Xh = g.X.cpu().numpy() if (g.md | g.relax)[done].any() else None
         Vh = g.V.cpu().numpy() if g.md[done].any() else None
         now = self.clock()
+        finished, freed = [], 0
         for k in np.flatnonzero(done):
             j = g.jobs[k]
             lo, hi = offsets[k], offsets[k + 1]
@@ -298,9 +320,10 @@ class Server:
                 r.positions = Xh[lo:hi].copy()
                 r.potential = np.array(j.potential)
                 r.converged = bool(fire_status[k] == 1)
-            self.responses.append(r)
-            self.used_bytes -= j.bytes
+            finished.append(r)
+            freed += j.bytes
         g.drop(done)
+        return n, finished, freed
 
     # ------------------------------------------------------------------- run
     def idle(self):
