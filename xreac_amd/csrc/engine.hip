@@ -783,7 +783,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   HIPCHK(hipEventRecord(ctx->bonds_ready, s), "event");
   HIPCHK(hipStreamWaitEvent(s2, ctx->bonds_ready, 0), "event");
   {
-    hipError_t e = hipSuccess;
+    hipError_t e = xr_vdw(at, ff, row_ptr, nbr_j, nbr_s, acc, s2);
+    HIPCHK(e, "van der Waals");
     XR_TIMED(XT_BOND_FORWARD, s2, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s2));
     HIPCHK(e, "bond orders");
     XR_TIMED(XT_MANYBODY, s2, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, bflag, lammps, acc, s2));

@@ -15,11 +15,19 @@ namespace {
 constexpr double C_ELE = 332.06371;
 constexpr double SELF_CONVERSION = 23.02;
 
-// When dEdq is given, lane 0 also writes (dE/dq_i, -1): the right-hand sides
-// of the charge-response adjoint solve.
+// PART selects the terms: NB_VDW (van der Waals, no charges needed: runs
+// beside QEq on the second stream and writes its forces into F_hb, which the
+// H-bond kernels then accumulate into), NB_COUL (Coulomb and the QEq self
+// energy, after QEq, forces into F_nb), or both into F_nb.  With NB_COUL,
+// lane 0 also writes (dE/dq_i, -1) when dEdq is given: the right-hand sides of
+// the charge-response adjoint solve.
+enum NbPart { NB_VDW = 1, NB_COUL = 2, NB_ALL = 3 };
+
+template <int PART>
 __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int* __restrict__ row_ptr,
                                                    const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
                                                    EnergyAcc acc, double2* __restrict__ dEdq) {
+  constexpr bool VDW = (PART & NB_VDW) != 0, COUL = (PART & NB_COUL) != 0;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i < at.n) {
@@ -29,7 +37,7 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
     const double pw = g[28];
     const double inv_pw = 1.0 / pw;
     const int ti = at.type[i];
-    const double qi = at.q[i];
+    const double qi = COUL ? at.q[i] : 0.0;
     const bool shielded = ff.vdw_type == 1 || ff.vdw_type == 3;
     const bool wall = ff.vdw_type == 2 || ff.vdw_type == 3;
     double ev_sum = 0.0, ec_sum = 0.0, qs_sum = 0.0, F0 = 0.0, F1 = 0.0, F2 = 0.0;
@@ -43,55 +51,67 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
       double t, dt;
       xr_taper(r, rc, t, dt);
       if (t == 0.0 && dt == 0.0) continue;
-      double f13, df13;
-      if (shielded) {
-        // f13 = (r^p + (1/gamma_w)^p)^(1/p); d f13/dr = r^(p-1) f13 / base
-        const double rp = exp(pw * log(r));
-        const double base = rp + ff.vdw_b[pij];
-        f13 = exp(log(base) * inv_pw);
-        df13 = (rp * f13) / (r * base);
-      } else {
-        f13 = r;
-        df13 = 1.0;
+      double dtot = 0.0;
+      if constexpr (VDW) {
+        double f13, df13;
+        if (shielded) {
+          // f13 = (r^p + (1/gamma_w)^p)^(1/p); d f13/dr = r^(p-1) f13 / base
+          const double rp = exp(pw * log(r));
+          const double base = rp + ff.vdw_b[pij];
+          f13 = exp(log(base) * inv_pw);
+          df13 = (rp * f13) / (r * base);
+        } else {
+          f13 = r;
+          df13 = 1.0;
+        }
+        const double ex = exp(0.5 * p[PP_ALPHA] * (1 - f13 / p[PP_RVDW]));
+        const double ev = p[PP_D] * (ex * ex - 2 * ex);
+        const double dev = p[PP_D] * (2 * ex - 2) * ex * (-0.5 * p[PP_ALPHA] / p[PP_RVDW]) * df13;
+        double phi = t * ev;  // per-direction vdW energy (times 2)
+        double dphi = dt * ev + t * dev;
+        if (wall) {
+          const double ecr = p[PP_ECORE] * exp(p[PP_ACORE] * (1 - r / p[PP_RCORE]));
+          phi += t * ecr;
+          dphi += dt * ecr + t * ecr * (-p[PP_ACORE] / p[PP_RCORE]);
+        }
+        ev_sum += phi;
+        dtot += dphi;
       }
-      const double ex = exp(0.5 * p[PP_ALPHA] * (1 - f13 / p[PP_RVDW]));
-      const double ev = p[PP_D] * (ex * ex - 2 * ex);
-      const double dev = p[PP_D] * (2 * ex - 2) * ex * (-0.5 * p[PP_ALPHA] / p[PP_RVDW]) * df13;
-      double phi = t * ev;  // per-direction vdW energy (times 2)
-      double dphi = dt * ev + t * dev;
-      if (wall) {
-        const double ecr = p[PP_ECORE] * exp(p[PP_ACORE] * (1 - r / p[PP_RCORE]));
-        phi += t * ecr;
-        dphi += dt * ecr + t * ecr * (-p[PP_ACORE] / p[PP_RCORE]);
+      if constexpr (COUL) {
+        double s, ds;
+        xr_shield_tapered(r, t, dt, p[PP_GAMMA], s, ds);
+        const double qj = at.q[j];
+        const double qq = qi * qj;
+        ec_sum += qq * s;
+        qs_sum += qj * s;
+        dtot += C_ELE * qq * ds;
       }
-      double s, ds;
-      xr_shield_tapered(r, t, dt, p[PP_GAMMA], s, ds);
-      const double qj = at.q[j];
-      const double qq = qi * qj;
-      ev_sum += phi;
-      ec_sum += qq * s;
-      qs_sum += qj * s;
-      const double dtot = (dphi + C_ELE * qq * ds) / r;
+      dtot /= r;
       F0 += dtot * v[0];
       F1 += dtot * v[1];
       F2 += dtot * v[2];
     }
-    ev_sum = xr_wave_sum(ev_sum);
-    ec_sum = xr_wave_sum(ec_sum);
-    if (dEdq) qs_sum = xr_wave_sum(qs_sum);
+    if constexpr (VDW) ev_sum = xr_wave_sum(ev_sum);
+    if constexpr (COUL) {
+      ec_sum = xr_wave_sum(ec_sum);
+      if (dEdq) qs_sum = xr_wave_sum(qs_sum);
+    }
     F0 = xr_wave_sum(F0);
     F1 = xr_wave_sum(F1);
     F2 = xr_wave_sum(F2);
     if (lane == 0) {
       const double* a = ff.atom + ti * AP_COUNT;
-      xr_energy(acc, CH_VDW, i, 0.5 * ev_sum);
-      xr_energy(acc, CH_COUL, i, 0.5 * C_ELE * ec_sum);
-      xr_energy(acc, CH_QEQ, i, SELF_CONVERSION * (a[AP_CHI] * qi + 0.5 * a[AP_ETA] * qi * qi));
-      // dE/dq_i = 332.06371 sum_e q_j s_e + 23.02 (chi_i + eta_i q_i)
-      if (dEdq) dEdq[i] = make_double2(C_ELE * qs_sum + SELF_CONVERSION * (a[AP_CHI] + a[AP_ETA] * qi), -1.0);
-      at.F_nb[3 * i] = F0;
-      at.F_nb[3 * i + 1] = F1;
-      at.F_nb[3 * i + 2] = F2;
+      if constexpr (VDW) xr_energy(acc, CH_VDW, i, 0.5 * ev_sum);
+      if constexpr (COUL) {
+        xr_energy(acc, CH_COUL, i, 0.5 * C_ELE * ec_sum);
+        xr_energy(acc, CH_QEQ, i, SELF_CONVERSION * (a[AP_CHI] * qi + 0.5 * a[AP_ETA] * qi * qi));
+        // dE/dq_i = 332.06371 sum_e q_j s_e + 23.02 (chi_i + eta_i q_i)
+        if (dEdq) dEdq[i] = make_double2(C_ELE * qs_sum + SELF_CONVERSION * (a[AP_CHI] + a[AP_ETA] * qi), -1.0);
+      }
+      double* F = PART == NB_VDW ? at.F_hb : at.F_nb;
+      F[3 * i] = F0;
+      F[3 * i + 1] = F1;
+      F[3 * i + 2] = F2;
     }
   }
 }
@@ -154,10 +174,18 @@ __global__ void __launch_bounds__(256) k_energy_reduce(Systems S, EnergyAcc acc,
 
 }  // namespace
 
+hipError_t xr_vdw(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s, EnergyAcc acc,
+                  hipStream_t s) {
+  const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
+  hipLaunchKernelGGL(k_nonbonded<NB_VDW>, grid, dim3(256), 0, s, at, ff, row_ptr, nbr_j, nbr_s, acc, nullptr);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
 hipError_t xr_nonbonded(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
                         EnergyAcc acc, double2* dEdq, hipStream_t s) {
   const dim3 grid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
-  XR_LAUNCH(XT_NONBONDED, k_nonbonded, grid, dim3(256), 0, s, at, ff, row_ptr, nbr_j, nbr_s, acc, dEdq);
+  XR_LAUNCH(XT_NONBONDED, k_nonbonded<NB_COUL>, grid, dim3(256), 0, s, at, ff, row_ptr, nbr_j, nbr_s, acc, dEdq);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

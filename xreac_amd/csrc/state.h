@@ -66,6 +66,10 @@ hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* ro
                        const int* nbr_s, const unsigned char* eflag, int lammps_angles, EnergyAcc acc,
                        hipStream_t s);
 // pairs.hip
+// van der Waals (positions only; second stream): forces written into F_hb, before the H-bond kernels add theirs.
+hipError_t xr_vdw(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s, EnergyAcc acc,
+                  hipStream_t s);
+// Coulomb and QEq self energy (after QEq): forces into F_nb.
 hipError_t xr_nonbonded(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s,
                         EnergyAcc acc, double2* dEdq, hipStream_t s);
 // Adds the charge-response term (full derivative through QEq) to forces.
