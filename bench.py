@@ -35,15 +35,15 @@ def log(msg):
 def kernel_bytes(name, n, m, nb):
     """Algorithmic HBM bytes per launch of a timed kernel class.
 
-    cg_direction: the implicit QEq matrix read once in half storage (value 8 B
-    + column 4 B per unordered pair, P = M/2) plus 72 B per atom (read z and
-    previous p, write p and A p as FP64 pairs, row pointer).
+    cg_iteration: the implicit QEq matrix read once in half storage (value 8 B
+    + column 4 B per unordered pair, P = M/2) plus 436 B per atom: the eight
+    recurrence vectors read and written (x, r, u, w, z, q, s, p as FP64
+    pairs: 256 B), m read once and written once (32 B), the block-Jacobi row
+    (16 doubles: 128 B), eta, block placement and row pointer (20 B).
     """
     P = m / 2
-    if name == "cg_direction":
-        return 12 * P + 72 * n
-    if name == "cg_update":
-        return 112 * n  # read/write x, r (FP64 pairs), read p, Ap, write z, diag
+    if name == "cg_iteration":
+        return 12 * P + 436 * n
     if name == "nonbonded":
         return 16 * P + 56 * n  # half list (j + image code) + positions, types, charges, forces
     if name == "edge":

@@ -45,14 +45,15 @@ enum {
 };
 
 /* Kernel timers (xreac_profile_read): neighbor count, neighbor fill, edge
- * pass, CG direction+A p sweep, CG update, nonbonded, bond-order forward,
- * many-body forward, adjoint. */
+ * pass, CG iteration (A m sweep and vector recurrences), CG set-up (chunks and
+ * block-Jacobi inverse), nonbonded, bond-order forward, many-body forward,
+ * adjoint. */
 enum {
   XREAC_TIMER_NEIGHBOR_COUNT = 0,
   XREAC_TIMER_NEIGHBOR_FILL = 1,
   XREAC_TIMER_EDGE = 2,
-  XREAC_TIMER_CG_DIRECTION = 3,
-  XREAC_TIMER_CG_UPDATE = 4,
+  XREAC_TIMER_CG_ITERATION = 3,
+  XREAC_TIMER_CG_INIT = 4,
   XREAC_TIMER_NONBONDED = 5,
   XREAC_TIMER_BOND_FORWARD = 6,
   XREAC_TIMER_MANYBODY = 7,

@@ -36,8 +36,8 @@ TIMERS = (
     "neighbor_count",
     "neighbor_fill",
     "edge",
-    "cg_direction",
-    "cg_update",
+    "cg_iteration",
+    "cg_init",
     "nonbonded",
     "bond_forward",
     "manybody",

@@ -92,7 +92,7 @@ struct xreac_ctx {
   char* ctl_host = nullptr;  // pinned mirror of the control block
   size_t ctl_host_bytes = 0;
   XrProfiler prof;
-  int last_cg_iters = 0;
+  int last_cg_launches = 0;
   double m_per_atom = 0.0;  // edges and bonds per atom of the last evaluation
   double b_per_atom = 0.0;
   int max_row = 0;  // longest neighbor row of the last built list
@@ -496,8 +496,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<int>(off, n + 1);           // bond_ptr
   carve_size<int>(off, scan_ints);       // scan scratch
   carve_size<char>(off, xr_bin_record_bytes() * n);  // bin-ordered atom records
-  carve_size<double2>(off, 7 * (size_t)n);  // CG vectors: x, r, Ap (pairs), two (z, p) records
-  carve_size<float4>(off, 2 * (size_t)n);   // FP32 (z, p) records (mixed precision)
+  carve_size<double2>(off, 10 * (size_t)n);  // CG vectors: x, r, u, w, z, q, s, p and two m (pairs)
+  carve_size<float2>(off, 2 * (size_t)n);    // FP32 copies of the two m (mixed precision)
+  carve_size<int2>(off, nu);                 // CG chunk ranges
   carve_size<double>(off, xr_cg_block_doubles(n));  // block-Jacobi inverse rows
   carve_size<int2>(off, n);              // block placement
   carve_size<unsigned char>(off, n);     // block join flags
@@ -520,8 +521,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   int* bond_ptr = cv.take<int>(n + 1);
   int* scan_tmp = cv.take<int>(scan_ints);
   void* binned = cv.take<char>(xr_bin_record_bytes() * n);
-  double2* cgv = cv.take<double2>(7 * (size_t)n);
-  float4* cgq = cv.take<float4>(2 * (size_t)n);
+  double2* cgv = cv.take<double2>(10 * (size_t)n);
+  float2* cgq = cv.take<float2>(2 * (size_t)n);
+  int2* cg_chunks = cv.take<int2>(nu);
   double* minv = cv.take<double>(xr_cg_block_doubles(n));
   int2* binfo = cv.take<int2>(n);
   unsigned char* join = cv.take<unsigned char>(n);
@@ -781,18 +783,28 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   }
   hipStream_t s2 = ctx->side;
   HIPCHK(hipEventRecord(ctx->bonds_ready, s), "event");
-  HIPCHK(hipStreamWaitEvent(s2, ctx->bonds_ready, 0), "event");
-  {
-    hipError_t e = xr_vdw(at, ff, row_ptr, nbr_j, nbr_s, acc, s2);
-    HIPCHK(e, "van der Waals");
+  // Issued from inside the QEq solve once its first iterations are queued, so
+  // the main stream does not sit idle while the host makes these launches.
+  const char* side_what = "side stream";
+  auto launch_side = [&]() -> hipError_t {
+    hipError_t e = hipStreamWaitEvent(s2, ctx->bonds_ready, 0);
+    if (e != hipSuccess) return e;
+    side_what = "van der Waals";
+    if ((e = xr_vdw(at, ff, row_ptr, nbr_j, nbr_s, acc, s2)) != hipSuccess) return e;
+    side_what = "bond orders";
     XR_TIMED(XT_BOND_FORWARD, s2, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s2));
-    HIPCHK(e, "bond orders");
+    if (e != hipSuccess) return e;
+    side_what = "many-body";
     XR_TIMED(XT_MANYBODY, s2, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, bflag, lammps, acc, s2));
-    HIPCHK(e, "many-body");
+    if (e != hipSuccess) return e;
+    side_what = "adjoint";
     XR_TIMED(XT_BACKWARD, s2, e = xr_bond_backward(at, bd, ff, s2));
-    HIPCHK(e, "adjoint");
-  }
-  HIPCHK(hipEventRecord(ctx->side_done, s2), "event");
+    if (e != hipSuccess) return e;
+    side_what = "event";
+    return hipEventRecord(ctx->side_done, s2);
+  };
+  using SideLaunch = decltype(launch_side);
+  bool side_issued = false;
   auto join_side = [&]() { return hipStreamWaitEvent(s, ctx->side_done, 0); };
 
   // ------------------------------------------------------------------ QEq
@@ -806,13 +818,13 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.nbr_j = nbr_j;
   cg.hval = hval;
   cg.hval32 = hval32;
-  cg.X = cgv;
-  cg.R = cgv + n;
-  cg.AP = cgv + 2 * (size_t)n;
-  cg.ZP0 = reinterpret_cast<double4*>(cgv + 3 * (size_t)n);
-  cg.ZP1 = reinterpret_cast<double4*>(cgv + 5 * (size_t)n);
-  cg.ZQ0 = cgq;
-  cg.ZQ1 = cgq + n;
+  {
+    double2* vecs[10];
+    for (int q = 0; q < 10; ++q) vecs[q] = cgv + q * (size_t)n;
+    cg.v = CgVec{vecs[0], vecs[1], vecs[2], vecs[3], vecs[4], vecs[5], vecs[6], vecs[7],
+                 {vecs[8], vecs[9]}, {cgq, cgq + n}};
+  }
+  cg.chunks = cg_chunks;
   cg.join = join;
   cg.minv = minv;
   cg.n = n;
@@ -822,8 +834,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.state = dc + L.cg;
   cg.tol = opt->qeq_tolerance > 0 ? opt->qeq_tolerance : 1e-10;
   cg.max_iter = opt->qeq_max_iterations > 0 ? opt->qeq_max_iterations : 2000;
-  // Read back convergence first near the previous solve's iteration count.
-  cg.first_check = ctx->last_cg_iters > 0 ? ctx->last_cg_iters + 2 : 16;
+  // Read back convergence first just past the launch the previous solve settled in.
+  cg.first_check = ctx->last_cg_launches > 0 ? ctx->last_cg_launches + 1 : 16;
   cg.check_every = 2;
   cg.guess = (opt->warm_start && out.qeq_state) ? reinterpret_cast<const double2*>(out.qeq_state) : nullptr;
   cg.keep = reinterpret_cast<double2*>(out.qeq_state);
@@ -833,15 +845,30 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.poll_host = ctx->poll_host;
   cg.poll_ev[0] = ctx->poll_ev[0];
   cg.poll_ev[1] = ctx->poll_ev[1];
+  hipError_t side_err = hipSuccess;
+  struct SideHook {
+    SideLaunch* fn;
+    bool* issued;
+    hipError_t* err;
+  } hook{&launch_side, &side_issued, &side_err};
+  cg.after_start = [](void* p) {
+    SideHook* h = static_cast<SideHook*>(p);
+    *h->issued = true;
+    return *h->err = (*h->fn)();
+  };
+  cg.after_start_arg = &hook;
   if (!given_q) {
     CgOutcome cgo{};
     const hipError_t e = xr_cg_solve(cg, s, &cgo);
-    HIPCHK(join_side(), "event");
+    if (side_issued) HIPCHK(join_side(), "event");
+    if (side_err != hipSuccess) return hip_fail(ctx, side_err, side_what);
     HIPCHK(e, "QEq");
     if (cgo.aborted) {
       *overflow = true;
       return XREAC_OK;
     }
+  } else {
+    HIPCHK(launch_side(), side_what);
   }
 
   // ----------------------------------------------------- energy and forces
@@ -858,6 +885,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     adj.q = lam;
     adj.mu = nullptr;
     adj.state = dc + L.cg2;
+    adj.after_start = nullptr;
     CgOutcome o2{};
     HIPCHK(xr_cg_solve(adj, s, &o2), "QEq adjoint");
     if (o2.aborted) {
@@ -890,15 +918,15 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   const CgSys* h_cg = xr_cg_systems(hc + L.cg);
   const CgSys* h_cg2 = xr_cg_systems(hc + L.cg2);
   ctx->sys_messages.assign(ns, std::string());
-  int max_iters = 0;
+  int launches = 0;
   for (int q = 0; q < ns; ++q) {
     out.iters[q] = given_q ? 0 : h_cg[q].iters;
-    max_iters = std::max(max_iters, h_cg[q].iters);
+    if (!given_q && h_cg[q].ctl[0].done == CG_DONE) launches = std::max(launches, h_cg[q].ks + 1);
     xr_components(h_comps + (size_t)q * NCH, out.comps + (size_t)q * NCOMP);
     if (h_serr[q]) {
       out.status[q] = XREAC_INVALID;
       ctx->sys_messages[q] = error_text(h_serr[q]);
-    } else if ((!given_q && h_cg[q].done != CG_DONE) || (full && h_cg2[q].done != CG_DONE)) {
+    } else if ((!given_q && h_cg[q].ctl[0].done != CG_DONE) || (full && h_cg2[q].ctl[0].done != CG_DONE)) {
       out.status[q] = XREAC_INVALID;
       ctx->sys_messages[q] = "QEq conjugate gradient did not converge; check the geometry";
     } else {
@@ -910,7 +938,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   ctx->m_per_atom = (double)m / n;
   ctx->b_per_atom = (double)nb_total / n;
   if (!supplied) ctx->max_row = host_counts[2];
-  ctx->last_cg_iters = max_iters;
+  ctx->last_cg_launches = launches;
 
   ctx->n = n;
   ctx->m = m;

@@ -11,7 +11,7 @@
 #include <vector>
 
 enum XrTimer {
-  XT_NBR_COUNT, XT_NBR_FILL, XT_EDGE, XT_CG_DIRECTION, XT_CG_UPDATE, XT_NONBONDED, XT_BOND_FORWARD, XT_MANYBODY,
+  XT_NBR_COUNT, XT_NBR_FILL, XT_EDGE, XT_CG_ITERATION, XT_CG_INIT, XT_NONBONDED, XT_BOND_FORWARD, XT_MANYBODY,
   XT_BACKWARD, XT_COUNT
 };
 

@@ -2,19 +2,31 @@
 #pragma once
 #include "common.h"
 
-// CG_REFRESH: the system's next direction launch replaces the recurrence
-// residual by the FP64 true residual b - A x (and z, p = 0), the update
-// launch after it reduces the new residual, and the recurrence restarts.
+// CG_REFRESH: the system's next launch replaces the recurrence residual by
+// the FP64 true residual r = b - A x (and u = M^-1 r), the launch after it
+// computes w = A u, and the recurrence restarts from there.
 enum CgState { CG_RUNNING = 0, CG_DONE = 1, CG_REFRESH = 2, CG_FAILED = 3, CG_STALLED = 4 };
 
-// Per-system solver state (device).
-struct CgSys {
+// Solver control of one system as one launch leaves it for the next.
+struct CgCtl {
   int done;       // CgState
-  int iters;      // iterations executed by this system
-  int k0;         // global iteration index of the last (re)start (set by the refresh)
+  int k0;         // launch of the last (re)start's first iteration
+  int kr;         // CG_REFRESH: launch that computes the true residual
   int restarts;
-  int kr;         // CG_REFRESH: launch whose direction kernel computes the true residual
-  double rr0[2];  // true squared residuals at the last restart (mixed precision)
+  int it;         // iterations executed
+  double rr0[2];  // squared residuals at the last (re)start (mixed precision)
+  double gam[2];  // r.u of the last iteration, per right-hand side
+  double alp[2];  // step of the last iteration
+};
+
+// Per-system solver state (device).  Launch k reads ctl[(k - 1) & 1] and
+// its first block writes ctl[k & 1], so no block reads a slot that another
+// block of the same launch writes (final states, which stop every block, go
+// to both).
+struct CgSys {
+  CgCtl ctl[2];
+  int iters;  // iterations when the system settled
+  int ks;     // launch in which it settled
 };
 
 struct CgCounters {
@@ -25,29 +37,35 @@ struct CgCounters {
 // The state block holds the counters, then CgSys[ns] at byte offset 64.
 inline CgSys* xr_cg_systems(void* state) { return reinterpret_cast<CgSys*>(static_cast<char*>(state) + 64); }
 
+// CG vectors, (s, t) pairs per atom (pipelined CG recurrences).
+struct CgVec {
+  double2 *X, *R, *U, *W, *Z, *Q, *S, *P;
+  double2* M[2];   // M^-1 w, alternating by launch (gathered by the next launch)
+  float2* M32[2];  // FP32 copies gathered by the mixed-precision sweep
+};
+
 struct CgArgs {
   Systems S;
-  int nd, nu;          // total direction / update blocks over all systems
+  int nd, nu;          // total iteration blocks / chunks over all systems
   const int* type;
   FFDev ff;
   const int* row_ptr;
   const int* nbr_j;
   const double* hval;
-  const float* hval32;  // non-null: mixed precision (FP32 A p sweeps, FP64 restarts)
-  double2 *X, *R, *AP;
-  double4 *ZP0, *ZP1;  // (z_s, z_t, p_s, p_t), alternating by iteration
-  float4 *ZQ0, *ZQ1;   // FP32 copies gathered by the mixed-precision sweep
+  const float* hval32;  // non-null: mixed precision (FP32 A m sweeps, FP64 restarts)
+  CgVec v;
   const unsigned char* join;  // per atom: strong bonds to the previous three atoms (edge pass)
   int n;                      // atoms over all systems
   double* minv;               // block-Jacobi inverse rows, column-major: entry c of row i at minv[c * n + i]
-  int2* binfo;                // per atom: (local row of its block start, block size)
+  int2* binfo;                // per atom: (chunk row of its block start, block size)
+  int2* chunks;               // per chunk: atom range [x, y)
   double* eta;
   double* partials;  // xr_cg_partials(nd, nu) doubles
   void* state;       // xr_cg_state_bytes(ns) bytes
   double tol;        // relative residual per right-hand side
   int max_iter;      // per system
-  int first_check;   // iterations before the first read-back
-  int check_every;   // iterations between later read-backs
+  int first_check;   // launches before the first read-back
+  int check_every;   // launches between later read-backs
   const double2* rhs;    // optional right-hand sides per atom (default (-chi, -1))
   bool neutral;          // normalise the result to zero sum instead of the systems' total charges
   const double2* guess;  // optional starting (s, t) per atom
@@ -57,6 +75,11 @@ struct CgArgs {
   const unsigned* abort_flag;  // device word; nonzero stops the solve (buffer overflow)
   int* poll_host;              // pinned host memory, two slots of two ints (settled, abort)
   hipEvent_t poll_ev[2];       // completion of each slot's copy
+  // Optional host call made once the first chunk of iterations is queued and
+  // before the first wait: the caller issues independent work to other streams
+  // there, so the solve starts without waiting for those launches.
+  hipError_t (*after_start)(void*);
+  void* after_start_arg;
 };
 
 struct CgOutcome {

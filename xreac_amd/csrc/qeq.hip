@@ -8,27 +8,29 @@
 // (self-image edges e = (i, i, S != 0) land on the diagonal exactly as
 // pair_sum does).  The constrained minimum is q = s + mu t with H s = -chi,
 // H t = -1, mu = (Q - sum s) / sum t; s and t share every sweep over the
-// edges (one 16-byte gather of (p_s, p_t) per edge).
+// edges (one 16-byte gather of (m_s, m_t) per edge).
 //
-// One CG iteration is two kernels.  Each consumer block re-reduces the
-// producer's per-block partials in a fixed order, so no separate reduction
+// The solver is the pipelined preconditioned CG of Ghysels and Vanroose
+// (2014): its only matrix product per iteration, n = A m, reads a vector the
+// previous iteration finished, and its two dot products come from the same
+// pass, so one CG iteration is one kernel.  Each block re-reduces the previous
+// launch's per-block partials in a fixed order, so no separate reduction
 // launches are needed and the result is deterministic.
 //
-// Batches: every system owns a fixed set of blocks in both CG kernels (sized
-// from its atom count only), reduces only its own partials and stops on its
-// own convergence test, so a system's charges are bitwise the same alone and
-// in any batch.
+// Batches: every system owns a fixed set of blocks and chunks (sized from its
+// atom count only), reduces only its own partials and stops on its own
+// convergence test, so a system's charges are bitwise the same alone and in
+// any batch.
 //
-// Mixed precision: the A p sweep reads FP32 copies of H and of the gathered
-// (z, p) records while x, r, z, p and every dot product stay FP64.  When the
+// Mixed precision: the A m sweep reads FP32 copies of H and of the gathered m
+// while the recurrence vectors and every dot product stay FP64.  When the
 // recurrence residual has dropped by the inner factor, the system's next
-// direction launch replaces it by the FP64 true residual r = b - A x (one
-// FP64 sweep over its rows), the update launch after it reduces the new
-// residual, and the recurrence restarts from there (iterative refinement); a
-// system is
-// converged only when a true residual meets the tolerance.  A warm start
-// begins with the same refresh.  All of it runs on the device: the host only
-// polls how many systems have settled.
+// launch replaces it by the FP64 true residual r = b - A x (one FP64 sweep),
+// the launch after it computes w = A u in FP64, and the recurrence restarts
+// from there (iterative refinement); a system is converged only when a true
+// residual meets the tolerance.  A warm start begins with the same refresh.
+// All of it runs on the device: the host only polls how many systems have
+// settled.
 #include <cstdlib>
 #include <type_traits>
 
@@ -123,20 +125,21 @@ __global__ void __launch_bounds__(256) k_bond_fill(int n, const int* __restrict_
 }
 
 // --------------------------------------------------------------------- CG
-// Kernel boundaries order all cross-block communication: each kernel writes
-// per-block partial dot products, and every block of the next kernel reduces
+// Kernel boundaries order all cross-block communication: each launch writes
+// per-block partial dot products, and every block of the next launch reduces
 // its system's partials again in the same fixed order (redundant but cheap,
 // and bitwise reproducible).  Device-scope fences inside a kernel are avoided
 // on purpose: on a multi-XCD part they write back the whole L2.
-constexpr int UPD_BLOCK = 256;      // rows per update block
-constexpr int DIR_BLOCK = 1024;     // 16 waves, one row per wave at a time
-constexpr int DIR_MAX_BLOCKS = 256;  // per system: 4096 waves; few partial sums to reduce
-// Rows per wave a direction block aims for: a block's prologue and partials
-// cost the same whatever its rows, so small systems get few, full blocks.
-constexpr int DIR_ROWS_PER_WAVE = 3;
-constexpr int INNER_MAX = 64;        // mixed precision: iterations between FP64 restarts
-constexpr double INNER_DROP2 = 1e-10;  // mixed precision: squared residual drop before a restart
+constexpr int CG_BLOCK = 1024;      // 16 waves; the rows of a chunk are spread over the waves
+constexpr int CG_MAX_BLOCKS = 256;  // per system: few partial sums to reduce
+constexpr int CHUNK = 48;           // nominal rows per chunk; chunks start at bonded-run starts
+constexpr int CHUNK_MAX = 64;       // CHUNK + RUN_MAX - 1 rows at most: one wave holds a chunk
+constexpr int CG_WAVES = CG_BLOCK / 64;
+constexpr int INNER_MAX = 64;          // mixed precision: iterations between FP64 restarts
+constexpr double INNER_DROP2 = 1e-8;   // mixed precision: squared residual drop before a restart
 constexpr int MAX_RESTARTS = 12;
+constexpr int NDOT = 6;                // per block: r.u, w.u, r.r for both right-hand sides
+static_assert(CHUNK + RUN_MAX - 1 <= CHUNK_MAX, "a chunk must fit one wave");
 
 // System owning global block b (first-block offsets ascend with the system index).
 __device__ inline int block_system(const Systems& S, int b, bool dir) {
@@ -179,105 +182,44 @@ __device__ inline void block_sums(int cnt, Load load, double (&out)[NV]) {
   }
 }
 
-// The direction kernel's prologue: current and previous residual dots (4 + 4)
-// and |b|^2 (2) over the system's update-block partials.
-__device__ inline void reduce_direction(const double* __restrict__ cur_p, const double* __restrict__ prev_p,
-                                        const double* __restrict__ bb_p, int stride, int b0, int cnt,
-                                        double (&cur)[4], double (&prev)[4], double (&bb)[2]) {
-  double res[10];
-  block_sums<10>(cnt, [&](int t, double (&v)[10]) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      v[q] += cur_p[q * stride + b0 + t];
-      v[4 + q] += prev_p[q * stride + b0 + t];
-    }
-    v[8] += bb_p[b0 + t];
-    v[9] += bb_p[stride + b0 + t];
-  }, res);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    cur[q] = res[q];
-    prev[q] = res[4 + q];
-  }
-  bb[0] = res[8];
-  bb[1] = res[9];
-}
-
-// The update kernel's prologue: residual dots (4) and |b|^2 (2) over the
-// system's update blocks, p.Ap (2) over its direction blocks.
-__device__ inline void reduce_update(const double* __restrict__ cur_p, const double* __restrict__ bb_p, int su,
-                                     int u0, int nu_cnt, const double* __restrict__ pap_p, int sd, int d0, int nd_cnt,
-                                     double (&cur)[4], double (&bb)[2], double (&pap)[2]) {
-  double res[8];
-  block_sums<8>(nu_cnt > nd_cnt ? nu_cnt : nd_cnt, [&](int t, double (&v)[8]) {
-    if (t < nu_cnt) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] += cur_p[q * su + u0 + t];
-      v[4] += bb_p[u0 + t];
-      v[5] += bb_p[su + u0 + t];
-    }
-    if (t < nd_cnt) {
-      v[6] += pap_p[d0 + t];
-      v[7] += pap_p[sd + d0 + t];
-    }
-  }, res);
-#pragma unroll
-  for (int q = 0; q < 4; ++q) cur[q] = res[q];
-  bb[0] = res[4];
-  bb[1] = res[5];
-  pap[0] = res[6];
-  pap[1] = res[7];
-}
-
-// Block partial of NV values into part[k * stride + blockIdx.x].
-template <int NV>
-__device__ inline void write_partials(double (&vals)[NV], double* __restrict__ part, int stride) {
-  xr_block_partials<NV>(vals, part, stride);
-}
-
-// Per right-hand side: has this system's residual reached its current target?
-// At a (re)start the residual is the true one and the target is the final
-// tolerance; in between, mixed precision aims for the inner drop.
-__device__ inline void rhs_converged(const double (&cur)[4], const double (&bb)[2], const CgSys& c, int j,
-                                     bool mixed, double tol2, bool (&conv)[2]) {
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    double target = tol2 * bb[r];
-    if (mixed && j > 0) target = fmax(target, INNER_DROP2 * c.rr0[r]);
-    conv[r] = cur[2 + r] <= target;
-  }
-}
-
-__device__ inline void settle(CgSys* c, CgCounters* cnt, int state, int k) {
-  if (state == CG_REFRESH) c->kr = k + 1;  // the next direction launch refreshes
-  c->done = state;
-  if (state != CG_REFRESH) atomicAdd(&cnt->settled, 1);
+// Local row where chunk c of a system starts: the nominal c * CHUNK, moved
+// forward to the first row within RUN_MAX rows that starts a bonded run (no
+// strong bond to any of the three rows before it), so that molecules are not
+// split between chunks.  Called by a whole wave.
+__device__ inline int chunk_start(const SysDev& sd, const unsigned char* __restrict__ join, int c) {
+  if (c <= 0) return 0;
+  const int t = c * CHUNK;
+  if (t >= sd.n) return sd.n;
+  const int lane = threadIdx.x & 63;
+  const int r = t + lane;
+  const bool ok = lane < RUN_MAX && r < sd.n && (join[sd.atom0 + r] & 7u) == 0;
+  const unsigned long long m = __ballot(ok);
+  return m ? t + __ffsll((long long)m) - 1 : t;
 }
 
 // Block-Jacobi preconditioner.  Consecutive atoms joined by a strong
 // same-image bond (raw order >= 0.3 to one of the previous three atoms of the
-// block) form blocks of at most four atoms, e.g. one water molecule; blocks
-// never cross an update block, so they depend only on the system's own atoms.
+// run) form bonded runs of at most RUN_MAX atoms (molecules); whole runs are
+// packed into blocks of at most BLOCK_MAX atoms that never cross a chunk.
 // M^-1 is the inverse of the blocks of H on its diagonal (image terms
-// included).  Every update block holds whole preconditioner blocks, so
-// applying M^-1 needs only the block's residuals, exchanged through LDS.
-struct BlockRows {
-  int first[UPD_BLOCK];  // local row of each row's block start
-  int size[UPD_BLOCK];
+// included).  A chunk is processed by one block of the iteration kernel, so
+// applying M^-1 needs only the chunk's own rows, exchanged through LDS.
+struct ChunkBlocks {
+  int first[CHUNK_MAX];  // local row of each row's block start
+  int size[CHUNK_MAX];
 };
 
-// Preconditioner blocks of an update block's rows [0, rows).  First the
-// bonded runs, as one pass in row order would find them: row t joins the
-// current run when it is fewer than RUN_MAX rows past the run's start and
-// bonded to an earlier row of it.  A row with no strong bond to the three
-// rows before it always starts a run, so each such row's thread walks on to
-// the next one and all walks run in parallel.  Then whole consecutive runs
-// are packed greedily into blocks of at most BLOCK_MAX rows (thread 0, over
-// the run starts held as a bit mask); a block row finds its block start and
-// size from the mask of block starts.
-__device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, int rows, BlockRows& br) {
-  __shared__ unsigned long long run_mask[UPD_BLOCK / 64], blk_mask[UPD_BLOCK / 64];
-  const int t = threadIdx.x;
+// Preconditioner blocks of a chunk's rows [0, rows), computed by one wave
+// (lane t owns row t).  First the bonded runs, as one pass in row order would
+// find them: row t joins the current run when it is fewer than RUN_MAX rows
+// past the run's start and bonded to an earlier row of it.  A row with no
+// strong bond to the three rows before it always starts a run, so each such
+// row's lane walks on to the next one and all walks run in parallel.  Then
+// whole consecutive runs are packed greedily into blocks of at most BLOCK_MAX
+// rows (the same scalar loop in every lane, over the run starts as a bit
+// mask); a row finds its block start and size from the mask of block starts.
+__device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, int rows, ChunkBlocks& br) {
+  const int t = threadIdx.x & 63;
   auto starts = [&](int r) { return r == 0 || (join_sh[r] & 7u) == 0; };
   if (t < rows && starts(t)) {
     int start = t;
@@ -292,99 +234,84 @@ __device__ inline void assign_blocks(const unsigned char* __restrict__ join_sh, 
       br.first[r] = start;
     }
   }
-  __syncthreads();
-  const unsigned long long m = __ballot(t < rows && br.first[t] == t);
-  if ((t & 63) == 0) run_mask[t >> 6] = m;
-  __syncthreads();
-  if (t == 0) {
-    // Runs start at s_0 = 0 < s_1 < ...; run k ends where run k+1 starts (or
-    // at rows).  A run that would take the current block past BLOCK_MAX rows
-    // opens a new block (runs are at most RUN_MAX <= BLOCK_MAX rows).
-    blk_mask[0] = 1ull;
-    for (int w = 1; w < UPD_BLOCK / 64; ++w) blk_mask[w] = 0ull;
-    int bstart = 0, run = 0;
-    for (int w = 0; w < UPD_BLOCK / 64; ++w) {
-      unsigned long long bits = run_mask[w];
-      while (bits) {
-        const int next = w * 64 + __ffsll((long long)bits) - 1;
-        bits &= bits - 1;
-        if (next == 0) continue;
-        if (next - bstart > BLOCK_MAX) {
-          bstart = run;
-          blk_mask[run >> 6] |= 1ull << (run & 63);
-        }
-        run = next;
-      }
+  xr_wave_sync();
+  const unsigned long long runs = __ballot(t < rows && br.first[t] == t);
+  // Runs start at s_0 = 0 < s_1 < ...; run k ends where run k+1 starts (or
+  // at rows).  A run that would take the current block past BLOCK_MAX rows
+  // opens a new block (runs are at most RUN_MAX <= BLOCK_MAX rows).
+  unsigned long long blk = 1ull, bits = runs & ~1ull;
+  int bstart = 0, run = 0;
+  while (bits) {
+    const int next = __ffsll((long long)bits) - 1;
+    bits &= bits - 1;
+    if (next - bstart > BLOCK_MAX) {
+      bstart = run;
+      blk |= 1ull << run;
     }
-    if (rows - bstart > BLOCK_MAX) blk_mask[run >> 6] |= 1ull << (run & 63);
+    run = next;
   }
-  __syncthreads();
+  if (rows - bstart > BLOCK_MAX) blk |= 1ull << run;
   if (t < rows) {
-    // block start: the highest block-start bit at or below t; end: the next one above t (or rows)
-    int f = 0;
-    for (int w = t >> 6; w >= 0; --w) {
-      unsigned long long bits = blk_mask[w];
-      if (w == (t >> 6)) bits &= (t & 63) == 63 ? ~0ull : ((1ull << ((t & 63) + 1)) - 1);
-      if (bits) {
-        f = w * 64 + 63 - __clzll((long long)bits);
-        break;
-      }
-    }
-    int e = rows;
-    for (int w = t >> 6; w < UPD_BLOCK / 64; ++w) {
-      unsigned long long bits = blk_mask[w];
-      if (w == (t >> 6)) bits &= (t & 63) == 63 ? 0ull : ~((1ull << ((t & 63) + 1)) - 1);
-      if (bits) {
-        e = w * 64 + __ffsll((long long)bits) - 1;
-        break;
-      }
-    }
+    const unsigned long long upto = t == 63 ? ~0ull : ((2ull << t) - 1);
+    const int f = 63 - __clzll((long long)(blk & upto));
+    const unsigned long long above = blk & ~upto;
+    const int e = above ? __ffsll((long long)above) - 1 : rows;
     br.first[t] = f;
     br.size[t] = (e < rows ? e : rows) - f;
   }
-  __syncthreads();
+  xr_wave_sync();
 }
 
+// Right-hand sides of atom i: the supplied pair, else (-chi_i, -1).
+__device__ inline double2 cg_rhs(const double2* __restrict__ rhs, const int* __restrict__ type, const FFDev& ff,
+                                 int i) {
+  if (rhs) return rhs[i];
+  return make_double2(-ff.atom[type[i] * AP_COUNT + AP_CHI], -1.0);
+}
 
-__global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __restrict__ type, FFDev ff,
-                                                       const double2* __restrict__ rhs,
-                                                       const double2* __restrict__ guess, double2* __restrict__ X,
-                                                       double2* __restrict__ R, double4* __restrict__ ZP,
-                                                       float4* __restrict__ ZQ, const int* __restrict__ row_ptr,
-                                                       const int* __restrict__ nbr_j, const double* __restrict__ hval,
-                                                       const unsigned char* __restrict__ join,
-                                                       double* __restrict__ minv, int2* __restrict__ binfo,
-                                                       double* __restrict__ eta_out, double* __restrict__ part_b,
-                                                       double* __restrict__ part_bb, int nu, CgSys* __restrict__ cg,
-                                                       CgCounters* __restrict__ cnt,
-                                                       const unsigned* __restrict__ abort_flag, int n_all) {
+// One block per chunk, one lane per row: chunk bounds, the preconditioner
+// blocks and their inverses, x0 and, for a cold start, r0 = b and u0 = M^-1 b.
+// The first iteration launch then computes w0 = A u0 (a warm start first
+// computes r0 = b - A x0 and u0 there) and the solve state starts as a
+// refresh that is already under way.
+__global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict__ type, FFDev ff,
+                                                const double2* __restrict__ rhs, const double2* __restrict__ guess,
+                                                CgVec v, const int* __restrict__ row_ptr,
+                                                const int* __restrict__ nbr_j, const double* __restrict__ hval,
+                                                const unsigned char* __restrict__ join, double* __restrict__ minv,
+                                                int2* __restrict__ binfo, int2* __restrict__ chunks,
+                                                double* __restrict__ eta_out, CgSys* __restrict__ cg,
+                                                CgCounters* __restrict__ cnt, const unsigned* __restrict__ abort_flag,
+                                                int n_all) {
   const int sys = block_system(S, blockIdx.x, false);
   const SysDev& sd = S.sys[sys];
-  const int local = blockIdx.x - sd.upd0;
-  const int row0 = sd.atom0 + local * UPD_BLOCK;
-  const int rows = min(UPD_BLOCK, sd.atom0 + sd.n - row0);
+  const int c = blockIdx.x - sd.upd0;
+  const int r0 = chunk_start(sd, join, c);
+  const int r1 = chunk_start(sd, join, c + 1);
+  const int rows = r1 - r0;
   const int t = threadIdx.x;
+  const int row0 = sd.atom0 + r0;
   const int i = row0 + t;
   const bool mine = t < rows;
-  __shared__ unsigned char join_sh[UPD_BLOCK];
-  __shared__ BlockRows br;
-  __shared__ double hb[UPD_BLOCK][BLOCK_MAX];  // each row's row of its H block, inverted in place
-  __shared__ double2 bsh[UPD_BLOCK];
+  __shared__ unsigned char join_sh[CHUNK_MAX];
+  __shared__ ChunkBlocks br;
+  __shared__ double hb[CHUNK_MAX][BLOCK_MAX];  // each row's row of its H block, inverted in place
+  __shared__ double2 bsh[CHUNK_MAX];
+  if (t == 0) chunks[blockIdx.x] = make_int2(row0, sd.atom0 + r1);
   join_sh[t] = mine ? join[i] : 0;
-  __syncthreads();
+  xr_wave_sync();
   assign_blocks(join_sh, rows, br);
   double eta = 0.0;
   double2 b = make_double2(0.0, 0.0);
   const int f = mine ? br.first[t] : t, m = mine ? br.size[t] : 0;
   if (mine) {
-    const double* a = ff.atom + type[i] * AP_COUNT;
-    eta = a[AP_ETA];
-    b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
+    eta = ff.atom[type[i] * AP_COUNT + AP_ETA];
+    b = cg_rhs(rhs, type, ff, i);
     bsh[t] = b;
     // row t of its block of H: eta on the diagonal plus h summed over all
     // images to each block mate, from the row's neighbors in [g0, g0 + m)
 #pragma unroll
-    for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = c == t - f ? eta : 0.0;
+    for (int q = 0; q < BLOCK_MAX; ++q) hb[t][q] = q == t - f ? eta : 0.0;
     const int g0 = row0 + f;
     int lo = row_ptr[i], hi = row_ptr[i + 1];
     const int e_end = hi;
@@ -399,8 +326,8 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
       hb[t][jj - g0] += hval[e];
     }
   }
-  __syncthreads();
-  // In-place Gauss-Jordan inversion of every block, one thread per block row
+  xr_wave_sync();
+  // In-place Gauss-Jordan inversion of every block, one lane per block row
   // (no pivoting: the blocks of H are symmetric positive definite).  Each
   // step reads the old pivot row before any row of the block is rewritten.
   for (int p = 0; p < BLOCK_MAX; ++p) {
@@ -409,98 +336,53 @@ __global__ void __launch_bounds__(UPD_BLOCK) k_cg_init(Systems S, const int* __r
     double pp = 1.0, fr = 0.0;
     if (act) {
 #pragma unroll
-      for (int c = 0; c < BLOCK_MAX; ++c) prow[c] = hb[f + p][c];
+      for (int q = 0; q < BLOCK_MAX; ++q) prow[q] = hb[f + p][q];
       pp = hb[f + p][p];
       fr = hb[t][p];
     }
-    __syncthreads();
+    xr_wave_sync();
     if (act) {
       const double piv = 1.0 / pp;
       if (t == f + p) {
 #pragma unroll
-        for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = (c == p ? 1.0 : prow[c]) * piv;
+        for (int q = 0; q < BLOCK_MAX; ++q) hb[t][q] = (q == p ? 1.0 : prow[q]) * piv;
       } else {
 #pragma unroll
-        for (int c = 0; c < BLOCK_MAX; ++c) hb[t][c] = (c == p ? 0.0 : hb[t][c]) - fr * ((c == p ? 1.0 : prow[c]) * piv);
+        for (int q = 0; q < BLOCK_MAX; ++q) hb[t][q] = (q == p ? 0.0 : hb[t][q]) - fr * ((q == p ? 1.0 : prow[q]) * piv);
       }
     }
-    __syncthreads();
+    xr_wave_sync();
   }
-  double vals[4] = {0, 0, 0, 0};
-  double bb[2] = {0, 0};
   if (mine) {
-    double2 z = make_double2(0.0, 0.0);
-    for (int c = 0; c < m; ++c) {
-      z.x += hb[t][c] * bsh[f + c].x;
-      z.y += hb[t][c] * bsh[f + c].y;
-      minv[(size_t)c * n_all + i] = hb[t][c];
+    double2 u = make_double2(0.0, 0.0);
+    for (int q = 0; q < m; ++q) {
+      u.x += hb[t][q] * bsh[f + q].x;
+      u.y += hb[t][q] * bsh[f + q].y;
+      minv[(size_t)q * n_all + i] = hb[t][q];
     }
     binfo[i] = make_int2(f, m);
-    X[i] = guess ? guess[i] : make_double2(0.0, 0.0);
-    R[i] = b;
-    ZP[i] = make_double4(z.x, z.y, 0.0, 0.0);  // (z_s, z_t, p_s, p_t), p = 0 before iteration 0
-    if (ZQ) ZQ[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
     eta_out[i] = eta;
-    vals[0] = b.x * z.x;
-    vals[1] = b.y * z.y;
-    vals[2] = b.x * b.x;
-    vals[3] = b.y * b.y;
-    bb[0] = vals[2];
-    bb[1] = vals[3];
+    v.X[i] = guess ? guess[i] : make_double2(0.0, 0.0);
+    if (!guess) {
+      v.R[i] = b;
+      v.U[i] = u;
+    }
   }
-  // slot 1 holds the residual state before iteration 0
-  write_partials<4>(vals, part_b + 4 * nu, nu);
-  write_partials<2>(bb, part_bb, nu);
-  if (blockIdx.x == 0 && threadIdx.x == 0) cnt->abort = abort_flag ? (int)(*abort_flag != 0) : 0;
-  if (local == 0 && threadIdx.x == 0) {
-    CgSys c{};
-    c.done = CG_RUNNING;
-    c.iters = 0;
-    c.k0 = 0;
-    c.restarts = 0;
-    cg[sys] = c;
+  if (blockIdx.x == 0 && t == 0) cnt->abort = abort_flag ? (int)(*abort_flag != 0) : 0;
+  if (c == 0 && t == 0) {
+    CgCtl s{};
+    s.done = CG_REFRESH;
+    s.kr = guess ? 0 : -1;  // launch kr computes r and u, launch kr + 1 computes w
+    CgSys init{};
     if (S.err[sys]) {
       // an input error was flagged upstream: nothing to solve
-      cg[sys].done = CG_FAILED;
+      s.done = CG_FAILED;
       atomicAdd(&cnt->settled, 1);
-    } else if (guess) {
-      // warm start: launch 0 computes the true residual of the guess
-      cg[sys].done = CG_REFRESH;
-      cg[sys].kr = 0;
-      cg[sys].k0 = -1;
     }
+    init.ctl[0] = s;
+    init.ctl[1] = s;
+    cg[sys] = init;
   }
-}
-
-// z = M^-1 r for one row of an update block, from the block's residuals in LDS
-// (bi = local row of the block start, block size).
-// A row's block-Jacobi inverse row (column-major storage: entry c of row i at
-// minv[c * stride + i]), loaded into registers in one unrolled batch so the
-// loads are in flight together.
-struct MinvRow {
-  double w[BLOCK_MAX];
-  int2 bi;
-};
-
-__device__ inline void load_minv(const double* __restrict__ minv, int stride, int i, bool mine,
-                                 const int2* __restrict__ binfo, MinvRow& m) {
-  m.bi = mine ? binfo[i] : make_int2(0, 0);
-#pragma unroll
-  for (int c = 0; c < BLOCK_MAX; ++c) m.w[c] = c < m.bi.y ? minv[(size_t)c * stride + i] : 0.0;
-}
-
-// z = M^-1 r for one row from the block's residuals in LDS
-__device__ inline double2 apply_block(const MinvRow& m, const double2* __restrict__ rsh) {
-  double2 z = make_double2(0.0, 0.0);
-#pragma unroll
-  for (int c = 0; c < BLOCK_MAX; ++c) {
-    if (c < m.bi.y) {
-      const double2 r = rsh[m.bi.x + c];
-      z.x += m.w[c] * r.x;
-      z.y += m.w[c] * r.y;
-    }
-  }
-  return z;
 }
 
 // Eight H entries per lane of one row (coalesced over the wave): columns and
@@ -523,330 +405,429 @@ __device__ inline void load_batch(const int* __restrict__ nbr_j, const HV* __res
   }
 }
 
-// sum_e h_e p_j for one batch, p_j = z_j + beta p_old_j from the gathered records
-__device__ inline void gather_batch(const RowBatch<double>& b, const double4* __restrict__ zp, double bs, double bt,
-                                    double& as, double& at) {
+// sum_e h_e v_j(e) over one batch, both right-hand sides
+template <class HV, class VT, class AT>
+__device__ inline void gather_sum(const RowBatch<HV>& b, const VT* __restrict__ vec, AT& as, AT& at) {
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
-    const double4 v = zp[b.j[u]];
-    as += b.h[u] * (v.x + bs * v.z);
-    at += b.h[u] * (v.y + bt * v.w);
+    const VT w = vec[b.j[u]];
+    as += b.h[u] * w.x;
+    at += b.h[u] * w.y;
   }
 }
 
-__device__ inline void gather_batch(const RowBatch<float>& b, const float4* __restrict__ zq, float bs, float bt,
-                                    float& as, float& at) {
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const float4 v = zq[b.j[u]];
-    as += b.h[u] * (v.x + bs * v.z);
-    at += b.h[u] * (v.y + bt * v.w);
-  }
-}
-
-// The same sums with z and p_old kept apart (a = sum h z, sum h p_old per
-// right-hand side), for the row gathered before beta is known.
-__device__ inline void gather_split(const RowBatch<double>& b, const double4* __restrict__ zp, double (&a)[4]) {
-  // two groups of four 32-B gathers: all eight in flight at once would not
-  // fit the 128 registers of a 1024-thread block next to the prefetched row
-#pragma unroll
-  for (int g = 0; g < 8; g += 4) {
-#pragma unroll
-    for (int u = g; u < g + 4; ++u) {
-      const double4 v = zp[b.j[u]];
-      a[0] += b.h[u] * v.x;
-      a[1] += b.h[u] * v.y;
-      a[2] += b.h[u] * v.z;
-      a[3] += b.h[u] * v.w;
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-}
-
-__device__ inline void gather_split(const RowBatch<float>& b, const float4* __restrict__ zq, float (&a)[4]) {
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const float4 v = zq[b.j[u]];
-    a[0] += b.h[u] * v.x;
-    a[1] += b.h[u] * v.y;
-    a[2] += b.h[u] * v.z;
-    a[3] += b.h[u] * v.w;
-  }
-}
-
-// Iteration k, first half: beta, p = z + beta p_old and A p (rows of the
-// system strided over its blocks, one wave per row), partial p.Ap per block.
-// zp_in holds (z, p_old) of every atom; the new p goes to the p half of
-// zp_out.  Rows are software-pipelined: each wave loads the first 512 H
-// entries of its next row before gathering for the current one, and those of
-// its first row before the prologue reduction, so the H stream overlaps the
-// gathers.
-template <bool MIXED>
-__global__ void __launch_bounds__(DIR_BLOCK) k_cg_direction(Systems S, int k, const int* __restrict__ row_ptr,
-                                                            const int* __restrict__ nbr_j,
-                                                            const double* __restrict__ hval,
-                                                            const float* __restrict__ hval32,
-                                                            const double* __restrict__ eta,
-                                                            const double4* __restrict__ zp_in,
-                                                            double4* __restrict__ zp_out,
-                                                            const float4* __restrict__ zq_in,
-                                                            float4* __restrict__ zq_out, double2* __restrict__ AP,
-                                                            const double* __restrict__ part_b,
-                                                            const double* __restrict__ part_bb, int nu,
-                                                            double* __restrict__ part_a, int nd, double tol2,
-                                                            int max_iter, CgSys* __restrict__ cg,
-                                                            CgCounters* __restrict__ cnt, const int* __restrict__ type,
-                                                            FFDev ff, const double2* __restrict__ rhs,
-                                                            const double2* __restrict__ X, double2* __restrict__ R) {
-  const int sys = block_system(S, blockIdx.x, true);
-  const CgSys c = cg[sys];
-  const SysDev& sd = S.sys[sys];
-  const int local = blockIdx.x - sd.dir0;
+// Off-diagonal row sums sum_e h_e v_j(e) for the rows [r0, r1) of a chunk:
+// wave w takes rows r0 + w, r0 + w + 16, ...; each row's first 512 entries
+// are in cur (loaded by the caller for the wave's first row), and the next
+// row's are loaded before the current row's gathers.  Lane 0 leaves each
+// row's pair of sums in out[row - r0].
+template <class HV, class VT, class AT>
+__device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
+                                  const HV* __restrict__ hv, const VT* __restrict__ vec, RowBatch<HV>& cur, int e0,
+                                  int e1, double2* out) {
   const int lane = threadIdx.x & 63;
-  const int waves = sd.ndir * (DIR_BLOCK >> 6);
-  const int end = sd.atom0 + sd.n;
-  if (c.done == CG_REFRESH && c.kr == k) {
-    // Refresh: FP64 true residual r = b - A x.  The update launch applies the
-    // preconditioner, writes (z, p = 0) and reduces r.z and r.r.
-    for (int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6); i < end; i += waves) {
-      double as = 0.0, at = 0.0;
-      for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
-        const double2 v = X[nbr_j[e]];
-        as += hval[e] * v.x;
-        at += hval[e] * v.y;
-      }
-      as = xr_wave_sum(as);
-      at = xr_wave_sum(at);
-      if (lane == 0) {
-        const double* a = ff.atom + type[i] * AP_COUNT;
-        const double2 b = rhs ? rhs[i] : make_double2(-a[AP_CHI], -1.0);
-        const double2 xv = X[i];
-        const double d = eta[i];
-        R[i] = make_double2(b.x - (d * xv.x + as), b.y - (d * xv.y + at));
-      }
-    }
-    return;
-  }
-  // A system refreshed by the previous launch (k0 == k, set by its update
-  // kernel) runs again from j = 0; one in any other state sits this launch out.
-  const bool resumed = c.done == CG_REFRESH && c.k0 == k;
-  if (c.done != CG_RUNNING && !resumed) return;
-  const int j = k - c.k0;
-  using HV = typename std::conditional<MIXED, float, double>::type;
-  const HV* hv;
-  if constexpr (MIXED) hv = hval32;
-  else hv = hval;
-  int i = sd.atom0 + local * (DIR_BLOCK >> 6) + (threadIdx.x >> 6);
-  int e0 = 0, e1 = 0;
-  RowBatch<HV> cur_b;
-  if (i < end) {
-    e0 = row_ptr[i];
-    e1 = row_ptr[i + 1];
-    load_batch(nbr_j, hv, e0, e1, i, lane, cur_b);
-  }
-  // The wave's first row is gathered before beta is known (sum h z and
-  // sum h p_old apart), so the prologue reduction overlaps it; the next row's
-  // first 512 entries are already on their way.
-  int inext = i + waves, n0 = 0, n1 = 0;
-  RowBatch<HV> next_b;
-  if (inext < end) {
-    n0 = row_ptr[inext];
-    n1 = row_ptr[inext + 1];
-    load_batch(nbr_j, hv, n0, n1, inext, lane, next_b);
-  }
-  double first[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i < end) {
-    if constexpr (MIXED) {
-      float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-      gather_split(cur_b, zq_in, f);
-      for (int base = e0 + 512; base < e1; base += 512) {
-        RowBatch<HV> more;
-        load_batch(nbr_j, hv, base, e1, i, lane, more);
-        gather_split(more, zq_in, f);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) first[q] = (double)f[q];
-    } else {
-      gather_split(cur_b, zp_in, first);
-      for (int base = e0 + 512; base < e1; base += 512) {
-        RowBatch<HV> more;
-        load_batch(nbr_j, hv, base, e1, i, lane, more);
-        gather_split(more, zp_in, first);
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) first[q] = xr_wave_sum(first[q]);
-  double cur[4], prev[4], bb[2];
-  reduce_direction(part_b + ((k - 1) & 1) * 4 * nu, part_b + (k & 1) * 4 * nu, part_bb, nu, sd.upd0, sd.nupd, cur,
-                   prev, bb);
-  bool conv[2];
-  rhs_converged(cur, bb, c, j, MIXED, tol2, conv);
-  if ((conv[0] && conv[1]) || c.iters >= max_iter || (MIXED && j >= INNER_MAX)) {
-    if (local == 0 && threadIdx.x == 0) {
-      int state;
-      if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
-      else if (c.iters >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
-      else state = CG_REFRESH;
-      settle(&cg[sys], cnt, state, k);
-    }
-    return;
-  }
-  if (j == 0 && local == 0 && threadIdx.x == 0) {
-    if (MIXED) {
-      cg[sys].rr0[0] = cur[2];
-      cg[sys].rr0[1] = cur[3];
-    }
-    if (resumed) cg[sys].done = CG_RUNNING;
-  }
-  double beta_s = 0.0, beta_t = 0.0;
-  if (j > 0) {
-    if (!conv[0] && prev[0] != 0.0) beta_s = cur[0] / prev[0];
-    if (!conv[1] && prev[1] != 0.0) beta_t = cur[1] / prev[1];
-  }
-  double pap_s = 0.0, pap_t = 0.0;
-  // p = z + beta p_old and A p = eta p + sum h p_j for row i from its sums
-  auto finish_row = [&](int row, double acc_s, double acc_t) {
-    const double4 v = zp_in[row];
-    const double2 pr = make_double2(v.x + beta_s * v.z, v.y + beta_t * v.w);
-    const double d = eta[row];
-    const double2 ap = make_double2(d * pr.x + acc_s, d * pr.y + acc_t);
-    reinterpret_cast<double2*>(zp_out + row)[1] = pr;
-    if (MIXED) reinterpret_cast<float2*>(zq_out + row)[1] = make_float2((float)pr.x, (float)pr.y);
-    AP[row] = ap;
-    pap_s += pr.x * ap.x;
-    pap_t += pr.y * ap.y;
-  };
-  if (i < end && lane == 0) finish_row(i, first[0] + beta_s * first[2], first[1] + beta_t * first[3]);
-  i = inext;
-  e0 = n0;
-  e1 = n1;
-  cur_b = next_b;
-  while (i < end) {
-    inext = i + waves;
-    n0 = 0;
-    n1 = 0;
-    if (inext < end) {
+  for (int i = r0 + (threadIdx.x >> 6); i < r1; i += CG_WAVES) {
+    const int inext = i + CG_WAVES;
+    int n0 = 0, n1 = 0;
+    RowBatch<HV> next;
+    if (inext < r1) {
       n0 = row_ptr[inext];
       n1 = row_ptr[inext + 1];
-      load_batch(nbr_j, hv, n0, n1, inext, lane, next_b);
+      load_batch(nbr_j, hv, n0, n1, inext, lane, next);
     }
-    double acc_s, acc_t;
-    if constexpr (MIXED) {
-      const float bs = (float)beta_s, bt = (float)beta_t;
-      float fs = 0.0f, ft = 0.0f;
-      gather_batch(cur_b, zq_in, bs, bt, fs, ft);
-      for (int base = e0 + 512; base < e1; base += 512) {
-        RowBatch<HV> more;
-        load_batch(nbr_j, hv, base, e1, i, lane, more);
-        gather_batch(more, zq_in, bs, bt, fs, ft);
-      }
-      acc_s = (double)fs;
-      acc_t = (double)ft;
-    } else {
-      acc_s = 0.0;
-      acc_t = 0.0;
-      gather_batch(cur_b, zp_in, beta_s, beta_t, acc_s, acc_t);
-      for (int base = e0 + 512; base < e1; base += 512) {
-        RowBatch<HV> more;
-        load_batch(nbr_j, hv, base, e1, i, lane, more);
-        gather_batch(more, zp_in, beta_s, beta_t, acc_s, acc_t);
-      }
+    AT as = AT(0), at = AT(0);
+    gather_sum(cur, vec, as, at);
+    for (int base = e0 + 512; base < e1; base += 512) {
+      RowBatch<HV> more;
+      load_batch(nbr_j, hv, base, e1, i, lane, more);
+      gather_sum(more, vec, as, at);
     }
-    acc_s = xr_wave_sum(acc_s);
-    acc_t = xr_wave_sum(acc_t);
-    if (lane == 0) finish_row(i, acc_s, acc_t);
-    i = inext;
+    const double s0 = xr_wave_sum((double)as), s1 = xr_wave_sum((double)at);
+    if (lane == 0) out[i - r0] = make_double2(s0, s1);
     e0 = n0;
     e1 = n1;
-    cur_b = next_b;
+    cur = next;
   }
-  double vals[2] = {pap_s, pap_t};
-  write_partials<2>(vals, part_a, nd);
 }
 
-// Iteration k, second half: alpha, x += alpha p, r -= alpha A p, z = M^-1 r
-// (into the z half of zp), partial r.z and r.r per block.
-template <bool MIXED>
-__global__ void __launch_bounds__(UPD_BLOCK) k_cg_update(Systems S, int k, int n_all, const double* __restrict__ minv,
-                                                         const int2* __restrict__ binfo,
-                                                         double2* __restrict__ X, double2* __restrict__ R,
-                                                         double4* __restrict__ zp, float4* __restrict__ zq,
-                                                         const double2* __restrict__ AP,
-                                                         const double* __restrict__ part_a, int nd,
-                                                         double* __restrict__ part_b,
-                                                         const double* __restrict__ part_bb, int nu, double tol2,
-                                                         CgSys* __restrict__ cg) {
-  const int sys = block_system(S, blockIdx.x, false);
-  const CgSys c = cg[sys];
-  const SysDev& sd = S.sys[sys];
-  const int local = blockIdx.x - sd.upd0;
-  __shared__ double2 rsh[UPD_BLOCK];
-  const int i = sd.atom0 + local * UPD_BLOCK + threadIdx.x;
-  const bool mine = i < sd.atom0 + sd.n;
-  if (c.done != CG_RUNNING && !(c.done == CG_REFRESH && c.kr == k)) return;
-  MinvRow mrow;
-  load_minv(minv, n_all, i, mine, binfo, mrow);
+template <class HV>
+__device__ inline void first_batch(int r0, int r1, const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
+                                   const HV* __restrict__ hv, RowBatch<HV>& b, int& e0, int& e1) {
+  const int i = r0 + (threadIdx.x >> 6);
+  e0 = e1 = 0;
+  if (i < r1) {
+    e0 = row_ptr[i];
+    e1 = row_ptr[i + 1];
+    load_batch(nbr_j, hv, e0, e1, i, threadIdx.x & 63, b);
+  }
+}
+
+// What a system's block does in launch k, from the state the previous launch
+// left (j: iterations since the last (re)start).
+enum CgMode { MODE_IDLE = 0, MODE_RESIDUAL = 1, MODE_START = 2, MODE_ITER = 3 };
+
+__device__ inline int cg_mode(const CgCtl& c, int k, int& j) {
+  j = 0;
+  if (c.done == CG_RUNNING) {
+    j = k - c.k0;
+    return MODE_ITER;
+  }
   if (c.done == CG_REFRESH) {
-    // A refresh requested by this iteration's direction launch waits for the
-    // next one; a true residual computed by this iteration's direction launch
-    // is preconditioned and reduced here.
-    if (c.kr != k) return;
-    double vals[4] = {0, 0, 0, 0};
-    const double2 r = mine ? R[i] : make_double2(0.0, 0.0);
-    rsh[threadIdx.x] = r;
-    __syncthreads();
-    if (mine) {
-      const double2 z = apply_block(mrow, rsh);
-      zp[i] = make_double4(z.x, z.y, 0.0, 0.0);
-      if (MIXED) zq[i] = make_float4((float)z.x, (float)z.y, 0.0f, 0.0f);
-      vals[0] = r.x * z.x;
-      vals[1] = r.y * z.y;
-      vals[2] = r.x * r.x;
-      vals[3] = r.y * r.y;
+    if (k == c.kr) return MODE_RESIDUAL;
+    if (k == c.kr + 1) return MODE_START;
+    if (k == c.kr + 2) return MODE_ITER;
+  }
+  return MODE_IDLE;
+}
+
+// Row operands staged through LDS: each thread loads one vector entry and one
+// inverse-block column before the chunk's sweep and stores them after it.
+enum CgItem { IT_Z = 0, IT_Q, IT_S, IT_P, IT_X, IT_R, IT_U, IT_W, IT_M, IT_B, NITEM };
+
+struct ChunkLds {
+  double2 st[NITEM][CHUNK_MAX];
+  double mv[BLOCK_MAX][CHUNK_MAX];
+  double eta[CHUNK_MAX];
+  int2 bi[CHUNK_MAX];
+  double2 sums[CHUNK_MAX];  // off-diagonal A v row sums
+  double2 ex[CHUNK_MAX];    // vector exchanged for the M^-1 apply
+};
+
+struct Staged {
+  double2 val;
+  double mv;
+  double eta;
+  int2 bi;
+};
+
+__device__ inline bool item_used(int mode, int item) {
+  if (mode == MODE_ITER) return item <= IT_M;
+  if (mode == MODE_RESIDUAL) return item == IT_X || item == IT_B;
+  return item == IT_R || item == IT_U || item == IT_B;  // MODE_START
+}
+
+__device__ inline void stage_load(int mode, int r0, int r1, const CgVec& v, const double2* __restrict__ m_in,
+                                  const double* __restrict__ minv, int n_all, const int2* __restrict__ binfo,
+                                  const double* __restrict__ eta, const double2* __restrict__ rhs,
+                                  const int* __restrict__ type, const FFDev& ff, Staged& s) {
+  const int row = threadIdx.x & 63, item = threadIdx.x >> 6;
+  const int i = r0 + row;
+  s.val = make_double2(0.0, 0.0);
+  s.mv = 0.0;
+  s.eta = 0.0;
+  s.bi = make_int2(0, 0);
+  if (i >= r1) return;
+  s.mv = minv[(size_t)item * n_all + i];  // entries past the block size are never used
+  if (item < NITEM && item_used(mode, item)) {
+    switch (item) {
+      case IT_Z: s.val = v.Z[i]; break;
+      case IT_Q: s.val = v.Q[i]; break;
+      case IT_S: s.val = v.S[i]; break;
+      case IT_P: s.val = v.P[i]; break;
+      case IT_X: s.val = v.X[i]; break;
+      case IT_R: s.val = v.R[i]; break;
+      case IT_U: s.val = v.U[i]; break;
+      case IT_W: s.val = v.W[i]; break;
+      case IT_M: s.val = m_in[i]; break;
+      default: s.val = cg_rhs(rhs, type, ff, i); break;
     }
-    write_partials<4>(vals, part_b + (k & 1) * 4 * nu, nu);
-    if (local == 0 && threadIdx.x == 0) {
-      cg[sys].k0 = k + 1;
-      cg[sys].restarts = c.restarts + 1;
+  }
+  if (item == NITEM) s.eta = eta[i];
+  if (item == NITEM + 1) s.bi = binfo[i];
+}
+
+__device__ inline void stage_store(int r0, int r1, const Staged& s, ChunkLds& L) {
+  const int row = threadIdx.x & 63, item = threadIdx.x >> 6;
+  if (r0 + row >= r1) return;
+  L.mv[item][row] = s.mv;
+  if (item < NITEM) L.st[item][row] = s.val;
+  if (item == NITEM) L.eta[row] = s.eta;
+  if (item == NITEM + 1) L.bi[row] = s.bi;
+}
+
+// M^-1 applied to the exchanged vector for row t of the chunk
+__device__ inline double2 apply_block(const ChunkLds& L, int t) {
+  const int2 bi = L.bi[t];
+  double2 z = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < BLOCK_MAX; ++c) {
+    if (c < bi.y) {
+      const double w = L.mv[c][t];
+      const double2 r = L.ex[bi.x + c];
+      z.x += w * r.x;
+      z.y += w * r.y;
     }
+  }
+  return z;
+}
+
+struct CgStep {
+  double alpha[2], beta[2];
+  bool fresh;  // first iteration since a (re)start: the old z, q, s, p are not read
+};
+
+// The chunk's rows after its sweep (wave 0, lane t owns row t):
+//   iteration: n = A m;  z = n + b z, q = m + b q, s = w + b s, p = u + b p;
+//              x += a p, r -= a s, u -= a q, w -= a z;  m = M^-1 w
+//   residual:  r = b - A x;  u = M^-1 r
+//   start:     w = A u;  m = M^-1 w
+// and the block's dots (r.u, w.u, r.r and, at a start, b.b) accumulate in dots.
+template <bool MIXED>
+__device__ inline void chunk_tail(int mode, int r0, int r1, const CgStep& st, const CgVec& v, double2* m_out,
+                                  float2* m32_out, ChunkLds& L, double (&dots)[8]) {
+  const int t = threadIdx.x;
+  const int i = r0 + t;
+  const bool mine = i < r1;
+  if (mine) {
+    const double eta = L.eta[t];
+    const double2 ax = L.sums[t];
+    if (mode == MODE_ITER) {
+      const double2 m = L.st[IT_M][t];
+      const double2 nn = make_double2(eta * m.x + ax.x, eta * m.y + ax.y);
+      double2 u = L.st[IT_U][t], w = L.st[IT_W][t], x = L.st[IT_X][t], r = L.st[IT_R][t];
+      double2 z = nn, q = m, s = w, p = u;
+      if (!st.fresh) {
+        const double2 z0 = L.st[IT_Z][t], q0 = L.st[IT_Q][t], s0 = L.st[IT_S][t], p0 = L.st[IT_P][t];
+        z = make_double2(nn.x + st.beta[0] * z0.x, nn.y + st.beta[1] * z0.y);
+        q = make_double2(m.x + st.beta[0] * q0.x, m.y + st.beta[1] * q0.y);
+        s = make_double2(w.x + st.beta[0] * s0.x, w.y + st.beta[1] * s0.y);
+        p = make_double2(u.x + st.beta[0] * p0.x, u.y + st.beta[1] * p0.y);
+      }
+      x = make_double2(x.x + st.alpha[0] * p.x, x.y + st.alpha[1] * p.y);
+      r = make_double2(r.x - st.alpha[0] * s.x, r.y - st.alpha[1] * s.y);
+      u = make_double2(u.x - st.alpha[0] * q.x, u.y - st.alpha[1] * q.y);
+      w = make_double2(w.x - st.alpha[0] * z.x, w.y - st.alpha[1] * z.y);
+      v.Z[i] = z;
+      v.Q[i] = q;
+      v.S[i] = s;
+      v.P[i] = p;
+      v.X[i] = x;
+      v.R[i] = r;
+      v.U[i] = u;
+      v.W[i] = w;
+      L.ex[t] = w;
+      L.st[IT_R][t] = r;
+      L.st[IT_U][t] = u;
+    } else if (mode == MODE_RESIDUAL) {
+      const double2 x = L.st[IT_X][t], b = L.st[IT_B][t];
+      const double2 r = make_double2(b.x - (eta * x.x + ax.x), b.y - (eta * x.y + ax.y));
+      v.R[i] = r;
+      L.ex[t] = r;
+    } else {
+      const double2 u = L.st[IT_U][t];
+      const double2 w = make_double2(eta * u.x + ax.x, eta * u.y + ax.y);
+      v.W[i] = w;
+      L.ex[t] = w;
+    }
+  }
+  xr_wave_sync();
+  if (!mine) return;
+  const double2 mz = apply_block(L, t);
+  if (mode == MODE_RESIDUAL) {
+    v.U[i] = mz;
     return;
   }
-  if (c.done != CG_RUNNING) return;
-  double cur[4], pap[2], bb[2];
-  reduce_update(part_b + ((k - 1) & 1) * 4 * nu, part_bb, nu, sd.upd0, sd.nupd, part_a, nd, sd.dir0, sd.ndir, cur,
-                bb, pap);
-  bool conv[2];
-  rhs_converged(cur, bb, c, k - c.k0, MIXED, tol2, conv);
-  const double alpha_s = (!conv[0] && pap[0] != 0.0) ? cur[0] / pap[0] : 0.0;
-  const double alpha_t = (!conv[1] && pap[1] != 0.0) ? cur[1] / pap[1] : 0.0;
-  double vals[4] = {0, 0, 0, 0};
-  double2 r = make_double2(0.0, 0.0);
-  if (mine) {
-    const double2 p = reinterpret_cast<const double2*>(zp + i)[1];
-    const double2 ap = AP[i];
-    double2 xv = X[i];
-    r = R[i];
-    xv.x += alpha_s * p.x;
-    xv.y += alpha_t * p.y;
-    r.x -= alpha_s * ap.x;
-    r.y -= alpha_t * ap.y;
-    X[i] = xv;
-    R[i] = r;
+  m_out[i] = mz;
+  if (MIXED) m32_out[i] = make_float2((float)mz.x, (float)mz.y);
+  const double2 r = L.st[IT_R][t], u = L.st[IT_U][t], w = L.ex[t];
+  dots[0] += r.x * u.x;
+  dots[1] += r.y * u.y;
+  dots[2] += w.x * u.x;
+  dots[3] += w.y * u.y;
+  dots[4] += r.x * r.x;
+  dots[5] += r.y * r.y;
+  if (mode == MODE_START) {
+    const double2 b = L.st[IT_B][t];
+    dots[6] += b.x * b.x;
+    dots[7] += b.y * b.y;
   }
-  rsh[threadIdx.x] = r;
-  __syncthreads();
-  if (mine) {
-    const double2 z = apply_block(mrow, rsh);
-    reinterpret_cast<double2*>(zp + i)[0] = z;
-    if (MIXED) reinterpret_cast<float2*>(zq + i)[0] = make_float2((float)z.x, (float)z.y);
-    vals[0] = r.x * z.x;
-    vals[1] = r.y * z.y;
-    vals[2] = r.x * r.x;
-    vals[3] = r.y * r.y;
+}
+
+__device__ inline void settle(CgSys* c, CgCounters* cnt, const CgCtl& cur, int state, int k) {
+  CgCtl s = cur;
+  s.done = state;
+  if (state == CG_REFRESH) {
+    s.kr = k + 1;  // the next launch computes the true residual
+    c->ctl[k & 1] = s;
+    return;
   }
-  write_partials<4>(vals, part_b + (k & 1) * 4 * nu, nu);
-  if (local == 0 && threadIdx.x == 0) cg[sys].iters = c.iters + 1;
+  // A final state is written to both slots: every block of this launch stops
+  // whichever slot it reads.
+  c->ctl[0] = s;
+  c->ctl[1] = s;
+  c->iters = cur.it;
+  c->ks = k;
+  atomicAdd(&cnt->settled, 1);
+}
+
+// Launch k of the pipelined preconditioned CG (Ghysels and Vanroose, 2014),
+// two right-hand sides.  With gamma = r.u, delta = w.u from the previous
+// launch, beta = gamma / gamma_old and alpha = gamma / (delta - beta gamma /
+// alpha_old) (alpha = gamma / delta at a start), every launch does one sweep
+// n = A m over its rows and the vector recurrences listed at chunk_tail.  The
+// sweep reads only m, computed for every row by the previous launch, so one
+// launch per iteration suffices; the dots for the next launch come out of the
+// same pass.  Each block of a system owns whole chunks.
+template <bool MIXED>
+__global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgVec v, const int* __restrict__ row_ptr,
+                                                           const int* __restrict__ nbr_j,
+                                                           const double* __restrict__ hval,
+                                                           const float* __restrict__ hval32,
+                                                           const double* __restrict__ eta,
+                                                           const double* __restrict__ minv,
+                                                           const int2* __restrict__ binfo,
+                                                           const int2* __restrict__ chunks,
+                                                           const int* __restrict__ type, FFDev ff,
+                                                           const double2* __restrict__ rhs, double* __restrict__ part,
+                                                           double* __restrict__ part_bb, int nd, double tol2,
+                                                           int max_iter, CgSys* __restrict__ cg,
+                                                           CgCounters* __restrict__ cnt, int n_all) {
+  const int sys = block_system(S, blockIdx.x, true);
+  const CgCtl c = cg[sys].ctl[(k - 1) & 1];
+  int j;
+  const int mode = cg_mode(c, k, j);
+  if (mode == MODE_IDLE) return;
+  const SysDev& sd = S.sys[sys];
+  const int local = blockIdx.x - sd.dir0;
+  const int per = (sd.nupd + sd.ndir - 1) / sd.ndir;
+  const int c0 = local * per, c1 = min(c0 + per, sd.nupd);
+  __shared__ ChunkLds L;
+  const double2* m_in = v.M[k & 1];
+  double2* m_out = v.M[(k + 1) & 1];
+  float2* m32_out = v.M32[(k + 1) & 1];
+  const bool lead = local == 0 && threadIdx.x == 0;
+  double dots[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  CgStep st{};
+  if (mode == MODE_ITER) {
+    using HV = typename std::conditional<MIXED, float, double>::type;
+    using VT = typename std::conditional<MIXED, float2, double2>::type;
+    using AT = HV;
+    const HV* hv;
+    const VT* vin;
+    if constexpr (MIXED) {
+      hv = hval32;
+      vin = v.M32[k & 1];
+    } else {
+      hv = hval;
+      vin = m_in;
+    }
+    // the first chunk's first rows and row operands are on their way while
+    // the previous launch's dots are reduced
+    int2 cr = c0 < c1 ? chunks[sd.upd0 + c0] : make_int2(0, 0);
+    RowBatch<HV> cur;
+    int e0, e1;
+    first_batch(cr.x, cr.y, row_ptr, nbr_j, hv, cur, e0, e1);
+    Staged sv;
+    stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+    double res[8];
+    const double* pp = part + ((k - 1) & 1) * NDOT * nd;
+    block_sums<8>(sd.ndir, [&](int t, double (&a)[8]) {
+#pragma unroll
+      for (int q = 0; q < NDOT; ++q) a[q] += pp[q * nd + sd.dir0 + t];
+      a[6] += part_bb[sd.dir0 + t];
+      a[7] += part_bb[nd + sd.dir0 + t];
+    }, res);
+    bool conv[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      double target = tol2 * res[6 + r];
+      if (MIXED && j > 0) target = fmax(target, INNER_DROP2 * c.rr0[r]);
+      conv[r] = res[4 + r] <= target;
+    }
+    if ((conv[0] && conv[1]) || c.it >= max_iter || (MIXED && j >= INNER_MAX)) {
+      if (lead) {
+        int state;
+        if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
+        else if (c.it >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
+        else state = CG_REFRESH;
+        settle(&cg[sys], cnt, c, state, k);
+      }
+      return;
+    }
+    st.fresh = j == 0;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const double g = res[r], d = res[2 + r];
+      double a = 0.0, b = 0.0;
+      if (!conv[r]) {
+        if (st.fresh || c.alp[r] == 0.0 || c.gam[r] == 0.0) {
+          a = d != 0.0 ? g / d : 0.0;
+        } else {
+          b = g / c.gam[r];
+          const double den = d - b * g / c.alp[r];
+          a = den != 0.0 ? g / den : 0.0;
+        }
+      }
+      st.alpha[r] = a;
+      st.beta[r] = b;
+    }
+    if (lead) {
+      CgCtl s = c;
+      s.done = CG_RUNNING;
+      if (st.fresh) {
+        s.k0 = k;
+        s.rr0[0] = res[4];
+        s.rr0[1] = res[5];
+      }
+      s.it = c.it + 1;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        // a converged right-hand side keeps its last step (alpha = 0 stops it)
+        s.gam[r] = conv[r] ? c.gam[r] : res[r];
+        s.alp[r] = conv[r] ? c.alp[r] : st.alpha[r];
+      }
+      cg[sys].ctl[k & 1] = s;
+    }
+    for (int ch = c0; ch < c1; ++ch) {
+      if (ch > c0) {
+        cr = chunks[sd.upd0 + ch];
+        first_batch(cr.x, cr.y, row_ptr, nbr_j, hv, cur, e0, e1);
+        stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+      }
+      chunk_spmv<HV, VT, AT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
+      stage_store(cr.x, cr.y, sv, L);
+      __syncthreads();
+      if (threadIdx.x < 64) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
+      if (ch + 1 < c1) __syncthreads();
+    }
+  } else {
+    // refresh in FP64: r = b - A x and u = M^-1 r, then w = A u and m = M^-1 w
+    if (lead) {
+      CgCtl s = c;
+      if (mode == MODE_RESIDUAL) s.restarts = c.restarts + 1;
+      cg[sys].ctl[k & 1] = s;
+    }
+    const double2* vin = mode == MODE_RESIDUAL ? v.X : v.U;
+    for (int ch = c0; ch < c1; ++ch) {
+      const int2 cr = chunks[sd.upd0 + ch];
+      RowBatch<double> cur;
+      int e0, e1;
+      first_batch(cr.x, cr.y, row_ptr, nbr_j, hval, cur, e0, e1);
+      Staged sv;
+      stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+      chunk_spmv<double, double2, double>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
+      stage_store(cr.x, cr.y, sv, L);
+      __syncthreads();
+      if (threadIdx.x < 64) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
+      if (ch + 1 < c1) __syncthreads();
+    }
+    if (mode == MODE_RESIDUAL) return;
+  }
+  if (threadIdx.x < 64) {
+    // wave 0 holds every row's dots: one wave sum per series, lane 0 stores
+#pragma unroll
+    for (int q = 0; q < 8; ++q) dots[q] = xr_wave_sum(dots[q]);
+    if (threadIdx.x == 0) {
+      double* out = part + (k & 1) * NDOT * nd;
+#pragma unroll
+      for (int q = 0; q < NDOT; ++q) out[q * nd + blockIdx.x] = dots[q];
+      if (mode == MODE_START) {
+        part_bb[blockIdx.x] = dots[6];
+        part_bb[nd + blockIdx.x] = dots[7];
+      }
+    }
+  }
 }
 
 // Per system: q = s + mu t with mu = (Q - sum s) / sum t; the (s, t) solution
@@ -909,23 +890,27 @@ hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, c
   return hipSuccess;
 }
 
-// Direction blocks per system: about DIR_ROWS_PER_WAVE rows per wave up to a cap (XREAC_CG_MAX_BLOCKS
-// overrides the cap for tuning; it must be the same for every call that is
-// compared bitwise, since a system's blocks depend only on its size and this cap).
+// Chunks and iteration blocks per system: ceil(n / CHUNK) chunks, spread over
+// at most CG_MAX_BLOCKS blocks with every block owning at least one chunk
+// (XREAC_CG_MAX_BLOCKS overrides the cap for tuning; it must be the same for
+// every call that is compared bitwise, since a system's blocks depend only on
+// its size and this cap).
 void xr_cg_blocks(int n, int* ndir, int* nupd) {
   static const int cap = [] {
     const char* v = std::getenv("XREAC_CG_MAX_BLOCKS");
-    const int c = v ? std::atoi(v) : DIR_MAX_BLOCKS;
-    return c > 0 ? c : DIR_MAX_BLOCKS;
+    const int c = v ? std::atoi(v) : CG_MAX_BLOCKS;
+    return c > 0 ? c : CG_MAX_BLOCKS;
   }();
-  const int rows_per_block = DIR_ROWS_PER_WAVE * (DIR_BLOCK / 64);
-  int d = (n + rows_per_block - 1) / rows_per_block;
-  *ndir = d < 1 ? 1 : (d > cap ? cap : d);
-  const int u = (n + UPD_BLOCK - 1) / UPD_BLOCK;
-  *nupd = u < 1 ? 1 : u;
+  const int chunks = n > 0 ? (n + CHUNK - 1) / CHUNK : 1;
+  const int per = (chunks + cap - 1) / cap;
+  *nupd = chunks;
+  *ndir = (chunks + per - 1) / per;
 }
 
-size_t xr_cg_partials(int nd, int nu) { return (size_t)(2 * 4 * nu + 2 * nu + 2 * nd) + 64; }
+size_t xr_cg_partials(int nd, int nu) {
+  (void)nu;
+  return (size_t)(2 * NDOT * nd + 2 * nd) + 64;
+}
 
 size_t xr_cg_block_doubles(int n) { return (size_t)n * BLOCK_MAX; }
 
@@ -933,44 +918,32 @@ size_t xr_cg_state_bytes(int ns) { return sizeof(CgSys) * (size_t)ns + sizeof(Cg
 
 hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   const int nu = a.nu, nd = a.nd, ns = a.S.ns;
-  double* part_b = a.partials;
-  double* part_bb = part_b + 2 * 4 * nu;
-  double* part_a = part_bb + 2 * nu;
+  double* part = a.partials;
+  double* part_bb = part + 2 * NDOT * nd;
   CgCounters* cnt = static_cast<CgCounters*>(a.state);
   CgSys* cg = xr_cg_systems(a.state);
   const double tol2 = a.tol * a.tol;
   const bool mixed = a.hval32 != nullptr;
-  float4* zq0 = mixed ? a.ZQ0 : nullptr;
-  float4* zq1 = mixed ? a.ZQ1 : nullptr;
   // counters start at zero (cleared with the control block)
-  hipLaunchKernelGGL(k_cg_init, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.X, a.R, a.ZP0,
-                     zq0, a.row_ptr, a.nbr_j, a.hval, a.join, a.minv, a.binfo, a.eta, part_b, part_bb, nu, cg, cnt,
-                     a.abort_flag, a.n);
+  XR_LAUNCH(XT_CG_INIT, k_cg_init, dim3(nu), dim3(64), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.v, a.row_ptr,
+            a.nbr_j, a.hval, a.join, a.minv, a.binfo, a.chunks, a.eta, cg, cnt, a.abort_flag, a.n);
   XR_LAUNCH_CHECK();
   int k = 0;
-  // Every system settles within max_iter executed iterations plus two
-  // launches per refresh (the request and the FP64 sweep; one more for a warm start).
+  // Every system settles within max_iter iterations plus three launches per
+  // refresh (the settling launch, the true residual and the start).
   const long long k_limit = (long long)a.max_iter + 3LL * (MAX_RESTARTS + 3) + a.first_check + a.check_every;
   out->aborted = false;
   out->exhausted = false;
   auto enqueue = [&](int iters) {
     for (int c = 0; c < iters && k < k_limit; ++c) {
-      double4* zp_in = (k & 1) ? a.ZP1 : a.ZP0;
-      double4* zp_out = (k & 1) ? a.ZP0 : a.ZP1;
-      float4* zq_in = (k & 1) ? zq1 : zq0;
-      float4* zq_out = (k & 1) ? zq0 : zq1;
       if (mixed) {
-        XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<true>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
-                  a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
-                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.X, a.R);
-        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<true>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.n, a.minv, a.binfo, a.X, a.R,
-                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
+        XR_LAUNCH(XT_CG_ITERATION, k_cg_iteration<true>, dim3(nd), dim3(CG_BLOCK), 0, s, a.S, k, a.v, a.row_ptr,
+                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.type, a.ff, a.rhs, part, part_bb,
+                  nd, tol2, a.max_iter, cg, cnt, a.n);
       } else {
-        XR_LAUNCH(XT_CG_DIRECTION, k_cg_direction<false>, dim3(nd), dim3(DIR_BLOCK), 0, s, a.S, k, a.row_ptr,
-                  a.nbr_j, a.hval, a.hval32, a.eta, zp_in, zp_out, zq_in, zq_out, a.AP, part_b, part_bb, nu, part_a,
-                  nd, tol2, a.max_iter, cg, cnt, a.type, a.ff, a.rhs, a.X, a.R);
-        XR_LAUNCH(XT_CG_UPDATE, k_cg_update<false>, dim3(nu), dim3(UPD_BLOCK), 0, s, a.S, k, a.n, a.minv, a.binfo, a.X, a.R,
-                  zp_out, zq_out, a.AP, part_a, nd, part_b, part_bb, nu, tol2, cg);
+        XR_LAUNCH(XT_CG_ITERATION, k_cg_iteration<false>, dim3(nd), dim3(CG_BLOCK), 0, s, a.S, k, a.v, a.row_ptr,
+                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.type, a.ff, a.rhs, part, part_bb,
+                  nd, tol2, a.max_iter, cg, cnt, a.n);
       }
       ++k;
     }
@@ -995,6 +968,7 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
     return e;
   };
   hipError_t e = issue(0, a.first_check);
+  if (e == hipSuccess && a.after_start) e = a.after_start(a.after_start_arg);
   if (e != hipSuccess) return e;
   int cur = 0;
   bool pipelined = false;
@@ -1023,7 +997,7 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
     }
   }
   out->sweeps = k;
-  hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.X, a.q, a.mu, a.keep, a.neutral ? 1 : 0);
+  hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.v.X, a.q, a.mu, a.keep, a.neutral ? 1 : 0);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
