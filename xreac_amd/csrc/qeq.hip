@@ -153,35 +153,6 @@ __device__ inline int block_system(const Systems& S, int b, bool dir) {
   return lo;
 }
 
-// Sums of NV series of cnt block partials in a fixed order: thread-strided
-// loads over the whole block (one latency), the wave tree in the waves that
-// hold data, then those waves in order.  One barrier; the order depends only
-// on cnt and the block size, so only on the system.
-template <int NV, class Load>
-__device__ inline void block_sums(int cnt, Load load, double (&out)[NV]) {
-  __shared__ double red[NV][16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int used = (cnt + 63) >> 6 < (int)(blockDim.x >> 6) ? (cnt + 63) >> 6 : (int)(blockDim.x >> 6);
-  if (wave < used) {
-    double v[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = 0.0;
-    for (int t = threadIdx.x; t < cnt; t += blockDim.x) load(t, v);
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      const double w = xr_wave_sum(v[q]);
-      if (lane == 0) red[q][wave] = w;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double x = 0.0;
-    for (int w = 0; w < used; ++w) x += red[q][w];
-    out[q] = x;
-  }
-}
-
 // Local row where chunk c of a system starts: the nominal c * CHUNK, moved
 // forward to the first row within RUN_MAX rows that starts a bonded run (no
 // strong bond to any of the three rows before it), so that molecules are not
@@ -437,10 +408,9 @@ __device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_pt
     }
     AT as = AT(0), at = AT(0);
     gather_sum(cur, vec, as, at);
-    for (int base = e0 + 512; base < e1; base += 512) {
-      RowBatch<HV> more;
-      load_batch(nbr_j, hv, base, e1, i, lane, more);
-      gather_sum(more, vec, as, at);
+    for (int base = e0 + 512; base < e1; base += 512) {  // rows longer than one batch reuse cur
+      load_batch(nbr_j, hv, base, e1, i, lane, cur);
+      gather_sum(cur, vec, as, at);
     }
     const double s0 = xr_wave_sum((double)as), s1 = xr_wave_sum((double)at);
     if (lane == 0) out[i - r0] = make_double2(s0, s1);
@@ -489,15 +459,16 @@ struct ChunkLds {
   double mv[BLOCK_MAX][CHUNK_MAX];
   double eta[CHUNK_MAX];
   int2 bi[CHUNK_MAX];
-  double2 sums[CHUNK_MAX];  // off-diagonal A v row sums
-  double2 ex[CHUNK_MAX];    // vector exchanged for the M^-1 apply
+  double2 sums[CHUNK_MAX];   // off-diagonal A v row sums
+  double ex[2][CHUNK_MAX];   // vector exchanged for the M^-1 apply, per right-hand side
 };
 
+// One vector entry (items below NITEM), eta (item NITEM, in val.x) or the
+// block placement (item NITEM + 1, its bits in val.x), and one inverse-block
+// column entry.
 struct Staged {
   double2 val;
   double mv;
-  double eta;
-  int2 bi;
 };
 
 __device__ inline bool item_used(int mode, int item) {
@@ -514,8 +485,6 @@ __device__ inline void stage_load(int mode, int r0, int r1, const CgVec& v, cons
   const int i = r0 + row;
   s.val = make_double2(0.0, 0.0);
   s.mv = 0.0;
-  s.eta = 0.0;
-  s.bi = make_int2(0, 0);
   if (i >= r1) return;
   s.mv = minv[(size_t)item * n_all + i];  // entries past the block size are never used
   if (item < NITEM && item_used(mode, item)) {
@@ -532,8 +501,8 @@ __device__ inline void stage_load(int mode, int r0, int r1, const CgVec& v, cons
       default: s.val = cg_rhs(rhs, type, ff, i); break;
     }
   }
-  if (item == NITEM) s.eta = eta[i];
-  if (item == NITEM + 1) s.bi = binfo[i];
+  if (item == NITEM) s.val.x = eta[i];
+  if (item == NITEM + 1) s.val.x = __longlong_as_double(*reinterpret_cast<const long long*>(binfo + i));
 }
 
 __device__ inline void stage_store(int r0, int r1, const Staged& s, ChunkLds& L) {
@@ -541,23 +510,17 @@ __device__ inline void stage_store(int r0, int r1, const Staged& s, ChunkLds& L)
   if (r0 + row >= r1) return;
   L.mv[item][row] = s.mv;
   if (item < NITEM) L.st[item][row] = s.val;
-  if (item == NITEM) L.eta[row] = s.eta;
-  if (item == NITEM + 1) L.bi[row] = s.bi;
+  if (item == NITEM) L.eta[row] = s.val.x;
+  if (item == NITEM + 1) *reinterpret_cast<long long*>(&L.bi[row]) = __double_as_longlong(s.val.x);
 }
 
-// M^-1 applied to the exchanged vector for row t of the chunk
-__device__ inline double2 apply_block(const ChunkLds& L, int t) {
+// M^-1 applied to the exchanged vector of right-hand side h for row t
+__device__ inline double apply_block(const ChunkLds& L, int h, int t) {
   const int2 bi = L.bi[t];
-  double2 z = make_double2(0.0, 0.0);
+  double z = 0.0;
 #pragma unroll
-  for (int c = 0; c < BLOCK_MAX; ++c) {
-    if (c < bi.y) {
-      const double w = L.mv[c][t];
-      const double2 r = L.ex[bi.x + c];
-      z.x += w * r.x;
-      z.y += w * r.y;
-    }
-  }
+  for (int c = 0; c < BLOCK_MAX; ++c)
+    if (c < bi.y) z += L.mv[c][t] * L.ex[h][bi.x + c];
   return z;
 }
 
@@ -566,80 +529,82 @@ struct CgStep {
   bool fresh;  // first iteration since a (re)start: the old z, q, s, p are not read
 };
 
-// The chunk's rows after its sweep (wave 0, lane t owns row t):
+// The chunk's rows after its sweep, waves 0 and 1 (lane t owns row t, wave h
+// right-hand side h):
 //   iteration: n = A m;  z = n + b z, q = m + b q, s = w + b s, p = u + b p;
 //              x += a p, r -= a s, u -= a q, w -= a z;  m = M^-1 w
 //   residual:  r = b - A x;  u = M^-1 r
 //   start:     w = A u;  m = M^-1 w
-// and the block's dots (r.u, w.u, r.r and, at a start, b.b) accumulate in dots.
+// and the wave's dots (r.u, w.u, r.r and, at a start, b.b) accumulate in dots.
 template <bool MIXED>
 __device__ inline void chunk_tail(int mode, int r0, int r1, const CgStep& st, const CgVec& v, double2* m_out,
-                                  float2* m32_out, ChunkLds& L, double (&dots)[8]) {
-  const int t = threadIdx.x;
+                                  float2* m32_out, ChunkLds& L, double (&dots)[4]) {
+  const int t = threadIdx.x & 63;
+  const int h = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar store addresses
   const int i = r0 + t;
   const bool mine = i < r1;
+  auto get = [&](int item) { return h ? L.st[item][t].y : L.st[item][t].x; };
+  auto put = [&](double2* vec, double val) { reinterpret_cast<double*>(vec + i)[h] = val; };
+  double r = 0.0, u = 0.0, w = 0.0;
   if (mine) {
     const double eta = L.eta[t];
-    const double2 ax = L.sums[t];
+    const double ax = h ? L.sums[t].y : L.sums[t].x;
     if (mode == MODE_ITER) {
-      const double2 m = L.st[IT_M][t];
-      const double2 nn = make_double2(eta * m.x + ax.x, eta * m.y + ax.y);
-      double2 u = L.st[IT_U][t], w = L.st[IT_W][t], x = L.st[IT_X][t], r = L.st[IT_R][t];
-      double2 z = nn, q = m, s = w, p = u;
+      const double a = h ? st.alpha[1] : st.alpha[0];
+      const double b = h ? st.beta[1] : st.beta[0];
+      const double m = get(IT_M);
+      const double nn = eta * m + ax;
+      u = get(IT_U);
+      w = get(IT_W);
+      r = get(IT_R);
+      double x = get(IT_X);
+      double z = nn, q = m, s = w, p = u;
       if (!st.fresh) {
-        const double2 z0 = L.st[IT_Z][t], q0 = L.st[IT_Q][t], s0 = L.st[IT_S][t], p0 = L.st[IT_P][t];
-        z = make_double2(nn.x + st.beta[0] * z0.x, nn.y + st.beta[1] * z0.y);
-        q = make_double2(m.x + st.beta[0] * q0.x, m.y + st.beta[1] * q0.y);
-        s = make_double2(w.x + st.beta[0] * s0.x, w.y + st.beta[1] * s0.y);
-        p = make_double2(u.x + st.beta[0] * p0.x, u.y + st.beta[1] * p0.y);
+        z = nn + b * get(IT_Z);
+        q = m + b * get(IT_Q);
+        s = w + b * get(IT_S);
+        p = u + b * get(IT_P);
       }
-      x = make_double2(x.x + st.alpha[0] * p.x, x.y + st.alpha[1] * p.y);
-      r = make_double2(r.x - st.alpha[0] * s.x, r.y - st.alpha[1] * s.y);
-      u = make_double2(u.x - st.alpha[0] * q.x, u.y - st.alpha[1] * q.y);
-      w = make_double2(w.x - st.alpha[0] * z.x, w.y - st.alpha[1] * z.y);
-      v.Z[i] = z;
-      v.Q[i] = q;
-      v.S[i] = s;
-      v.P[i] = p;
-      v.X[i] = x;
-      v.R[i] = r;
-      v.U[i] = u;
-      v.W[i] = w;
-      L.ex[t] = w;
-      L.st[IT_R][t] = r;
-      L.st[IT_U][t] = u;
+      x += a * p;
+      r -= a * s;
+      u -= a * q;
+      w -= a * z;
+      put(v.Z, z);
+      put(v.Q, q);
+      put(v.S, s);
+      put(v.P, p);
+      put(v.X, x);
+      put(v.R, r);
+      put(v.U, u);
+      put(v.W, w);
+      L.ex[h][t] = w;
     } else if (mode == MODE_RESIDUAL) {
-      const double2 x = L.st[IT_X][t], b = L.st[IT_B][t];
-      const double2 r = make_double2(b.x - (eta * x.x + ax.x), b.y - (eta * x.y + ax.y));
-      v.R[i] = r;
-      L.ex[t] = r;
+      r = get(IT_B) - (eta * get(IT_X) + ax);
+      put(v.R, r);
+      L.ex[h][t] = r;
     } else {
-      const double2 u = L.st[IT_U][t];
-      const double2 w = make_double2(eta * u.x + ax.x, eta * u.y + ax.y);
-      v.W[i] = w;
-      L.ex[t] = w;
+      r = get(IT_R);
+      u = get(IT_U);
+      w = eta * u + ax;
+      put(v.W, w);
+      L.ex[h][t] = w;
     }
   }
   xr_wave_sync();
   if (!mine) return;
-  const double2 mz = apply_block(L, t);
+  const double mz = apply_block(L, h, t);
   if (mode == MODE_RESIDUAL) {
-    v.U[i] = mz;
+    put(v.U, mz);
     return;
   }
-  m_out[i] = mz;
-  if (MIXED) m32_out[i] = make_float2((float)mz.x, (float)mz.y);
-  const double2 r = L.st[IT_R][t], u = L.st[IT_U][t], w = L.ex[t];
-  dots[0] += r.x * u.x;
-  dots[1] += r.y * u.y;
-  dots[2] += w.x * u.x;
-  dots[3] += w.y * u.y;
-  dots[4] += r.x * r.x;
-  dots[5] += r.y * r.y;
+  put(m_out, mz);
+  if (MIXED) reinterpret_cast<float*>(m32_out + i)[h] = (float)mz;
+  dots[0] += r * u;
+  dots[1] += w * u;
+  dots[2] += r * r;
   if (mode == MODE_START) {
-    const double2 b = L.st[IT_B][t];
-    dots[6] += b.x * b.x;
-    dots[7] += b.y * b.y;
+    const double b = get(IT_B);
+    dots[3] += b * b;
   }
 }
 
@@ -696,7 +661,7 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
   double2* m_out = v.M[(k + 1) & 1];
   float2* m32_out = v.M32[(k + 1) & 1];
   const bool lead = local == 0 && threadIdx.x == 0;
-  double dots[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double dots[4] = {0, 0, 0, 0};
 
   CgStep st{};
   if (mode == MODE_ITER) {
@@ -712,73 +677,34 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
       hv = hval;
       vin = m_in;
     }
-    // the first chunk's first rows and row operands are on their way while
-    // the previous launch's dots are reduced
+    // The first chunk's first rows, its row operands and the previous
+    // launch's partials are all loaded before the sweep; the partials are
+    // reduced after it (the sweep n = A m needs neither alpha nor beta), so
+    // their latency hides under the gathers.  Thread t holds partial t % 256
+    // of the series pair 2 (t / 256): waves 0-3 the r.u pair, 4-7 w.u, 8-11
+    // r.r, 12-15 b.b.
     int2 cr = c0 < c1 ? chunks[sd.upd0 + c0] : make_int2(0, 0);
     RowBatch<HV> cur;
     int e0, e1;
     first_batch(cr.x, cr.y, row_ptr, nbr_j, hv, cur, e0, e1);
     Staged sv;
     stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
-    double res[8];
-    const double* pp = part + ((k - 1) & 1) * NDOT * nd;
-    block_sums<8>(sd.ndir, [&](int t, double (&a)[8]) {
-#pragma unroll
-      for (int q = 0; q < NDOT; ++q) a[q] += pp[q * nd + sd.dir0 + t];
-      a[6] += part_bb[sd.dir0 + t];
-      a[7] += part_bb[nd + sd.dir0 + t];
-    }, res);
-    bool conv[2];
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      double target = tol2 * res[6 + r];
-      if (MIXED && j > 0) target = fmax(target, INNER_DROP2 * c.rr0[r]);
-      conv[r] = res[4 + r] <= target;
-    }
-    if ((conv[0] && conv[1]) || c.it >= max_iter || (MIXED && j >= INNER_MAX)) {
-      if (lead) {
-        int state;
-        if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
-        else if (c.it >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
-        else state = CG_REFRESH;
-        settle(&cg[sys], cnt, c, state, k);
+    double pa = 0.0, pb = 0.0;
+    {
+      const int series = (threadIdx.x >> 8) * 2;
+      const double* base = series < NDOT ? part + ((k - 1) & 1) * NDOT * nd + series * nd : part_bb;
+      const int t0 = threadIdx.x & 255;
+      if (t0 < sd.ndir) {
+        pa = base[sd.dir0 + t0];
+        pb = base[nd + sd.dir0 + t0];
       }
-      return;
-    }
-    st.fresh = j == 0;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const double g = res[r], d = res[2 + r];
-      double a = 0.0, b = 0.0;
-      if (!conv[r]) {
-        if (st.fresh || c.alp[r] == 0.0 || c.gam[r] == 0.0) {
-          a = d != 0.0 ? g / d : 0.0;
-        } else {
-          b = g / c.gam[r];
-          const double den = d - b * g / c.alp[r];
-          a = den != 0.0 ? g / den : 0.0;
-        }
+      for (int t = t0 + 256; t < sd.ndir; t += 256) {  // only with a raised block cap
+        pa += base[sd.dir0 + t];
+        pb += base[nd + sd.dir0 + t];
       }
-      st.alpha[r] = a;
-      st.beta[r] = b;
     }
-    if (lead) {
-      CgCtl s = c;
-      s.done = CG_RUNNING;
-      if (st.fresh) {
-        s.k0 = k;
-        s.rr0[0] = res[4];
-        s.rr0[1] = res[5];
-      }
-      s.it = c.it + 1;
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        // a converged right-hand side keeps its last step (alpha = 0 stops it)
-        s.gam[r] = conv[r] ? c.gam[r] : res[r];
-        s.alp[r] = conv[r] ? c.alp[r] : st.alpha[r];
-      }
-      cg[sys].ctl[k & 1] = s;
-    }
+    __shared__ double red[8][4];
+    bool stepped = false;
     for (int ch = c0; ch < c1; ++ch) {
       if (ch > c0) {
         cr = chunks[sd.upd0 + ch];
@@ -787,8 +713,73 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
       }
       chunk_spmv<HV, VT, AT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
       stage_store(cr.x, cr.y, sv, L);
+      if (!stepped) {
+        pa = xr_wave_sum(pa);
+        pb = xr_wave_sum(pb);
+        if ((threadIdx.x & 63) == 0) {
+          red[(threadIdx.x >> 8) * 2][(threadIdx.x >> 6) & 3] = pa;
+          red[(threadIdx.x >> 8) * 2 + 1][(threadIdx.x >> 6) & 3] = pb;
+        }
+      }
       __syncthreads();
-      if (threadIdx.x < 64) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
+      if (!stepped) {
+        stepped = true;
+        double res[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) res[q] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+        bool conv[2];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          double target = tol2 * res[6 + r];
+          if (MIXED && j > 0) target = fmax(target, INNER_DROP2 * c.rr0[r]);
+          conv[r] = res[4 + r] <= target;
+        }
+        if ((conv[0] && conv[1]) || c.it >= max_iter || (MIXED && j >= INNER_MAX)) {
+          if (lead) {
+            int state;
+            if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
+            else if (c.it >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
+            else state = CG_REFRESH;
+            settle(&cg[sys], cnt, c, state, k);
+          }
+          return;
+        }
+        st.fresh = j == 0;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const double g = res[r], d = res[2 + r];
+          double a = 0.0, b = 0.0;
+          if (!conv[r]) {
+            if (st.fresh || c.alp[r] == 0.0 || c.gam[r] == 0.0) {
+              a = d != 0.0 ? g / d : 0.0;
+            } else {
+              b = g / c.gam[r];
+              const double den = d - b * g / c.alp[r];
+              a = den != 0.0 ? g / den : 0.0;
+            }
+          }
+          st.alpha[r] = a;
+          st.beta[r] = b;
+        }
+        if (lead) {
+          CgCtl s = c;
+          s.done = CG_RUNNING;
+          if (st.fresh) {
+            s.k0 = k;
+            s.rr0[0] = res[4];
+            s.rr0[1] = res[5];
+          }
+          s.it = c.it + 1;
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            // a converged right-hand side keeps its last step (alpha = 0 stops it)
+            s.gam[r] = conv[r] ? c.gam[r] : res[r];
+            s.alp[r] = conv[r] ? c.alp[r] : st.alpha[r];
+          }
+          cg[sys].ctl[k & 1] = s;
+        }
+      }
+      if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
       if (ch + 1 < c1) __syncthreads();
     }
   } else {
@@ -809,23 +800,22 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
       chunk_spmv<double, double2, double>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
       stage_store(cr.x, cr.y, sv, L);
       __syncthreads();
-      if (threadIdx.x < 64) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
+      if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
       if (ch + 1 < c1) __syncthreads();
     }
     if (mode == MODE_RESIDUAL) return;
   }
-  if (threadIdx.x < 64) {
-    // wave 0 holds every row's dots: one wave sum per series, lane 0 stores
+  if (threadIdx.x < 128) {
+    // wave h holds right-hand side h of every row: one wave sum per series,
+    // lane 0 stores series 2q + h
+    const int h = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) dots[q] = xr_wave_sum(dots[q]);
-    if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; ++q) dots[q] = xr_wave_sum(dots[q]);
+    if ((threadIdx.x & 63) == 0) {
       double* out = part + (k & 1) * NDOT * nd;
 #pragma unroll
-      for (int q = 0; q < NDOT; ++q) out[q * nd + blockIdx.x] = dots[q];
-      if (mode == MODE_START) {
-        part_bb[blockIdx.x] = dots[6];
-        part_bb[nd + blockIdx.x] = dots[7];
-      }
+      for (int q = 0; q < 3; ++q) out[(2 * q + h) * nd + blockIdx.x] = dots[q];
+      if (mode == MODE_START) part_bb[h * nd + blockIdx.x] = dots[3];
     }
   }
 }
