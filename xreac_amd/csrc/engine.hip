@@ -499,6 +499,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double2>(off, 10 * (size_t)n);  // CG vectors: x, r, u, w, z, q, s, p and two m (pairs)
   carve_size<float2>(off, 2 * (size_t)n);    // FP32 copies of the two m (mixed precision)
   carve_size<int2>(off, nu);                 // CG chunk ranges
+  carve_size<int4>(off, nd);                 // CG iteration block descriptors
   carve_size<double>(off, xr_cg_block_doubles(n));  // block-Jacobi inverse rows
   carve_size<int2>(off, n);              // block placement
   carve_size<unsigned char>(off, n);     // block join flags
@@ -524,6 +525,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   double2* cgv = cv.take<double2>(10 * (size_t)n);
   float2* cgq = cv.take<float2>(2 * (size_t)n);
   int2* cg_chunks = cv.take<int2>(nu);
+  int4* cg_blocks = cv.take<int4>(nd);
   double* minv = cv.take<double>(xr_cg_block_doubles(n));
   int2* binfo = cv.take<int2>(n);
   unsigned char* join = cv.take<unsigned char>(n);
@@ -825,6 +827,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
                  {vecs[8], vecs[9]}, {cgq, cgq + n}};
   }
   cg.chunks = cg_chunks;
+  cg.blocks = cg_blocks;
   cg.join = join;
   cg.minv = minv;
   cg.n = n;

@@ -59,6 +59,7 @@ struct CgArgs {
   double* minv;               // block-Jacobi inverse rows, column-major: entry c of row i at minv[c * n + i]
   int2* binfo;                // per atom: (chunk row of its block start, block size)
   int2* chunks;               // per chunk: atom range [x, y)
+  int4* blocks;               // per iteration block: system, first chunk's atom range, its chunk index
   double* eta;
   double* partials;  // xr_cg_partials(nd, nu) doubles
   void* state;       // xr_cg_state_bytes(ns) bytes

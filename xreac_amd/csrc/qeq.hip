@@ -251,7 +251,8 @@ __global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict
                                                 const int* __restrict__ nbr_j, const double* __restrict__ hval,
                                                 const unsigned char* __restrict__ join, double* __restrict__ minv,
                                                 int2* __restrict__ binfo, int2* __restrict__ chunks,
-                                                double* __restrict__ eta_out, CgSys* __restrict__ cg,
+                                                int4* __restrict__ blocks, double* __restrict__ eta_out,
+                                                CgSys* __restrict__ cg,
                                                 CgCounters* __restrict__ cnt, const unsigned* __restrict__ abort_flag,
                                                 int n_all) {
   const int sys = block_system(S, blockIdx.x, false);
@@ -268,7 +269,13 @@ __global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict
   __shared__ ChunkBlocks br;
   __shared__ double hb[CHUNK_MAX][BLOCK_MAX];  // each row's row of its H block, inverted in place
   __shared__ double2 bsh[CHUNK_MAX];
-  if (t == 0) chunks[blockIdx.x] = make_int2(row0, sd.atom0 + r1);
+  if (t == 0) {
+    chunks[blockIdx.x] = make_int2(row0, sd.atom0 + r1);
+    // the iteration block owning this chunk first: its system, this chunk's
+    // rows and the system-local index of this chunk
+    const int per = (sd.nupd + sd.ndir - 1) / sd.ndir;
+    if (c % per == 0) blocks[sd.dir0 + c / per] = make_int4(sys, row0, sd.atom0 + r1, c);
+  }
   join_sh[t] = mine ? join[i] : 0;
   xr_wave_sync();
   assign_blocks(join_sh, rows, br);
@@ -420,16 +427,21 @@ __device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_pt
   }
 }
 
-template <class HV>
-__device__ inline void first_batch(int r0, int r1, const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                   const HV* __restrict__ hv, RowBatch<HV>& b, int& e0, int& e1) {
+// The wave's first row of a chunk: its entry range, then its first batch.
+__device__ inline void first_row(int r0, int r1, const int* __restrict__ row_ptr, int& e0, int& e1) {
   const int i = r0 + (threadIdx.x >> 6);
   e0 = e1 = 0;
   if (i < r1) {
     e0 = row_ptr[i];
     e1 = row_ptr[i + 1];
-    load_batch(nbr_j, hv, e0, e1, i, threadIdx.x & 63, b);
   }
+}
+
+template <class HV>
+__device__ inline void first_batch(int r0, int r1, const int* __restrict__ nbr_j, const HV* __restrict__ hv, int e0,
+                                   int e1, RowBatch<HV>& b) {
+  const int i = r0 + (threadIdx.x >> 6);
+  if (i < r1) load_batch(nbr_j, hv, e0, e1, i, threadIdx.x & 63, b);
 }
 
 // What a system's block does in launch k, from the state the previous launch
@@ -642,12 +654,18 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
                                                            const double* __restrict__ minv,
                                                            const int2* __restrict__ binfo,
                                                            const int2* __restrict__ chunks,
+                                                           const int4* __restrict__ blocks,
                                                            const int* __restrict__ type, FFDev ff,
                                                            const double2* __restrict__ rhs, double* __restrict__ part,
                                                            double* __restrict__ part_bb, int nd, double tol2,
                                                            int max_iter, CgSys* __restrict__ cg,
                                                            CgCounters* __restrict__ cnt, int n_all) {
-  const int sys = block_system(S, blockIdx.x, true);
+  // The block's descriptor gives its system and first chunk; the wave's first
+  // row range is loaded alongside the system's control state.
+  const int4 desc = blocks[blockIdx.x];
+  const int sys = desc.x;
+  int e0, e1;
+  first_row(desc.y, desc.z, row_ptr, e0, e1);
   const CgCtl c = cg[sys].ctl[(k - 1) & 1];
   int j;
   const int mode = cg_mode(c, k, j);
@@ -655,7 +673,7 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
   const SysDev& sd = S.sys[sys];
   const int local = blockIdx.x - sd.dir0;
   const int per = (sd.nupd + sd.ndir - 1) / sd.ndir;
-  const int c0 = local * per, c1 = min(c0 + per, sd.nupd);
+  const int c0 = desc.w, c1 = min(c0 + per, sd.nupd);
   __shared__ ChunkLds L;
   const double2* m_in = v.M[k & 1];
   double2* m_out = v.M[(k + 1) & 1];
@@ -683,10 +701,9 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
     // their latency hides under the gathers.  Thread t holds partial t % 256
     // of the series pair 2 (t / 256): waves 0-3 the r.u pair, 4-7 w.u, 8-11
     // r.r, 12-15 b.b.
-    int2 cr = c0 < c1 ? chunks[sd.upd0 + c0] : make_int2(0, 0);
+    int2 cr = make_int2(desc.y, desc.z);
     RowBatch<HV> cur;
-    int e0, e1;
-    first_batch(cr.x, cr.y, row_ptr, nbr_j, hv, cur, e0, e1);
+    first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, cur);
     Staged sv;
     stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
     double pa = 0.0, pb = 0.0;
@@ -708,7 +725,8 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
     for (int ch = c0; ch < c1; ++ch) {
       if (ch > c0) {
         cr = chunks[sd.upd0 + ch];
-        first_batch(cr.x, cr.y, row_ptr, nbr_j, hv, cur, e0, e1);
+        first_row(cr.x, cr.y, row_ptr, e0, e1);
+        first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, cur);
         stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
       }
       chunk_spmv<HV, VT, AT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
@@ -791,10 +809,10 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
     }
     const double2* vin = mode == MODE_RESIDUAL ? v.X : v.U;
     for (int ch = c0; ch < c1; ++ch) {
-      const int2 cr = chunks[sd.upd0 + ch];
+      const int2 cr = ch == c0 ? make_int2(desc.y, desc.z) : chunks[sd.upd0 + ch];
+      if (ch > c0) first_row(cr.x, cr.y, row_ptr, e0, e1);
       RowBatch<double> cur;
-      int e0, e1;
-      first_batch(cr.x, cr.y, row_ptr, nbr_j, hval, cur, e0, e1);
+      first_batch(cr.x, cr.y, nbr_j, hval, e0, e1, cur);
       Staged sv;
       stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
       chunk_spmv<double, double2, double>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
@@ -916,7 +934,7 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   const bool mixed = a.hval32 != nullptr;
   // counters start at zero (cleared with the control block)
   XR_LAUNCH(XT_CG_INIT, k_cg_init, dim3(nu), dim3(64), 0, s, a.S, a.type, a.ff, a.rhs, a.guess, a.v, a.row_ptr,
-            a.nbr_j, a.hval, a.join, a.minv, a.binfo, a.chunks, a.eta, cg, cnt, a.abort_flag, a.n);
+            a.nbr_j, a.hval, a.join, a.minv, a.binfo, a.chunks, a.blocks, a.eta, cg, cnt, a.abort_flag, a.n);
   XR_LAUNCH_CHECK();
   int k = 0;
   // Every system settles within max_iter iterations plus three launches per
@@ -928,11 +946,11 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
     for (int c = 0; c < iters && k < k_limit; ++c) {
       if (mixed) {
         XR_LAUNCH(XT_CG_ITERATION, k_cg_iteration<true>, dim3(nd), dim3(CG_BLOCK), 0, s, a.S, k, a.v, a.row_ptr,
-                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.type, a.ff, a.rhs, part, part_bb,
+                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff, a.rhs, part, part_bb,
                   nd, tol2, a.max_iter, cg, cnt, a.n);
       } else {
         XR_LAUNCH(XT_CG_ITERATION, k_cg_iteration<false>, dim3(nd), dim3(CG_BLOCK), 0, s, a.S, k, a.v, a.row_ptr,
-                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.type, a.ff, a.rhs, part, part_bb,
+                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff, a.rhs, part, part_bb,
                   nd, tol2, a.max_iter, cg, cnt, a.n);
       }
       ++k;
