@@ -778,8 +778,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // waits for them.  Every return below joins the side stream first.
   if (!ctx->side) {
     HIPCHK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "side stream");
-    HIPCHK(hipEventCreateWithFlags(&ctx->bonds_ready, hipEventDisableTiming), "event");
-    HIPCHK(hipEventCreateWithFlags(&ctx->side_done, hipEventDisableTiming), "event");
+    // ordering between two streams of one device only: no system-scope fence
+    HIPCHK(hipEventCreateWithFlags(&ctx->bonds_ready, hipEventDisableTiming | hipEventDisableSystemFence), "event");
+    HIPCHK(hipEventCreateWithFlags(&ctx->side_done, hipEventDisableTiming | hipEventDisableSystemFence), "event");
     for (hipEvent_t& ev : ctx->poll_ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&ctx->poll_host), 16 * sizeof(int)), "pinned poll slots");
   }

@@ -15,6 +15,11 @@ enum XrTimer {
   XT_BACKWARD, XT_COUNT
 };
 
+// Timing events need no system-scope fence: their timestamps are read after a
+// stream synchronisation, and a system-scope release at the end of every timed
+// dispatch would write back L2 and slow the very kernel being timed.
+constexpr unsigned kTimerEventFlags = hipEventDisableSystemFence;
+
 struct XrProfiler {
   bool on = false;
   unsigned mask = 0;  // bit k: timer k is recorded
@@ -35,7 +40,7 @@ struct XrProfiler {
     if (used + 2 > pool.size()) {
       for (int k = 0; k < 64; ++k) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) {
+        if (hipEventCreateWithFlags(&e, kTimerEventFlags) != hipSuccess) {
           on = false;
           return false;
         }
@@ -53,7 +58,7 @@ struct XrProfiler {
     if (used + 2 > pool.size()) {
       for (int k = 0; k < 64; ++k) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) {
+        if (hipEventCreateWithFlags(&e, kTimerEventFlags) != hipSuccess) {
           on = false;
           return false;
         }
