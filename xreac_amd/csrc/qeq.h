@@ -76,7 +76,7 @@ struct CgArgs {
   const unsigned* abort_flag;  // device word; nonzero stops the solve (buffer overflow)
   int* poll_host;              // pinned host memory, two slots of two ints (settled, abort)
   hipEvent_t poll_ev[2];       // completion of each slot's copy
-  // Optional host call made once the first chunk of iterations is queued and
+  // Optional host call made once the solve's first few launches are queued and
   // before the first wait: the caller issues independent work to other streams
   // there, so the solve starts without waiting for those launches.
   hipError_t (*after_start)(void*);

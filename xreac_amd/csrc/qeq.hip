@@ -31,6 +31,7 @@
 // residual meets the tolerance.  A warm start begins with the same refresh.
 // All of it runs on the device: the host only polls how many systems have
 // settled.
+#include <algorithm>
 #include <cstdlib>
 #include <type_traits>
 
@@ -139,6 +140,7 @@ constexpr int INNER_MAX = 64;          // mixed precision: iterations between FP
 constexpr double INNER_DROP2 = 1e-8;   // mixed precision: squared residual drop before a restart
 constexpr int MAX_RESTARTS = 12;
 constexpr int NDOT = 6;                // per block: r.u, w.u, r.r for both right-hand sides
+constexpr int CG_LEAD_LAUNCHES = 4;    // launches queued before the caller's side-stream work
 static_assert(CHUNK + RUN_MAX - 1 <= CHUNK_MAX, "a chunk must fit one wave");
 
 // System owning global block b (first-block offsets ascend with the system index).
@@ -975,8 +977,12 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
     if (e == hipSuccess) pending[slot] = true;
     return e;
   };
-  hipError_t e = issue(0, a.first_check);
+  // A few launches go out before the caller's side-stream work, so the device
+  // is busy while the host issues it, and the rest of the first chunk after.
+  const int lead = a.after_start ? std::min(a.first_check, CG_LEAD_LAUNCHES) : 0;
+  hipError_t e = enqueue(lead);
   if (e == hipSuccess && a.after_start) e = a.after_start(a.after_start_arg);
+  if (e == hipSuccess) e = issue(0, a.first_check - lead);
   if (e != hipSuccess) return e;
   int cur = 0;
   bool pipelined = false;
