@@ -861,51 +861,90 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     return *h->err = (*h->fn)();
   };
   cg.after_start_arg = &hook;
+  // After the charges: Coulomb, forces, energies and the copies back.  With no
+  // charge response to solve in between, this is issued behind the CG's first
+  // chunk of launches when that chunk is expected to finish the solve (one
+  // system, chunk sized from the previous solve), so the device goes on without
+  // waiting for the host to see the convergence.  Every step overwrites its
+  // outputs, so a speculation that the solve outran is simply issued again.
+  const char* tail_what = "post-QEq";
+  auto tail_front = [&]() -> hipError_t {
+    hipError_t e = xr_nonbonded(at, ff, row_ptr, nbr_j, nbr_s, acc, dEdq, s);
+    tail_what = "nonbonded";
+    if (e == hipSuccess) e = join_side();
+    if (e == hipSuccess) e = xr_force_assembly(at, bd, out.forces, s);
+    if (e != hipSuccess) tail_what = "forces";
+    return e;
+  };
+  auto tail_back = [&]() -> hipError_t {
+    tail_what = "energy";
+    hipError_t e = xr_energy_reduce(S, acc, reinterpret_cast<double*>(dc + L.comps), s);
+    if (e == hipSuccess && out.tbo) e = hipMemcpyAsync(out.tbo, at.total, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && out.nlp) e = hipMemcpyAsync(out.nlp, at.nlp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && out.bcounts)
+      e = hipMemcpyAsync(out.bcounts, at.bond_count, sizeof(int) * n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(hc, dc, L.sys, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess && tail_what[0] == 'e') tail_what = "copy";
+    return e;
+  };
+  auto tail_all = [&]() -> hipError_t {
+    hipError_t e = tail_front();
+    return e == hipSuccess ? tail_back() : e;
+  };
+  using TailAll = decltype(tail_all);
+  struct TailHook {
+    TailAll* fn;
+    hipError_t err;
+  } tail{&tail_all, hipSuccess};
+  if (!full && ns == 1 && ctx->last_cg_launches > 0) {
+    cg.tail = [](void* p) {
+      TailHook* t = static_cast<TailHook*>(p);
+      return t->err = (*t->fn)();
+    };
+    cg.tail_arg = &tail;
+  }
+  bool tail_done = false;
   if (!given_q) {
     CgOutcome cgo{};
     const hipError_t e = xr_cg_solve(cg, s, &cgo);
     if (side_issued) HIPCHK(join_side(), "event");
     if (side_err != hipSuccess) return hip_fail(ctx, side_err, side_what);
+    if (tail.err != hipSuccess) return hip_fail(ctx, tail.err, tail_what);
     HIPCHK(e, "QEq");
     if (cgo.aborted) {
       *overflow = true;
       return XREAC_OK;
     }
+    tail_done = cgo.tail_done;
   } else {
     HIPCHK(launch_side(), side_what);
   }
 
   // ----------------------------------------------------- energy and forces
-  HIPCHK(xr_nonbonded(at, ff, row_ptr, nbr_j, nbr_s, acc, dEdq, s), "nonbonded");
-  HIPCHK(join_side(), "event");
-  HIPCHK(xr_force_assembly(at, bd, out.forces, s), "forces");
-  if (full) {
-    // Charge response: neutral adjoint KKT solve with right-hand sides (dE/dq, -1).
-    CgArgs adj = cg;
-    adj.rhs = dEdq;
-    adj.neutral = true;
-    adj.guess = nullptr;
-    adj.keep = nullptr;
-    adj.q = lam;
-    adj.mu = nullptr;
-    adj.state = dc + L.cg2;
-    adj.after_start = nullptr;
-    CgOutcome o2{};
-    HIPCHK(xr_cg_solve(adj, s, &o2), "QEq adjoint");
-    if (o2.aborted) {
-      *overflow = true;
-      return XREAC_OK;
+  if (!tail_done) {
+    if (hipError_t e = tail_front(); e != hipSuccess) return hip_fail(ctx, e, tail_what);
+    if (full) {
+      // Charge response: neutral adjoint KKT solve with right-hand sides (dE/dq, -1).
+      CgArgs adj = cg;
+      adj.rhs = dEdq;
+      adj.neutral = true;
+      adj.guess = nullptr;
+      adj.keep = nullptr;
+      adj.q = lam;
+      adj.mu = nullptr;
+      adj.state = dc + L.cg2;
+      adj.after_start = nullptr;
+      adj.tail = nullptr;
+      CgOutcome o2{};
+      HIPCHK(xr_cg_solve(adj, s, &o2), "QEq adjoint");
+      if (o2.aborted) {
+        *overflow = true;
+        return XREAC_OK;
+      }
+      HIPCHK(xr_charge_response(at, ff, row_ptr, nbr_j, nbr_s, lam, out.forces, s), "charge response");
     }
-    HIPCHK(xr_charge_response(at, ff, row_ptr, nbr_j, nbr_s, lam, out.forces, s), "charge response");
+    if (hipError_t e = tail_back(); e != hipSuccess) return hip_fail(ctx, e, tail_what);
   }
-  HIPCHK(xr_energy_reduce(S, acc, reinterpret_cast<double*>(dc + L.comps), s), "energy");
-  if (out.tbo)
-    HIPCHK(hipMemcpyAsync(out.tbo, at.total, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
-  if (out.nlp)
-    HIPCHK(hipMemcpyAsync(out.nlp, at.nlp, sizeof(double) * n, hipMemcpyDeviceToDevice, s), "copy");
-  if (out.bcounts)
-    HIPCHK(hipMemcpyAsync(out.bcounts, at.bond_count, sizeof(int) * n, hipMemcpyDeviceToDevice, s), "copy");
-  HIPCHK(hipMemcpyAsync(hc, dc, L.sys, hipMemcpyDeviceToHost, s), "copy");
   HIPCHK(hipStreamSynchronize(s), "final sync");
   ctx->prof.resolve();
   const unsigned* h_flags = reinterpret_cast<const unsigned*>(hc + L.flags);

@@ -81,12 +81,19 @@ struct CgArgs {
   // there, so the solve starts without waiting for those launches.
   hipError_t (*after_start)(void*);
   void* after_start_arg;
+  // Optional work that follows the solve on the same stream.  When given, it is
+  // issued (after the charges) behind the first chunk of launches, before the
+  // first wait; CgOutcome::tail_done says whether that issue stands or more
+  // launches followed it, in which case the caller issues the work again.
+  hipError_t (*tail)(void*);
+  void* tail_arg;
 };
 
 struct CgOutcome {
   bool aborted;    // stopped on abort_flag
   bool exhausted;  // launch bound reached before every system settled
   int sweeps;      // iterations launched
+  bool tail_done;  // the tail issued behind the first chunk is final
 };
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,

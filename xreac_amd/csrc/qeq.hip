@@ -984,6 +984,16 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   if (e == hipSuccess && a.after_start) e = a.after_start(a.after_start_arg);
   if (e == hipSuccess) e = issue(0, a.first_check - lead);
   if (e != hipSuccess) return e;
+  auto charges = [&]() {
+    hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.v.X, a.q, a.mu, a.keep, a.neutral ? 1 : 0);
+    return hipGetLastError();
+  };
+  if (a.tail) {
+    e = charges();
+    if (e == hipSuccess) e = a.tail(a.tail_arg);
+    if (e != hipSuccess) return e;
+  }
+  const int k_tail = k;
   int cur = 0;
   bool pipelined = false;
   for (;;) {
@@ -1011,8 +1021,8 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
     }
   }
   out->sweeps = k;
-  hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.v.X, a.q, a.mu, a.keep, a.neutral ? 1 : 0);
-  XR_LAUNCH_CHECK();
+  out->tail_done = a.tail && k == k_tail && !out->aborted && !out->exhausted;
+  if (!out->tail_done) return charges();
   return hipSuccess;
 }
 
