@@ -184,23 +184,6 @@ __global__ void k_zero_ranges(ZeroRanges z) {
 
 // Record total = ptr[n] and clamp every pointer to `cap`, so later kernels
 // stay inside buffers sized for `cap` entries even when the total exceeds it.
-__global__ void k_clamp_ptr(int* __restrict__ ptr, int n, int cap, int* __restrict__ total,
-                            unsigned* __restrict__ err) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  const int v = ptr[i];
-  if (i == n) {
-    *total = v;
-    if (v > cap) atomicOr(err, (unsigned)XR_ERR_CAPACITY);
-  }
-  if (v > cap) ptr[i] = cap;
-}
-
-hipError_t clamp_ptr(int* ptr, int n, int cap, int* total, unsigned* err, hipStream_t s) {
-  hipLaunchKernelGGL(k_clamp_ptr, dim3((n + 1 + 255) / 256), dim3(256), 0, s, ptr, n, cap, total, err);
-  return hipGetLastError();
-}
-
 template <class T>
 hipError_t upload(xreac_ff* ff, const T* src, size_t count, const T** dst) {
   if (count == 0) count = 1;
@@ -646,8 +629,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // A supplied list still gets the count pass: it checks every pair for
   // coincident atoms, as validate_input does whatever list is supplied.
   const bool supplied = in.nbr_row_ptr != nullptr;
-  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, supplied ? -1 : 2048, counts,
-                           dev_counts + 2, s),
+  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, supplied ? -1 : 2048, counts, s),
          "neighbor count");
   int max_row = ctx->max_row;
   int64_t m_cap = 0;
@@ -655,14 +637,16 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     m_cap = in.nbr_count;
     HIPCHK(hipMemcpyAsync(row_ptr, in.nbr_row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToDevice, s), "copy");
     // clamped to the supplied length, so a malformed pointer array cannot reach past the list
-    HIPCHK(clamp_ptr(row_ptr, n, (int)m_cap, dev_counts, d_flags, s), "clamp");
+    HIPCHK(xr_clamp_ptr(row_ptr, n, (int)m_cap, dev_counts, d_flags, s), "clamp");
   } else {
-    HIPCHK(xr_exclusive_scan(counts, n, row_ptr, scan_tmp, s), "neighbor scan");
     const double slack = 1.10;
     m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
     exact = exact || m_cap == 0;
     if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
-    HIPCHK(clamp_ptr(row_ptr, n, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags, s), "clamp");
+    // row pointers clamped to the edge buffer, total edges and longest row in one pass
+    HIPCHK(xr_scan_clamped(counts, n, row_ptr, scan_tmp, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags,
+                           dev_counts + 2, s),
+           "neighbor scan");
     if (exact) {
       int h[3] = {0, 0, 0};
       HIPCHK(hipMemcpyAsync(h, dev_counts, sizeof(h), hipMemcpyDeviceToHost, s), "copy");
@@ -700,13 +684,14 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, hval32, bflag, bond_count,
                       join, s),
          "edge pass");
-  HIPCHK(xr_exclusive_scan(bond_count, n, bond_ptr, scan_tmp, s), "bond scan");
   int64_t b_cap = 0;
   if (!exact) {
     b_cap = (int64_t)std::ceil(ctx->b_per_atom * n * 1.25) + 1024;
     if (b_cap > m_cap) b_cap = m_cap;
   }
-  HIPCHK(clamp_ptr(bond_ptr, n, exact ? INT32_MAX : (int)b_cap, dev_counts + 1, d_flags, s), "clamp");
+  HIPCHK(xr_scan_clamped(bond_count, n, bond_ptr, scan_tmp, exact ? INT32_MAX : (int)b_cap, dev_counts + 1, d_flags,
+                         nullptr, s),
+         "bond scan");
   if (exact) {
     int nb_host = 0;
     HIPCHK(hipMemcpyAsync(&nb_host, dev_counts + 1, sizeof(int), hipMemcpyDeviceToHost, s), "copy");

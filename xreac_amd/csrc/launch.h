@@ -6,6 +6,13 @@
 // scan.hip
 size_t xr_scan_scratch_ints(int n);
 hipError_t xr_exclusive_scan(const int* in, int n, int* out, int* scratch, hipStream_t s);
+// The scan with every pointer clamped to cap; *total receives the unclamped
+// total and err the XR_ERR_CAPACITY bit when it exceeds cap; *max_out (when
+// not null) the largest count.
+hipError_t xr_scan_clamped(const int* in, int n, int* out, int* scratch, int cap, int* total, unsigned* err,
+                           int* max_out, hipStream_t s);
+// Clamp of existing pointers ptr[0..n] to cap, with the same total and error word.
+hipError_t xr_clamp_ptr(int* ptr, int n, int cap, int* total, unsigned* err, hipStream_t s);
 
 // neighbor.hip
 hipError_t xr_coord_bounds(const Systems& S, const double* x, double* bounds, hipStream_t s);
@@ -16,7 +23,7 @@ hipError_t xr_bin_scatter(int n, const double* x, const Systems& S, const int* h
                           const int* bin_start, int* bin_fill, void* recs, hipStream_t s);
 hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int* home, const int* bin,
                              const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
-                             int* max_row, hipStream_t s);
+                             hipStream_t s);
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
                             int* nbr_s, int max_row, unsigned* flags, hipStream_t s);

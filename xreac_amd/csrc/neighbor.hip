@@ -259,22 +259,6 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
 }
 
 // Longest row of the list (one block; the fill instance is chosen from it).
-__global__ void __launch_bounds__(1024) k_row_max(int n, const int* __restrict__ counts, int* __restrict__ out) {
-  __shared__ int red[16];
-  int m = 0;
-  for (int t = threadIdx.x; t < n; t += blockDim.x) m = counts[t] > m ? counts[t] : m;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const int o = __shfl_down(m, off, 64);
-    m = o > m ? o : m;
-  }
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = red[w] > m ? red[w] : m;
-    *out = m;
-  }
-}
 
 // Sort the first `count` keys of s (lane-major, K keys per lane) ascending
 // with a bitonic network held in registers: partners closer than K sit in the
@@ -452,13 +436,11 @@ hipError_t xr_bin_scatter(int n, const double* x, const Systems& S, const int* h
 
 hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int* home, const int* bin,
                              const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
-                             int* max_row, hipStream_t s) {
+                             hipStream_t s) {
   // row_cap < 0: count only (coincidence and image checks), no row-length limit
   const int cap = row_cap < 0 ? 0x7fffffff : (row_cap < ROW_CAP ? row_cap : ROW_CAP);
   XR_LAUNCH(XT_NBR_COUNT, k_nbr_count, dim3((n + 3) / 4), dim3(256), 0, s, n, x, S, home, bin, bin_start,
             static_cast<const BinRec*>(binned), cutoff * cutoff, cap, counts);
-  XR_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_row_max, dim3(1), dim3(1024), 0, s, n, counts, max_row);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
