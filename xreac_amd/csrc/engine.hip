@@ -88,7 +88,7 @@ struct xreac_ctx {
   hipStream_t stream = nullptr;
   std::string message;
   std::vector<std::string> sys_messages;  // per system of the last batch
-  Buffer atom_buf, bin_buf, edge_buf, bond_buf, ctl_buf;
+  Buffer atom_buf, bin_buf, edge_buf, bond_buf, ctl_buf, slab_buf;
   char* ctl_host = nullptr;  // pinned mirror of the control block
   size_t ctl_host_bytes = 0;
   XrProfiler prof;
@@ -626,14 +626,17 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   HIPCHK(xr_bin_scatter(n, in.x, S, home, bin, bin_start, bin_fill, binned, s), "bin scatter");
 
   // --------------------------------------------------------- neighbor rows
-  // A supplied list still gets the count pass: it checks every pair for
+  // The fill writes each row, sorted, to a slab of fixed stride and its length
+  // to counts; one scan gives the row pointers and the edge pass moves the rows
+  // into place.  Buffers are sized from the last evaluation; an exact
+  // evaluation (the first, or after an overflow) counts the rows first.  A
+  // supplied list gets the count pass too: it checks every pair for
   // coincident atoms, as validate_input does whatever list is supplied.
   const bool supplied = in.nbr_row_ptr != nullptr;
-  HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, supplied ? -1 : 2048, counts, s),
-         "neighbor count");
   int max_row = ctx->max_row;
   int64_t m_cap = 0;
   if (supplied) {
+    HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, -1, counts, s), "neighbor count");
     m_cap = in.nbr_count;
     HIPCHK(hipMemcpyAsync(row_ptr, in.nbr_row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToDevice, s), "copy");
     // clamped to the supplied length, so a malformed pointer array cannot reach past the list
@@ -643,11 +646,10 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
     exact = exact || m_cap == 0;
     if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
-    // row pointers clamped to the edge buffer, total edges and longest row in one pass
-    HIPCHK(xr_scan_clamped(counts, n, row_ptr, scan_tmp, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags,
-                           dev_counts + 2, s),
-           "neighbor scan");
     if (exact) {
+      HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, 2048, counts, s), "neighbor count");
+      HIPCHK(xr_scan_clamped(counts, n, row_ptr, scan_tmp, INT32_MAX, dev_counts, d_flags, dev_counts + 2, s),
+             "neighbor scan");
       int h[3] = {0, 0, 0};
       HIPCHK(hipMemcpyAsync(h, dev_counts, sizeof(h), hipMemcpyDeviceToHost, s), "copy");
       HIPCHK(hipStreamSynchronize(s), "neighbor sync");
@@ -669,20 +671,30 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   double* hval = ce.take<double>(m_cap);
   float* hval32 = mixed ? ce.take<float>(m_cap) : nullptr;
   unsigned char* bflag = ce.take<unsigned char>(m_cap);
+  void* slab = nullptr;
+  int slab_stride = 0;
   if (supplied) {
     HIPCHK(xr_pack_supplied(n, S, row_ptr, in.nbr_j, in.nbr_shifts, nbr_j, nbr_s, s), "supplied neighbors");
   } else {
-    // The fill instance is sized from the longest row: known in the exact
-    // pass, else the last evaluation's plus a margin.  A longer row makes the
-    // instance raise the capacity bit and the evaluation is repeated exactly.
-    HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, row_ptr, nbr_j, nbr_s,
-                            exact ? max_row : (max_row > 0 ? max_row + 64 : 2048), d_flags, s),
+    // The fill instance (and slab stride) is sized from the longest row: known
+    // in an exact evaluation, else the last evaluation's plus a margin.  A
+    // longer row makes the instance raise the capacity bit and the evaluation
+    // is repeated exactly.
+    const int hint = exact ? max_row : (max_row > 0 ? max_row + 64 : 2048);
+    slab_stride = xr_fill_stride(hint);
+    HIPCHK(ctx->slab_buf.ensure(sizeof(unsigned long long) * (size_t)slab_stride * n + 4096), "neighbor slab");
+    slab = ctx->slab_buf.ptr;
+    HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, counts, slab, hint, d_flags, s),
            "neighbor fill");
+    // row pointers clamped to the edge buffer, total edges and longest row in one pass
+    HIPCHK(xr_scan_clamped(counts, n, row_ptr, scan_tmp, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags,
+                           dev_counts + 2, s),
+           "neighbor scan");
   }
 
   // ------------------------------------------------------- edges and bonds
-  HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, nbr_j, nbr_s, bond_cut, hval, hval32, bflag, bond_count,
-                      join, s),
+  HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, slab, slab_stride, nbr_j, nbr_s, bond_cut, hval, hval32,
+                      bflag, bond_count, join, s),
          "edge pass");
   int64_t b_cap = 0;
   if (!exact) {

@@ -24,8 +24,11 @@ hipError_t xr_bin_scatter(int n, const double* x, const Systems& S, const int* h
 hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int* home, const int* bin,
                              const int* bin_start, const void* binned, double cutoff, int row_cap, int* counts,
                              hipStream_t s);
+// Slab row stride (keys per row) of the fill instance for rows up to max_row.
+int xr_fill_stride(int max_row);
+// Sorted rows to slab[i * stride, i * stride + counts[i]) as (j << 32 | image code).
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
-                            const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
-                            int* nbr_s, int max_row, unsigned* flags, hipStream_t s);
+                            const int* bin_start, const void* binned, double cutoff, int* counts, void* slab,
+                            int max_row, unsigned* flags, hipStream_t s);
 hipError_t xr_pack_supplied(int n, const Systems& S, const int* row_ptr, const int* in_j, const int* in_s,
                             int* nbr_j, int* nbr_s, hipStream_t s);

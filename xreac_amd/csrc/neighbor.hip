@@ -306,45 +306,58 @@ __device__ inline void wave_sort(unsigned long long* s, int count) {
   for (int k = 0; k < K; ++k) s[lane * K + k] = v[k];
 }
 
-// Rows are staged in LDS of CAP keys: the 512-key (4 KB) and 1024-key (8 KB)
-// instances sort in registers and leave room for many waves per CU; a launch
-// that meets a row longer than its CAP raises the capacity bit, and the
-// evaluation is repeated with the instance the longest row needs.
+// One wave per row: the row's neighbors, sorted into canonical (j, image)
+// order, go to the row's slab slots [i * CAP, i * CAP + count) as 64-bit keys
+// (j << 32 | image code) and the count to counts[i]; a scan of the counts then
+// gives the CSR row pointers, and the edge pass moves the keys into place.
+// The same walk flags coincident atoms and over-long rows.  Rows are staged in
+// LDS of CAP keys: the 512-key (4 KB) and 1024-key (8 KB) instances sort in
+// registers and leave room for many waves per CU; a launch that meets a row
+// longer than its CAP drops the row (count 0) and raises the capacity bit, and
+// the evaluation is repeated with the instance the longest row needs.
 template <int CAP>
 __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, Systems S,
                                                  const int* __restrict__ home, const int* __restrict__ bin,
                                                  const int* __restrict__ bin_start, const BinRec* __restrict__ binned,
-                                                 double rc2, const int* __restrict__ row_ptr, int* __restrict__ nbr_j,
-                                                 int* __restrict__ nbr_s, unsigned* __restrict__ flags) {
+                                                 double rc2, int* __restrict__ counts,
+                                                 unsigned long long* __restrict__ slab, unsigned* __restrict__ flags) {
   __shared__ unsigned long long keys[CAP];
   const int i = blockIdx.x;
   if (i >= n) return;
   const int lane = threadIdx.x & 63;
-  const int begin = row_ptr[i], count = row_ptr[i + 1] - begin;
-  if (count > CAP) {
-    // Over ROW_CAP: reported by the count pass and the system's results are
-    // discarded; over CAP only: the evaluation is repeated with the long-row
-    // instance.  Either way the row gets in-range entries (the atom itself,
-    // zero shift) so later kernels index valid memory.
-    for (int t2 = lane; t2 < count; t2 += 64) {
-      nbr_j[begin + t2] = i;
-      nbr_s[begin + t2] = xr_pack_shift(0, 0, 0);
-    }
-    if (count <= ROW_CAP && lane == 0) atomicOr(flags, (unsigned)XR_ERR_CAPACITY);
-    return;
-  }
+  const int sys = __builtin_amdgcn_readfirstlane(S.asys[i]);
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   unsigned err = 0;
   int cursor = 0;  // identical in every lane
-  for_each_neighbor(i, x, S.sys[S.asys[i]], home, bin, bin_start, binned, rc2, err,
-                    [&](bool hit, int j, int code, double) {
+  for_each_neighbor(i, x, S.sys[sys], home, bin, bin_start, binned, rc2, err,
+                    [&](bool hit, int j, int code, double r2) {
                       const unsigned long long m = __ballot(hit);
                       if (hit) {
                         const int p = cursor + __popcll(m & below);
                         if (p < CAP) keys[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
+                        if (j != i && r2 < 1e-12) err |= XR_ERR_COINCIDENT;
                       }
                       cursor += __popcll(m);
                     });
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) err |= __shfl_down(err, off, 64);
+  const int count = cursor;
+  if (count > CAP) {
+    // Over ROW_CAP: the system's results are discarded; over CAP only: the
+    // evaluation is repeated with the long-row instance.  The row is left
+    // empty, so later kernels stay in bounds.
+    if (lane == 0) {
+      counts[i] = 0;
+      if (count > ROW_CAP) err |= XR_ERR_ROW_TOO_LONG;
+      else atomicOr(flags, (unsigned)XR_ERR_CAPACITY);
+      if (err) atomicOr(&S.err[sys], err);
+    }
+    return;
+  }
+  if (lane == 0) {
+    counts[i] = count;
+    if (err) atomicOr(&S.err[sys], err);
+  }
   __syncthreads();
   if (count <= 128) {
     wave_sort<2>(keys, count);
@@ -377,11 +390,8 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
     }
   }
   __syncthreads();
-  for (int t2 = lane; t2 < count; t2 += 64) {
-    const unsigned long long key = keys[t2];
-    nbr_j[begin + t2] = (int)(key >> 32);
-    nbr_s[begin + t2] = (int)(key & 0xffffffffu);
-  }
+  unsigned long long* out = slab + (size_t)i * CAP;
+  for (int t2 = lane; t2 < count; t2 += 64) out[t2] = keys[t2];
 }
 
 // Supplied list: copy j and pack the shifts into the device codes, one wave
@@ -445,18 +455,22 @@ hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int
   return hipSuccess;
 }
 
+int xr_fill_stride(int max_row) { return max_row <= 512 ? 512 : (max_row <= 1024 ? 1024 : ROW_CAP); }
+
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
-                            const int* bin_start, const void* binned, double cutoff, const int* row_ptr, int* nbr_j,
-                            int* nbr_s, int max_row, unsigned* flags, hipStream_t s) {
-  if (max_row <= 512) {
+                            const int* bin_start, const void* binned, double cutoff, int* counts, void* slab,
+                            int max_row, unsigned* flags, hipStream_t s) {
+  unsigned long long* keys = static_cast<unsigned long long*>(slab);
+  const int stride = xr_fill_stride(max_row);
+  if (stride == 512) {
     XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<512>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
-              static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s, flags);
-  } else if (max_row <= 1024) {
+              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags);
+  } else if (stride == 1024) {
     XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<1024>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
-              static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s, flags);
+              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags);
   } else {
     XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<ROW_CAP>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
-              static_cast<const BinRec*>(binned), cutoff * cutoff, row_ptr, nbr_j, nbr_s, flags);
+              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags);
   }
   XR_LAUNCH_CHECK();
   return hipSuccess;

@@ -96,9 +96,11 @@ struct CgOutcome {
   bool tail_done;  // the tail issued behind the first chunk is final
 };
 
+// slab: the fill's sorted rows (row i at slab[i * slab_stride]), moved to nbr_j / nbr_s on the way; null
+// when nbr_j / nbr_s already hold the rows (a supplied list).
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
-                        const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s);
+                        const void* slab, int slab_stride, int* nbr_j, int* nbr_s, double bond_cut, double* hval,
+                        float* hval32, unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s);
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int cap,
                         int* bond_edge, int* bond_i, hipStream_t s);
 void xr_cg_blocks(int n, int* ndir, int* nupd);

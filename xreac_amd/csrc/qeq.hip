@@ -47,9 +47,13 @@ constexpr int RUN_MAX = 16;         // atoms per bonded run (molecule) of the CG
 constexpr int BLOCK_MAX = 16;       // atoms per CG preconditioner block (whole runs packed together)
 
 // ---------------------------------------------------------------- edge pass
+// With a slab (the fill's sorted rows), the pass first moves each row's keys
+// to their CSR places, nbr_j[e] and nbr_s[e]; without one (a supplied list)
+// it reads them there.
 __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restrict__ x, const int* __restrict__ type,
                                                    Systems S, FFDev ff, const int* __restrict__ row_ptr,
-                                                   const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
+                                                   const unsigned long long* __restrict__ slab, int slab_stride,
+                                                   int* __restrict__ nbr_j, int* __restrict__ nbr_s,
                                                    double bond_cut, double* __restrict__ hval,
                                                    float* __restrict__ hval32, unsigned char* __restrict__ bflag,
                                                    int* __restrict__ bond_count, unsigned char* __restrict__ join) {
@@ -65,11 +69,23 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   int cnt = 0;
   unsigned jm = 0;  // bit d-1: strong same-image bond to atom i - d (d = 1..3), for the CG blocks
   const int zero = xr_pack_shift(0, 0, 0);
-  for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
-    const int j = nbr_j[e];
+  const int e_begin = row_ptr[i], e_end = row_ptr[i + 1];
+  const unsigned long long* srow = slab ? slab + (size_t)i * slab_stride - e_begin : nullptr;
+  for (int e = e_begin + lane; e < e_end; e += 64) {
+    int j, code;
+    if (srow) {
+      const unsigned long long key = srow[e];
+      j = (int)(key >> 32);
+      code = (int)(key & 0xffffffffu);
+      nbr_j[e] = j;
+      nbr_s[e] = code;
+    } else {
+      j = nbr_j[e];
+      code = nbr_s[e];
+    }
     const int tj = type[j];
     double v[3];
-    const double r = xr_edge(x, cell, i, j, nbr_s[e], v);
+    const double r = xr_edge(x, cell, i, j, code, v);
     double s, ds;
     xr_shield(r, rc, ff.pair[(ti * ff.nt + tj) * PP_COUNT + PP_GAMMA], s, ds);
     hval[e] = 14.4 * s;
@@ -80,7 +96,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
       xr_bond_raw(ff, ti, tj, r, b);
       const double raw = (b.sig + b.pi) + b.pp;
       active = raw >= cut;
-      if (raw >= BLOCK_BOND && j < i && j >= i - 3 && nbr_s[e] == zero) jm |= 1u << (i - j - 1);
+      if (raw >= BLOCK_BOND && j < i && j >= i - 3 && code == zero) jm |= 1u << (i - j - 1);
     }
     unsigned char flag = active ? EF_BOND : 0;
     if (r <= hcut) {
@@ -882,11 +898,12 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
 }  // namespace
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
-                        const int* nbr_j, const int* nbr_s, double bond_cut, double* hval, float* hval32,
-                        unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s) {
+                        const void* slab, int slab_stride, int* nbr_j, int* nbr_s, double bond_cut, double* hval,
+                        float* hval32, unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
-  XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr, nbr_j, nbr_s,
-            bond_cut, hval, hval32, bflag, bond_count, join);
+  XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr,
+            static_cast<const unsigned long long*>(slab), slab_stride, nbr_j, nbr_s, bond_cut, hval, hval32, bflag,
+            bond_count, join);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
