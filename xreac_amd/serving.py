@@ -76,6 +76,7 @@ class _Job:
     evals: int = 0
     potential: list = field(default_factory=list)
     kinetic: list = field(default_factory=list)
+    prep: tuple | None = None  # host arrays for _Group.append, built ahead of admission
 
 
 class _Group:
@@ -106,25 +107,38 @@ class _Group:
         self.FS = torch.zeros((0, 4), **f64)
         self.ST = torch.zeros(0, dtype=torch.int32, device=self.dev)
 
+    def prepare(self, job, ff):
+        """Host arrays of one job (positions, type ids, velocities, FIRE state); pure host work."""
+        req = job.req
+        x = np.asarray(req["positions"], dtype=np.float64).reshape(job.n, 3)
+        t = self.eng.type_ids(req["symbols"])
+        if req.get("md_steps", 0) > 0:
+            v0 = req.get("velocities")
+            if v0 is None:
+                masses = [ff.atoms[s]["mass"] for s in req["symbols"]]
+                v0 = maxwell_boltzmann(masses, req.get("temperature", 300.0), req.get("velocity_seed", 0))
+            v = np.asarray(v0, dtype=np.float64).reshape(job.n, 3)
+        else:
+            v = np.zeros((job.n, 3))
+        relax = bool(req.get("relax"))
+        fs = [0.02, 0.1, 0.0, 0.0] if relax else [0.0, 0.0, 0.0, 0.0]
+        st = 0 if relax else 1  # status 1: not relaxing, FIRE leaves the system alone
+        job.prep = (x, t, v, fs, st)
+
     def append(self, jobs, ff):
         """Pack newly admitted jobs behind the active ones (one host-to-device copy per array)."""
         torch = self.torch
         x, t, v, fs, st = [], [], [], [], []
         for job in jobs:
-            req = job.req
-            x.append(np.asarray(req["positions"], dtype=np.float64).reshape(job.n, 3))
-            t.append(self.eng.type_ids(req["symbols"]))
-            if req.get("md_steps", 0) > 0:
-                v0 = req.get("velocities")
-                if v0 is None:
-                    masses = [ff.atoms[s]["mass"] for s in req["symbols"]]
-                    v0 = maxwell_boltzmann(masses, req.get("temperature", 300.0), req.get("velocity_seed", 0))
-                v.append(np.asarray(v0, dtype=np.float64).reshape(job.n, 3))
-            else:
-                v.append(np.zeros((job.n, 3)))
-            relax = bool(req.get("relax"))
-            fs.append([0.02, 0.1, 0.0, 0.0] if relax else [0.0, 0.0, 0.0, 0.0])
-            st.append(0 if relax else 1)  # status 1: not relaxing, FIRE leaves the system alone
+            if job.prep is None:
+                self.prepare(job, ff)
+            xj, tj, vj, fsj, stj = job.prep
+            job.prep = None
+            x.append(xj)
+            t.append(tj)
+            v.append(vj)
+            fs.append(fsj)
+            st.append(stj)
         m = sum(job.n for job in jobs)
         f64 = dict(dtype=torch.float64, device=self.dev)
         self.X = torch.cat([self.X, torch.as_tensor(np.concatenate(x), **f64)])
@@ -243,15 +257,26 @@ class Server:
         """
         self._admit()
         groups = [g for g in self.active.values() if g.jobs]
-        if len(groups) > 1:
-            results = list(self._pool.map(self._run_group, groups))
-        else:
-            results = [self._run_group(g) for g in groups]
+        futures = [self._pool.submit(self._run_group, g) for g in groups]
+        # while the groups run, build the host arrays of the requests likely
+        # to be admitted next
+        self._prepare_ahead()
+        results = [f.result() for f in futures]
         for evals, responses, freed in results:
             self.atom_evals += evals
             self.responses.extend(responses)
             self.used_bytes -= freed
         self.steps += 1
+
+    def _prepare_ahead(self):
+        atoms = 0
+        for job in self.queue:
+            if atoms >= self.max_batch_atoms:
+                break
+            if job.prep is None:
+                name = job.req["force_field"]
+                self.active[name].prepare(job, self._ff[name])
+            atoms += job.n
 
     def _run_group(self, g):
         """One step of a group on its engine's stream; returns (atom evaluations, finished responses, bytes freed)."""
