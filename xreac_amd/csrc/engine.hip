@@ -594,7 +594,10 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   at.a_ov2 = nxt();
   at.a_totp = nxt();
   if (sp > scal_end) return fail(ctx, XREAC_INVALID, "internal error: per-atom workspace layout");
-  at.bond_count = bcount03;
+  // Per-atom properties the caller asked for are computed in its buffers directly.
+  if (out.tbo) at.total = out.tbo;
+  if (out.nlp) at.nlp = out.nlp;
+  at.bond_count = out.bcounts ? out.bcounts : bcount03;
   at.F_nb = fbuf;
   at.F_hb = fbuf + 3 * (size_t)n;
   at.q = out.charges;
@@ -876,10 +879,6 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   auto tail_back = [&]() -> hipError_t {
     tail_what = "energy";
     hipError_t e = xr_energy_reduce(S, acc, reinterpret_cast<double*>(dc + L.comps), s);
-    if (e == hipSuccess && out.tbo) e = hipMemcpyAsync(out.tbo, at.total, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess && out.nlp) e = hipMemcpyAsync(out.nlp, at.nlp, sizeof(double) * n, hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess && out.bcounts)
-      e = hipMemcpyAsync(out.bcounts, at.bond_count, sizeof(int) * n, hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(hc, dc, L.sys, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess && tail_what[0] == 'e') tail_what = "copy";
     return e;

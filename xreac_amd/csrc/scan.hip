@@ -11,9 +11,9 @@
 
 namespace {
 constexpr int ONE_BLOCK = 1024;
-constexpr int ONE_ITEMS = 8;
+constexpr int ONE_ITEMS = 16;
 constexpr int ONE_TILE = ONE_BLOCK * ONE_ITEMS;
-constexpr int ONE_MAX = 8 * ONE_TILE;  // 65,536 counts
+constexpr int ONE_MAX = 4 * ONE_TILE;  // 65,536 counts
 constexpr int SCAN_BLOCK = 256;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS;
@@ -104,8 +104,9 @@ __device__ inline int block_max(int m) {
   return m;
 }
 
-// One block, n <= ONE_MAX: each thread scans 8 consecutive counts of a tile,
-// the block scans the thread sums, and a carry runs from tile to tile.
+// One block, n <= ONE_MAX: each thread scans 16 consecutive counts of a tile,
+// the block scans the thread sums, and a carry runs from tile to tile; the
+// next tile's counts are loaded before the current tile is scanned.
 // Pointers above cap are written as cap; the unclamped total goes to *total
 // and raises XR_ERR_CAPACITY when it exceeds cap.  The largest count goes to
 // *max_out.  (total, err, max_out may be null.)
@@ -114,13 +115,22 @@ __global__ void __launch_bounds__(ONE_BLOCK) k_scan_one(const int* __restrict__ 
                                                         unsigned* __restrict__ err, int* __restrict__ max_out) {
   __shared__ int tmp[17];
   int carry = 0, mx = 0;
+  int nv[ONE_ITEMS];
+  auto load = [&](int t0) {
+    const int base = t0 + threadIdx.x * ONE_ITEMS;
+#pragma unroll
+    for (int k = 0; k < ONE_ITEMS; ++k) nv[k] = base + k < n ? in[base + k] : 0;
+  };
+  load(0);
   for (int t0 = 0; t0 < n; t0 += ONE_TILE) {
     const int base = t0 + threadIdx.x * ONE_ITEMS;
     int v[ONE_ITEMS];
+#pragma unroll
+    for (int k = 0; k < ONE_ITEMS; ++k) v[k] = nv[k];
+    if (t0 + ONE_TILE < n) load(t0 + ONE_TILE);
     int s = 0;
 #pragma unroll
     for (int k = 0; k < ONE_ITEMS; ++k) {
-      v[k] = base + k < n ? in[base + k] : 0;
       s += v[k];
       mx = v[k] > mx ? v[k] : mx;
     }
