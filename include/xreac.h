@@ -101,7 +101,7 @@ typedef struct {
 
 enum {
   XREAC_PRECISION_FP64 = 0,    /* FP64 CG throughout */
-  XREAC_PRECISION_MIXED = 1    /* FP32 A p sweeps, FP64 vectors and dots, FP64 true-residual restarts */
+  XREAC_PRECISION_MIXED = 1    /* FP32 A m sweeps, FP64 vectors and dots, FP64 true-residual restarts */
 };
 
 typedef struct {

@@ -92,7 +92,7 @@ struct CgArgs {
 struct CgOutcome {
   bool aborted;    // stopped on abort_flag
   bool exhausted;  // launch bound reached before every system settled
-  int sweeps;      // iterations launched
+  int sweeps;      // iteration launches issued
   bool tail_done;  // the tail issued behind the first chunk is final
 };
 
