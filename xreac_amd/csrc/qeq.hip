@@ -45,6 +45,7 @@ constexpr int ROWS_PER_BLOCK = 4;  // one wave per row, 256 threads
 constexpr double BLOCK_BOND = 0.3;  // raw bond order joining consecutive atoms into one CG block
 constexpr int RUN_MAX = 16;         // atoms per bonded run (molecule) of the CG preconditioner
 constexpr int BLOCK_MAX = 16;       // atoms per CG preconditioner block (whole runs packed together)
+constexpr int BOND_QUEUE = 128;     // edge pass: per-wave queue of edges within the bond cutoff
 
 // ---------------------------------------------------------------- edge pass
 // With a slab (the fill's sorted rows), the pass first moves each row's keys
@@ -71,42 +72,90 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   const int zero = xr_pack_shift(0, 0, 0);
   const int e_begin = row_ptr[i], e_end = row_ptr[i + 1];
   const unsigned long long* srow = slab ? slab + (size_t)i * slab_stride - e_begin : nullptr;
-  for (int e = e_begin + lane; e < e_end; e += 64) {
-    int j, code;
-    if (srow) {
-      const unsigned long long key = srow[e];
-      j = (int)(key >> 32);
-      code = (int)(key & 0xffffffffu);
-      nbr_j[e] = j;
-      nbr_s[e] = code;
-    } else {
-      j = nbr_j[e];
-      code = nbr_s[e];
-    }
-    const int tj = type[j];
-    double v[3];
-    const double r = xr_edge(x, cell, i, j, code, v);
-    double s, ds;
-    xr_shield(r, rc, ff.pair[(ti * ff.nt + tj) * PP_COUNT + PP_GAMMA], s, ds);
-    hval[e] = 14.4 * s;
-    if (hval32) hval32[e] = (float)(14.4 * s);
-    unsigned char active = 0;
-    if (r <= bond_cut) {
+  // Edges within the bond cutoff (about one in eight) wait in a wave-private
+  // LDS queue and have their raw bond order evaluated 64 at a time, so the
+  // exp-heavy code runs on full waves instead of masked ones.
+  const int w = threadIdx.x >> 6;
+  __shared__ int q_e[ROWS_PER_BLOCK][BOND_QUEUE], q_j[ROWS_PER_BLOCK][BOND_QUEUE], q_t[ROWS_PER_BLOCK][BOND_QUEUE];
+  __shared__ double q_r[ROWS_PER_BLOCK][BOND_QUEUE];
+  __shared__ unsigned char q_f[ROWS_PER_BLOCK][BOND_QUEUE];  // H-bond flags, 0x80: zero image shift
+  int qn = 0;  // identical in every lane
+  auto flush = [&](int count) {  // the first count (<= 64) queued edges, one per lane
+    if (lane < count) {
+      const int e = q_e[w][lane], j = q_j[w][lane];
+      const double r = q_r[w][lane];
+      const unsigned char f = q_f[w][lane];
       BondRaw b;
-      xr_bond_raw(ff, ti, tj, r, b);
+      xr_bond_raw(ff, ti, q_t[w][lane], r, b);
       const double raw = (b.sig + b.pi) + b.pp;
-      active = raw >= cut;
-      if (raw >= BLOCK_BOND && j < i && j >= i - 3 && code == zero) jm |= 1u << (i - j - 1);
+      const bool active = raw >= cut;
+      if (raw >= BLOCK_BOND && j < i && j >= i - 3 && (f & 0x80)) jm |= 1u << (i - j - 1);
+      bflag[e] = (unsigned char)((f & 0x7f) | (active ? EF_BOND : 0));
+      cnt += active;
     }
-    unsigned char flag = active ? EF_BOND : 0;
-    if (r <= hcut) {
-      const double hb_j = ff.atom[tj * AP_COUNT + AP_HBOND];
-      if (hb_i == 1.0 && hb_j == 2.0) flag |= EF_HB_DONOR;
-      if (hb_i == 2.0 && hb_j == 1.0) flag |= EF_HB_ACCEPTOR;
+  };
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int base = e_begin; base < e_end; base += 64) {
+    const int e = base + lane;
+    const bool valid = e < e_end;
+    bool cand = false;
+    int j = 0, tj = 0, code = 0;
+    double r = 0.0;
+    unsigned char flag = 0;
+    if (valid) {
+      if (srow) {
+        const unsigned long long key = srow[e];
+        j = (int)(key >> 32);
+        code = (int)(key & 0xffffffffu);
+        nbr_j[e] = j;
+        nbr_s[e] = code;
+      } else {
+        j = nbr_j[e];
+        code = nbr_s[e];
+      }
+      tj = type[j];
+      double v[3];
+      r = xr_edge(x, cell, i, j, code, v);
+      double s, ds;
+      xr_shield(r, rc, ff.pair[(ti * ff.nt + tj) * PP_COUNT + PP_GAMMA], s, ds);
+      hval[e] = 14.4 * s;
+      if (hval32) hval32[e] = (float)(14.4 * s);
+      if (r <= hcut) {
+        const double hb_j = ff.atom[tj * AP_COUNT + AP_HBOND];
+        if (hb_i == 1.0 && hb_j == 2.0) flag |= EF_HB_DONOR;
+        if (hb_i == 2.0 && hb_j == 1.0) flag |= EF_HB_ACCEPTOR;
+      }
+      cand = r <= bond_cut;
+      if (!cand) bflag[e] = flag;
     }
-    bflag[e] = flag;
-    cnt += active;
+    const unsigned long long cm = __ballot(cand);
+    if (cand) {
+      const int slot = qn + __popcll(cm & below);
+      q_e[w][slot] = e;
+      q_j[w][slot] = j;
+      q_t[w][slot] = tj;
+      q_r[w][slot] = r;
+      q_f[w][slot] = (unsigned char)(flag | (code == zero ? 0x80 : 0));
+    }
+    qn += __popcll(cm);
+    if (qn >= 64) {
+      xr_wave_sync();
+      flush(64);
+      xr_wave_sync();
+      // queue entries past the batch move to the front (reads from 64.., writes below 64)
+      if (lane < qn - 64) {
+        q_e[w][lane] = q_e[w][64 + lane];
+        q_j[w][lane] = q_j[w][64 + lane];
+        q_t[w][lane] = q_t[w][64 + lane];
+        q_r[w][lane] = q_r[w][64 + lane];
+        q_f[w][lane] = q_f[w][64 + lane];
+      }
+      xr_wave_sync();
+      qn -= 64;
+    }
   }
+  xr_wave_sync();
+  flush(qn);
   cnt = xr_wave_sum_int(cnt);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) jm |= __shfl_down(jm, off, 64);
