@@ -360,15 +360,40 @@ __global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict
     const int g0 = row0 + f;
     int lo = row_ptr[i], hi = row_ptr[i + 1];
     const int e_end = hi;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (nbr_j[mid] < g0) lo = mid + 1;
-      else hi = mid;
+    // First entry with j >= g0 (rows are sorted by j): an 8-way search whose
+    // seven probes per round are loaded together, then the last few entries.
+    while (hi - lo > 8) {
+      const int step = (hi - lo) >> 3;
+      int below = 0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) below += nbr_j[lo + (k + 1) * step] < g0;
+      const int nhi = below < 7 ? lo + (below + 1) * step : hi;
+      lo = below > 0 ? lo + below * step + 1 : lo;
+      hi = nhi;
     }
-    for (int e = lo; e < e_end; ++e) {
-      const int jj = nbr_j[e];
-      if (jj >= g0 + m) break;
-      hb[t][jj - g0] += hval[e];
+    while (lo < hi && nbr_j[lo] < g0) ++lo;
+    // The block's entries follow: a window of 16 loaded together, then one by one.
+    constexpr int WIN = 16;
+    int wj[WIN];
+    double wh[WIN];
+#pragma unroll
+    for (int k = 0; k < WIN; ++k) {
+      const bool ok = lo + k < e_end;
+      wj[k] = ok ? nbr_j[lo + k] : 0x7fffffff;
+      wh[k] = ok ? hval[lo + k] : 0.0;
+    }
+    bool more = true;
+#pragma unroll
+    for (int k = 0; k < WIN; ++k) {
+      if (more && wj[k] < g0 + m) hb[t][wj[k] - g0] += wh[k];
+      else more = false;
+    }
+    if (more) {
+      for (int e = lo + WIN; e < e_end; ++e) {
+        const int jj = nbr_j[e];
+        if (jj >= g0 + m) break;
+        hb[t][jj - g0] += hval[e];
+      }
     }
   }
   xr_wave_sync();
