@@ -268,14 +268,15 @@ __device__ inline void xr_bond_raw(const FFDev& ff, int ti, int tj, double r, Bo
 // Taper (energy.py:90-91): 1 + u^4(-35 + u(84 + u(-70 + 20u))), u = r/rc, 0 at r >= rc.
 __device__ inline void xr_taper(double r, double rc, double& t, double& dt) {
   if (r < rc) {
-    const double u = r / rc;
+    const double inv_rc = 1.0 / rc;  // loop-invariant in the edge loops: hoisted
+    const double u = r * inv_rc;
     const double u2 = u * u;
     const double u4 = u2 * u2;
     const double inner = -35 + u * (84 + u * (-70 + 20 * u));
     t = 1 + u4 * inner;
     // d/du [u^4 inner] = 4u^3 inner + u^4 (84 - 140u + 60u^2)
     const double du = 4 * (u2 * u) * inner + u4 * (84 + u * (-140 + 60 * u));
-    dt = du / rc;
+    dt = du * inv_rc;
   } else {
     t = 0.0;
     dt = 0.0;
@@ -287,10 +288,10 @@ __device__ inline void xr_taper(double r, double rc, double& t, double& dt) {
 __device__ inline void xr_shield_tapered(double r, double t, double dt, double gamma, double& s, double& ds) {
   const double r2 = r * r;
   const double base = r2 * r + gamma;
-  const double inv = 1.0 / cbrt(base);
+  const double inv = rcbrt(base);  // base^(-1/3)
   s = t * inv;
-  // d/dr base^(1/3) = r^2 base^(1/3) / base
-  ds = (dt - t * (r2 / base)) * inv;
+  // d/dr base^(1/3) = r^2 base^(1/3) / base, and 1 / base = inv^3
+  ds = (dt - t * (r2 * ((inv * inv) * inv))) * inv;
 }
 
 __device__ inline void xr_shield(double r, double rc, double gamma, double& s, double& ds) {
