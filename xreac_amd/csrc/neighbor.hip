@@ -258,20 +258,19 @@ __global__ void __launch_bounds__(256) k_nbr_count(int n, const double* __restri
   }
 }
 
-// Longest row of the list (one block; the fill instance is chosen from it).
-
-// Sort the first `count` keys of s (lane-major, K keys per lane) ascending
-// with a bitonic network held in registers: partners closer than K sit in the
-// same lane, farther ones are exchanged with __shfl_xor.  One wave.
-template <int K>
-__device__ inline void wave_sort(unsigned long long* s, int count) {
+// Sort `count` keys (lane-major, K keys per lane) ascending with a bitonic
+// network held in registers: partners closer than K sit in the same lane,
+// farther ones are exchanged with __shfl_xor.  One wave; load(idx) gives key
+// idx < count, store(idx, key) takes the sorted key idx.
+template <int K, class T, class Load, class Store>
+__device__ inline void wave_sort(int count, Load load, Store store) {
   constexpr int N = 64 * K;
   const int lane = threadIdx.x & 63;
-  unsigned long long v[K];
+  T v[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int idx = lane * K + k;
-    v[k] = idx < count ? s[idx] : ~0ull;
+    v[k] = idx < count ? load(idx) : T(~T(0));
   }
 #pragma unroll
   for (int size = 2; size <= N; size <<= 1) {
@@ -283,7 +282,7 @@ __device__ inline void wave_sort(unsigned long long* s, int count) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const bool up = ((lane * K + k) & size) == 0;
-          const unsigned long long o = __shfl_xor(v[k], ls, 64);
+          const T o = __shfl_xor(v[k], ls, 64);
           const bool keep_min = lower == up;
           v[k] = keep_min ? (v[k] < o ? v[k] : o) : (v[k] > o ? v[k] : o);
         }
@@ -293,7 +292,7 @@ __device__ inline void wave_sort(unsigned long long* s, int count) {
           const int q = k ^ stride;
           if (q > k) {
             const bool up = ((lane * K + k) & size) == 0;
-            const unsigned long long a = v[k], b = v[q];
+            const T a = v[k], b = v[q];
             const bool swap = (a > b) == up;
             v[k] = swap ? b : a;
             v[q] = swap ? a : b;
@@ -303,7 +302,31 @@ __device__ inline void wave_sort(unsigned long long* s, int count) {
     }
   }
 #pragma unroll
-  for (int k = 0; k < K; ++k) s[lane * K + k] = v[k];
+  for (int k = 0; k < K; ++k) {
+    const int idx = lane * K + k;
+    if (idx < count) store(idx, v[k]);
+  }
+}
+
+// A row whose image shifts all lie in {-1, 0, 1} (any cell taller than the
+// cutoff) sorts 32-bit keys (j - atom0) << 5 | (9 (sx + 1) + 3 (sy + 1) +
+// (sz + 1)), ordered as the full (j << 32 | packed shifts) keys.
+__device__ inline bool compact_shift(int code) {
+  return (unsigned)(((code >> 20) & 1023) - 511) <= 2u && (unsigned)(((code >> 10) & 1023) - 511) <= 2u &&
+         (unsigned)((code & 1023) - 511) <= 2u;
+}
+
+__device__ inline unsigned compact_key(unsigned long long key, int atom0) {
+  const int code = (int)(key & 0xffffffffu);
+  const int img = 9 * (((code >> 20) & 1023) - 511) + 3 * (((code >> 10) & 1023) - 511) + ((code & 1023) - 511);
+  return ((unsigned)((int)(key >> 32) - atom0) << 5) | (unsigned)img;
+}
+
+__device__ inline unsigned long long full_key(unsigned c, int atom0) {
+  const int img = (int)(c & 31u);
+  const int j = (int)(c >> 5) + atom0;
+  const int code = xr_pack_shift(img / 9 - 1, (img / 3) % 3 - 1, img % 3 - 1);
+  return ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
 }
 
 // One wave per row: the row's neighbors, sorted into canonical (j, image)
@@ -329,6 +352,7 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   unsigned err = 0;
   int cursor = 0;  // identical in every lane
+  bool wide = false;  // a shift outside {-1, 0, 1}: full 64-bit sort keys
   for_each_neighbor(i, x, S.sys[sys], home, bin, bin_start, binned, rc2, err,
                     [&](bool hit, int j, int code, double r2) {
                       const unsigned long long m = __ballot(hit);
@@ -336,9 +360,12 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
                         const int p = cursor + __popcll(m & below);
                         if (p < CAP) keys[p] = ((unsigned long long)(unsigned)j << 32) | (unsigned)code;
                         if (j != i && r2 < 1e-12) err |= XR_ERR_COINCIDENT;
+                        if (!compact_shift(code)) wide = true;
                       }
                       cursor += __popcll(m);
                     });
+  const int atom0 = S.sys[sys].atom0;
+  const bool compact = __ballot(wide) == 0ull && S.sys[sys].n <= (1 << 27);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) err |= __shfl_down(err, off, 64);
   const int count = cursor;
@@ -359,15 +386,27 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
     if (err) atomicOr(&S.err[sys], err);
   }
   __syncthreads();
-  if (count <= 128) {
-    wave_sort<2>(keys, count);
-  } else if (count <= 256) {
-    wave_sort<4>(keys, count);
-  } else if (count <= 512) {
-    wave_sort<8>(keys, count);
-  } else if constexpr (CAP == 1024) {
-    wave_sort<16>(keys, count);
-  } else if constexpr (CAP > 1024) {
+  unsigned long long* out = slab + (size_t)i * CAP;
+  // Register sorts store straight to the row's slab slots.
+  auto ld64 = [&](int idx) { return keys[idx]; };
+  auto st64 = [&](int idx, unsigned long long key) { out[idx] = key; };
+  auto ld32 = [&](int idx) { return compact_key(keys[idx], atom0); };
+  auto st32 = [&](int idx, unsigned key) { out[idx] = full_key(key, atom0); };
+  if (count <= 512 || (CAP == 1024 && count <= 1024)) {
+    if (compact) {
+      if (count <= 128) wave_sort<2, unsigned>(count, ld32, st32);
+      else if (count <= 256) wave_sort<4, unsigned>(count, ld32, st32);
+      else if (count <= 512) wave_sort<8, unsigned>(count, ld32, st32);
+      else if constexpr (CAP == 1024) wave_sort<16, unsigned>(count, ld32, st32);
+    } else {
+      if (count <= 128) wave_sort<2, unsigned long long>(count, ld64, st64);
+      else if (count <= 256) wave_sort<4, unsigned long long>(count, ld64, st64);
+      else if (count <= 512) wave_sort<8, unsigned long long>(count, ld64, st64);
+      else if constexpr (CAP == 1024) wave_sort<16, unsigned long long>(count, ld64, st64);
+    }
+    return;
+  }
+  if constexpr (CAP > 1024) {
     // Long rows: bitonic network in LDS.
     int size = 1024;
     while (size < count) size <<= 1;
@@ -390,7 +429,6 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
     }
   }
   __syncthreads();
-  unsigned long long* out = slab + (size_t)i * CAP;
   for (int t2 = lane; t2 < count; t2 += 64) out[t2] = keys[t2];
 }
 
