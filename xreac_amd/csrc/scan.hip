@@ -116,10 +116,24 @@ __global__ void __launch_bounds__(ONE_BLOCK) k_scan_one(const int* __restrict__ 
   __shared__ int tmp[17];
   int carry = 0, mx = 0;
   int nv[ONE_ITEMS];
+  // A thread's 16 counts are four 16-byte loads when all lie inside a 16-byte
+  // aligned input, else guarded single loads.
   auto load = [&](int t0) {
     const int base = t0 + threadIdx.x * ONE_ITEMS;
+    if (base + ONE_ITEMS <= n && (reinterpret_cast<size_t>(in) & 15) == 0) {
+      const int4* v4 = reinterpret_cast<const int4*>(in + base);
 #pragma unroll
-    for (int k = 0; k < ONE_ITEMS; ++k) nv[k] = base + k < n ? in[base + k] : 0;
+      for (int q = 0; q < ONE_ITEMS / 4; ++q) {
+        const int4 w = v4[q];
+        nv[4 * q] = w.x;
+        nv[4 * q + 1] = w.y;
+        nv[4 * q + 2] = w.z;
+        nv[4 * q + 3] = w.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < ONE_ITEMS; ++k) nv[k] = base + k < n ? in[base + k] : 0;
+    }
   };
   load(0);
   for (int t0 = 0; t0 < n; t0 += ONE_TILE) {
@@ -136,10 +150,20 @@ __global__ void __launch_bounds__(ONE_BLOCK) k_scan_one(const int* __restrict__ 
     }
     int total;
     int e = block_exclusive_scan(s, tmp, total) + carry;
+    int o[ONE_ITEMS];
 #pragma unroll
     for (int k = 0; k < ONE_ITEMS; ++k) {
-      if (base + k < n) out[base + k] = e < cap ? e : cap;
+      o[k] = e < cap ? e : cap;
       e += v[k];
+    }
+    if (base + ONE_ITEMS <= n && (reinterpret_cast<size_t>(out + base) & 15) == 0) {
+      int4* o4 = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+      for (int q = 0; q < ONE_ITEMS / 4; ++q) o4[q] = make_int4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < ONE_ITEMS; ++k)
+        if (base + k < n) out[base + k] = o[k];
     }
     carry += total;
   }
