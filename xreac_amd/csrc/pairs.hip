@@ -159,17 +159,29 @@ __global__ void __launch_bounds__(256) k_charge_response(Atoms at, FFDev ff, con
 // One block per (system, channel): the channel summed over the system's atoms
 // in a fixed order (lane-strided, then a fixed tree), independent of where
 // the system sits in the batch.
-__global__ void __launch_bounds__(256) k_energy_reduce(Systems S, EnergyAcc acc, double* __restrict__ chan) {
+__global__ void __launch_bounds__(1024) k_energy_reduce(Systems S, EnergyAcc acc, double* __restrict__ chan) {
   const SysDev& sd = S.sys[blockIdx.x];
   const int c = blockIdx.y;
-  __shared__ double red[4];
+  __shared__ double red[16];
   const double* e = acc.e + (size_t)c * acc.n + sd.atom0;
-  double s = 0.0;
-  for (int t = threadIdx.x; t < sd.n; t += blockDim.x) s += e[t];
-  s = xr_wave_sum(s);
+  // four independent partial sums per thread, so the loads are in flight together
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int t = threadIdx.x;
+  for (; t + 3 * 1024 < sd.n; t += 4 * 1024) {
+    a0 += e[t];
+    a1 += e[t + 1024];
+    a2 += e[t + 2048];
+    a3 += e[t + 3072];
+  }
+  for (; t < sd.n; t += 1024) a0 += e[t];
+  double s = xr_wave_sum((a0 + a1) + (a2 + a3));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) chan[(size_t)blockIdx.x * NCH + c] = ((red[0] + red[1]) + red[2]) + red[3];
+  if (threadIdx.x == 0) {
+    double x = 0.0;
+    for (int w = 0; w < 16; ++w) x += red[w];
+    chan[(size_t)blockIdx.x * NCH + c] = x;
+  }
 }
 
 }  // namespace
@@ -199,7 +211,7 @@ hipError_t xr_charge_response(const Atoms& at, FFDev ff, const int* row_ptr, con
 }
 
 hipError_t xr_energy_reduce(const Systems& S, EnergyAcc acc, double* chan, hipStream_t s) {
-  hipLaunchKernelGGL(k_energy_reduce, dim3(S.ns, NCH), dim3(256), 0, s, S, acc, chan);
+  hipLaunchKernelGGL(k_energy_reduce, dim3(S.ns, NCH), dim3(1024), 0, s, S, acc, chan);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

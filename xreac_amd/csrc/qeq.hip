@@ -937,11 +937,23 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
                                                       double2* __restrict__ keep, int neutral) {
   const SysDev& sd = S.sys[blockIdx.x];
   __shared__ double red[2][16];
-  double ss = 0.0, st = 0.0;
-  for (int i = sd.atom0 + threadIdx.x; i < sd.atom0 + sd.n; i += blockDim.x) {
+  // two independent partial sums per thread and sum, so the loads are in flight together
+  double ss = 0.0, st = 0.0, ss2 = 0.0, st2 = 0.0;
+  const int end = sd.atom0 + sd.n;
+  int i = sd.atom0 + threadIdx.x;
+  for (; i + 1024 < end; i += 2048) {
+    const double2 a = X[i], b = X[i + 1024];
+    ss += a.x;
+    st += a.y;
+    ss2 += b.x;
+    st2 += b.y;
+  }
+  for (; i < end; i += 1024) {
     ss += X[i].x;
     st += X[i].y;
   }
+  ss += ss2;
+  st += st2;
   ss = xr_wave_sum(ss);
   st = xr_wave_sum(st);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
