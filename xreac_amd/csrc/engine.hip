@@ -699,15 +699,17 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, slab, slab_stride, nbr_j, nbr_s, bond_cut, hval, hval32,
                       bflag, bond_count, join, s),
          "edge pass");
+  // The bond list (scan of the active-bond counts, compaction) feeds only the
+  // bonded terms, so with buffers sized from the last evaluation it is built on
+  // the second stream with them; an exact evaluation reads its size back first.
+  const bool bonds_on_side = !exact;
   int64_t b_cap = 0;
   if (!exact) {
     b_cap = (int64_t)std::ceil(ctx->b_per_atom * n * 1.25) + 1024;
     if (b_cap > m_cap) b_cap = m_cap;
-  }
-  HIPCHK(xr_scan_clamped(bond_count, n, bond_ptr, scan_tmp, exact ? INT32_MAX : (int)b_cap, dev_counts + 1, d_flags,
-                         nullptr, s),
-         "bond scan");
-  if (exact) {
+  } else {
+    HIPCHK(xr_scan_clamped(bond_count, n, bond_ptr, scan_tmp, INT32_MAX, dev_counts + 1, d_flags, nullptr, s),
+           "bond scan");
     int nb_host = 0;
     HIPCHK(hipMemcpyAsync(&nb_host, dev_counts + 1, sizeof(int), hipMemcpyDeviceToHost, s), "copy");
     HIPCHK(hipStreamSynchronize(s), "bond sync");
@@ -770,7 +772,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   bd.c_i = bn();
   bd.c_j = bn();
   if (bs > bs_end) return fail(ctx, XREAC_INVALID, "internal error: per-bond workspace layout");
-  if (nbond > 0) HIPCHK(xr_bond_fill(n, row_ptr, bflag, bond_ptr, (int)nbond, b_edge, b_i, s), "bond fill");
+  if (!bonds_on_side && nbond > 0) HIPCHK(xr_bond_fill(n, row_ptr, bflag, bond_ptr, (int)nbond, b_edge, b_i, s), "bond fill");
 
   // ------------------------------------------- bonded terms on the side stream
   // Bond orders, many-body terms and their adjoints depend on positions only;
@@ -792,6 +794,14 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   auto launch_side = [&]() -> hipError_t {
     hipError_t e = hipStreamWaitEvent(s2, ctx->bonds_ready, 0);
     if (e != hipSuccess) return e;
+    if (bonds_on_side) {
+      side_what = "bond scan";
+      e = xr_scan_clamped(bond_count, n, bond_ptr, scan_tmp, (int)b_cap, dev_counts + 1, d_flags, nullptr, s2);
+      if (e != hipSuccess) return e;
+      side_what = "bond fill";
+      if (nbond > 0 && (e = xr_bond_fill(n, row_ptr, bflag, bond_ptr, (int)nbond, b_edge, b_i, s2)) != hipSuccess)
+        return e;
+    }
     side_what = "van der Waals";
     if ((e = xr_vdw(at, ff, row_ptr, nbr_j, nbr_s, acc, s2)) != hipSuccess) return e;
     side_what = "bond orders";
