@@ -62,6 +62,7 @@ struct FFDev {
   const double* hb;            // [nt^3][4] keyed (t_donor, t_H, t_acceptor)
   const unsigned char* hb_on;  // [nt^3]
   const double* vdw_b;         // [nt*nt] (1/gamma_w)^p_vdw, the shielding offset of each pair
+  const double* vdw_c;         // [nt*nt][4] 1/r_vdw, -alpha/(2 r_vdw), 1/r_core, -a_core/r_core
 };
 
 struct CellDev {

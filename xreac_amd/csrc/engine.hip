@@ -228,6 +228,19 @@ int xreac_ff_create(const xreac_ff_tables* t, xreac_ff** out) {
     for (size_t k = 0; k < nt * nt; ++k) b[k] = std::pow(1.0 / t->pair[k * PP_COUNT + PP_GAMMAW], t->general[28]);
     e = upload(ff, b.data(), nt * nt, &ff->dev.vdw_b);
   }
+  if (e == hipSuccess) {
+    // per-pair ratios of the van der Waals and inner-wall terms, so the edge
+    // loop multiplies instead of dividing
+    std::vector<double> c(4 * nt * nt);
+    for (size_t k = 0; k < nt * nt; ++k) {
+      const double* pp = t->pair + k * PP_COUNT;
+      c[4 * k] = 1.0 / pp[PP_RVDW];
+      c[4 * k + 1] = -0.5 * pp[PP_ALPHA] / pp[PP_RVDW];
+      c[4 * k + 2] = 1.0 / pp[PP_RCORE];
+      c[4 * k + 3] = -pp[PP_ACORE] / pp[PP_RCORE];
+    }
+    e = upload(ff, c.data(), 4 * nt * nt, &ff->dev.vdw_c);
+  }
   if (e != hipSuccess) {
     xreac_ff_destroy(ff);
     return e == hipErrorOutOfMemory ? XREAC_NOMEM : XREAC_DEVICE;

@@ -64,15 +64,16 @@ __global__ void __launch_bounds__(256) k_nonbonded(Atoms at, FFDev ff, const int
           f13 = r;
           df13 = 1.0;
         }
-        const double ex = exp(0.5 * p[PP_ALPHA] * (1 - f13 / p[PP_RVDW]));
+        const double* cv = ff.vdw_c + 4 * pij;
+        const double ex = exp(0.5 * p[PP_ALPHA] * (1 - f13 * cv[0]));
         const double ev = p[PP_D] * (ex * ex - 2 * ex);
-        const double dev = p[PP_D] * (2 * ex - 2) * ex * (-0.5 * p[PP_ALPHA] / p[PP_RVDW]) * df13;
+        const double dev = p[PP_D] * (2 * ex - 2) * ex * cv[1] * df13;
         double phi = t * ev;  // per-direction vdW energy (times 2)
         double dphi = dt * ev + t * dev;
         if (wall) {
-          const double ecr = p[PP_ECORE] * exp(p[PP_ACORE] * (1 - r / p[PP_RCORE]));
+          const double ecr = p[PP_ECORE] * exp(p[PP_ACORE] * (1 - r * cv[2]));
           phi += t * ecr;
-          dphi += dt * ecr + t * ecr * (-p[PP_ACORE] / p[PP_RCORE]);
+          dphi += dt * ecr + t * ecr * cv[3];
         }
         ev_sum += phi;
         dtot += dphi;
