@@ -94,6 +94,32 @@ def test_batch_is_bitwise_alone(precision):
         assert res["qeq_iterations"][k] == a["qeq_iterations"], system[0]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_with_both_cg_block_shapes(precision):
+    # Systems below 8,192 atoms take the small CG block shape and larger ones
+    # the large shape, so this batch runs two iteration launches per CG
+    # iteration, with the large system's blocks numbered after both small
+    # systems' although it sits between them.
+    eng = _engine()
+    s, x, cell = workloads.water_box(side=14, seed=7)
+    assert len(s) >= 8192
+    systems = [SYSTEMS[0], ("box8232", s, x, cell, (True, True, True), 0.0), SYSTEMS[-1]]
+    print(f"[batch] both shapes, {precision}", flush=True)
+    x, t, off, cells, pbcs, qs = _pack(eng, systems)
+    res = eng.evaluate_batch(x, t, off, cells, pbcs, qs, tol=1e-12, precision=precision)
+    assert all(e is None for e in res["errors"]), res["errors"]
+    forces = res["forces"].cpu().numpy()
+    charges = res["charges"].cpu().numpy()
+    for k, system in enumerate(systems):
+        print(f"[batch] alone: {system[0]}", flush=True)
+        a = _alone(eng, system, precision)
+        lo, hi = off[k], off[k + 1]
+        np.testing.assert_array_equal(res["components"][k], a["components"], err_msg=system[0])
+        np.testing.assert_array_equal(forces[lo:hi], a["forces"].cpu().numpy(), err_msg=system[0])
+        np.testing.assert_array_equal(charges[lo:hi], a["charges"].cpu().numpy(), err_msg=system[0])
+        assert res["qeq_iterations"][k] == a["qeq_iterations"], system[0]
+
+
 def test_batch_failure_stays_with_its_system():
     eng = _engine()
     bad = ("coincident", ["O", "H"], [[0.0, 0, 0], [12.0, 0, 0]], np.diag([12.0] * 3), (True, True, True), 0.0)

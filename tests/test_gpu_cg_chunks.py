@@ -1,10 +1,11 @@
 """CG iteration blocks that own several chunks.
 
-A system gets one CG chunk per 48 rows and at most XREAC_CG_MAX_BLOCKS
-iteration blocks (256 by default), so systems above 12,288 atoms give some
-blocks several chunks, which they sweep one after the other.  The cap is read
-once per process, so a child process with a cap of 4 evaluates the 3150-atom
-CHO mixture (66 chunks, 17 per block) and its energy, forces and charges must
+A system gets one CG chunk per 48 rows (per 24 rows below 8,192 atoms) and at
+most XREAC_CG_MAX_BLOCKS iteration blocks (256 by default, 512 below 8,192
+atoms), so systems above 12,288 atoms give some blocks several chunks, which
+they sweep one after the other.  The cap is read once per process, so a child
+process with a cap of 4 evaluates the 3150-atom CHO mixture (132 chunks, 33
+per block) and its energy, forces and charges must
 agree with this process's evaluation (one chunk per block).  The dot partials
 are grouped differently, so the two solves differ in their last bits only;
 both are converged to tolerance 1e-12.

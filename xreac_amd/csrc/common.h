@@ -85,8 +85,8 @@ struct BinGeom {
 
 // One system of a batch.  Atoms of system s occupy [atom0, atom0 + n) of the
 // packed per-atom arrays; its bins occupy [bin0, bin0 + nb0*nb1*nb2) of the
-// global bin arrays; its CG blocks are [dir0, dir0 + ndir) and
-// [upd0, upd0 + nupd) of the two CG kernels' grids.
+// global bin arrays; its CG iteration blocks are [dir0, dir0 + ndir) (small
+// systems' blocks first) and its chunks [upd0, upd0 + nupd).
 struct SysDev {
   CellDev cell;
   BinGeom g;
@@ -94,6 +94,7 @@ struct SysDev {
   int bin0;
   int dir0, ndir;
   int upd0, nupd;
+  int chunk;  // nominal rows per CG chunk (by system size)
   double total_charge;
 };
 

@@ -446,7 +446,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   unsigned* d_flags = reinterpret_cast<unsigned*>(dc + L.flags);
   SysDev* hsys = reinterpret_cast<SysDev*>(hc + L.sys);
   std::vector<double> heights(3 * (size_t)ns);
-  int nd = 0, nu = 0;
+  int nd = 0, nu = 0, nd_small = 0;
   bool need_bounds = false;
   for (int q = 0; q < ns; ++q) {
     SysDev sd;
@@ -471,13 +471,19 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
       heights[3 * q + k] = 1.0 / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
       need_bounds |= !cell.pbc[k];
     }
-    xr_cg_blocks(sd.n, &sd.ndir, &sd.nupd);
-    sd.dir0 = nd;
+    xr_cg_blocks(sd.n, &sd.ndir, &sd.nupd, &sd.chunk);
     sd.upd0 = nu;
     nd += sd.ndir;
     nu += sd.nupd;
+    if (xr_cg_small(sd.n)) nd_small += sd.ndir;
     sd.total_charge = in.charges ? in.charges[q] : 0.0;
     hsys[q] = sd;
+  }
+  // CG iteration blocks: small systems' first, then large ones', each in batch order
+  for (int q = 0, ds = 0, dl = nd_small; q < ns; ++q) {
+    int& d = xr_cg_small(hsys[q].n) ? ds : dl;
+    hsys[q].dir0 = d;
+    d += hsys[q].ndir;
   }
 
   // ------------------------------------------------------ atom workspace
@@ -837,6 +843,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   CgArgs cg{};
   cg.S = S;
   cg.nd = nd;
+  cg.nd_small = nd_small;
   cg.nu = nu;
   cg.type = in.types;
   cg.ff = ff;

@@ -47,6 +47,7 @@ struct CgVec {
 struct CgArgs {
   Systems S;
   int nd, nu;          // total iteration blocks / chunks over all systems
+  int nd_small;        // iteration blocks of small systems, numbered first
   const int* type;
   FFDev ff;
   const int* row_ptr;
@@ -103,7 +104,10 @@ hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& 
                         float* hval32, unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s);
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int cap,
                         int* bond_edge, int* bond_i, hipStream_t s);
-void xr_cg_blocks(int n, int* ndir, int* nupd);
+// Iteration blocks (ndir), chunks (nupd) and nominal rows per chunk of a
+// system of n atoms; small systems (xr_cg_small) take the 512-thread shape.
+void xr_cg_blocks(int n, int* ndir, int* nupd, int* chunk);
+bool xr_cg_small(int n);
 size_t xr_cg_partials(int nd, int nu);
 size_t xr_cg_state_bytes(int ns);
 size_t xr_cg_block_doubles(int n);  // doubles of the block-Jacobi inverse rows of n atoms

@@ -196,17 +196,28 @@ __global__ void __launch_bounds__(256) k_bond_fill(int n, const int* __restrict_
 // its system's partials again in the same fixed order (redundant but cheap,
 // and bitwise reproducible).  Device-scope fences inside a kernel are avoided
 // on purpose: on a multi-XCD part they write back the whole L2.
-constexpr int CG_BLOCK = 1024;      // 16 waves; the rows of a chunk are spread over the waves
-constexpr int CG_MAX_BLOCKS = 256;  // per system: few partial sums to reduce
-constexpr int CHUNK = 48;           // nominal rows per chunk; chunks start at bonded-run starts
-constexpr int CHUNK_MAX = 64;       // CHUNK + RUN_MAX - 1 rows at most: one wave holds a chunk
-constexpr int CG_WAVES = CG_BLOCK / 64;
+// Two block shapes, chosen by a system's own size (so a system is solved the
+// same way alone and in any batch): systems of CG_SMALL_ATOMS atoms and more
+// take 1024-thread blocks over 48-row chunks, which fill the chip from
+// 12,288 atoms on; smaller systems take 512-thread blocks over 24-row chunks,
+// which spread a system over twice as many CUs and pack a batch of small
+// systems more tightly (two blocks per CU).  Each iteration launches one
+// kernel per shape present in the batch; small systems' blocks come first.
+constexpr int CG_BLOCK = 1024;            // 16 waves; the rows of a chunk are spread over the waves
+constexpr int CG_MAX_BLOCKS = 256;        // per system: few partial sums to reduce
+constexpr int CHUNK = 48;                 // nominal rows per chunk; chunks start at bonded-run starts
+constexpr int CG_BLOCK_SMALL = 512;       // 8 waves
+constexpr int CG_MAX_BLOCKS_SMALL = 512;
+constexpr int CHUNK_SMALL = 24;
+constexpr int CG_SMALL_ATOMS = 8192;
+constexpr int CHUNK_MAX = 64;             // CHUNK + RUN_MAX - 1 rows at most: one wave holds a chunk
 constexpr int INNER_MAX = 64;          // mixed precision: iterations between FP64 restarts
 constexpr double INNER_DROP2 = 1e-8;   // mixed precision: squared residual drop before a restart
 constexpr int MAX_RESTARTS = 12;
 constexpr int NDOT = 6;                // per block: r.u, w.u, r.r for both right-hand sides
 constexpr int CG_LEAD_LAUNCHES = 4;    // launches queued before the caller's side-stream work
-static_assert(CHUNK + RUN_MAX - 1 <= CHUNK_MAX, "a chunk must fit one wave");
+static_assert(CHUNK + RUN_MAX - 1 <= CHUNK_MAX && CHUNK_SMALL + RUN_MAX - 1 <= CHUNK_MAX,
+              "a chunk must fit one wave");
 
 // System owning global block b (first-block offsets ascend with the system index).
 __device__ inline int block_system(const Systems& S, int b, bool dir) {
@@ -220,13 +231,13 @@ __device__ inline int block_system(const Systems& S, int b, bool dir) {
   return lo;
 }
 
-// Local row where chunk c of a system starts: the nominal c * CHUNK, moved
+// Local row where chunk c of a system starts: the nominal c * sd.chunk, moved
 // forward to the first row within RUN_MAX rows that starts a bonded run (no
 // strong bond to any of the three rows before it), so that molecules are not
 // split between chunks.  Called by a whole wave.
 __device__ inline int chunk_start(const SysDev& sd, const unsigned char* __restrict__ join, int c) {
   if (c <= 0) return 0;
-  const int t = c * CHUNK;
+  const int t = c * sd.chunk;
   if (t >= sd.n) return sd.n;
   const int lane = threadIdx.x & 63;
   const int r = t + lane;
@@ -487,17 +498,17 @@ __device__ inline void gather_sum(const RowBatch<HV>& b, const VT* __restrict__ 
 }
 
 // Off-diagonal row sums sum_e h_e v_j(e) for the rows [r0, r1) of a chunk:
-// wave w takes rows r0 + w, r0 + w + 16, ...; each row's first 512 entries
+// wave w takes rows r0 + w, r0 + w + WAVES, ...; each row's first 512 entries
 // are in cur (loaded by the caller for the wave's first row), and the next
 // row's are loaded before the current row's gathers.  Lane 0 leaves each
 // row's pair of sums in out[row - r0].
-template <class HV, class VT, class AT>
+template <int WAVES, class HV, class VT, class AT>
 __device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
                                   const HV* __restrict__ hv, const VT* __restrict__ vec, RowBatch<HV>& cur, int e0,
                                   int e1, double2* out) {
   const int lane = threadIdx.x & 63;
-  for (int i = r0 + (threadIdx.x >> 6); i < r1; i += CG_WAVES) {
-    const int inext = i + CG_WAVES;
+  for (int i = r0 + (threadIdx.x >> 6); i < r1; i += WAVES) {
+    const int inext = i + WAVES;
     int n0 = 0, n1 = 0;
     RowBatch<HV> next;
     if (inext < r1) {
@@ -567,12 +578,19 @@ struct ChunkLds {
   double ex[2][CHUNK_MAX];   // vector exchanged for the M^-1 apply, per right-hand side
 };
 
-// One vector entry (items below NITEM), eta (item NITEM, in val.x) or the
-// block placement (item NITEM + 1, its bits in val.x), and one inverse-block
-// column entry.
+// Items staged per row: NITEM vectors, eta, the block placement and
+// BLOCK_MAX inverse-block columns, item w + u WAVES by wave w in pass u.
+constexpr int STAGE_ITEMS = 16;
+static_assert(NITEM + 2 <= STAGE_ITEMS && BLOCK_MAX <= STAGE_ITEMS, "staged items");
+
+// Per pass: one vector entry (items below NITEM), eta (item NITEM, in val.x)
+// or the block placement (item NITEM + 1, its bits in val.x), and one
+// inverse-block column entry.
+template <int WAVES>
 struct Staged {
-  double2 val;
-  double mv;
+  static_assert(STAGE_ITEMS % WAVES == 0, "whole passes");
+  double2 val[STAGE_ITEMS / WAVES];
+  double mv[STAGE_ITEMS / WAVES];
 };
 
 __device__ inline bool item_used(int mode, int item) {
@@ -581,41 +599,54 @@ __device__ inline bool item_used(int mode, int item) {
   return item == IT_R || item == IT_U || item == IT_B;  // MODE_START
 }
 
+template <int WAVES>
 __device__ inline void stage_load(int mode, int r0, int r1, const CgVec& v, const double2* __restrict__ m_in,
                                   const double* __restrict__ minv, int n_all, const int2* __restrict__ binfo,
                                   const double* __restrict__ eta, const double2* __restrict__ rhs,
-                                  const int* __restrict__ type, const FFDev& ff, Staged& s) {
-  const int row = threadIdx.x & 63, item = threadIdx.x >> 6;
+                                  const int* __restrict__ type, const FFDev& ff, Staged<WAVES>& s) {
+  const int row = threadIdx.x & 63;
   const int i = r0 + row;
-  s.val = make_double2(0.0, 0.0);
-  s.mv = 0.0;
-  if (i >= r1) return;
-  s.mv = minv[(size_t)item * n_all + i];  // entries past the block size are never used
-  if (item < NITEM && item_used(mode, item)) {
-    switch (item) {
-      case IT_Z: s.val = v.Z[i]; break;
-      case IT_Q: s.val = v.Q[i]; break;
-      case IT_S: s.val = v.S[i]; break;
-      case IT_P: s.val = v.P[i]; break;
-      case IT_X: s.val = v.X[i]; break;
-      case IT_R: s.val = v.R[i]; break;
-      case IT_U: s.val = v.U[i]; break;
-      case IT_W: s.val = v.W[i]; break;
-      case IT_M: s.val = m_in[i]; break;
-      default: s.val = cg_rhs(rhs, type, ff, i); break;
+#pragma unroll
+  for (int u = 0; u < STAGE_ITEMS / WAVES; ++u) {
+    const int item = (threadIdx.x >> 6) + u * WAVES;
+    double2 val = make_double2(0.0, 0.0);
+    double mv = 0.0;
+    if (i < r1) {
+      mv = minv[(size_t)item * n_all + i];  // entries past the block size are never used
+      if (item < NITEM && item_used(mode, item)) {
+        switch (item) {
+          case IT_Z: val = v.Z[i]; break;
+          case IT_Q: val = v.Q[i]; break;
+          case IT_S: val = v.S[i]; break;
+          case IT_P: val = v.P[i]; break;
+          case IT_X: val = v.X[i]; break;
+          case IT_R: val = v.R[i]; break;
+          case IT_U: val = v.U[i]; break;
+          case IT_W: val = v.W[i]; break;
+          case IT_M: val = m_in[i]; break;
+          default: val = cg_rhs(rhs, type, ff, i); break;
+        }
+      }
+      if (item == NITEM) val.x = eta[i];
+      if (item == NITEM + 1) val.x = __longlong_as_double(*reinterpret_cast<const long long*>(binfo + i));
     }
+    s.val[u] = val;
+    s.mv[u] = mv;
   }
-  if (item == NITEM) s.val.x = eta[i];
-  if (item == NITEM + 1) s.val.x = __longlong_as_double(*reinterpret_cast<const long long*>(binfo + i));
 }
 
-__device__ inline void stage_store(int r0, int r1, const Staged& s, ChunkLds& L) {
-  const int row = threadIdx.x & 63, item = threadIdx.x >> 6;
+template <int WAVES>
+__device__ inline void stage_store(int r0, int r1, const Staged<WAVES>& s, ChunkLds& L) {
+  const int row = threadIdx.x & 63;
   if (r0 + row >= r1) return;
-  L.mv[item][row] = s.mv;
-  if (item < NITEM) L.st[item][row] = s.val;
-  if (item == NITEM) L.eta[row] = s.val.x;
-  if (item == NITEM + 1) *reinterpret_cast<long long*>(&L.bi[row]) = __double_as_longlong(s.val.x);
+#pragma unroll
+  for (int u = 0; u < STAGE_ITEMS / WAVES; ++u) {
+    const int item = (threadIdx.x >> 6) + u * WAVES;
+    L.mv[item][row] = s.mv[u];
+    if (item < NITEM) L.st[item][row] = s.val[u];
+    if (item == NITEM) L.eta[row] = s.val[u].x;
+    if (item == NITEM + 1) *reinterpret_cast<long long*>(&L.bi[row]) = __double_as_longlong(s.val[u].x);
+  }
 }
 
 // M^-1 applied to the exchanged vector of right-hand side h for row t
@@ -736,9 +767,11 @@ __device__ inline void settle(CgSys* c, CgCounters* cnt, const CgCtl& cur, int s
 // n = A m over its rows and the vector recurrences listed at chunk_tail.  The
 // sweep reads only m, computed for every row by the previous launch, so one
 // launch per iteration suffices; the dots for the next launch come out of the
-// same pass.  Each block of a system owns whole chunks.
-template <bool MIXED>
-__global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgVec v, const int* __restrict__ row_ptr,
+// same pass.  Each block of a system owns whole chunks.  The launch covers
+// iteration blocks [base, base + gridDim.x), all of one shape (BLOCK threads,
+// at most 128 VGPRs).
+template <bool MIXED, int BLOCK>
+__global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgVec v, const int* __restrict__ row_ptr,
                                                            const int* __restrict__ nbr_j,
                                                            const double* __restrict__ hval,
                                                            const float* __restrict__ hval32,
@@ -751,10 +784,14 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
                                                            const double2* __restrict__ rhs, double* __restrict__ part,
                                                            double* __restrict__ part_bb, int nd, double tol2,
                                                            int max_iter, CgSys* __restrict__ cg,
-                                                           CgCounters* __restrict__ cnt, int n_all) {
+                                                           CgCounters* __restrict__ cnt, int n_all, int base) {
+  constexpr int WAVES = BLOCK / 64;
+  constexpr int QUARTER = BLOCK / 4, QW = WAVES / 4;  // threads and waves per series pair of partials
+  static_assert(WAVES % 4 == 0, "a quarter of the block per series pair");
+  const int b = base + blockIdx.x;
   // The block's descriptor gives its system and first chunk; the wave's first
   // row range is loaded alongside the system's control state.
-  const int4 desc = blocks[blockIdx.x];
+  const int4 desc = blocks[b];
   const int sys = desc.x;
   int e0, e1;
   first_row(desc.y, desc.z, row_ptr, e0, e1);
@@ -763,7 +800,7 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
   const int mode = cg_mode(c, k, j);
   if (mode == MODE_IDLE) return;
   const SysDev& sd = S.sys[sys];
-  const int local = blockIdx.x - sd.dir0;
+  const int local = b - sd.dir0;
   const int per = (sd.nupd + sd.ndir - 1) / sd.ndir;
   const int c0 = desc.w, c1 = min(c0 + per, sd.nupd);
   __shared__ ChunkLds L;
@@ -790,45 +827,45 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
     // The first chunk's first rows, its row operands and the previous
     // launch's partials are all loaded before the sweep; the partials are
     // reduced after it (the sweep n = A m needs neither alpha nor beta), so
-    // their latency hides under the gathers.  Thread t holds partial t % 256
-    // of the series pair 2 (t / 256): waves 0-3 the r.u pair, 4-7 w.u, 8-11
-    // r.r, 12-15 b.b.
+    // their latency hides under the gathers.  Each quarter of the block holds
+    // one series pair (in order r.u, w.u, r.r, b.b); thread t of a quarter
+    // sums partials t, t + QUARTER, ... of it.
     int2 cr = make_int2(desc.y, desc.z);
     RowBatch<HV> cur;
     first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, cur);
-    Staged sv;
-    stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+    Staged<WAVES> sv;
+    stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
     double pa = 0.0, pb = 0.0;
     {
-      const int series = (threadIdx.x >> 8) * 2;
-      const double* base = series < NDOT ? part + ((k - 1) & 1) * NDOT * nd + series * nd : part_bb;
-      const int t0 = threadIdx.x & 255;
+      const int series = (threadIdx.x / QUARTER) * 2;
+      const double* pbase = series < NDOT ? part + ((k - 1) & 1) * NDOT * nd + series * nd : part_bb;
+      const int t0 = threadIdx.x % QUARTER;
       if (t0 < sd.ndir) {
-        pa = base[sd.dir0 + t0];
-        pb = base[nd + sd.dir0 + t0];
+        pa = pbase[sd.dir0 + t0];
+        pb = pbase[nd + sd.dir0 + t0];
       }
-      for (int t = t0 + 256; t < sd.ndir; t += 256) {  // only with a raised block cap
-        pa += base[sd.dir0 + t];
-        pb += base[nd + sd.dir0 + t];
+      for (int t = t0 + QUARTER; t < sd.ndir; t += QUARTER) {  // only with a raised block cap
+        pa += pbase[sd.dir0 + t];
+        pb += pbase[nd + sd.dir0 + t];
       }
     }
-    __shared__ double red[8][4];
+    __shared__ double red[8][QW];
     bool stepped = false;
     for (int ch = c0; ch < c1; ++ch) {
       if (ch > c0) {
         cr = chunks[sd.upd0 + ch];
         first_row(cr.x, cr.y, row_ptr, e0, e1);
         first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, cur);
-        stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+        stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
       }
-      chunk_spmv<HV, VT, AT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
-      stage_store(cr.x, cr.y, sv, L);
+      chunk_spmv<WAVES, HV, VT, AT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
+      stage_store<WAVES>(cr.x, cr.y, sv, L);
       if (!stepped) {
         pa = xr_wave_sum(pa);
         pb = xr_wave_sum(pb);
         if ((threadIdx.x & 63) == 0) {
-          red[(threadIdx.x >> 8) * 2][(threadIdx.x >> 6) & 3] = pa;
-          red[(threadIdx.x >> 8) * 2 + 1][(threadIdx.x >> 6) & 3] = pb;
+          red[(threadIdx.x / QUARTER) * 2][(threadIdx.x >> 6) % QW] = pa;
+          red[(threadIdx.x / QUARTER) * 2 + 1][(threadIdx.x >> 6) % QW] = pb;
         }
       }
       __syncthreads();
@@ -836,7 +873,12 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
         stepped = true;
         double res[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) res[q] = ((red[q][0] + red[q][1]) + red[q][2]) + red[q][3];
+        for (int q = 0; q < 8; ++q) {
+          double x = red[q][0];
+#pragma unroll
+          for (int w = 1; w < QW; ++w) x += red[q][w];
+          res[q] = x;
+        }
         bool conv[2];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
@@ -905,10 +947,10 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
       if (ch > c0) first_row(cr.x, cr.y, row_ptr, e0, e1);
       RowBatch<double> cur;
       first_batch(cr.x, cr.y, nbr_j, hval, e0, e1, cur);
-      Staged sv;
-      stage_load(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
-      chunk_spmv<double, double2, double>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
-      stage_store(cr.x, cr.y, sv, L);
+      Staged<WAVES> sv;
+      stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+      chunk_spmv<WAVES, double, double2, double>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
+      stage_store<WAVES>(cr.x, cr.y, sv, L);
       __syncthreads();
       if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
       if (ch + 1 < c1) __syncthreads();
@@ -924,8 +966,8 @@ __global__ void __launch_bounds__(CG_BLOCK) k_cg_iteration(Systems S, int k, CgV
     if ((threadIdx.x & 63) == 0) {
       double* out = part + (k & 1) * NDOT * nd;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) out[(2 * q + h) * nd + blockIdx.x] = dots[q];
-      if (mode == MODE_START) part_bb[h * nd + blockIdx.x] = dots[3];
+      for (int q = 0; q < 3; ++q) out[(2 * q + h) * nd + b] = dots[q];
+      if (mode == MODE_START) part_bb[h * nd + b] = dots[3];
     }
   }
 }
@@ -1008,13 +1050,19 @@ hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, c
 // (XREAC_CG_MAX_BLOCKS overrides the cap for tuning; it must be the same for
 // every call that is compared bitwise, since a system's blocks depend only on
 // its size and this cap).
-void xr_cg_blocks(int n, int* ndir, int* nupd) {
-  static const int cap = [] {
+bool xr_cg_small(int n) { return n < CG_SMALL_ATOMS; }
+
+void xr_cg_blocks(int n, int* ndir, int* nupd, int* chunk) {
+  static const int cap_env = [] {
     const char* v = std::getenv("XREAC_CG_MAX_BLOCKS");
-    const int c = v ? std::atoi(v) : CG_MAX_BLOCKS;
-    return c > 0 ? c : CG_MAX_BLOCKS;
+    const int c = v ? std::atoi(v) : 0;
+    return c > 0 ? c : 0;
   }();
-  const int chunks = n > 0 ? (n + CHUNK - 1) / CHUNK : 1;
+  const bool small = xr_cg_small(n);
+  const int rows = small ? CHUNK_SMALL : CHUNK;
+  const int cap = cap_env ? cap_env : small ? CG_MAX_BLOCKS_SMALL : CG_MAX_BLOCKS;
+  *chunk = rows;
+  const int chunks = n > 0 ? (n + rows - 1) / rows : 1;
   const int per = (chunks + cap - 1) / cap;
   *nupd = chunks;
   *ndir = (chunks + per - 1) / per;
@@ -1047,16 +1095,28 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   const long long k_limit = (long long)a.max_iter + 3LL * (MAX_RESTARTS + 3) + a.first_check + a.check_every;
   out->aborted = false;
   out->exhausted = false;
+  // iteration blocks [0, nd_small) belong to small systems, the rest to large ones
+  const int nd_small = a.nd_small, nd_large = nd - a.nd_small;
   auto enqueue = [&](int iters) {
     for (int c = 0; c < iters && k < k_limit; ++c) {
       if (mixed) {
-        XR_LAUNCH(XT_CG_ITERATION, k_cg_iteration<true>, dim3(nd), dim3(CG_BLOCK), 0, s, a.S, k, a.v, a.row_ptr,
-                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff, a.rhs, part, part_bb,
-                  nd, tol2, a.max_iter, cg, cnt, a.n);
+        if (nd_small > 0)
+          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<true, CG_BLOCK_SMALL>), dim3(nd_small), dim3(CG_BLOCK_SMALL), 0,
+                    s, a.S, k, a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks,
+                    a.type, a.ff, a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, 0);
+        if (nd_large > 0)
+          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<true, CG_BLOCK>), dim3(nd_large), dim3(CG_BLOCK), 0, s, a.S, k,
+                    a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff,
+                    a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, nd_small);
       } else {
-        XR_LAUNCH(XT_CG_ITERATION, k_cg_iteration<false>, dim3(nd), dim3(CG_BLOCK), 0, s, a.S, k, a.v, a.row_ptr,
-                  a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff, a.rhs, part, part_bb,
-                  nd, tol2, a.max_iter, cg, cnt, a.n);
+        if (nd_small > 0)
+          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<false, CG_BLOCK_SMALL>), dim3(nd_small), dim3(CG_BLOCK_SMALL), 0,
+                    s, a.S, k, a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks,
+                    a.type, a.ff, a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, 0);
+        if (nd_large > 0)
+          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<false, CG_BLOCK>), dim3(nd_large), dim3(CG_BLOCK), 0, s, a.S, k,
+                    a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff,
+                    a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, nd_small);
       }
       ++k;
     }
