@@ -1045,9 +1045,10 @@ hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, c
   return hipSuccess;
 }
 
-// Chunks and iteration blocks per system: ceil(n / CHUNK) chunks, spread over
-// at most CG_MAX_BLOCKS blocks with every block owning at least one chunk
-// (XREAC_CG_MAX_BLOCKS overrides the cap for tuning; it must be the same for
+// Chunks and iteration blocks per system: ceil(n / CHUNK) chunks (CHUNK_SMALL
+// below CG_SMALL_ATOMS atoms), spread over at most CG_MAX_BLOCKS blocks
+// (CG_MAX_BLOCKS_SMALL) with every block owning at least one chunk
+// (XREAC_CG_MAX_BLOCKS overrides both caps for tuning; it must be the same for
 // every call that is compared bitwise, since a system's blocks depend only on
 // its size and this cap).
 bool xr_cg_small(int n) { return n < CG_SMALL_ATOMS; }
