@@ -111,3 +111,54 @@ def charged_cases():
         _, s, x, cell, pbc = small_cell_cases()[name]
         cases["charged_" + name] = (s, x, cell, pbc, -1.0)
     return cases
+
+
+# ------------------------------------------------ closed-form pin geometries
+def modified(ff, general=None, atoms=None, angles=None, torsions=None):
+    """Copy of a ForceField with edited parameters.
+
+    general: {index: value}; atoms: {label: {key: value}}; angles: {(a, b, c): [row, ...]};
+    torsions: {(a, b, c, d): row}.  Rows follow the file's column order.
+    """
+    from copy import deepcopy
+    from dataclasses import replace
+
+    g = ff.general.copy()
+    for k, v in (general or {}).items():
+        g[k] = v
+    at = deepcopy(ff.atoms)
+    for label, kv in (atoms or {}).items():
+        at[label].update(kv)
+    an = dict(ff.angles)
+    for key, rows in (angles or {}).items():
+        an[key] = [np.asarray(r, dtype=float) for r in rows]
+    to = dict(ff.torsions)
+    for key, row in (torsions or {}).items():
+        to[key] = np.asarray(row, dtype=float)
+    return replace(ff, general=g, atoms=at, angles=an, torsions=to)
+
+
+def dihedral_chain(omega, lengths=(1.09, 1.54, 1.09), angles=(125.0, 135.0)):
+    """Four atoms i-j-k-l with bond lengths |ij|, |jk|, |kl|, bond angles ijk and jkl
+    (degrees) and IUPAC dihedral omega (radians; 0 = cis)."""
+    t1, t2 = np.radians(angles)
+    j = np.zeros(3)
+    k = np.array([0.0, 0.0, lengths[1]])
+    i = j + lengths[0] * np.array([np.sin(t1), 0.0, np.cos(t1)])
+    l = k + lengths[2] * np.array([np.sin(t2) * np.cos(omega), np.sin(t2) * np.sin(omega), -np.cos(t2)])
+    return np.array([i, j, k, l])
+
+
+def iupac_dihedral(p):
+    """Signed dihedral of four points (atan2 form, 0 = cis)."""
+    b0, b1, b2 = p[0] - p[1], p[2] - p[1], p[3] - p[2]
+    b1 = b1 / np.linalg.norm(b1)
+    v = b0 - np.dot(b0, b1) * b1
+    w = b2 - np.dot(b2, b1) * b1
+    return np.arctan2(np.dot(np.cross(b1, v), w), np.dot(v, w))
+
+
+def bent_triatomic(theta, lengths=(1.4, 1.4)):
+    """End-centre-end with the given bond lengths and angle (radians) at the centre (atom 1)."""
+    return np.array([[lengths[0], 0.0, 0.0], [0.0, 0.0, 0.0],
+                     [lengths[1] * np.cos(theta), lengths[1] * np.sin(theta), 0.0]])

@@ -65,13 +65,10 @@ def _calc(ffname):
     return _CALCS[ffname]
 
 
-@pytest.mark.parametrize("name", list(SYSTEMS))
-def test_parity(name):
-    ffname, s, x, cell, pbc, q = SYSTEMS[name]
-    print(f"[parity] {name}: n={len(s)}", flush=True)
-    calc = _calc(ffname)
-    gpu = calc.evaluate(s, x, cell=cell, pbc=pbc, total_charge=q)
-    ref = orc.evaluate(calc.force_field, s, x, cell=cell, pbc=pbc, total_charge=q)
+def assert_matches_oracle(calc, s, x, cell=None, pbc=None, q=0.0, full_derivative=False):
+    """One evaluation on the device against the oracle at every bar of the module docstring."""
+    gpu = calc.evaluate(s, x, cell=cell, pbc=pbc, total_charge=q, full_derivative=full_derivative)
+    ref = orc.evaluate(calc.force_field, s, x, cell=cell, pbc=pbc, total_charge=q, full_derivative=full_derivative)
     model = ref["model"]
     # neighbor list: identical canonical (i, j, S)
     eng = calc.engine
@@ -98,6 +95,14 @@ def test_parity(name):
     np.testing.assert_allclose(gpu.bond_orders, ref["bond_orders"], rtol=0, atol=1e-8)
     np.testing.assert_allclose(gpu.dipole, ref["dipole"], rtol=0, atol=1e-7)
     assert abs(gpu.charges.sum() - q) < 1e-10
+    return gpu, ref
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_parity(name):
+    ffname, s, x, cell, pbc, q = SYSTEMS[name]
+    print(f"[parity] {name}: n={len(s)}", flush=True)
+    assert_matches_oracle(_calc(ffname), s, x, cell, pbc, q)
 
 
 def test_golden_monomer_on_gpu():
