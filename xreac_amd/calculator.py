@@ -14,7 +14,7 @@ from numbers import Real
 import numpy as np
 
 from .engine import COMPONENTS, Engine
-from .geometry import Boundary, canonical_neighbors, validate_expansion_limit
+from .geometry import Boundary, canonical_neighbors, fit_device_range, validate_expansion_limit
 
 
 @dataclass(frozen=True)
@@ -103,10 +103,12 @@ class Calculator:
         else:
             repetitions = boundary.repetitions(self.force_field.general[12], len(x), self.max_expanded_atoms)
             backend = "replicated"
+        # whole-cell moves of far atoms and of huge supplied shifts (results unchanged)
+        x_dev, csr = fit_device_range(x, cell, pbc, self.force_field.general[12], csr)
         eng = self.engine
         torch = eng.torch
         types = torch.as_tensor(eng.type_ids(symbols), device=eng.device)
-        xd = torch.as_tensor(x, device=eng.device)
+        xd = torch.as_tensor(x_dev, device=eng.device)
         out = eng.evaluate(
             xd,
             types,

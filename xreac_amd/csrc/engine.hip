@@ -93,6 +93,8 @@ struct xreac_ctx {
   size_t ctl_host_bytes = 0;
   XrProfiler prof;
   int last_cg_launches = 0;
+  int cg_history[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // launches of the last eight solves (0: none yet)
+  int cg_history_next = 0;
   double m_per_atom = 0.0;  // edges and bonds per atom of the last evaluation
   double b_per_atom = 0.0;
   int max_row = 0;  // longest neighbor row of the last built list
@@ -139,6 +141,10 @@ int hip_fail(xreac_ctx* c, hipError_t e, const char* where) {
 
 std::string error_text(unsigned bits) {
   if (bits & XR_ERR_COINCIDENT) return "Coincident atoms are not supported";
+  // a malformed supplied list comes next: it can also trip every later check
+  if (bits & XR_ERR_NEIGHBORS)
+    return "Neighbor atom indices are out of range, an image shift exceeds 511 cells, or the neighbor row "
+           "pointers are malformed";
   if (bits & XR_ERR_SHIFT_RANGE)
     return "Positions lie more than 511 periodic images from the cell; wrap them before evaluating";
   if (bits & XR_ERR_ROW_TOO_LONG) return "More than 2048 neighbors within the cutoff for one atom";
@@ -148,8 +154,6 @@ std::string error_text(unsigned bits) {
   if (bits & XR_ERR_COLLINEAR)
     return "Collinear atoms in an active torsion: the dihedral derivative is undefined; perturb the geometry";
   if (bits & XR_ERR_REVERSE) return "Internal error: bond without a reverse edge";
-  if (bits & XR_ERR_NEIGHBORS)
-    return "Neighbor atom indices are out of range, or an image shift exceeds 511 cells";
   return "Non-finite energy, charges, or forces; check the geometry";
 }
 
@@ -661,8 +665,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, -1, counts, s), "neighbor count");
     m_cap = in.nbr_count;
     HIPCHK(hipMemcpyAsync(row_ptr, in.nbr_row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToDevice, s), "copy");
-    // clamped to the supplied length, so a malformed pointer array cannot reach past the list
-    HIPCHK(xr_clamp_ptr(row_ptr, n, (int)m_cap, dev_counts, d_flags, s), "clamp");
+    // clamped to [0, supplied length], so a malformed pointer array cannot reach outside the list;
+    // a first pointer other than 0 or a decreasing pointer fails the system
+    HIPCHK(xr_clamp_supplied(row_ptr, n, (int)m_cap, dev_counts, d_flags, S, s), "clamp");
   } else {
     const double slack = 1.10;
     m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
@@ -868,8 +873,20 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.state = dc + L.cg;
   cg.tol = opt->qeq_tolerance > 0 ? opt->qeq_tolerance : 1e-10;
   cg.max_iter = opt->qeq_max_iterations > 0 ? opt->qeq_max_iterations : 2000;
-  // Read back convergence first just past the launch the previous solve settled in.
-  cg.first_check = ctx->last_cg_launches > 0 ? ctx->last_cg_launches + 1 : 16;
+  // Read back convergence first just past the launch the previous solve settled in, but no later than
+  // twice the median of the recent solves (and never past 128 launches): one slow solve must not make
+  // the next, easier one queue a long run of idle launches before its first read-back.
+  {
+    int recent[8], count = 0;
+    for (int h : ctx->cg_history)
+      if (h > 0) recent[count++] = h;
+    int first = ctx->last_cg_launches > 0 ? ctx->last_cg_launches + 1 : 16;
+    if (count > 0) {
+      std::sort(recent, recent + count);
+      first = std::min(first, 2 * recent[count / 2] + 1);
+    }
+    cg.first_check = std::max(2, std::min(first, 128));
+  }
   cg.check_every = 2;
   cg.guess = (opt->warm_start && out.qeq_state) ? reinterpret_cast<const double2*>(out.qeq_state) : nullptr;
   cg.keep = reinterpret_cast<double2*>(out.qeq_state);
@@ -997,7 +1014,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
       ctx->sys_messages[q] = error_text(h_serr[q]);
     } else if ((!given_q && h_cg[q].ctl[0].done != CG_DONE) || (full && h_cg2[q].ctl[0].done != CG_DONE)) {
       out.status[q] = XREAC_INVALID;
-      ctx->sys_messages[q] = "QEq conjugate gradient did not converge; check the geometry";
+      // calculator.py:149-150 raises this text when the dense solve fails
+      ctx->sys_messages[q] =
+          "Singular QEq system; check the geometry and parameters (the QEq conjugate gradient did not converge)";
     } else {
       out.status[q] = XREAC_OK;
     }
@@ -1008,6 +1027,10 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   ctx->b_per_atom = (double)nb_total / n;
   if (!supplied) ctx->max_row = host_counts[2];
   ctx->last_cg_launches = launches;
+  if (launches > 0) {
+    ctx->cg_history[ctx->cg_history_next] = launches;
+    ctx->cg_history_next = (ctx->cg_history_next + 1) % 8;
+  }
 
   ctx->n = n;
   ctx->m = m;

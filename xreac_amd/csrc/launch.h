@@ -30,5 +30,10 @@ int xr_fill_stride(int max_row);
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, int* counts, void* slab,
                             int max_row, unsigned* flags, hipStream_t s);
+// Supplied row pointers ptr[0..n] clamped to [0, cap]; a malformed pointer
+// array (first entry not 0, negative or decreasing entries) flags the row's
+// system with XR_ERR_NEIGHBORS.  *total and the capacity bit as xr_clamp_ptr.
+hipError_t xr_clamp_supplied(int* row_ptr, int n, int cap, int* total, unsigned* err, const Systems& S,
+                             hipStream_t s);
 hipError_t xr_pack_supplied(int n, const Systems& S, const int* row_ptr, const int* in_j, const int* in_s,
                             int* nbr_j, int* nbr_s, hipStream_t s);

@@ -432,6 +432,32 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
   for (int t2 = lane; t2 < count; t2 += 64) out[t2] = keys[t2];
 }
 
+// Supplied list: clamp every row pointer to [0, cap], so every later row
+// walk stays inside the edge buffers whatever the caller passed.  A first
+// pointer other than 0, a negative pointer or a pointer below its
+// predecessor flags the row's system (such a row then walks nothing).  The
+// clamp is idempotent, so a thread that reads a neighbour's pointer before or
+// after that neighbour's thread clamped it compares the same two values.
+__global__ void k_clamp_supplied(int* __restrict__ ptr, int n, int cap, int* __restrict__ total,
+                                 unsigned* __restrict__ err, Systems S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int v = ptr[i];
+  const int c = v < 0 ? 0 : (v > cap ? cap : v);
+  bool bad = v < 0 || (i == 0 && v != 0);
+  if (i < n) {
+    const int w = ptr[i + 1];
+    const int cw = w < 0 ? 0 : (w > cap ? cap : w);
+    bad |= cw < c;
+  }
+  if (i == n) {
+    *total = v;
+    if (v > cap) atomicOr(err, (unsigned)XR_ERR_CAPACITY);
+  }
+  if (bad && n > 0) xr_flag(S, i < n ? i : n - 1, XR_ERR_NEIGHBORS);
+  if (c != v) ptr[i] = c;
+}
+
 // Supplied list: copy j and pack the shifts into the device codes, one wave
 // per row.  An entry whose atom lies outside the row's system or whose shift
 // does not fit 10 bits is replaced by the row atom with zero shift (so later
@@ -510,6 +536,13 @@ hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int*
     XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<ROW_CAP>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
               static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags);
   }
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_clamp_supplied(int* row_ptr, int n, int cap, int* total, unsigned* err, const Systems& S,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_clamp_supplied, dim3((n + 1 + 255) / 256), dim3(256), 0, s, row_ptr, n, cap, total, err, S);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

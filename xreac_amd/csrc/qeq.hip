@@ -219,12 +219,13 @@ constexpr int CG_LEAD_LAUNCHES = 4;    // launches queued before the caller's si
 static_assert(CHUNK + RUN_MAX - 1 <= CHUNK_MAX && CHUNK_SMALL + RUN_MAX - 1 <= CHUNK_MAX,
               "a chunk must fit one wave");
 
-// System owning global block b (first-block offsets ascend with the system index).
-__device__ inline int block_system(const Systems& S, int b, bool dir) {
+// System owning global chunk b: chunk offsets (upd0) ascend with the system
+// index.  (Iteration-block offsets do not: small systems' blocks come first.)
+__device__ inline int chunk_system(const Systems& S, int b) {
   int lo = 0, hi = S.ns - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    const int start = dir ? S.sys[mid].dir0 : S.sys[mid].upd0;
+    const int start = S.sys[mid].upd0;
     if (start <= b) lo = mid;
     else hi = mid - 1;
   }
@@ -333,7 +334,7 @@ __global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict
                                                 CgSys* __restrict__ cg,
                                                 CgCounters* __restrict__ cnt, const unsigned* __restrict__ abort_flag,
                                                 int n_all) {
-  const int sys = block_system(S, blockIdx.x, false);
+  const int sys = chunk_system(S, blockIdx.x);
   const SysDev& sd = S.sys[sys];
   const int c = blockIdx.x - sd.upd0;
   const int r0 = chunk_start(sd, join, c);
@@ -844,7 +845,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
         pa = pbase[sd.dir0 + t0];
         pb = pbase[nd + sd.dir0 + t0];
       }
-      for (int t = t0 + QUARTER; t < sd.ndir; t += QUARTER) {  // only with a raised block cap
+      for (int t = t0 + QUARTER; t < sd.ndir; t += QUARTER) {  // systems with more blocks than QUARTER
         pa += pbase[sd.dir0 + t];
         pb += pbase[nd + sd.dir0 + t];
       }

@@ -178,14 +178,28 @@ class Engine:
         )
 
     def _device_csr(self, neighbors, n):
-        """Device int32 copies of a canonical CSR (row_ptr [n+1], j [M], S [M, 3])."""
+        """Device int32 copies of a canonical CSR (row_ptr [n+1], j [M], S [M, 3]).
+
+        The CSR is checked on the host before any kernel runs: row_ptr starts
+        at 0, never decreases and ends at len(j); every j is an atom index; every
+        shift fits the device's 10-bit fields.  (The device clamps the pointers
+        again, so a list passed straight to the C ABI cannot reach outside its
+        buffers either.)
+        """
         torch = self.torch
-        row_ptr, j, S = neighbors
-        t = [torch.as_tensor(np.asarray(a.cpu() if hasattr(a, "cpu") else a, dtype=np.int32), device=self.device)
-             .contiguous() for a in (row_ptr, j, np.reshape(S, (-1, 3)))]
-        if t[0].shape[0] != n + 1 or t[2].shape[0] != t[1].shape[0]:
+        host = [np.asarray(a.cpu() if hasattr(a, "cpu") else a) for a in neighbors]
+        if len(host) != 3:
             raise ValueError("neighbors must be a CSR (row_ptr [n + 1], j [M], S [M, 3])")
-        return t
+        row_ptr, j, S = host[0].reshape(-1), host[1].reshape(-1), host[2].reshape(-1, 3)
+        if row_ptr.shape[0] != n + 1 or S.shape[0] != j.shape[0]:
+            raise ValueError("neighbors must be a CSR (row_ptr [n + 1], j [M], S [M, 3])")
+        if row_ptr[0] != 0 or np.any(np.diff(row_ptr.astype(np.int64)) < 0) or row_ptr[-1] != len(j):
+            raise ValueError("neighbor row pointers must start at 0, never decrease and end at the number of edges")
+        if len(j) and (j.min() < 0 or j.max() >= n):
+            raise ValueError("Neighbor atom indices are out of range")
+        if len(j) and np.max(np.abs(S.astype(np.int64))) > 511:
+            raise ValueError("Neighbor shifts beyond 511 cells are not supported on the device")
+        return [torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=self.device) for a in (row_ptr, j, S)]
 
     @staticmethod
     def _csr_fields(nbr):

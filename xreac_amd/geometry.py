@@ -66,6 +66,7 @@ class Boundary:
 
 
 SHIFT_LIMIT = 511  # image shifts are packed as three 10-bit fields on the device
+WRAP_LIMIT = 256  # positions further than this many cells from the cell are wrapped before the device build
 
 
 def canonical_neighbors(neighbors, atoms, cell=None, pbc=None):
@@ -74,7 +75,10 @@ def canonical_neighbors(neighbors, atoms, cell=None, pbc=None):
     Same checks, order and messages as the reference's ``Neighbors``
     (src/xreac/neighbors.py:101-131, duplicates and both directions from
     ``_edge_order``, neighbors.py:63-98).  Returns ``row_ptr`` int32 (atoms + 1),
-    ``j`` int32 (M,) and ``S`` int32 (M, 3), rows in (i, j, Sx, Sy, Sz) order.
+    ``j`` int32 (M,) and ``S`` int64 (M, 3), rows in (i, j, Sx, Sy, Sz) order;
+    shifts of any size are accepted, as the reference's overflow-safe order
+    does (neighbors.py:63-98).  ``fit_device_range`` then maps them into the
+    device's image range.
     """
     boundary = Boundary(cell, pbc)
     if not isinstance(neighbors, (tuple, list)) or len(neighbors) != 3:
@@ -93,8 +97,6 @@ def canonical_neighbors(neighbors, atoms, cell=None, pbc=None):
     i = i.astype(np.int64)
     j = j.astype(np.int64)
     s = shifts.astype(np.int64)
-    if len(i) and np.max(np.abs(s)) > SHIFT_LIMIT:
-        raise ValueError(f"Neighbor shifts beyond {SHIFT_LIMIT} cells are not supported on the device")
     order = np.lexsort((s[:, 2], s[:, 1], s[:, 0], j, i))
     i, j, s = i[order], j[order], s[order]
     forward = np.column_stack((i, j, s))
@@ -107,4 +109,58 @@ def canonical_neighbors(neighbors, atoms, cell=None, pbc=None):
         raise ValueError("neighbors must include both directions: (i, j, S) and (j, i, -S)")
     row_ptr = np.zeros(atoms + 1, dtype=np.int32)
     np.cumsum(np.bincount(i, minlength=atoms), out=row_ptr[1:])
-    return row_ptr, j.astype(np.int32), s.astype(np.int32)
+    return row_ptr, j.astype(np.int32), s
+
+
+def fit_device_range(x, cell, pbc, cutoff, csr=None):
+    """Positions (and a supplied canonical CSR) inside the device's image range.
+
+    The device packs image shifts into 10-bit fields (|S_k| <= 511) and builds
+    its own lists from positions within 511 cells of the cell.  Moving atom a
+    by whole cells, x'_a = x_a - w_a L, leaves every edge vector unchanged when
+    its shift becomes S' = S + w_j - w_i, so results do not change (up to the
+    rounding of the subtraction).  Positions are wrapped only when some atom
+    lies more than WRAP_LIMIT cells out or a supplied shift exceeds 511; after
+    wrapping, an edge with |S'_k| > 1 is at least (|S'_k| - 1) h_k long (h_k the
+    cell height), and one longer than the cutoff contributes nothing (every
+    term is zero there, energy.py:89-103, 105-165, 248-276), so such edges are
+    dropped with their reverses.  Returns (x, csr) with csr shifts as int32.
+    """
+    boundary = Boundary(cell, pbc)
+    x = np.asarray(x, dtype=np.float64)
+    if csr is None and not boundary.periodic:
+        return x, None
+    if csr is not None and not boundary.periodic:
+        row_ptr, j, s = csr
+        return x, (row_ptr, j, np.asarray(s).astype(np.int32))
+    flags = boundary.pbc
+    frac = x @ boundary.inverse
+    far = np.any(np.abs(frac[:, flags]) > WRAP_LIMIT)
+    big = csr is not None and len(csr[1]) > 0 and int(np.max(np.abs(csr[2]))) > SHIFT_LIMIT
+    if not far and not big:
+        if csr is None:
+            return x, None
+        row_ptr, j, s = csr
+        return x, (row_ptr, j, np.asarray(s).astype(np.int32))
+    w = np.zeros_like(frac)
+    w[:, flags] = np.floor(frac[:, flags])
+    w = w.astype(np.int64)
+    x = x - w @ boundary.cell
+    if csr is None:
+        return x, None
+    row_ptr, j, s = csr
+    rows = np.repeat(np.arange(len(row_ptr) - 1), np.diff(row_ptr))
+    s = np.asarray(s, dtype=np.int64) + w[j] - w[rows]
+    # lower bound of each edge's length after wrapping; fractional offsets lie in (-1, 1)
+    reach = np.max(np.maximum(np.abs(s) - 1 - 1e-9, 0) * boundary.heights, axis=1)
+    keep = reach <= cutoff * (1 + 1e-8)
+    if np.any(np.abs(s[keep]) > SHIFT_LIMIT):
+        raise ValueError(
+            f"Neighbor shifts beyond {SHIFT_LIMIT} cells within the cutoff are not supported on the device"
+        )
+    counts = np.bincount(rows[keep], minlength=len(row_ptr) - 1)
+    new_ptr = np.zeros_like(row_ptr)
+    np.cumsum(counts, out=new_ptr[1:])
+    # within a row and a neighbor j the shift moves by the constant w_j - w_i, so the kept
+    # edges stay in canonical order
+    return x, (new_ptr, j[keep], s[keep].astype(np.int32))
