@@ -177,6 +177,26 @@ const char* xreac_error_message(const xreac_ctx* ctx);
 int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ff, const xreac_system* sys, const xreac_options* opt,
                    xreac_result* out);
 
+/* Bytes of the static device scratch array behind xreac_evaluate_static
+ * (64 MiB per device, held from the library's first use on a device until
+ * the process ends). */
+#define XREAC_STATIC_SCRATCH_BYTES (64u << 20)
+
+/* xreac_evaluate with no allocation of any kind: no counterpart in the
+ * reference.  It is for callers that cannot allocate device or pinned memory
+ * mid-run (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Every workspace comes from a file-scope static device
+ * array of XREAC_STATIC_SCRATCH_BYTES; the side stream, events and pinned
+ * read-back slots are made by xreac_ctx_create.  A system whose workspace
+ * (about 15 KB per atom of liquid water, i.e. up to roughly 4,000 such atoms)
+ * does not fit returns XREAC_NOMEM with a message naming the size.  Results
+ * are bitwise those of xreac_evaluate.  All calls on one device share the
+ * array: calls that overlap in time on one device (two contexts, two host
+ * threads) race; the last call's lists (xreac_copy_neighbors,
+ * xreac_copy_bonds) stay readable until the next call on that device. */
+int xreac_evaluate_static(xreac_ctx* ctx, const xreac_ff* ff, const xreac_system* sys, const xreac_options* opt,
+                          xreac_result* out);
+
 /* The same evaluation over a batch (calculator.py:76-165 per system).  Returns
  * XREAC_OK when the batch ran; each system then has its own status, and a
  * failed system (e.g. coincident atoms) does not affect the others.  Other

@@ -1,0 +1,67 @@
+"""xreac_evaluate_static (no allocation, static device scratch) against xreac_evaluate.
+
+Both entry points are called through the C ABI (ctypes, ``Engine.evaluate``
+with ``no_alloc``) on the same inputs; the static one must give bitwise the
+same results.  Two calls in a row, the second on a smaller system, check that
+the scratch's leftovers from the first call do not leak into the second.
+"""
+
+import numpy as np
+import pytest
+
+from xreac_amd import ForceField, workloads
+from xreac_amd.engine import DeviceError, Engine
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("forces", "charges", "total_bond_orders", "lone_pairs", "bond_counts")
+
+
+def _evaluate(eng, symbols, x, cell, no_alloc, precision="fp64"):
+    import torch
+
+    xd = torch.as_tensor(np.asarray(x, dtype=np.float64), device=eng.device).contiguous()
+    types = torch.as_tensor(eng.type_ids(symbols), device=eng.device)
+    out = eng.outputs(len(symbols), properties=True)
+    pbc = (cell is not None,) * 3
+    r = eng.evaluate(xd, types, cell=cell, pbc=pbc, no_alloc=no_alloc, out=out, precision=precision)
+    arrays = {k: r[k].cpu().numpy().copy() for k in KEYS}
+    return arrays, r
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_static_scratch_matches_evaluate(precision):
+    ff = ForceField.bundled("ffield.reax.HO.2015")
+    eng = Engine(ff)
+    s1, x1, c1 = workloads.water_box(side=8, seed=2)  # 1,536 atoms, periodic
+    s2, x2 = workloads.water_cluster()  # 192 atoms, nonperiodic: the second call is smaller
+    for symbols, x, cell in ((s1, x1, c1), (s2, x2, None)):
+        print(f"[no-alloc] {precision} n={len(symbols)}", flush=True)
+        got, g = _evaluate(eng, symbols, x, cell, True, precision)
+        ref, r = _evaluate(eng, symbols, x, cell, False, precision)
+        np.testing.assert_array_equal(g["components"], r["components"])
+        for k in KEYS:
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+        for k in ("num_edges", "num_bonds", "qeq_iterations"):
+            assert g[k] == r[k], k
+        # the context's last lists are the static call's until the next call
+        again, g2 = _evaluate(eng, symbols, x, cell, True, precision)
+        row_ptr, j, S = eng.neighbors(len(symbols), g2["num_edges"])
+        assert row_ptr[-1] == g2["num_edges"]
+        for k in KEYS:
+            np.testing.assert_array_equal(again[k], ref[k], err_msg=k)
+
+
+def test_static_scratch_rejects_large_systems():
+    ff = ForceField.bundled("ffield.reax.HO.2015")
+    eng = Engine(ff)
+    s, x, c = workloads.water_box(side=16, seed=1)  # 12,288 atoms: about 180 MB of workspace
+    print(f"[no-alloc] oversize n={len(s)}", flush=True)
+    with pytest.raises(DeviceError, match="static scratch"):
+        _evaluate(eng, s, x, c, True)
+    # the context still evaluates, with and without the static scratch
+    s2, x2 = workloads.water_cluster()
+    got, _ = _evaluate(eng, s2, x2, None, True)
+    ref, _ = _evaluate(eng, s2, x2, None, False)
+    for k in KEYS:
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)

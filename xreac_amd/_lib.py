@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libxreac_hi
 XREAC_OK, XREAC_INVALID, XREAC_DEVICE, XREAC_NOMEM = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 NCOMP = 14
+STATIC_SCRATCH_BYTES = 64 << 20  # XREAC_STATIC_SCRATCH_BYTES
 
 EXPORTED = (
     "xreac_ff_create",
@@ -22,6 +23,7 @@ EXPORTED = (
     "xreac_ctx_destroy",
     "xreac_error_message",
     "xreac_evaluate",
+    "xreac_evaluate_static",
     "xreac_evaluate_batch",
     "xreac_batch_error",
     "xreac_verlet",
@@ -162,6 +164,8 @@ def load():
     lib.xreac_error_message.restype = C.c_char_p
     lib.xreac_evaluate.argtypes = [vp, vp, C.POINTER(System), C.POINTER(Options), C.POINTER(Result)]
     lib.xreac_evaluate.restype = C.c_int
+    lib.xreac_evaluate_static.argtypes = [vp, vp, C.POINTER(System), C.POINTER(Options), C.POINTER(Result)]
+    lib.xreac_evaluate_static.restype = C.c_int
     lib.xreac_evaluate_batch.argtypes = [vp, vp, C.POINTER(Batch), C.POINTER(Options), C.POINTER(BatchResult)]
     lib.xreac_evaluate_batch.restype = C.c_int
     lib.xreac_batch_error.argtypes = [vp, C.c_int32]

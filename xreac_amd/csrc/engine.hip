@@ -81,6 +81,23 @@ size_t carve_size(size_t& off, size_t count) {
   return off;
 }
 
+// Sequential carving of the static scratch array for xreac_evaluate_static:
+// each workspace the evaluation needs is taken from the front of what is left.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  size_t wanted = 0;  // bytes the evaluation asked for in total (reported when cap is exceeded)
+};
+
+// Workspace of xreac_evaluate_static.  A host reference to this name is a
+// host-side stand-in, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for the current device (scratch_ptr below).
+static __device__ __align__(256) unsigned char scratch[XREAC_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of scratch on each device, once resolved
+
 }  // namespace
 
 struct xreac_ctx {
@@ -116,6 +133,7 @@ struct xreac_ctx {
   int* bond_j = nullptr;
   int* bond_code = nullptr;
   double* bond_BO = nullptr;
+  void* scratch = nullptr;  // this device's scratch array (resolved at creation)
 };
 
 namespace {
@@ -261,10 +279,35 @@ void xreac_ff_destroy(xreac_ff* ff) {
 
 int xreac_ctx_create(int device, void* stream, xreac_ctx** out) {
   if (!out) return XREAC_INVALID;
+  if (device < 0 || device >= kMaxDevices) return XREAC_INVALID;
   if (hipSetDevice(device) != hipSuccess) return XREAC_DEVICE;
   auto* c = new xreac_ctx();
   c->device = device;
   c->stream = static_cast<hipStream_t>(stream);
+  // Everything an evaluation needs besides device workspace is made here, so
+  // that xreac_evaluate_static can run without allocating: the side stream
+  // (default priority) and its events (ordering between two streams of one
+  // device only: no system-scope fence), the pinned QEq poll slots and a
+  // pinned control-block mirror, and this device's scratch address.
+  bool ok = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&c->bonds_ready, hipEventDisableTiming | hipEventDisableSystemFence) ==
+                hipSuccess &&
+            hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming | hipEventDisableSystemFence) == hipSuccess;
+  for (hipEvent_t& ev : c->poll_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+  ok = ok && hipHostMalloc(reinterpret_cast<void**>(&c->poll_host), 16 * sizeof(int)) == hipSuccess;
+  const size_t ctl_bytes = 64 * 1024;
+  if (ok && hipHostMalloc(reinterpret_cast<void**>(&c->ctl_host), ctl_bytes) == hipSuccess) c->ctl_host_bytes = ctl_bytes;
+  else ok = false;
+  if (ok && !g_scratch_ptr[device]) {
+    void* p = nullptr;
+    ok = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p != nullptr;
+    if (ok) g_scratch_ptr[device] = p;
+  }
+  c->scratch = g_scratch_ptr[device];
+  if (!ok) {
+    xreac_ctx_destroy(c);
+    return XREAC_DEVICE;
+  }
   *out = c;
   return XREAC_OK;
 }
@@ -403,13 +446,34 @@ void bin_geometry(SysDev& sd, const double* height, const double* bounds, double
   g.kappa = kappa * (1.0 + 1e-12);
 }
 
+// Workspace of `need` bytes: the context's grow-only buffer, or the next piece
+// of the arena when the evaluation runs in static scratch.
+hipError_t workspace(Arena* arena, Buffer& buf, size_t need, void** out) {
+  if (!arena) {
+    hipError_t e = buf.ensure(need);
+    *out = buf.ptr;
+    return e;
+  }
+  const size_t off = (arena->used + 255) & ~size_t(255);
+  arena->wanted = off + need;
+  if (off + need > arena->cap) return hipErrorOutOfMemory;
+  *out = arena->base + off;
+  arena->used = off + need;
+  return hipSuccess;
+}
+
 // One attempt.  With exact == false, edge and bond buffers are sized from the
 // previous evaluation's per-atom counts and no mid-pipeline read-back happens;
 // if a count exceeds its buffer, *overflow is set and the caller repeats the
 // evaluation with exact == true (sizes read back before allocation).
 int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out,
-                  bool exact, bool* overflow) {
+                  bool exact, bool* overflow, Arena* arena) {
   *overflow = false;
+  if (arena) {
+    arena->used = 0;
+    arena->wanted = 0;
+    exact = true;  // sizes read back before carving: no hints, no second attempt
+  }
   const int ns = in.ns;
   const int n = in.offsets[ns];
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
@@ -421,11 +485,13 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     explicit ActiveProfiler(XrProfiler* q) : p(q) { xr_active_profiler = q; }
     ~ActiveProfiler() {
       // An early return leaves recorded events unresolved; drop them.
-      p->pending.clear();
-      p->used = 0;
+      if (p) {
+        p->pending.clear();
+        p->used = 0;
+      }
       xr_active_profiler = nullptr;
     }
-  } active(&ctx->prof);
+  } active(arena ? nullptr : &ctx->prof);  // no timer events (which would be created) in static scratch
   const double rc = ffh->cutoff;
   const double bond_cut = std::min(5.0, rc);
   const double rcb = rc * (1 + 1e-8) + 1e-12;
@@ -436,7 +502,23 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
 
   // ------------------------------------------------------- control block
   const CtlLayout L = ctl_layout(ns);
-  HIPCHK(ctx->ctl_buf.ensure(L.total), "control block");
+  // The arena's workspaces are checked against its capacity as they are taken.
+#define XR_WORKSPACE(buf, need, out, what)                                                                    \
+  do {                                                                                                        \
+    hipError_t e_ = workspace(arena, (buf), (need), (out));                                                   \
+    if (e_ != hipSuccess && arena)                                                                            \
+      return fail(ctx, XREAC_NOMEM,                                                                           \
+                  "xreac_evaluate_static: the system needs more than the " +                                 \
+                      std::to_string(XREAC_STATIC_SCRATCH_BYTES >> 20) +                                      \
+                      " MiB of static scratch (at least " + std::to_string(arena->wanted >> 20) +            \
+                      " MiB); evaluate it with xreac_evaluate");                                              \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, what);                                                     \
+  } while (0)
+  void* ctl_dev = nullptr;
+  XR_WORKSPACE(ctx->ctl_buf, L.total, &ctl_dev, "control block");
+  if (arena && ctx->ctl_host_bytes < L.total)
+    return fail(ctx, XREAC_NOMEM, "xreac_evaluate_static: the pinned control block made at context creation is "
+                                  "too small for this call");
   if (ctx->ctl_host_bytes < L.total) {
     if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
     ctx->ctl_host = nullptr;
@@ -445,7 +527,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&ctx->ctl_host), want), "pinned control block");
     ctx->ctl_host_bytes = want;
   }
-  char* dc = static_cast<char*>(ctx->ctl_buf.ptr);
+  char* dc = static_cast<char*>(ctl_dev);
   char* hc = ctx->ctl_host;
   unsigned* d_flags = reinterpret_cast<unsigned*>(dc + L.flags);
   SysDev* hsys = reinterpret_cast<SysDev*>(hc + L.sys);
@@ -517,8 +599,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double>(off, (size_t)NCH * n);  // per-atom energy channels
   carve_size<double2>(off, full ? n : 0);    // charge-response right-hand sides (dE/dq, -1)
   carve_size<double>(off, full ? n : 0);     // charge-response adjoint lambda
-  HIPCHK(ctx->atom_buf.ensure(off + 4096), "atom workspace");
-  Carver cv{static_cast<char*>(ctx->atom_buf.ptr)};
+  void* atom_ws = nullptr;
+  XR_WORKSPACE(ctx->atom_buf, off + 4096, &atom_ws, "atom workspace");
+  Carver cv{static_cast<char*>(atom_ws)};
   int* asys = cv.take<int>(n);
   int* home = cv.take<int>(3 * (size_t)n);
   int* bin = cv.take<int>(n);
@@ -577,8 +660,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<int>(off, nbins + 1);
   carve_size<int>(off, nbins + 1);
   carve_size<int>(off, xr_scan_scratch_ints((int)nbins));
-  HIPCHK(ctx->bin_buf.ensure(off + 4096), "bin workspace");
-  Carver cb{static_cast<char*>(ctx->bin_buf.ptr)};
+  void* bin_ws = nullptr;
+  XR_WORKSPACE(ctx->bin_buf, off + 4096, &bin_ws, "bin workspace");
+  Carver cb{static_cast<char*>(bin_ws)};
   int* bin_count = cb.take<int>(nbins + 1);
   int* bin_start = cb.take<int>(nbins + 1);
   int* bin_fill = cb.take<int>(nbins + 1);
@@ -691,8 +775,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double>(off, m_cap);
   carve_size<float>(off, mixed ? m_cap : 0);
   carve_size<unsigned char>(off, m_cap);
-  HIPCHK(ctx->edge_buf.ensure(off + 4096), "edge workspace");
-  Carver ce{static_cast<char*>(ctx->edge_buf.ptr)};
+  void* edge_ws = nullptr;
+  XR_WORKSPACE(ctx->edge_buf, off + 4096, &edge_ws, "edge workspace");
+  Carver ce{static_cast<char*>(edge_ws)};
   int* nbr_j = ce.take<int>(m_cap);
   int* nbr_s = ce.take<int>(m_cap);
   double* hval = ce.take<double>(m_cap);
@@ -709,8 +794,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     // is repeated exactly.
     const int hint = exact ? max_row : (max_row > 0 ? max_row + 64 : 2048);
     slab_stride = xr_fill_stride(hint);
-    HIPCHK(ctx->slab_buf.ensure(sizeof(unsigned long long) * (size_t)slab_stride * n + 4096), "neighbor slab");
-    slab = ctx->slab_buf.ptr;
+    XR_WORKSPACE(ctx->slab_buf, sizeof(unsigned long long) * (size_t)slab_stride * n + 4096, &slab,
+                 "neighbor slab");
     HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, counts, slab, hint, d_flags, s),
            "neighbor fill");
     // row pointers clamped to the edge buffer, total edges and longest row in one pass
@@ -746,8 +831,10 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double>(off, 3 * (size_t)nbond);               // vec
   carve_size<double>(off, 3 * (size_t)nbond);               // G
   carve_size<double>(off, (size_t)kBondScalars * nbond);    // per-bond scalars
-  HIPCHK(ctx->bond_buf.ensure(off + 4096), "bond workspace");
-  Carver cbd{static_cast<char*>(ctx->bond_buf.ptr)};
+  void* bond_ws = nullptr;
+  XR_WORKSPACE(ctx->bond_buf, off + 4096, &bond_ws, "bond workspace");
+#undef XR_WORKSPACE
+  Carver cbd{static_cast<char*>(bond_ws)};
   Bonds bd{};
   bd.cap = (int)nbond;
   bd.nbp = dev_counts + 1;
@@ -802,14 +889,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // Bond orders, many-body terms and their adjoints depend on positions only;
   // they run on a second stream while QEq runs here, and the force assembly
   // waits for them.  Every return below joins the side stream first.
-  if (!ctx->side) {
-    HIPCHK(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "side stream");
-    // ordering between two streams of one device only: no system-scope fence
-    HIPCHK(hipEventCreateWithFlags(&ctx->bonds_ready, hipEventDisableTiming | hipEventDisableSystemFence), "event");
-    HIPCHK(hipEventCreateWithFlags(&ctx->side_done, hipEventDisableTiming | hipEventDisableSystemFence), "event");
-    for (hipEvent_t& ev : ctx->poll_ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&ctx->poll_host), 16 * sizeof(int)), "pinned poll slots");
-  }
+  if (!ctx->side) return fail(ctx, XREAC_DEVICE, "internal error: context without its side stream");
   hipStream_t s2 = ctx->side;
   HIPCHK(hipEventRecord(ctx->bonds_ready, s), "event");
   // Issued from inside the QEq solve once its first iterations are queued, so
@@ -989,7 +1069,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     if (hipError_t e = tail_back(); e != hipSuccess) return hip_fail(ctx, e, tail_what);
   }
   HIPCHK(hipStreamSynchronize(s), "final sync");
-  ctx->prof.resolve();
+  if (!arena) ctx->prof.resolve();
   const unsigned* h_flags = reinterpret_cast<const unsigned*>(hc + L.flags);
   const int* host_counts = reinterpret_cast<const int*>(h_flags) + 1;
   const int64_t m = host_counts[0], nb_total = host_counts[1];
@@ -1046,10 +1126,11 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   return XREAC_OK;
 }
 
-int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out) {
+int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out,
+              Arena* arena = nullptr) {
   bool overflow = false;
-  int rc = evaluate_once(ctx, ffh, in, opt, out, false, &overflow);
-  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ffh, in, opt, out, true, &overflow);
+  int rc = evaluate_once(ctx, ffh, in, opt, out, false, &overflow, arena);
+  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ffh, in, opt, out, true, &overflow, arena);
   if (rc == XREAC_OK && overflow) return fail(ctx, XREAC_DEVICE, "internal error: buffer sizing");
   return rc;
 }
@@ -1058,8 +1139,12 @@ int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xrea
 
 extern "C" {
 
-int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
-                   xreac_result* out) {
+}  // extern "C"
+
+namespace {
+
+int evaluate_single(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
+                    xreac_result* out, Arena* arena) {
   if (!ctx || !ffh || !sys || !opt || !out) return XREAC_INVALID;
   ctx->message.clear();
   if (sys->n <= 0 || !sys->positions || !sys->types || !out->forces || !out->charges)
@@ -1085,13 +1170,32 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys,
   bo.qeq_state = nullptr;
   xreac_options o = *opt;
   o.warm_start = 0;
-  const int rc = run_batch(ctx, ffh, in, &o, bo);
+  const int rc = run_batch(ctx, ffh, in, &o, bo, arena);
   if (rc != XREAC_OK) return rc;
   out->num_edges = bo.num_edges;
   out->num_bonds = bo.num_bonds;
   out->qeq_iterations = iters;
   if (status != XREAC_OK) return fail(ctx, status, ctx->sys_messages[0]);
   return XREAC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
+                   xreac_result* out) {
+  return evaluate_single(ctx, ffh, sys, opt, out, nullptr);
+}
+
+int xreac_evaluate_static(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_system* sys, const xreac_options* opt,
+                          xreac_result* out) {
+  if (!ctx) return XREAC_INVALID;
+  if (!ctx->scratch) return fail(ctx, XREAC_DEVICE, "xreac_evaluate_static: no static scratch on this device");
+  Arena arena;
+  arena.base = static_cast<char*>(ctx->scratch);
+  arena.cap = XREAC_STATIC_SCRATCH_BYTES;
+  return evaluate_single(ctx, ffh, sys, opt, out, &arena);
 }
 
 int xreac_evaluate_batch(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_batch* b, const xreac_options* opt,

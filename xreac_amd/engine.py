@@ -118,7 +118,7 @@ class Engine:
 
     def evaluate(self, positions, types, cell=None, pbc=(False, False, False), total_charge=0.0, *, tol=1e-10,
                  max_iter=2000, lammps_angles=True, properties=True, out=None, precision="fp64",
-                 full_derivative=False, charges=None, neighbors=None):
+                 full_derivative=False, charges=None, neighbors=None, no_alloc=False):
         """Evaluate one system whose positions (n,3 float64) and types (n int32) are device tensors.
 
         Returns a dict of device tensors (forces, charges and optionally
@@ -127,6 +127,9 @@ class Engine:
         ``outputs``) is filled in place instead of allocating new buffers.
         ``neighbors`` = canonical CSR ``(row_ptr, j, S)`` (host or device
         int32, see ``geometry.canonical_neighbors``) replaces the device build.
+        ``no_alloc`` runs ``xreac_evaluate_static``: the library takes every
+        workspace from its static device scratch and allocates nothing (pass
+        ``out`` to keep the call itself free of PyTorch allocations too).
         """
         torch = self.torch
         n = int(positions.shape[0])
@@ -162,7 +165,8 @@ class Engine:
         res.total_bond_orders = _ptr(tbo) if tbo is not None else None
         res.lone_pairs = _ptr(lp) if lp is not None else None
         res.bond_counts = _ptr(bc) if bc is not None else None
-        status = self.lib.xreac_evaluate(self._ctx, self._ff, C.byref(sysd), C.byref(opt), C.byref(res))
+        entry = self.lib.xreac_evaluate_static if no_alloc else self.lib.xreac_evaluate
+        status = entry(self._ctx, self._ff, C.byref(sysd), C.byref(opt), C.byref(res))
         self._check(status, self._ctx)
         return dict(
             forces=forces,
