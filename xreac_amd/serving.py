@@ -186,12 +186,14 @@ class Server:
     precision: "fp64" or "mixed" QEq; warm_start: start each QEq solve from the
     request's previous (s, t); max_systems: cap on requests per batch (1 gives
     the one-request-at-a-time plain path); budget_bytes / max_batch_atoms bound
-    the active set.
+    the active set; full_derivative: charge-response forces (the exact gradient
+    of the QEq-equilibrated energy, calculator.py:79-86) for every request.
     """
 
     def __init__(self, force_fields=("ffield.reax.HO.2015", "ffield.reax.CHO.2008"), device=0, *,
                  precision="fp64", warm_start=True, tol=1e-10, max_systems=None, max_batch_atoms=65536,
-                 budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, clock=time.perf_counter):
+                 budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, full_derivative=False,
+                 clock=time.perf_counter):
         import torch
 
         self.torch = torch
@@ -210,6 +212,7 @@ class Server:
         self.budget = budget_bytes
         self.relax_tolerance = relax_tolerance
         self.relax_iterations = relax_iterations
+        self.full_derivative = bool(full_derivative)
         self.clock = clock
         self.queue = deque()
         self.active = {k: _Group(torch, eng) for k, eng in self.engines.items()}
@@ -300,7 +303,8 @@ class Server:
                    charges=torch.empty(n, dtype=torch.float64, device=dev),
                    total_bond_orders=None, lone_pairs=None, bond_counts=None, qeq_state=g.QS)
         res = eng.evaluate_batch(g.X, g.T, offsets, g.cells, g.pbcs, g.charges, tol=self.tol,
-                                 precision=self.precision, warm_start=self.warm_start, out=out)
+                                 precision=self.precision, warm_start=self.warm_start,
+                                 full_derivative=self.full_derivative, out=out)
         kin = None
         if g.md.any():
             kin_t = torch.zeros(ns, dtype=torch.float64, device=dev)
