@@ -1,9 +1,11 @@
-"""Write tests/golden/water_box_12288.npz: CPU-oracle results for the 12,288-atom box.
+"""Write tests/golden/water_box_12288.npz: CPU-oracle results for the benchmarked 12,288-atom box.
 
-The geometry is regenerated from its seed (xreac_amd.workloads.water_box(side=16,
-seed=1), no pre-relax), so only results are stored: energy, components, the
-neighbor-list and active-bond-set digests, and charges/forces/bond properties
-on 512 seeded atoms.  Run: python tests/golden/make_water_box_fixture.py
+The geometry is the pre-relaxed box the benchmark times
+(data/water_box_12288_prerelaxed.npz: workloads.water_box(side=16, seed=1)
+after 200 steepest-descent steps), so only results are stored: energy,
+components, the neighbor-list and active-bond-set digests, and
+charges/forces/bond properties on 512 seeded atoms.  Run (CPU, about two
+minutes on eight cores): python tests/golden/make_water_box_fixture.py
 """
 
 import hashlib
@@ -28,7 +30,7 @@ def digest(*arrays):
 
 def main():
     ff = ForceField.bundled("ffield.reax.HO.2015")
-    s, x, cell = workloads.water_box(side=16, seed=1)
+    s, x, cell = workloads.prerelaxed_water_box()
     r = orc.evaluate(ff, s, x, cell=cell, pbc=True)
     model = r["model"]
     active = model._bond_candidates[model._bond_mask]
@@ -50,6 +52,7 @@ def main():
         bond_counts=r["bond_counts"][subset],
         charge_sum=r["charges"].sum(),
         force_sum=r["forces"].sum(axis=0),
+        positions_sha256=hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest(),
     )
     print(out, r["energy"], len(model.i), len(active))
 

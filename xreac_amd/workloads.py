@@ -11,6 +11,8 @@ Every generator takes a seed and uses numpy.random.default_rng.
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 AVOGADRO_PER_A3 = 0.602214076  # g/cm^3 -> amu / A^3 conversion factor
@@ -91,6 +93,18 @@ def water_box(side=16, density=0.997, jitter=0.1, seed=1):
     centers = (grid + 0.5) * a + rng.normal(0.0, jitter, (n, 3))
     x = np.concatenate([mono @ random_rotation(rng).T + c for c in centers])
     return ["O", "H", "H"] * n, x, np.diag([L, L, L])
+
+
+PRERELAXED_BOX = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data",
+                              "water_box_12288_prerelaxed.npz")
+
+
+def prerelaxed_water_box(path=PRERELAXED_BOX):
+    """The 12,288-atom box after its 200-step steepest-descent pre-relax (data/make_water_box_prerelaxed.py):
+    symbols, positions (12,288 x 3) and the cubic cell."""
+    data = np.load(path)
+    x = np.array(data["positions"], dtype=np.float64)
+    return ["O", "H", "H"] * (len(x) // 3), x, np.array(data["cell"], dtype=np.float64)
 
 
 def steepest_descent(evaluate_forces, x, steps=200, gain=0.002, max_step=0.05):
