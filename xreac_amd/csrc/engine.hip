@@ -774,6 +774,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<int>(off, m_cap);
   carve_size<double>(off, m_cap);
   carve_size<float>(off, mixed ? m_cap : 0);
+  // 2-byte CG columns: every global atom index of the batch fits 16 bits
+  const bool narrow = n <= 65536;
+  carve_size<unsigned short>(off, narrow ? m_cap : 0);
   carve_size<unsigned char>(off, m_cap);
   void* edge_ws = nullptr;
   XR_WORKSPACE(ctx->edge_buf, off + 4096, &edge_ws, "edge workspace");
@@ -782,6 +785,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   int* nbr_s = ce.take<int>(m_cap);
   double* hval = ce.take<double>(m_cap);
   float* hval32 = mixed ? ce.take<float>(m_cap) : nullptr;
+  unsigned short* col16 = narrow ? ce.take<unsigned short>(m_cap) : nullptr;
   unsigned char* bflag = ce.take<unsigned char>(m_cap);
   void* slab = nullptr;
   int slab_stride = 0;
@@ -805,8 +809,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   }
 
   // ------------------------------------------------------- edges and bonds
-  HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, slab, slab_stride, nbr_j, nbr_s, bond_cut, hval, hval32,
-                      bflag, bond_count, join, s),
+  HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, slab, slab_stride, nbr_j, nbr_s, col16, bond_cut, hval,
+                      hval32, bflag, bond_count, join, s),
          "edge pass");
   // The bond list (scan of the active-bond counts, compaction) feeds only the
   // bonded terms, so with buffers sized from the last evaluation it is built on
@@ -934,6 +938,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   cg.ff = ff;
   cg.row_ptr = row_ptr;
   cg.nbr_j = nbr_j;
+  cg.col16 = col16;
   cg.hval = hval;
   cg.hval32 = hval32;
   {

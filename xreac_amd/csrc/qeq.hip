@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
                                                    Systems S, FFDev ff, const int* __restrict__ row_ptr,
                                                    const unsigned long long* __restrict__ slab, int slab_stride,
                                                    int* __restrict__ nbr_j, int* __restrict__ nbr_s,
-                                                   double bond_cut, double* __restrict__ hval,
+                                                   unsigned short* __restrict__ col16, double bond_cut, double* __restrict__ hval,
                                                    float* __restrict__ hval32, unsigned char* __restrict__ bflag,
                                                    int* __restrict__ bond_count, unsigned char* __restrict__ join) {
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
@@ -113,6 +113,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
         j = nbr_j[e];
         code = nbr_s[e];
       }
+      if (col16) col16[e] = (unsigned short)j;
       tj = type[j];
       double v[3];
       r = xr_edge(x, cell, i, j, code, v);
@@ -475,14 +476,16 @@ struct RowBatch {
   HV h[8];
 };
 
-template <class HV>
-__device__ inline void load_batch(const int* __restrict__ nbr_j, const HV* __restrict__ hv, int base, int e1, int i,
+// Columns are the 4-byte neighbor indices or, in batches of at most 65,536
+// atoms, their 2-byte copies (IT = unsigned short).
+template <class HV, class IT>
+__device__ inline void load_batch(const IT* __restrict__ cols, const HV* __restrict__ hv, int base, int e1, int i,
                                   int lane, RowBatch<HV>& b) {
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
     const int e = base + u * 64 + lane;
     const bool ok = e < e1;
-    b.j[u] = ok ? nbr_j[e] : i;
+    b.j[u] = ok ? (int)cols[e] : i;
     b.h[u] = ok ? hv[e] : HV(0);
   }
 }
@@ -503,8 +506,8 @@ __device__ inline void gather_sum(const RowBatch<HV>& b, const VT* __restrict__ 
 // are in cur (loaded by the caller for the wave's first row), and the next
 // row's are loaded before the current row's gathers.  Lane 0 leaves each
 // row's pair of sums in out[row - r0].
-template <int WAVES, class HV, class VT, class AT>
-__device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
+template <int WAVES, class HV, class VT, class AT, class IT>
+__device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_ptr, const IT* __restrict__ nbr_j,
                                   const HV* __restrict__ hv, const VT* __restrict__ vec, RowBatch<HV>& cur, int e0,
                                   int e1, double2* out) {
   const int lane = threadIdx.x & 63;
@@ -541,8 +544,8 @@ __device__ inline void first_row(int r0, int r1, const int* __restrict__ row_ptr
   }
 }
 
-template <class HV>
-__device__ inline void first_batch(int r0, int r1, const int* __restrict__ nbr_j, const HV* __restrict__ hv, int e0,
+template <class HV, class IT>
+__device__ inline void first_batch(int r0, int r1, const IT* __restrict__ nbr_j, const HV* __restrict__ hv, int e0,
                                    int e1, RowBatch<HV>& b) {
   const int i = r0 + (threadIdx.x >> 6);
   if (i < r1) load_batch(nbr_j, hv, e0, e1, i, threadIdx.x & 63, b);
@@ -771,9 +774,9 @@ __device__ inline void settle(CgSys* c, CgCounters* cnt, const CgCtl& cur, int s
 // same pass.  Each block of a system owns whole chunks.  The launch covers
 // iteration blocks [base, base + gridDim.x), all of one shape (BLOCK threads,
 // at most 128 VGPRs).
-template <bool MIXED, int BLOCK>
+template <bool MIXED, int BLOCK, class IT>
 __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgVec v, const int* __restrict__ row_ptr,
-                                                           const int* __restrict__ nbr_j,
+                                                           const IT* __restrict__ nbr_j,
                                                            const double* __restrict__ hval,
                                                            const float* __restrict__ hval32,
                                                            const double* __restrict__ eta,
@@ -859,7 +862,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
         first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, cur);
         stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
       }
-      chunk_spmv<WAVES, HV, VT, AT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
+      chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
       if (!stepped) {
         pa = xr_wave_sum(pa);
@@ -950,7 +953,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       first_batch(cr.x, cr.y, nbr_j, hval, e0, e1, cur);
       Staged<WAVES> sv;
       stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
-      chunk_spmv<WAVES, double, double2, double>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
+      chunk_spmv<WAVES, double, double2, double, IT>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
       __syncthreads();
       if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
@@ -1027,12 +1030,13 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
 }  // namespace
 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
-                        const void* slab, int slab_stride, int* nbr_j, int* nbr_s, double bond_cut, double* hval,
-                        float* hval32, unsigned char* bflag, int* bond_count, unsigned char* join, hipStream_t s) {
+                        const void* slab, int slab_stride, int* nbr_j, int* nbr_s, unsigned short* col16,
+                        double bond_cut, double* hval, float* hval32, unsigned char* bflag, int* bond_count,
+                        unsigned char* join, hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
   XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr,
-            static_cast<const unsigned long long*>(slab), slab_stride, nbr_j, nbr_s, bond_cut, hval, hval32, bflag,
-            bond_count, join);
+            static_cast<const unsigned long long*>(slab), slab_stride, nbr_j, nbr_s, col16, bond_cut, hval, hval32,
+            bflag, bond_count, join);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
@@ -1079,6 +1083,21 @@ size_t xr_cg_block_doubles(int n) { return (size_t)n * BLOCK_MAX; }
 
 size_t xr_cg_state_bytes(int ns) { return sizeof(CgSys) * (size_t)ns + sizeof(CgCounters) + 64; }
 
+// One launch of iteration k over `blocks` iteration blocks of one shape,
+// starting at block `base`; columns from the 2-byte copy when there is one.
+template <bool MIXED, int BLOCK>
+void cg_launch(const CgArgs& a, int k, double* part, double* part_bb, int nd, double tol2, CgSys* cg,
+               CgCounters* cnt, int blocks, int base, hipStream_t s) {
+  if (a.col16)
+    XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<MIXED, BLOCK, unsigned short>), dim3(blocks), dim3(BLOCK), 0, s, a.S,
+              k, a.v, a.row_ptr, a.col16, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff,
+              a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, base);
+  else
+    XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<MIXED, BLOCK, int>), dim3(blocks), dim3(BLOCK), 0, s, a.S, k, a.v,
+              a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff, a.rhs,
+              part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, base);
+}
+
 hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   const int nu = a.nu, nd = a.nd, ns = a.S.ns;
   double* part = a.partials;
@@ -1102,23 +1121,11 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   auto enqueue = [&](int iters) {
     for (int c = 0; c < iters && k < k_limit; ++c) {
       if (mixed) {
-        if (nd_small > 0)
-          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<true, CG_BLOCK_SMALL>), dim3(nd_small), dim3(CG_BLOCK_SMALL), 0,
-                    s, a.S, k, a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks,
-                    a.type, a.ff, a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, 0);
-        if (nd_large > 0)
-          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<true, CG_BLOCK>), dim3(nd_large), dim3(CG_BLOCK), 0, s, a.S, k,
-                    a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff,
-                    a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, nd_small);
+        if (nd_small > 0) cg_launch<true, CG_BLOCK_SMALL>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
+        if (nd_large > 0) cg_launch<true, CG_BLOCK>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_large, nd_small, s);
       } else {
-        if (nd_small > 0)
-          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<false, CG_BLOCK_SMALL>), dim3(nd_small), dim3(CG_BLOCK_SMALL), 0,
-                    s, a.S, k, a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks,
-                    a.type, a.ff, a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, 0);
-        if (nd_large > 0)
-          XR_LAUNCH(XT_CG_ITERATION, (k_cg_iteration<false, CG_BLOCK>), dim3(nd_large), dim3(CG_BLOCK), 0, s, a.S, k,
-                    a.v, a.row_ptr, a.nbr_j, a.hval, a.hval32, a.eta, a.minv, a.binfo, a.chunks, a.blocks, a.type, a.ff,
-                    a.rhs, part, part_bb, nd, tol2, a.max_iter, cg, cnt, a.n, nd_small);
+        if (nd_small > 0) cg_launch<false, CG_BLOCK_SMALL>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
+        if (nd_large > 0) cg_launch<false, CG_BLOCK>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_large, nd_small, s);
       }
       ++k;
     }
