@@ -28,6 +28,7 @@ enum XrErr : unsigned {
   XR_ERR_NONFINITE = 1u << 8,
   XR_ERR_CAPACITY = 1u << 9,  // a buffer sized from the previous call was too small
   XR_ERR_NEIGHBORS = 1u << 10,  // a supplied neighbor entry points outside its system or its shift is too large
+  XR_ERR_POSITIONS = 1u << 11,  // non-finite coordinates along a nonperiodic direction
 };
 
 // ------------------------------------------------------- parameter layouts
@@ -92,6 +93,7 @@ struct SysDev {
   BinGeom g;
   int atom0, n;
   int bin0;
+  int bin_budget;  // bins reserved at bin0; nonperiodic directions are fitted into it on the device
   int dir0, ndir;
   int upd0, nupd;
   int chunk;  // nominal rows per CG chunk (by system size)

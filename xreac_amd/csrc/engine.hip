@@ -158,6 +158,7 @@ int hip_fail(xreac_ctx* c, hipError_t e, const char* where) {
   } while (0)
 
 std::string error_text(unsigned bits) {
+  if (bits & XR_ERR_POSITIONS) return "positions must be a finite (N, 3) array in Angstrom";
   if (bits & XR_ERR_COINCIDENT) return "Coincident atoms are not supported";
   // a malformed supplied list comes next: it can also trip every later check
   if (bits & XR_ERR_NEIGHBORS)
@@ -397,7 +398,9 @@ __global__ void k_fill_systems(Systems S, int* __restrict__ asys) {
 }
 
 // Bin grid of one system (rc/2-thick bins, searched two deep when periodic).
-void bin_geometry(SysDev& sd, const double* height, const double* bounds, double rcb) {
+// The nonperiodic directions depend on the coordinate bounds and are fitted
+// on the device (xr_bin_grid); here they get placeholders.
+void bin_geometry(SysDev& sd, const double* height, double rcb) {
   BinGeom& g = sd.g;
   for (int k = 0; k < 3; ++k) {
     g.pbc[k] = sd.cell.pbc[k];
@@ -410,17 +413,10 @@ void bin_geometry(SysDev& sd, const double* height, const double* bounds, double
       const double h = height[k] / nbk;
       g.reach[k] = h >= rcb ? 1 : (int)std::floor(rcb / h) + 1;
     } else {
-      double w = 0.5 * rcb / height[k];
-      const double span = bounds[3 + k] - bounds[k];
-      long long nbk = (long long)std::floor(span / w) + 1;
-      if (nbk > 128) {
-        w = span / 127.0;
-        nbk = 128;
-      }
-      g.nb[k] = (int)nbk;
-      g.w[k] = w;
-      g.lo[k] = bounds[k];
-      g.reach[k] = (int)std::floor(rcb / (w * height[k])) + 1;
+      g.nb[k] = 1;
+      g.w[k] = 1.0;
+      g.lo[k] = 0.0;
+      g.reach[k] = 1;
     }
     g.thick[k] = g.w[k] * height[k];
   }
@@ -631,26 +627,23 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   Systems S{reinterpret_cast<const SysDev*>(dc + L.sys), asys, reinterpret_cast<unsigned*>(dc + L.serr), ns};
 
   // ------------------------------------------------------- bin grids
-  if (need_bounds) {
-    HIPCHK(hipMemcpyAsync(dc + L.sys, hc + L.sys, sizeof(SysDev) * ns, hipMemcpyHostToDevice, s), "system table");
-    HIPCHK(xr_coord_bounds(S, in.x, reinterpret_cast<double*>(dc + L.bounds), s), "bounds");
-    HIPCHK(hipMemcpyAsync(hc + L.bounds, dc + L.bounds, sizeof(double) * 6 * ns, hipMemcpyDeviceToHost, s),
-           "bounds copy");
-    HIPCHK(hipStreamSynchronize(s), "bounds sync");
-  }
-  const double* hbounds = reinterpret_cast<const double*>(hc + L.bounds);
+  // Periodic directions are fixed by the cell.  A system with a nonperiodic
+  // direction gets a budget of bins (4 per atom + 64, whole multiples of its
+  // periodic bin count), and its grid is fitted to its coordinate bounds on
+  // the device, so the host never waits for the bounds.
   long long nbins = 0;
   for (int q = 0; q < ns; ++q) {
     SysDev& sd = hsys[q];
-    const bool all_pbc = sd.cell.pbc[0] && sd.cell.pbc[1] && sd.cell.pbc[2];
-    if (!all_pbc) {
-      for (int k = 0; k < 6; ++k)
-        if (!std::isfinite(hbounds[6 * q + k]))
-          return fail(ctx, XREAC_INVALID, "positions must be a finite (N, 3) array in Angstrom");
+    bin_geometry(sd, &heights[3 * (size_t)q], rcb);
+    const long long periodic = (long long)sd.g.nb[0] * sd.g.nb[1] * sd.g.nb[2];  // placeholders count 1
+    long long budget = periodic;
+    if (!(sd.cell.pbc[0] && sd.cell.pbc[1] && sd.cell.pbc[2])) {
+      const long long np = std::max(1LL, std::min(128LL * 128 * 128, (4LL * sd.n + 64) / periodic));
+      budget = periodic * np;
     }
-    bin_geometry(sd, &heights[3 * (size_t)q], hbounds + 6 * (size_t)q, rcb);
     sd.bin0 = (int)nbins;
-    nbins += (long long)sd.g.nb[0] * sd.g.nb[1] * sd.g.nb[2];
+    sd.bin_budget = (int)std::min<long long>(budget, INT32_MAX / 2);
+    nbins += budget;
     if (nbins > INT32_MAX / 2) return fail(ctx, XREAC_INVALID, "too many cell-list bins in one batch");
   }
   HIPCHK(hipMemcpyAsync(dc + L.sys, hc + L.sys, sizeof(SysDev) * ns, hipMemcpyHostToDevice, s), "system table");
@@ -731,6 +724,12 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   }
   hipLaunchKernelGGL(k_fill_systems, dim3(ns), dim3(256), 0, s, S, asys);
   HIPCHK(hipGetLastError(), "system index");
+  if (need_bounds) {
+    HIPCHK(xr_coord_bounds(S, in.x, reinterpret_cast<double*>(dc + L.bounds), s), "bounds");
+    HIPCHK(xr_bin_grid(reinterpret_cast<SysDev*>(dc + L.sys), ns, reinterpret_cast<const double*>(dc + L.bounds),
+                       rcb, reinterpret_cast<unsigned*>(dc + L.serr), s),
+           "bin grid");
+  }
   HIPCHK(xr_bin_atoms(n, in.x, S, home, bin, bin_count, s), "bin");
   HIPCHK(xr_exclusive_scan(bin_count, (int)nbins, bin_start, bin_scan, s), "bin scan");
   HIPCHK(xr_bin_scatter(n, in.x, S, home, bin, bin_start, bin_fill, binned, s), "bin scatter");

@@ -16,6 +16,8 @@ hipError_t xr_clamp_ptr(int* ptr, int n, int cap, int* total, unsigned* err, hip
 
 // neighbor.hip
 hipError_t xr_coord_bounds(const Systems& S, const double* x, double* bounds, hipStream_t s);
+// Nonperiodic directions' bin grids from the bounds, in the device system table (no host read-back).
+hipError_t xr_bin_grid(SysDev* sys, int ns, const double* bounds, double rcb, unsigned* err, hipStream_t s);
 hipError_t xr_bin_atoms(int n, const double* x, const Systems& S, int* home, int* bin, int* bin_count,
                         hipStream_t s);
 size_t xr_bin_record_bytes();

@@ -70,6 +70,73 @@ __global__ void __launch_bounds__(256) k_coord_bounds(Systems S, const double* _
   if (threadIdx.x < 6) out[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// Bin grid along the nonperiodic directions of each system, from its
+// coordinate bounds (fractional units of its cell), one thread per system:
+// bins at least rc/2 thick (at most 128 per direction), widened by steps of
+// 1.25 until the system's bins fit the budget reserved for it on the host
+// (the periodic directions' counts are fixed there).  Non-finite bounds flag
+// the system and leave one bin per direction, so later kernels stay in bounds.
+__global__ void k_bin_grid(SysDev* __restrict__ sys, int ns, const double* __restrict__ bounds, double rcb,
+                           unsigned* __restrict__ err) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ns) return;
+  SysDev& sd = sys[q];
+  BinGeom& g = sd.g;
+  if (g.pbc[0] && g.pbc[1] && g.pbc[2]) return;
+  const double* b = bounds + 6 * q;
+  double height[3], span[3], w[3];
+  long long nb[3];
+  int periodic = 1;
+  bool finite = true;
+  for (int k = 0; k < 3; ++k) {
+    const double c0 = sd.cell.inv[k], c1 = sd.cell.inv[3 + k], c2 = sd.cell.inv[6 + k];
+    height[k] = 1.0 / sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    if (g.pbc[k]) {
+      periodic *= g.nb[k];
+      continue;
+    }
+    span[k] = b[3 + k] - b[k];
+    finite = finite && isfinite(span[k]) && isfinite(b[k]);
+  }
+  if (!finite) {
+    atomicOr(&err[q], (unsigned)XR_ERR_POSITIONS);
+    for (int k = 0; k < 3; ++k) {
+      if (g.pbc[k]) continue;
+      g.nb[k] = 1;
+      g.w[k] = 1.0;
+      g.lo[k] = 0.0;
+      g.reach[k] = 1;
+      g.thick[k] = height[k];
+    }
+    return;
+  }
+  const long long budget = sd.bin_budget / periodic > 0 ? sd.bin_budget / periodic : 1;
+  double scale = 1.0;
+  for (int round = 0; round < 200; ++round) {
+    long long total = 1;
+    for (int k = 0; k < 3; ++k) {
+      if (g.pbc[k]) continue;
+      w[k] = 0.5 * rcb / height[k] * scale;
+      nb[k] = (long long)floor(span[k] / w[k]) + 1;
+      if (nb[k] > 128) {
+        w[k] = span[k] / 127.0;
+        nb[k] = 128;
+      }
+      total *= nb[k];
+    }
+    if (total <= budget) break;
+    scale *= 1.25;
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (g.pbc[k]) continue;
+    g.nb[k] = (int)nb[k];
+    g.w[k] = w[k];
+    g.lo[k] = b[k];
+    g.reach[k] = (int)floor(rcb / (w[k] * height[k])) + 1;
+    g.thick[k] = w[k] * height[k];
+  }
+}
+
 __global__ void k_bin_atoms(int n, const double* __restrict__ x, Systems S, int* __restrict__ home,
                             int* __restrict__ bin, int* __restrict__ bin_count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -487,6 +554,12 @@ __global__ void __launch_bounds__(256) k_pack_supplied(int n, Systems S, const i
 
 hipError_t xr_coord_bounds(const Systems& S, const double* x, double* bounds, hipStream_t s) {
   hipLaunchKernelGGL(k_coord_bounds, dim3(S.ns), dim3(256), 0, s, S, x, bounds);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_bin_grid(SysDev* sys, int ns, const double* bounds, double rcb, unsigned* err, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_grid, dim3((ns + 63) / 64), dim3(64), 0, s, sys, ns, bounds, rcb, err);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
