@@ -21,6 +21,7 @@ form of that call.
 
 from __future__ import annotations
 
+import threading
 import time
 from collections import deque
 from concurrent.futures import ThreadPoolExecutor
@@ -214,8 +215,10 @@ class Server:
         self.relax_iterations = relax_iterations
         self.full_derivative = bool(full_derivative)
         self.clock = clock
-        self.queue = deque()
+        self.queues = {k: deque() for k in self.engines}  # per force field, in arrival order
         self.active = {k: _Group(torch, eng) for k, eng in self.engines.items()}
+        self._lock = threading.Lock()  # queues, budget and results, shared by the groups' loops
+        self._closed = False
         self.used_bytes = 0
         self.responses = []
         self.atom_evals = 0
@@ -228,58 +231,62 @@ class Server:
         n = len(req["symbols"])
         periodic = req.get("cell") is not None and any(req.get("pbc", (True, True, True)))
         job = _Job(req, n, periodic, footprint(n, periodic), self.clock() if arrival is None else arrival)
-        self.queue.append(job)
+        with self._lock:
+            self.queues[req["force_field"]].append(job)
 
-    def _admit(self):
+    @property
+    def queue(self):
+        """Requests waiting for admission, all force fields."""
+        return [job for q in self.queues.values() for job in q]
+
+    def _admit(self, name):
+        """Admit queued requests of force field ``name`` in arrival order while the shared budget allows
+        (the caller holds the lock); a group with nothing active always takes its next request."""
+        group = self.active[name]
+        queue = self.queues[name]
         active = sum(len(g.jobs) for g in self.active.values())
         atoms = sum(int(g.sizes.sum()) for g in self.active.values())
-        admitted = {}
-        while self.queue:
-            job = self.queue[0]
+        admitted = []
+        while queue:
+            job = queue[0]
             if self.max_systems is not None and active >= self.max_systems:
                 break
-            if active and (self.used_bytes + job.bytes > self.budget or atoms + job.n > self.max_batch_atoms):
+            if (group.jobs or admitted) and (self.used_bytes + job.bytes > self.budget or
+                                              atoms + job.n > self.max_batch_atoms):
                 break
-            self.queue.popleft()
-            admitted.setdefault(job.req["force_field"], []).append(job)
+            queue.popleft()
+            admitted.append(job)
             self.used_bytes += job.bytes
             active += 1
             atoms += job.n
-        for name, jobs in admitted.items():
-            group = self.active[name]
-            with self.torch.cuda.stream(group.eng.stream):  # the group's tensors live on its stream
-                group.append(jobs, self._ff[name])
+        return admitted
 
-    def step(self):
-        """Admit what fits, then evaluate every active request once.
+    def _loop(self, name):
+        """One force field's serving loop: admit, evaluate its batch once, hand back finished requests.
 
-        The force fields' groups run on their own streams from a thread each
-        (the library releases the interpreter lock while it works), so one
-        group's host work overlaps the other's kernels; results are merged in
-        a fixed order.
+        The groups' loops run side by side on their own streams, each at its
+        own pace (the library releases the interpreter lock while it works),
+        so one group never waits at the other's step boundary.
         """
-        self._admit()
-        groups = [g for g in self.active.values() if g.jobs]
-        futures = [self._pool.submit(self._run_group, g) for g in groups]
-        # while the groups run, build the host arrays of the requests likely
-        # to be admitted next
-        self._prepare_ahead()
-        results = [f.result() for f in futures]
-        for evals, responses, freed in results:
-            self.atom_evals += evals
-            self.responses.extend(responses)
-            self.used_bytes -= freed
-        self.steps += 1
-
-    def _prepare_ahead(self):
-        atoms = 0
-        for job in self.queue:
-            if atoms >= self.max_batch_atoms:
-                break
-            if job.prep is None:
-                name = job.req["force_field"]
-                self.active[name].prepare(job, self._ff[name])
-            atoms += job.n
+        group = self.active[name]
+        while True:
+            with self._lock:
+                admitted = self._admit(name)
+                closed = self._closed and not self.queues[name]
+            if admitted:
+                with self.torch.cuda.stream(group.eng.stream):  # the group's tensors live on its stream
+                    group.append(admitted, self._ff[name])
+            if not group.jobs:
+                if closed:
+                    return
+                time.sleep(2e-4)
+                continue
+            evals, responses, freed = self._run_group(group)
+            with self._lock:
+                self.atom_evals += evals
+                self.responses.extend(responses)
+                self.used_bytes -= freed
+                self.steps += 1
 
     def _run_group(self, g):
         """One step of a group on its engine's stream; returns (atom evaluations, finished responses, bytes freed)."""
@@ -356,26 +363,32 @@ class Server:
 
     # ------------------------------------------------------------------- run
     def idle(self):
-        return not self.queue and not any(g.jobs for g in self.active.values())
+        return not any(self.queues.values()) and not any(g.jobs for g in self.active.values())
 
     def run(self, requests, arrivals=None):
         """Serve ``requests`` arriving at ``arrivals`` (seconds from now; all at once if None).
 
-        Returns the responses in completion order.
+        Returns the responses in completion order; ``steps`` counts the batched
+        evaluations of all groups.
         """
         t0 = self.clock()
         order = np.argsort(arrivals, kind="stable") if arrivals is not None else np.arange(len(requests))
         times = np.zeros(len(requests)) if arrivals is None else np.asarray(arrivals, dtype=float)
+        with self._lock:
+            self._closed = False
+        loops = [self._pool.submit(self._loop, name) for name in self.active]
         i = 0
-        while i < len(order) or not self.idle():
-            now = self.clock() - t0
-            while i < len(order) and times[order[i]] <= now:
-                self.submit(requests[order[i]], arrival=t0 + times[order[i]])
-                i += 1
-            if self.idle():
-                wait = times[order[i]] - (self.clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
-                continue
-            self.step()
+        try:
+            while i < len(order):
+                now = self.clock() - t0
+                while i < len(order) and times[order[i]] <= now:
+                    self.submit(requests[order[i]], arrival=t0 + times[order[i]])
+                    i += 1
+                if i < len(order):
+                    time.sleep(min(max(times[order[i]] - (self.clock() - t0), 0.0), 0.001))
+        finally:
+            with self._lock:
+                self._closed = True
+            for f in loops:
+                f.result()
         return self.responses
