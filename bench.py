@@ -483,7 +483,7 @@ def main():
     ap.add_argument("--no-systems", action="store_true", help="skip the CHO mixture and the anatase slab")
     ap.add_argument("--only-box", action="store_true", help="measure only the 12,288-atom box")
     ap.add_argument("--stream-requests", type=int, default=20000)
-    ap.add_argument("--stream-batch-atoms", type=int, default=65536)
+    ap.add_argument("--stream-batch-atoms", type=int, default=262144)
     ap.add_argument("--stream-loads", type=float, nargs="+", default=[0.5, 0.7, 0.9],
                     help="Poisson rates as fractions of the saturated request rate")
     ap.add_argument("--poisson-requests", type=int, default=5000)

@@ -120,6 +120,35 @@ def test_batch_with_both_cg_block_shapes(precision):
         assert res["qeq_iterations"][k] == a["qeq_iterations"], system[0]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_beyond_16_bit_atom_indices(precision):
+    # 67,854 atoms: global atom indices no longer fit 16 bits, but every
+    # system's do, so the CG sweeps read 2-byte columns relative to each
+    # system's first atom; systems placed past atom 65,536 must still give
+    # bitwise their results alone.
+    eng = _engine()
+    systems = []
+    for side, seed in ((21, 8), (16, 10), (21, 9)):
+        s, x, cell = workloads.water_box(side=side, seed=seed)
+        systems.append((f"box{len(s)}_{seed}", s, x, cell, (True, True, True), 0.0))
+    systems.append(SYSTEMS[-1])
+    x, t, off, cells, pbcs, qs = _pack(eng, systems)
+    assert off[-1] > 65536 and off[2] < 65536 < off[3]
+    print(f"[batch] {off[-1]} atoms, {precision}", flush=True)
+    res = eng.evaluate_batch(x, t, off, cells, pbcs, qs, tol=1e-12, precision=precision)
+    assert all(e is None for e in res["errors"]), res["errors"]
+    forces = res["forces"].cpu().numpy()
+    charges = res["charges"].cpu().numpy()
+    for k, system in enumerate(systems):
+        print(f"[batch] alone: {system[0]}", flush=True)
+        a = _alone(eng, system, precision)
+        lo, hi = off[k], off[k + 1]
+        np.testing.assert_array_equal(res["components"][k], a["components"], err_msg=system[0])
+        np.testing.assert_array_equal(forces[lo:hi], a["forces"].cpu().numpy(), err_msg=system[0])
+        np.testing.assert_array_equal(charges[lo:hi], a["charges"].cpu().numpy(), err_msg=system[0])
+        assert res["qeq_iterations"][k] == a["qeq_iterations"], system[0]
+
+
 def test_batch_failure_stays_with_its_system():
     eng = _engine()
     bad = ("coincident", ["O", "H"], [[0.0, 0, 0], [12.0, 0, 0]], np.diag([12.0] * 3), (True, True, True), 0.0)

@@ -528,7 +528,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   unsigned* d_flags = reinterpret_cast<unsigned*>(dc + L.flags);
   SysDev* hsys = reinterpret_cast<SysDev*>(hc + L.sys);
   std::vector<double> heights(3 * (size_t)ns);
-  int nd = 0, nu = 0, nd_small = 0;
+  int nd = 0, nu = 0, nd_small = 0, max_sys = 0;
   bool need_bounds = false;
   for (int q = 0; q < ns; ++q) {
     SysDev sd;
@@ -553,6 +553,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
       heights[3 * q + k] = 1.0 / std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
       need_bounds |= !cell.pbc[k];
     }
+    max_sys = std::max(max_sys, sd.n);
     xr_cg_blocks(sd.n, &sd.ndir, &sd.nupd, &sd.chunk);
     sd.upd0 = nu;
     nd += sd.ndir;
@@ -773,8 +774,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<int>(off, m_cap);
   carve_size<double>(off, m_cap);
   carve_size<float>(off, mixed ? m_cap : 0);
-  // 2-byte CG columns: every global atom index of the batch fits 16 bits
-  const bool narrow = n <= 65536;
+  // 2-byte CG columns (relative to the system's first atom): every system has at most 65,536 atoms
+  const bool narrow = max_sys <= 65536;
   carve_size<unsigned short>(off, narrow ? m_cap : 0);
   carve_size<unsigned char>(off, m_cap);
   void* edge_ws = nullptr;

@@ -52,7 +52,7 @@ struct CgArgs {
   FFDev ff;
   const int* row_ptr;
   const int* nbr_j;
-  const unsigned short* col16;  // 2-byte copy of nbr_j (batches of at most 65,536 atoms) or null
+  const unsigned short* col16;  // nbr_j minus the system's first atom, 2 bytes (systems of at most 65,536 atoms) or null
   const double* hval;
   const float* hval32;  // non-null: mixed precision (FP32 A m sweeps, FP64 restarts)
   CgVec v;
@@ -100,7 +100,8 @@ struct CgOutcome {
 
 // slab: the fill's sorted rows (row i at slab[i * slab_stride]), moved to nbr_j / nbr_s on the way; null
 // when nbr_j / nbr_s already hold the rows (a supplied list).  col16 (or null): receives the
-// neighbor indices as 2-byte columns for the CG sweeps (n <= 65,536).
+// neighbor indices relative to the system's first atom as 2-byte columns for the CG sweeps (every
+// system of at most 65,536 atoms).
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const void* slab, int slab_stride, int* nbr_j, int* nbr_s, unsigned short* col16,
                         double bond_cut, double* hval, float* hval32, unsigned char* bflag, int* bond_count,

@@ -61,7 +61,8 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
-  const CellDev& cell = S.sys[__builtin_amdgcn_readfirstlane(S.asys[i])].cell;
+  const SysDev& sd = S.sys[__builtin_amdgcn_readfirstlane(S.asys[i])];
+  const CellDev& cell = sd.cell;
   const int ti = type[i];
   const double rc = ff.gen[12];
   const double cut = 0.01 * ff.gen[29];
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
         j = nbr_j[e];
         code = nbr_s[e];
       }
-      if (col16) col16[e] = (unsigned short)j;
+      if (col16) col16[e] = (unsigned short)(j - sd.atom0);  // system-relative
       tj = type[j];
       double v[3];
       r = xr_edge(x, cell, i, j, code, v);
@@ -508,9 +509,10 @@ __device__ inline void gather_sum(const RowBatch<HV>& b, const VT* __restrict__ 
 // row's pair of sums in out[row - r0].
 template <int WAVES, class HV, class VT, class AT, class IT>
 __device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_ptr, const IT* __restrict__ nbr_j,
-                                  const HV* __restrict__ hv, const VT* __restrict__ vec, RowBatch<HV>& cur, int e0,
-                                  int e1, double2* out) {
+                                  const HV* __restrict__ hv, const VT* __restrict__ vec, int off, RowBatch<HV>& cur,
+                                  int e0, int e1, double2* out) {
   const int lane = threadIdx.x & 63;
+  vec += off;  // columns are relative to off (the system's first atom for 2-byte columns, else 0)
   for (int i = r0 + (threadIdx.x >> 6); i < r1; i += WAVES) {
     const int inext = i + WAVES;
     int n0 = 0, n1 = 0;
@@ -518,12 +520,12 @@ __device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_pt
     if (inext < r1) {
       n0 = row_ptr[inext];
       n1 = row_ptr[inext + 1];
-      load_batch(nbr_j, hv, n0, n1, inext, lane, next);
+      load_batch(nbr_j, hv, n0, n1, inext - off, lane, next);
     }
     AT as = AT(0), at = AT(0);
     gather_sum(cur, vec, as, at);
     for (int base = e0 + 512; base < e1; base += 512) {  // rows longer than one batch reuse cur
-      load_batch(nbr_j, hv, base, e1, i, lane, cur);
+      load_batch(nbr_j, hv, base, e1, i - off, lane, cur);
       gather_sum(cur, vec, as, at);
     }
     const double s0 = xr_wave_sum((double)as), s1 = xr_wave_sum((double)at);
@@ -546,9 +548,9 @@ __device__ inline void first_row(int r0, int r1, const int* __restrict__ row_ptr
 
 template <class HV, class IT>
 __device__ inline void first_batch(int r0, int r1, const IT* __restrict__ nbr_j, const HV* __restrict__ hv, int e0,
-                                   int e1, RowBatch<HV>& b) {
+                                   int e1, int off, RowBatch<HV>& b) {
   const int i = r0 + (threadIdx.x >> 6);
-  if (i < r1) load_batch(nbr_j, hv, e0, e1, i, threadIdx.x & 63, b);
+  if (i < r1) load_batch(nbr_j, hv, e0, e1, i - off, threadIdx.x & 63, b);
 }
 
 // What a system's block does in launch k, from the state the previous launch
@@ -804,6 +806,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
   const int mode = cg_mode(c, k, j);
   if (mode == MODE_IDLE) return;
   const SysDev& sd = S.sys[sys];
+  const int coff = std::is_same<IT, unsigned short>::value ? sd.atom0 : 0;  // 2-byte columns are system-relative
   const int local = b - sd.dir0;
   const int per = (sd.nupd + sd.ndir - 1) / sd.ndir;
   const int c0 = desc.w, c1 = min(c0 + per, sd.nupd);
@@ -836,7 +839,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     // sums partials t, t + QUARTER, ... of it.
     int2 cr = make_int2(desc.y, desc.z);
     RowBatch<HV> cur;
-    first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, cur);
+    first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, coff, cur);
     Staged<WAVES> sv;
     stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
     double pa = 0.0, pb = 0.0;
@@ -859,10 +862,10 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       if (ch > c0) {
         cr = chunks[sd.upd0 + ch];
         first_row(cr.x, cr.y, row_ptr, e0, e1);
-        first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, cur);
+        first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, coff, cur);
         stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
       }
-      chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, cur, e0, e1, L.sums);
+      chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, coff, cur, e0, e1, L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
       if (!stepped) {
         pa = xr_wave_sum(pa);
@@ -950,10 +953,11 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       const int2 cr = ch == c0 ? make_int2(desc.y, desc.z) : chunks[sd.upd0 + ch];
       if (ch > c0) first_row(cr.x, cr.y, row_ptr, e0, e1);
       RowBatch<double> cur;
-      first_batch(cr.x, cr.y, nbr_j, hval, e0, e1, cur);
+      first_batch(cr.x, cr.y, nbr_j, hval, e0, e1, coff, cur);
       Staged<WAVES> sv;
       stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
-      chunk_spmv<WAVES, double, double2, double, IT>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, cur, e0, e1, L.sums);
+      chunk_spmv<WAVES, double, double2, double, IT>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, coff, cur, e0, e1,
+                                                     L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
       __syncthreads();
       if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
