@@ -219,6 +219,19 @@ int xreac_verlet(xreac_ctx* ctx, const xreac_ff* ff, int32_t num_systems, const 
                  const int32_t* types, const double* dt_fs, double* positions, double* velocities,
                  const double* forces, int32_t stage, double* kinetic);
 
+/* Starting guesses for the QEq solves of the next velocity-Verlet step (no
+ * reference counterpart: the reference has no dynamics).  Per system s of the
+ * packed batch, with k = order[s] (clamped to 0..3), the (s, t) state of each
+ * atom is replaced by the degree-k polynomial through its current value and
+ * the k newest of the three previous solutions in history, evaluated one
+ * equal time step ahead (k = 0 keeps it); history then shifts by one, the
+ * current value becoming its newest entry.  Use it between solves of systems
+ * stepped with a fixed dt, k at most the number of earlier solutions.  All
+ * pointers are device memory: offsets [num_systems + 1], order
+ * [num_systems], qeq_state [num_atoms*2], history [3][num_atoms*2]. */
+int xreac_qeq_extrapolate(xreac_ctx* ctx, int32_t num_systems, int32_t num_atoms, const int32_t* offsets,
+                          const int32_t* order, double* qeq_state, double* history);
+
 /* One FIRE iteration per system of a packed batch, as the reference's native
  * relaxation loop (calculator.py:213-246): if max |F| <= force_tolerance the
  * system is marked converged and not moved; otherwise velocities are mixed

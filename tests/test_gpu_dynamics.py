@@ -124,3 +124,43 @@ def test_calculator_relax_native_matches_oracle_fire():
     assert converged and rel.iterations == it
     np.testing.assert_allclose(rel.positions, xr, rtol=0, atol=1e-7)
     assert np.max(np.abs(rel.evaluation.forces)) <= 1e-4
+
+
+def test_qeq_extrapolation_is_exact_on_polynomials():
+    """The degree-k guess reproduces any polynomial of degree <= k sampled at equal steps (integer
+    coefficients: exact in binary), leaves order 0 untouched and shifts the history by one."""
+    import torch
+
+    eng = _engine()
+    dev = eng.device
+    sizes = [5, 7, 3, 9, 4]
+    order = np.array([0, 1, 2, 3, 3], dtype=np.int32)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    n = int(off[-1])
+    rng = np.random.default_rng(3)
+    # per atom and right-hand side: p(t) = c0 + c1 t + c2 t^2 + c3 t^3 with small integer c, the
+    # coefficients above each system's order zero; solutions at t = 0 (current), -1, -2, -3 (history)
+    coef = rng.integers(-9, 10, size=(n, 2, 4)).astype(np.float64)
+    deg = np.repeat(order, sizes)
+    for d in range(4):
+        coef[deg < d, :, d] = 0.0
+
+    def p(t):
+        return coef[..., 0] + coef[..., 1] * t + coef[..., 2] * t**2 + coef[..., 3] * t**3
+
+    state = torch.as_tensor(p(0.0), device=dev).contiguous()
+    hist = torch.as_tensor(np.stack([p(-1.0), p(-2.0), p(-3.0)]), device=dev).contiguous()
+    eng.qeq_extrapolate(torch.as_tensor(off, device=dev), torch.as_tensor(order, device=dev), state, hist)
+    got = state.cpu().numpy()
+    np.testing.assert_array_equal(got, p(1.0))
+    h = hist.cpu().numpy()
+    np.testing.assert_array_equal(h[0], p(0.0))
+    np.testing.assert_array_equal(h[1], p(-1.0))
+    np.testing.assert_array_equal(h[2], p(-2.0))
+    # a degree-k guess is not exact one degree up: the cubic term shows up with coefficient 6 = 3!
+    coef[:, :, 3] = 1.0
+    state = torch.as_tensor(p(0.0), device=dev).contiguous()
+    hist = torch.as_tensor(np.stack([p(-1.0), p(-2.0), p(-3.0)]), device=dev).contiguous()
+    two = np.full(len(sizes), 2, dtype=np.int32)
+    eng.qeq_extrapolate(torch.as_tensor(off, device=dev), torch.as_tensor(two, device=dev), state, hist)
+    np.testing.assert_array_equal(state.cpu().numpy(), p(1.0) - 6.0)

@@ -163,6 +163,47 @@ def test_nve_conserves_energy_with_exact_gradient():
         assert 0.15 < spread_h / spread < 0.32  # O(dt^2): a quarter, up to higher-order terms
 
 
+def test_extrapolated_qeq_guesses_keep_results_and_cut_iterations():
+    """NVE segments started from extrapolated QEq guesses give the results of the plain warm start, to
+    the solve's tolerance (the server's default 1e-10 relative residual), with clearly fewer CG
+    iterations (a guess closer by a fixed factor saves a fixed number of iterations, so the saving
+    is larger at looser tolerances)."""
+    reqs = []
+    for rid, (s, x, cell) in enumerate((
+        (*workloads.water_cluster_of(40, 5), None),
+        workloads.water_cell(50, seed=4),
+    )):
+        reqs.append(dict(rid=rid, force_field="ffield.reax.HO.2015", symbols=list(s), positions=x, cell=cell,
+                         pbc=(cell is not None,) * 3, total_charge=0.0, md_steps=30, dt=0.25, temperature=300.0,
+                         velocity_seed=5 + rid))
+    iters, out = {}, {}
+    for extrapolate in (False, True):
+        from xreac_amd.serving import Server
+
+        srv = Server(extrapolate=extrapolate, tol=1e-10)
+        count = [0]
+        for eng in srv.engines.values():
+            run = eng.evaluate_batch
+
+            def counted(*a, _run=run, **k):
+                res = _run(*a, **k)
+                count[0] += int(np.sum(res["qeq_iterations"]))
+                return res
+
+            eng.evaluate_batch = counted
+        out[extrapolate] = {r.rid: r for r in srv.run(reqs)}
+        iters[extrapolate] = count[0]
+    print(f"[serving] CG iterations over the segments: plain warm start {iters[False]}, "
+          f"extrapolated {iters[True]}", flush=True)
+    assert iters[True] < 0.85 * iters[False]
+    for req in reqs:
+        a, b = out[False][req["rid"]], out[True][req["rid"]]
+        assert a.evaluations == b.evaluations == 31
+        np.testing.assert_allclose(b.positions, a.positions, rtol=0, atol=1e-8)
+        np.testing.assert_allclose(b.potential, a.potential, rtol=0, atol=1e-6)
+        np.testing.assert_allclose(b.charges, a.charges, rtol=0, atol=1e-8)
+
+
 def test_served_relaxation_follows_oracle_fire():
     reqs = [r for r in _requests(20, seed=13, max_atoms=40) if r["md_steps"] == 0][:2]
     for r in reqs:

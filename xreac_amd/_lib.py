@@ -28,6 +28,7 @@ EXPORTED = (
     "xreac_batch_error",
     "xreac_verlet",
     "xreac_fire_step",
+    "xreac_qeq_extrapolate",
     "xreac_copy_neighbors",
     "xreac_copy_bonds",
     "xreac_profile_enable",
@@ -174,6 +175,8 @@ def load():
     lib.xreac_verlet.restype = C.c_int
     lib.xreac_fire_step.argtypes = [vp, C.c_int32, vp, vp, vp, vp, vp, vp, C.c_double, C.c_int32]
     lib.xreac_fire_step.restype = C.c_int
+    lib.xreac_qeq_extrapolate.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
+    lib.xreac_qeq_extrapolate.restype = C.c_int
     lib.xreac_copy_neighbors.argtypes = [vp, vp, vp, vp, C.c_int64]
     lib.xreac_copy_neighbors.restype = C.c_int
     lib.xreac_copy_bonds.argtypes = [vp, vp, vp, vp, vp, C.c_int64]

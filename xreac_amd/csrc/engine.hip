@@ -1253,6 +1253,15 @@ int xreac_verlet(xreac_ctx* ctx, const xreac_ff* ffh, int32_t ns, const int32_t*
   return XREAC_OK;
 }
 
+int xreac_qeq_extrapolate(xreac_ctx* ctx, int32_t ns, int32_t num_atoms, const int32_t* offsets,
+                          const int32_t* order, double* qeq_state, double* history) {
+  if (!ctx || ns <= 0 || num_atoms < 0 || !offsets || !order || !qeq_state || !history) return XREAC_INVALID;
+  ctx->message.clear();
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(xr_qeq_extrapolate(ns, num_atoms, offsets, order, qeq_state, history, ctx->stream), "QEq extrapolation");
+  return XREAC_OK;
+}
+
 int xreac_fire_step(xreac_ctx* ctx, int32_t ns, const int32_t* offsets, double* positions, double* velocities,
                     const double* forces, double* state, int32_t* status, double force_tolerance,
                     int32_t max_iterations) {

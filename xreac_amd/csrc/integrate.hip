@@ -10,7 +10,8 @@
 // after more than five positive-power steps, alpha *= 0.99, halving on
 // negative power, displacement capped at 0.1 A per atom and iteration.
 //
-// Velocity Verlet has no counterpart in the reference (it has no dynamics).
+// Velocity Verlet, and the extrapolated QEq guesses of its steps, have no
+// counterpart in the reference (it has no dynamics).
 // Units: A, fs, amu, kcal/mol; F/m in kcal/mol/A/amu times 4.184e-4 is A/fs^2.
 #include "integrate.h"
 
@@ -65,6 +66,41 @@ __global__ void __launch_bounds__(IBLK) k_verlet(const int* __restrict__ offsets
   if (stage == 1 && kinetic) {
     ke = block_reduce(ke, Add{}, red);
     if (threadIdx.x == 0) kinetic[s] = 0.5 * ke / ACCEL;
+  }
+}
+
+// Starting guess of the QEq solve of the next MD step: the polynomial of
+// degree k through the system's last k + 1 solutions (equal time steps),
+// evaluated one step ahead.  With h0 the current solution and h1..h3 the
+// previous ones (newest first), the guess is sum_q (-1)^q C(k+1, q+1) h_q:
+//   k = 1: 2 h0 - h1,  k = 2: 3 (h0 - h1) + h2,  k = 3: 4 (h0 + h2) - (6 h1 + h3),
+// and k = 0 keeps h0.  The history then shifts by one (h0 becomes h1).
+__global__ void __launch_bounds__(IBLK) k_qeq_extrapolate(const int* __restrict__ offsets,
+                                                          const int* __restrict__ order, double2* __restrict__ state,
+                                                          double2* __restrict__ hist, int n) {
+  const int s = blockIdx.x;
+  const int a0 = offsets[s], a1 = offsets[s + 1];
+  const int k = min(max(order[s], 0), 3);
+  double2* h1p = hist;
+  double2* h2p = hist + n;
+  double2* h3p = hist + 2 * (size_t)n;
+  for (int i = a0 + threadIdx.x; i < a1; i += blockDim.x) {
+    const double2 h0 = state[i], h1 = h1p[i], h2 = h2p[i], h3 = h3p[i];
+    double2 g = h0;
+    if (k == 1) {
+      g.x = 2.0 * h0.x - h1.x;
+      g.y = 2.0 * h0.y - h1.y;
+    } else if (k == 2) {
+      g.x = 3.0 * (h0.x - h1.x) + h2.x;
+      g.y = 3.0 * (h0.y - h1.y) + h2.y;
+    } else if (k == 3) {
+      g.x = 4.0 * (h0.x + h2.x) - (6.0 * h1.x + h3.x);
+      g.y = 4.0 * (h0.y + h2.y) - (6.0 * h1.y + h3.y);
+    }
+    h3p[i] = h2;
+    h2p[i] = h1;
+    h1p[i] = h0;
+    state[i] = g;
   }
 }
 
@@ -164,6 +200,13 @@ __global__ void __launch_bounds__(IBLK) k_fire(const int* __restrict__ offsets, 
 hipError_t xr_verlet(const FFDev& ff, int ns, const int* offsets, const int* types, const double* dt_fs, double* x,
                      double* v, const double* F, int stage, double* kinetic, hipStream_t s) {
   hipLaunchKernelGGL(k_verlet, dim3(ns), dim3(IBLK), 0, s, offsets, types, ff.atom, dt_fs, x, v, F, stage, kinetic);
+  return hipGetLastError();
+}
+
+hipError_t xr_qeq_extrapolate(int ns, int n, const int* offsets, const int* order, double* state, double* hist,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_qeq_extrapolate, dim3(ns), dim3(IBLK), 0, s, offsets, order, reinterpret_cast<double2*>(state),
+                     reinterpret_cast<double2*>(hist), n);
   return hipGetLastError();
 }
 

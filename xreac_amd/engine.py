@@ -306,6 +306,19 @@ class Engine:
                                    _ptr(kinetic) if kinetic is not None else None)
         self._check(rc, self._ctx)
 
+    def qeq_extrapolate(self, offsets, order, qeq_state, history):
+        """Extrapolated QEq starting guesses for the next equal MD step (device tensors: offsets int32
+        [ns+1], order int32 [ns] in 0..3, qeq_state float64 [N, 2], history float64 [3, N, 2], newest
+        first); qeq_state becomes the guess and history shifts by one."""
+        ns = int(offsets.shape[0]) - 1
+        n = int(qeq_state.shape[0])
+        for t in (offsets, order, qeq_state, history):
+            assert t.is_cuda and t.is_contiguous()
+        assert order.shape == (ns,) and qeq_state.shape == (n, 2) and history.shape == (3, n, 2)
+        rc = self.lib.xreac_qeq_extrapolate(self._ctx, ns, n, _ptr(offsets), _ptr(order), _ptr(qeq_state),
+                                            _ptr(history))
+        self._check(rc, self._ctx)
+
     def fire_step(self, offsets, positions, velocities, forces, state, status, force_tolerance, max_iterations):
         """One FIRE iteration per system (device tensors; state float64 [ns, 4], status int32 [ns])."""
         ns = int(offsets.shape[0]) - 1

@@ -104,6 +104,7 @@ class _Group:
         self.V = torch.zeros((0, 3), **f64)
         self.F = torch.zeros((0, 3), **f64)
         self.QS = torch.zeros((0, 2), **f64)
+        self.QH = torch.zeros((3, 0, 2), **f64)  # previous QEq solutions, newest first (MD extrapolation)
         self.T = torch.zeros(0, dtype=torch.int32, device=self.dev)
         self.FS = torch.zeros((0, 4), **f64)
         self.ST = torch.zeros(0, dtype=torch.int32, device=self.dev)
@@ -147,6 +148,7 @@ class _Group:
         self.T = torch.cat([self.T, torch.as_tensor(np.concatenate(t), dtype=torch.int32, device=self.dev)])
         self.F = torch.cat([self.F, torch.zeros((m, 3), **f64)])
         self.QS = torch.cat([self.QS, torch.zeros((m, 2), **f64)])
+        self.QH = torch.cat([self.QH, torch.zeros((3, m, 2), **f64)], dim=1)
         self.FS = torch.cat([self.FS, torch.as_tensor(np.array(fs, dtype=np.float64), **f64)])
         self.ST = torch.cat([self.ST, torch.as_tensor(np.array(st, dtype=np.int32), device=self.dev)])
         self.jobs.extend(jobs)
@@ -173,6 +175,7 @@ class _Group:
         self.V = self.V.index_select(0, ia)
         self.F = self.F.index_select(0, ia)
         self.QS = self.QS.index_select(0, ia)
+        self.QH = self.QH.index_select(1, ia)
         self.T = self.T.index_select(0, ia)
         self.FS = self.FS.index_select(0, isys)
         self.ST = self.ST.index_select(0, isys)
@@ -185,14 +188,16 @@ class Server:
     """Continuous batching over one GPU, one engine per bundled force field.
 
     precision: "fp64" or "mixed" QEq; warm_start: start each QEq solve from the
-    request's previous (s, t); max_systems: cap on requests per batch (1 gives
+    request's previous (s, t); extrapolate: for NVE segments, start instead
+    from the polynomial through the last (up to four) solutions, one step
+    ahead, which roughly halves the warm solve's iterations; max_systems: cap on requests per batch (1 gives
     the one-request-at-a-time plain path); budget_bytes / max_batch_atoms bound
     the active set; full_derivative: charge-response forces (the exact gradient
     of the QEq-equilibrated energy, calculator.py:79-86) for every request.
     """
 
     def __init__(self, force_fields=("ffield.reax.HO.2015", "ffield.reax.CHO.2008"), device=0, *,
-                 precision="fp64", warm_start=True, tol=1e-10, max_systems=None, max_batch_atoms=262144,
+                 precision="fp64", warm_start=True, extrapolate=True, tol=1e-10, max_systems=None, max_batch_atoms=262144,
                  budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, full_derivative=False,
                  clock=time.perf_counter):
         import torch
@@ -207,6 +212,7 @@ class Server:
         self._pool = ThreadPoolExecutor(max_workers=max(1, len(self.engines)))
         self.precision = precision
         self.warm_start = warm_start
+        self.extrapolate = bool(extrapolate)
         self.tol = tol
         self.max_systems = max_systems
         self.max_batch_atoms = max_batch_atoms
@@ -306,6 +312,10 @@ class Server:
         dt_dev = torch.as_tensor(np.where(moving, g.dt, 0.0), dtype=torch.float64, device=dev) if g.md.any() else None
         if moving.any():
             eng.verlet(off_dev, g.T, dt_dev, g.X, g.V, g.F, 0)
+            if self.warm_start and self.extrapolate:
+                # degree min(evaluations, 4) - 1 through the segment's solutions so far (fixed dt)
+                order = np.where(moving, np.minimum(evals, 4) - 1, 0).astype(np.int32)
+                eng.qeq_extrapolate(off_dev, torch.as_tensor(order, device=dev), g.QS, g.QH)
         out = dict(forces=torch.empty((n, 3), dtype=torch.float64, device=dev),
                    charges=torch.empty(n, dtype=torch.float64, device=dev),
                    total_bond_orders=None, lone_pairs=None, bond_counts=None, qeq_state=g.QS)
