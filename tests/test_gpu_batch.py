@@ -149,6 +149,36 @@ def test_batch_beyond_16_bit_atom_indices(precision):
         assert res["qeq_iterations"][k] == a["qeq_iterations"], system[0]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_of_many_small_systems_is_bitwise_alone(precision):
+    # More than 1,024 small-system CG blocks: the batch runs them as 256-thread
+    # blocks while each system alone runs 512-thread ones; the liquid cell of
+    # 1,650 atoms has more than 64 blocks, so its partial sums take the
+    # strided reduction.  Results must not depend on the shape.
+    eng = _engine()
+    clusters = [workloads.water_cluster_of(256, seed) for seed in range(6)]
+    systems = [(f"cluster768_{k % 6}_{k}", *clusters[k % 6], None, None, 0.0) for k in range(36)]
+    s, x, cell = workloads.water_cell(550, seed=1)
+    systems.insert(17, ("cell1650", s, x, cell, (True, True, True), 0.0))
+    systems.append(SYSTEMS[-1])
+    x, t, off, cells, pbcs, qs = _pack(eng, systems)
+    assert sum(-(-(hi - lo) // 24) for lo, hi in zip(off[:-1], off[1:])) > 1100
+    print(f"[batch] {len(systems)} small systems, {off[-1]} atoms, {precision}", flush=True)
+    res = eng.evaluate_batch(x, t, off, cells, pbcs, qs, tol=1e-12, precision=precision)
+    assert all(e is None for e in res["errors"]), res["errors"]
+    forces = res["forces"].cpu().numpy()
+    charges = res["charges"].cpu().numpy()
+    for k in (0, 17, 30, len(systems) - 1):
+        system = systems[k]
+        print(f"[batch] alone: {system[0]}", flush=True)
+        a = _alone(eng, system, precision)
+        lo, hi = off[k], off[k + 1]
+        np.testing.assert_array_equal(res["components"][k], a["components"], err_msg=system[0])
+        np.testing.assert_array_equal(forces[lo:hi], a["forces"].cpu().numpy(), err_msg=system[0])
+        np.testing.assert_array_equal(charges[lo:hi], a["charges"].cpu().numpy(), err_msg=system[0])
+        assert res["qeq_iterations"][k] == a["qeq_iterations"], system[0]
+
+
 def test_batch_failure_stays_with_its_system():
     eng = _engine()
     bad = ("coincident", ["O", "H"], [[0.0, 0, 0], [12.0, 0, 0]], np.diag([12.0] * 3), (True, True, True), 0.0)

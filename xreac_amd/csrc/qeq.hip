@@ -201,14 +201,22 @@ __global__ void __launch_bounds__(256) k_bond_fill(int n, const int* __restrict_
 // Two block shapes, chosen by a system's own size (so a system is solved the
 // same way alone and in any batch): systems of CG_SMALL_ATOMS atoms and more
 // take 1024-thread blocks over 48-row chunks, which fill the chip from
-// 12,288 atoms on; smaller systems take 512-thread blocks over 24-row chunks,
-// which spread a system over twice as many CUs and pack a batch of small
-// systems more tightly (two blocks per CU).  Each iteration launches one
-// kernel per shape present in the batch; small systems' blocks come first.
+// 12,288 atoms on; smaller systems take 512-thread (or, in a batch of many,
+// 256-thread) blocks over 24-row chunks, which spread a system over twice as
+// many CUs and pack a batch of small systems more tightly.  Each iteration
+// launches one kernel per shape present in the batch; small systems' blocks
+// come first.
 constexpr int CG_BLOCK = 1024;            // 16 waves; the rows of a chunk are spread over the waves
 constexpr int CG_MAX_BLOCKS = 256;        // per system: few partial sums to reduce
 constexpr int CHUNK = 48;                 // nominal rows per chunk; chunks start at bonded-run starts
 constexpr int CG_BLOCK_SMALL = 512;       // 8 waves
+// A batch with many small-system blocks launches them as 256-thread blocks
+// instead (same chunks and arithmetic, so bitwise the same results): under a
+// concurrent flood of one-wave workgroups from the side streams, a 4-wave
+// block finds room on a CU far sooner than an 8-wave one, while a lone small
+// system keeps the 8-wave shape (fewer rows per wave, shorter latency chain).
+constexpr int CG_BLOCK_NARROW = 256;
+constexpr int CG_NARROW_MIN_BLOCKS = 1024;
 constexpr int CG_MAX_BLOCKS_SMALL = 512;
 constexpr int CHUNK_SMALL = 24;
 constexpr int CG_SMALL_ATOMS = 8192;
@@ -792,7 +800,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
                                                            int max_iter, CgSys* __restrict__ cg,
                                                            CgCounters* __restrict__ cnt, int n_all, int base) {
   constexpr int WAVES = BLOCK / 64;
-  constexpr int QUARTER = BLOCK / 4, QW = WAVES / 4;  // threads and waves per series pair of partials
+  constexpr int QUARTER = BLOCK / 4;  // threads per series pair of partials
   static_assert(WAVES % 4 == 0, "a quarter of the block per series pair");
   const int b = base + blockIdx.x;
   // The block's descriptor gives its system and first chunk; the wave's first
@@ -835,28 +843,31 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     // launch's partials are all loaded before the sweep; the partials are
     // reduced after it (the sweep n = A m needs neither alpha nor beta), so
     // their latency hides under the gathers.  Each quarter of the block holds
-    // one series pair (in order r.u, w.u, r.r, b.b); thread t of a quarter
-    // sums partials t, t + QUARTER, ... of it.
+    // one series pair (in order r.u, w.u, r.r, b.b); lane l of the quarter's
+    // first wave sums partials l, l + 64, ... of it, then the wave sums its
+    // lanes: the same order whatever the block shape, so a system's result
+    // does not depend on the shape its batch runs in.
     int2 cr = make_int2(desc.y, desc.z);
     RowBatch<HV> cur;
     first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, coff, cur);
     Staged<WAVES> sv;
     stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
     double pa = 0.0, pb = 0.0;
-    {
+    const bool series_wave = threadIdx.x % QUARTER < 64;  // the first wave of its quarter
+    if (series_wave) {
       const int series = (threadIdx.x / QUARTER) * 2;
       const double* pbase = series < NDOT ? part + ((k - 1) & 1) * NDOT * nd + series * nd : part_bb;
-      const int t0 = threadIdx.x % QUARTER;
-      if (t0 < sd.ndir) {
-        pa = pbase[sd.dir0 + t0];
-        pb = pbase[nd + sd.dir0 + t0];
+      const int l = threadIdx.x & 63;
+      if (l < sd.ndir) {
+        pa = pbase[sd.dir0 + l];
+        pb = pbase[nd + sd.dir0 + l];
       }
-      for (int t = t0 + QUARTER; t < sd.ndir; t += QUARTER) {  // systems with more blocks than QUARTER
+      for (int t = l + 64; t < sd.ndir; t += 64) {  // systems with more than 64 blocks
         pa += pbase[sd.dir0 + t];
         pb += pbase[nd + sd.dir0 + t];
       }
     }
-    __shared__ double red[8][QW];
+    __shared__ double red[8];
     bool stepped = false;
     for (int ch = c0; ch < c1; ++ch) {
       if (ch > c0) {
@@ -867,12 +878,12 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       }
       chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, coff, cur, e0, e1, L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
-      if (!stepped) {
+      if (!stepped && series_wave) {
         pa = xr_wave_sum(pa);
         pb = xr_wave_sum(pb);
         if ((threadIdx.x & 63) == 0) {
-          red[(threadIdx.x / QUARTER) * 2][(threadIdx.x >> 6) % QW] = pa;
-          red[(threadIdx.x / QUARTER) * 2 + 1][(threadIdx.x >> 6) % QW] = pb;
+          red[(threadIdx.x / QUARTER) * 2] = pa;
+          red[(threadIdx.x / QUARTER) * 2 + 1] = pb;
         }
       }
       __syncthreads();
@@ -880,12 +891,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
         stepped = true;
         double res[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          double x = red[q][0];
-#pragma unroll
-          for (int w = 1; w < QW; ++w) x += red[q][w];
-          res[q] = x;
-        }
+        for (int q = 0; q < 8; ++q) res[q] = red[q];
         bool conv[2];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
@@ -1122,13 +1128,20 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
   out->exhausted = false;
   // iteration blocks [0, nd_small) belong to small systems, the rest to large ones
   const int nd_small = a.nd_small, nd_large = nd - a.nd_small;
+  const bool narrow = nd_small >= CG_NARROW_MIN_BLOCKS;
   auto enqueue = [&](int iters) {
     for (int c = 0; c < iters && k < k_limit; ++c) {
       if (mixed) {
-        if (nd_small > 0) cg_launch<true, CG_BLOCK_SMALL>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
+        if (nd_small > 0 && narrow)
+          cg_launch<true, CG_BLOCK_NARROW>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
+        else if (nd_small > 0)
+          cg_launch<true, CG_BLOCK_SMALL>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
         if (nd_large > 0) cg_launch<true, CG_BLOCK>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_large, nd_small, s);
       } else {
-        if (nd_small > 0) cg_launch<false, CG_BLOCK_SMALL>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
+        if (nd_small > 0 && narrow)
+          cg_launch<false, CG_BLOCK_NARROW>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
+        else if (nd_small > 0)
+          cg_launch<false, CG_BLOCK_SMALL>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_small, 0, s);
         if (nd_large > 0) cg_launch<false, CG_BLOCK>(a, k, part, part_bb, nd, tol2, cg, cnt, nd_large, nd_small, s);
       }
       ++k;
