@@ -482,6 +482,7 @@ def main():
     ap.add_argument("--no-stream", action="store_true", help="skip the serving stream (value is then the box's)")
     ap.add_argument("--no-systems", action="store_true", help="skip the CHO mixture and the anatase slab")
     ap.add_argument("--only-box", action="store_true", help="measure only the 12,288-atom box")
+    ap.add_argument("--only-stream", action="store_true", help="measure only the serving stream (for profiling)")
     ap.add_argument("--stream-requests", type=int, default=20000)
     ap.add_argument("--stream-batch-atoms", type=int, default=262144)
     ap.add_argument("--stream-loads", type=float, nargs="+", default=[0.5, 0.7, 0.9],
@@ -510,6 +511,8 @@ def main():
         sys.exit(2)
     if args.only_box:
         args.no_stream = args.no_systems = args.no_cpu_baseline = True
+    if args.only_stream:
+        args.no_systems = args.no_cpu_baseline = True
 
     # Process pools and CPU children start before this process touches the GPU.
     reqs = None if args.no_stream else build_stream(args)
@@ -518,8 +521,10 @@ def main():
         log("[bench] timing the CPU oracle")
         baseline = cpu_baseline(args)
 
-    box, roofline = water_box_bench(args)
-    log(f"[box] {box['ms_per_step']:.3f} ms per evaluation, {box['atom_evals_per_s']:.4g} atom-evals/s")
+    box = roofline = None
+    if not args.only_stream:
+        box, roofline = water_box_bench(args)
+        log(f"[box] {box['ms_per_step']:.3f} ms per evaluation, {box['atom_evals_per_s']:.4g} atom-evals/s")
     systems = {}
     if not args.no_systems:
         from xreac_amd import workloads

@@ -111,6 +111,37 @@ def test_served_evaluations_match_oracle(precision):
             np.testing.assert_allclose(r.forces, ref["forces"], rtol=0, atol=1e-3)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_large_requests_match_oracle(precision):
+    """Requests near the stream's 1,536-atom cap, served in one mix with small ones: a periodic water cell
+    (1,527 atoms), a periodic CHO cell (1,393), a charged water cluster (1,058) and a CHO cluster (1,231)
+    of the seeded stream, against the FP64 oracle."""
+    plan = workloads.stream_plan(400, seed=4)
+    big = [workloads.build_request(plan[rid]) for rid in (230, 376, 48, 272)]
+    assert [len(r["symbols"]) for r in big] == [1527, 1393, 1058, 1231]
+    assert big[0]["cell"] is not None and big[1]["cell"] is not None and big[2]["total_charge"] == -1.0
+    small = [r for r in _requests(30, seed=6, max_atoms=60) if r["md_steps"] == 0][:6]
+    for k, r in enumerate(small):
+        r["rid"] = 10**6 + k
+    res, _ = _serve(big + small, precision=precision, tol=1e-10)
+    ffs = {}
+    for req in big:
+        ff = ffs.setdefault(req["force_field"], ForceField.bundled(req["force_field"]))
+        n = len(req["symbols"])
+        print(f"[serving] oracle check rid={req['rid']} n={n} {req['force_field']} {precision}", flush=True)
+        ref = orc.evaluate(ff, req["symbols"], req["positions"], cell=req["cell"], pbc=req["pbc"],
+                           total_charge=req["total_charge"])
+        r = res[req["rid"]]
+        assert r.error is None
+        if precision == "fp64":
+            assert abs(r.energy - ref["energy"]) <= 1e-9 * max(1.0, abs(ref["energy"]))
+            np.testing.assert_allclose(r.forces, ref["forces"], rtol=0, atol=1e-6)
+            np.testing.assert_allclose(r.charges, ref["charges"], rtol=0, atol=1e-7)
+        else:
+            assert abs(r.energy - ref["energy"]) <= 1e-4 * n
+            np.testing.assert_allclose(r.forces, ref["forces"], rtol=0, atol=1e-3)
+
+
 def test_nve_segment_stays_bounded():
     """50-step NVE segments: fixed-charge forces are not the exact gradient of the QEq energy (14.4 in H
     against 332.06371/23.02 in the energy), so the total drifts a little; it must stay a small fraction of
