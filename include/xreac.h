@@ -114,6 +114,14 @@ typedef struct {
                                   the plain angle derivative is then used, as calculator.py:79-86 does */
   int32_t fixed_charges;       /* 1: no QEq; the charges array is read as the atomic charges (the
                                   EnergyModel.components(x, fixed_charges) hook, energy.py:180-219) */
+  double neighbor_skin;        /* > 0: ASE's per-atom displacement allowance (Angstrom), as the
+                                  reference's ASE calculator uses it (ase.py:39-40, 90-93, 107-131):
+                                  keep the device-built list within rc + 2 skin and reuse it on later
+                                  calls of the same shape (same force field, atom offsets, cells and
+                                  pbc) until an atom has moved more than skin or a type changed.  Each
+                                  call filters the kept rows to the cutoff, so results are bitwise those
+                                  of a fresh build.  Ignored with a supplied list; 0 (the default): a
+                                  fresh list every call */
 } xreac_options;
 
 typedef struct {
@@ -206,6 +214,10 @@ int xreac_evaluate_batch(xreac_ctx* ctx, const xreac_ff* ff, const xreac_batch* 
 
 /* Message of system s of the last batch ("" if it succeeded). */
 const char* xreac_batch_error(const xreac_ctx* ctx, int32_t system);
+
+/* Skin lists built on this context so far, one per system and rebuild
+ * (the reference's ReaxFFCalculator.neighbor_list_builds, ase.py:130). */
+int64_t xreac_neighbor_list_builds(const xreac_ctx* ctx);
 
 /* Velocity Verlet over a packed batch, one step in two stages around a force
  * evaluation (no reference counterpart: the reference has no dynamics).

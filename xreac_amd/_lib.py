@@ -29,6 +29,7 @@ EXPORTED = (
     "xreac_verlet",
     "xreac_fire_step",
     "xreac_qeq_extrapolate",
+    "xreac_neighbor_list_builds",
     "xreac_copy_neighbors",
     "xreac_copy_bonds",
     "xreac_profile_enable",
@@ -88,6 +89,7 @@ class Options(C.Structure):
         ("warm_start", C.c_int32),
         ("full_derivative", C.c_int32),
         ("fixed_charges", C.c_int32),
+        ("neighbor_skin", C.c_double),
     ]
 
 
@@ -177,6 +179,8 @@ def load():
     lib.xreac_fire_step.restype = C.c_int
     lib.xreac_qeq_extrapolate.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp]
     lib.xreac_qeq_extrapolate.restype = C.c_int
+    lib.xreac_neighbor_list_builds.argtypes = [vp]
+    lib.xreac_neighbor_list_builds.restype = C.c_int64
     lib.xreac_copy_neighbors.argtypes = [vp, vp, vp, vp, C.c_int64]
     lib.xreac_copy_neighbors.restype = C.c_int
     lib.xreac_copy_bonds.argtypes = [vp, vp, vp, vp, vp, C.c_int64]

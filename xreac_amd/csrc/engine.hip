@@ -106,6 +106,24 @@ struct xreac_ctx {
   std::string message;
   std::vector<std::string> sys_messages;  // per system of the last batch
   Buffer atom_buf, bin_buf, edge_buf, bond_buf, ctl_buf, slab_buf;
+  // Skin reuse (xreac_options.neighbor_skin): the kept rows within rc + 2 skin
+  // (slab of skin_stride keys per atom), their lengths, and the positions and
+  // types they were built at; valid for calls of the same shape as the last.
+  Buffer skin_slab_buf, skin_counts_buf, skin_xref_buf, skin_tref_buf;
+  struct SkinKey {
+    const void* ff = nullptr;
+    double skin = 0.0;
+    std::vector<int32_t> offsets;
+    std::vector<double> cells;
+    std::vector<int32_t> pbc;
+    bool operator==(const SkinKey& o) const {
+      return ff == o.ff && skin == o.skin && offsets == o.offsets && cells == o.cells && pbc == o.pbc;
+    }
+  } skin_key;
+  bool skin_valid = false;
+  int skin_stride = 0;
+  int skin_max_row = 0;
+  int64_t neighbor_list_builds = 0;
   char* ctl_host = nullptr;  // pinned mirror of the control block
   size_t ctl_host_bytes = 0;
   XrProfiler prof;
@@ -374,7 +392,7 @@ size_t carve_at(size_t& off, size_t count) {
 // Control block: small per-call state, mirrored in pinned host memory so
 // one copy brings every per-system result back.
 struct CtlLayout {
-  size_t flags, serr, comps, bounds, mu, cg, cg2, sys, total;
+  size_t flags, serr, comps, bounds, mu, cg, cg2, sys, gate, total;
 };
 
 CtlLayout ctl_layout(int ns) {
@@ -387,7 +405,8 @@ CtlLayout ctl_layout(int ns) {
   c.mu = carve_at<double>(off, ns);
   c.cg = carve_at<char>(off, xr_cg_state_bytes(ns));
   c.cg2 = carve_at<char>(off, xr_cg_state_bytes(ns));  // charge-response adjoint solve
-  c.sys = carve_at<SysDev>(off, ns);
+  c.gate = carve_at<unsigned>(off, ns);  // skin reuse: 1 where the system's skin list was rebuilt
+  c.sys = carve_at<SysDev>(off, ns);     // last: the copy back covers everything before it
   c.total = (off + 255) & ~size_t(255);
   return c;
 }
@@ -490,7 +509,16 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   } active(arena ? nullptr : &ctx->prof);  // no timer events (which would be created) in static scratch
   const double rc = ffh->cutoff;
   const double bond_cut = std::min(5.0, rc);
-  const double rcb = rc * (1 + 1e-8) + 1e-12;
+  // Skin reuse applies to lists built here (not to supplied ones); bins are then wide enough for rc + skin.
+  // ase.py:90-93 (the same check and text)
+  if (!(opt->neighbor_skin >= 0.0) || !std::isfinite(opt->neighbor_skin))
+    return fail(ctx, XREAC_INVALID, "neighbor_skin must be a finite nonnegative number");
+  // Skin reuse applies to lists built here (not to supplied ones).  As ASE's neighbor_skin (ase.py:39-40,
+  // 112-124): a per-atom displacement allowance, so the kept rows reach rc + 2 skin, and bins are that wide.
+  const double skin =
+      (in.nbr_row_ptr == nullptr && arena == nullptr && opt->neighbor_skin > 0.0) ? opt->neighbor_skin : 0.0;
+  const double rc_keep = rc + 2.0 * skin;
+  const double rcb = rc_keep * (1 + 1e-8) + 1e-12;
   const bool mixed = opt->precision == XREAC_PRECISION_MIXED;
   const bool given_q = opt->fixed_charges != 0;
   const bool full = opt->full_derivative != 0 && !given_q;
@@ -744,6 +772,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // coincident atoms, as validate_input does whatever list is supplied.
   const bool supplied = in.nbr_row_ptr != nullptr;
   int max_row = ctx->max_row;
+  int skin_max_row = 0;
   int64_t m_cap = 0;
   if (supplied) {
     HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, -1, counts, s), "neighbor count");
@@ -758,14 +787,21 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     exact = exact || m_cap == 0;
     if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
     if (exact) {
+      if (skin > 0.0) {  // the longest row within rc + 2 skin sizes the kept rows
+        HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc_keep, 2048, counts, s),
+               "neighbor count");
+        HIPCHK(xr_scan_clamped(counts, n, row_ptr, scan_tmp, INT32_MAX, dev_counts + 4, d_flags, dev_counts + 5, s),
+               "neighbor scan");
+      }
       HIPCHK(xr_neighbor_count(n, in.x, S, home, bin, bin_start, binned, rc, 2048, counts, s), "neighbor count");
       HIPCHK(xr_scan_clamped(counts, n, row_ptr, scan_tmp, INT32_MAX, dev_counts, d_flags, dev_counts + 2, s),
              "neighbor scan");
-      int h[3] = {0, 0, 0};
+      int h[6] = {0, 0, 0, 0, 0, 0};
       HIPCHK(hipMemcpyAsync(h, dev_counts, sizeof(h), hipMemcpyDeviceToHost, s), "copy");
       HIPCHK(hipStreamSynchronize(s), "neighbor sync");
       m_cap = h[0];
       max_row = h[2];
+      skin_max_row = h[5];
     }
   }
 
@@ -800,8 +836,54 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     slab_stride = xr_fill_stride(hint);
     XR_WORKSPACE(ctx->slab_buf, sizeof(unsigned long long) * (size_t)slab_stride * n + 4096, &slab,
                  "neighbor slab");
-    HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, counts, slab, hint, d_flags, s),
-           "neighbor fill");
+    if (skin > 0.0) {
+      // Skin reuse.  The kept rows' stride comes from an exact count, else stays as kept, else from the
+      // longest rc row scaled by the volume ratio; a row longer than it repeats the call exactly.
+      int hint_s;
+      if (exact) {
+        hint_s = skin_max_row;
+      } else if (ctx->skin_stride > 0) {
+        hint_s = ctx->skin_stride;
+      } else {
+        const double grow = std::pow(rc_keep / rc, 3.0);
+        hint_s = max_row > 0 ? (int)std::min(2048.0, std::ceil(max_row * grow * 1.1) + 64) : 2048;
+      }
+      const int stride_s = xr_fill_stride(hint_s);
+      xreac_ctx::SkinKey key;
+      key.ff = ffh;
+      key.skin = skin;
+      key.offsets.assign(in.offsets, in.offsets + ns + 1);
+      if (in.cells) key.cells.assign(in.cells, in.cells + 9 * (size_t)ns);
+      if (in.pbc) key.pbc.assign(in.pbc, in.pbc + 3 * (size_t)ns);
+      bool force = exact || !ctx->skin_valid || stride_s != ctx->skin_stride || !(key == ctx->skin_key);
+      const void* kept[4] = {ctx->skin_slab_buf.ptr, ctx->skin_counts_buf.ptr, ctx->skin_xref_buf.ptr,
+                             ctx->skin_tref_buf.ptr};
+      HIPCHK(ctx->skin_slab_buf.ensure(sizeof(unsigned long long) * (size_t)stride_s * n + 4096), "skin rows");
+      HIPCHK(ctx->skin_counts_buf.ensure(sizeof(int) * (size_t)n + 256), "skin rows");
+      HIPCHK(ctx->skin_xref_buf.ensure(sizeof(double) * 3 * (size_t)n + 256), "skin rows");
+      HIPCHK(ctx->skin_tref_buf.ensure(sizeof(int) * (size_t)n + 256), "skin rows");
+      force = force || kept[0] != ctx->skin_slab_buf.ptr || kept[1] != ctx->skin_counts_buf.ptr ||
+              kept[2] != ctx->skin_xref_buf.ptr || kept[3] != ctx->skin_tref_buf.ptr;
+      ctx->skin_valid = false;  // until this call completes cleanly
+      ctx->skin_stride = stride_s;
+      ctx->skin_key = std::move(key);
+      unsigned* gate = reinterpret_cast<unsigned*>(dc + L.gate);
+      int* skin_counts = static_cast<int*>(ctx->skin_counts_buf.ptr);
+      double* xref = static_cast<double*>(ctx->skin_xref_buf.ptr);
+      int* tref = static_cast<int*>(ctx->skin_tref_buf.ptr);
+      // atoms that moved at most skin (less a rounding margin) keep every pair within rc in the kept rows
+      HIPCHK(xr_skin_check(S, in.x, in.types, xref, tref, skin * (1.0 - 1e-9), force, gate, s), "skin check");
+      HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc_keep, skin_counts,
+                              ctx->skin_slab_buf.ptr, stride_s, d_flags, s, gate),
+             "neighbor fill");
+      HIPCHK(xr_skin_commit(n, S, in.x, in.types, xref, tref, gate, s), "skin commit");
+      HIPCHK(xr_skin_filter(n, in.x, S, ctx->skin_slab_buf.ptr, stride_s, skin_counts, rc, counts, slab,
+                            slab_stride, d_flags, s),
+             "skin filter");
+    } else {
+      HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc, counts, slab, hint, d_flags, s),
+             "neighbor fill");
+    }
     // row pointers clamped to the edge buffer, total edges and longest row in one pass
     HIPCHK(xr_scan_clamped(counts, n, row_ptr, scan_tmp, exact ? INT32_MAX : (int)m_cap, dev_counts, d_flags,
                            dev_counts + 2, s),
@@ -1106,6 +1188,15 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
       out.status[q] = XREAC_OK;
     }
   }
+  if (skin > 0.0) {
+    const unsigned* h_gate = reinterpret_cast<const unsigned*>(hc + L.gate);
+    bool clean = true;
+    for (int q = 0; q < ns; ++q) {
+      ctx->neighbor_list_builds += h_gate[q] ? 1 : 0;
+      clean = clean && h_serr[q] == 0;
+    }
+    ctx->skin_valid = clean;  // a failed system may have left its kept rows incomplete
+  }
   out.num_edges = m;
   out.num_bonds = nb_total;
   ctx->m_per_atom = (double)m / n;
@@ -1197,6 +1288,8 @@ int xreac_evaluate_static(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_syste
                           xreac_result* out) {
   if (!ctx) return XREAC_INVALID;
   if (!ctx->scratch) return fail(ctx, XREAC_DEVICE, "xreac_evaluate_static: no static scratch on this device");
+  if (opt && opt->neighbor_skin != 0.0)
+    return fail(ctx, XREAC_INVALID, "xreac_evaluate_static keeps no skin lists: neighbor_skin must be 0");
   Arena arena;
   arena.base = static_cast<char*>(ctx->scratch);
   arena.cap = XREAC_STATIC_SCRATCH_BYTES;
@@ -1275,6 +1368,8 @@ int xreac_fire_step(xreac_ctx* ctx, int32_t ns, const int32_t* offsets, double* 
          "FIRE");
   return XREAC_OK;
 }
+
+int64_t xreac_neighbor_list_builds(const xreac_ctx* ctx) { return ctx ? ctx->neighbor_list_builds : -1; }
 
 const char* xreac_batch_error(const xreac_ctx* ctx, int32_t system) {
   if (!ctx || system < 0 || system >= (int32_t)ctx->sys_messages.size()) return "no such system";

@@ -28,10 +28,22 @@ hipError_t xr_neighbor_count(int n, const double* x, const Systems& S, const int
                              hipStream_t s);
 // Slab row stride (keys per row) of the fill instance for rows up to max_row.
 int xr_fill_stride(int max_row);
-// Sorted rows to slab[i * stride, i * stride + counts[i]) as (j << 32 | image code).
+// Sorted rows to slab[i * stride, i * stride + counts[i]) as (j << 32 | image code).  With a gate (one
+// word per system), only the rows of systems whose word is nonzero are written.
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, int* counts, void* slab,
-                            int max_row, unsigned* flags, hipStream_t s);
+                            int max_row, unsigned* flags, hipStream_t s, const unsigned* gate = nullptr);
+// Skin reuse: gate[s] = 1 when system s needs a new skin list (force, an atom moved more than allowance
+// from xref, or a type differs from tref); the gated copy of positions and types after a rebuild; and
+// the kept skin rows (stride skin_stride) filtered to the cutoff into the slab rows (stride `stride`)
+// and counts, as the fill writes them.
+hipError_t xr_skin_check(const Systems& S, const double* x, const int* types, const double* xref, const int* tref,
+                         double allowance, bool force, unsigned* gate, hipStream_t s);
+hipError_t xr_skin_commit(int n, const Systems& S, const double* x, const int* types, double* xref, int* tref,
+                          const unsigned* gate, hipStream_t s);
+hipError_t xr_skin_filter(int n, const double* x, const Systems& S, const void* skin, int skin_stride,
+                          const int* skin_counts, double cutoff, int* counts, void* slab, int stride, unsigned* flags,
+                          hipStream_t s);
 // Supplied row pointers ptr[0..n] clamped to [0, cap]; a malformed pointer
 // array (first entry not 0, negative or decreasing entries) flags the row's
 // system with XR_ERR_NEIGHBORS.  *total and the capacity bit as xr_clamp_ptr.

@@ -405,17 +405,21 @@ __device__ inline unsigned long long full_key(unsigned c, int atom0) {
 // registers and leave room for many waves per CU; a launch that meets a row
 // longer than its CAP drops the row (count 0) and raises the capacity bit, and
 // the evaluation is repeated with the instance the longest row needs.
+// With a gate (one word per system), rows of systems whose word is 0 are left
+// as they are: a kept skin list that is still valid.
 template <int CAP>
 __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict__ x, Systems S,
                                                  const int* __restrict__ home, const int* __restrict__ bin,
                                                  const int* __restrict__ bin_start, const BinRec* __restrict__ binned,
                                                  double rc2, int* __restrict__ counts,
-                                                 unsigned long long* __restrict__ slab, unsigned* __restrict__ flags) {
+                                                 unsigned long long* __restrict__ slab, unsigned* __restrict__ flags,
+                                                 const unsigned* __restrict__ gate) {
   __shared__ unsigned long long keys[CAP];
   const int i = blockIdx.x;
   if (i >= n) return;
   const int lane = threadIdx.x & 63;
   const int sys = __builtin_amdgcn_readfirstlane(S.asys[i]);
+  if (gate && gate[sys] == 0u) return;
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   unsigned err = 0;
   int cursor = 0;  // identical in every lane
@@ -497,6 +501,111 @@ __global__ void __launch_bounds__(64) k_nbr_fill(int n, const double* __restrict
   }
   __syncthreads();
   for (int t2 = lane; t2 < count; t2 += 64) out[t2] = keys[t2];
+}
+
+// ---------------------------------------------------------- skin reuse
+// A list kept from an earlier call holds, per atom, every image within
+// rc + 2 skin of the positions it was built at (xref), sorted canonically.
+// While no atom of a system has moved more than skin since then (and no type
+// has changed), every pair now within rc is in it, so filtering it by the
+// current distances gives exactly the rows a fresh build gives.
+
+// gate[s] = 1 when system s needs a new skin list: forced by the host, an atom
+// moved by more than sqrt(lim2) (or to a non-finite position), or a type changed.
+__global__ void __launch_bounds__(256) k_skin_check(Systems S, const double* __restrict__ x,
+                                                    const int* __restrict__ types, const double* __restrict__ xref,
+                                                    const int* __restrict__ tref, double lim2, int force,
+                                                    unsigned* __restrict__ gate) {
+  const SysDev& sd = S.sys[blockIdx.x];
+  __shared__ int moved;
+  if (threadIdx.x == 0) moved = 0;
+  __syncthreads();
+  bool m = false;
+  if (!force) {
+    for (int i = sd.atom0 + threadIdx.x; i < sd.atom0 + sd.n; i += blockDim.x) {
+      const double d0 = x[3 * i] - xref[3 * i], d1 = x[3 * i + 1] - xref[3 * i + 1],
+                   d2 = x[3 * i + 2] - xref[3 * i + 2];
+      m = m || !((d0 * d0 + d1 * d1) + d2 * d2 <= lim2) || types[i] != tref[i];
+    }
+  }
+  if (m) moved = 1;  // every writer stores the same value
+  __syncthreads();
+  if (threadIdx.x == 0) gate[blockIdx.x] = (force || moved) ? 1u : 0u;
+}
+
+// Positions and types of the systems whose skin list was just rebuilt.
+__global__ void k_skin_commit(int n, Systems S, const double* __restrict__ x, const int* __restrict__ types,
+                              double* __restrict__ xref, int* __restrict__ tref, const unsigned* __restrict__ gate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || gate[S.asys[i]] == 0u) return;
+  xref[3 * i] = x[3 * i];
+  xref[3 * i + 1] = x[3 * i + 1];
+  xref[3 * i + 2] = x[3 * i + 2];
+  tref[i] = types[i];
+}
+
+// One wave per atom: the kept skin row filtered to rc with the fill's own
+// distance arithmetic, in its canonical order, to the row's slab slots and
+// its length to counts; coincident pairs and over-long rows are flagged as
+// the fill flags them.
+__global__ void __launch_bounds__(256) k_skin_filter(int n, const double* __restrict__ x, Systems S,
+                                                     const unsigned long long* __restrict__ skin, int skin_stride,
+                                                     const int* __restrict__ skin_counts, double rc2,
+                                                     int* __restrict__ counts, unsigned long long* __restrict__ slab,
+                                                     int stride, unsigned* __restrict__ flags) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const int sys = __builtin_amdgcn_readfirstlane(S.asys[i]);
+  const SysDev& sd = S.sys[sys];
+  const CellDev& cell = sd.cell;
+  const double xi0 = x[3 * i], xi1 = x[3 * i + 1], xi2 = x[3 * i + 2];
+  const int zero = xr_pack_shift(0, 0, 0);
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int have = min(max(skin_counts[i], 0), skin_stride);
+  const unsigned long long* row = skin + (size_t)i * skin_stride;
+  unsigned long long* out = slab + (size_t)i * stride;
+  unsigned err = 0;
+  int cursor = 0;  // identical in every lane
+  for (int b = 0; b < have; b += 64) {
+    const int e = b + lane;
+    bool hit = false;
+    unsigned long long key = 0;
+    if (e < have) {
+      key = row[e];
+      const int j = (int)(key >> 32), code = (int)(key & 0xffffffffu);
+      if ((unsigned)(j - sd.atom0) < (unsigned)sd.n) {
+        double off[3];
+        xr_offset(cell, code, off);
+        const double v0 = (x[3 * j] - xi0) + off[0];
+        const double v1 = (x[3 * j + 1] - xi1) + off[1];
+        const double v2 = (x[3 * j + 2] - xi2) + off[2];
+        const double r2 = (v0 * v0 + v1 * v1) + v2 * v2;
+        hit = r2 <= rc2 && !(j == i && code == zero);
+        if (hit && j != i && r2 < 1e-12) err |= XR_ERR_COINCIDENT;
+      } else {
+        err |= XR_ERR_NEIGHBORS;  // not a row of this system: the kept list is not valid
+      }
+    }
+    const unsigned long long m = __ballot(hit);
+    if (hit) {
+      const int p = cursor + __popcll(m & below);
+      if (p < stride) out[p] = key;
+    }
+    cursor += __popcll(m);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) err |= __shfl_down(err, off, 64);
+  if (lane == 0) {
+    if (cursor > stride) {
+      counts[i] = 0;  // the row is dropped and the evaluation repeated (or the system failed)
+      if (cursor > ROW_CAP) err |= XR_ERR_ROW_TOO_LONG;
+      else atomicOr(flags, (unsigned)XR_ERR_CAPACITY);
+    } else {
+      counts[i] = cursor;
+    }
+    if (err) atomicOr(&S.err[sys], err);
+  }
 }
 
 // Supplied list: clamp every row pointer to [0, cap], so every later row
@@ -596,19 +705,44 @@ int xr_fill_stride(int max_row) { return max_row <= 512 ? 512 : (max_row <= 1024
 
 hipError_t xr_neighbor_fill(int n, const double* x, const Systems& S, const int* home, const int* bin,
                             const int* bin_start, const void* binned, double cutoff, int* counts, void* slab,
-                            int max_row, unsigned* flags, hipStream_t s) {
+                            int max_row, unsigned* flags, hipStream_t s, const unsigned* gate) {
   unsigned long long* keys = static_cast<unsigned long long*>(slab);
   const int stride = xr_fill_stride(max_row);
   if (stride == 512) {
     XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<512>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
-              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags);
+              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags, gate);
   } else if (stride == 1024) {
     XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<1024>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
-              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags);
+              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags, gate);
   } else {
     XR_LAUNCH(XT_NBR_FILL, k_nbr_fill<ROW_CAP>, dim3(n), dim3(64), 0, s, n, x, S, home, bin, bin_start,
-              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags);
+              static_cast<const BinRec*>(binned), cutoff * cutoff, counts, keys, flags, gate);
   }
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_skin_check(const Systems& S, const double* x, const int* types, const double* xref, const int* tref,
+                         double allowance, bool force, unsigned* gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_skin_check, dim3(S.ns), dim3(256), 0, s, S, x, types, xref, tref, allowance * allowance,
+                     force ? 1 : 0, gate);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_skin_commit(int n, const Systems& S, const double* x, const int* types, double* xref, int* tref,
+                          const unsigned* gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_skin_commit, dim3((n + 255) / 256), dim3(256), 0, s, n, S, x, types, xref, tref, gate);
+  XR_LAUNCH_CHECK();
+  return hipSuccess;
+}
+
+hipError_t xr_skin_filter(int n, const double* x, const Systems& S, const void* skin, int skin_stride,
+                          const int* skin_counts, double cutoff, int* counts, void* slab, int stride, unsigned* flags,
+                          hipStream_t s) {
+  XR_LAUNCH(XT_NBR_FILL, k_skin_filter, dim3((n + 3) / 4), dim3(256), 0, s, n, x, S,
+            static_cast<const unsigned long long*>(skin), skin_stride, skin_counts, cutoff * cutoff, counts,
+            static_cast<unsigned long long*>(slab), stride, flags);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

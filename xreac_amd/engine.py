@@ -118,7 +118,7 @@ class Engine:
 
     def evaluate(self, positions, types, cell=None, pbc=(False, False, False), total_charge=0.0, *, tol=1e-10,
                  max_iter=2000, lammps_angles=True, properties=True, out=None, precision="fp64",
-                 full_derivative=False, charges=None, neighbors=None, no_alloc=False):
+                 full_derivative=False, charges=None, neighbors=None, no_alloc=False, neighbor_skin=0.0):
         """Evaluate one system whose positions (n,3 float64) and types (n int32) are device tensors.
 
         Returns a dict of device tensors (forces, charges and optionally
@@ -130,6 +130,11 @@ class Engine:
         ``no_alloc`` runs ``xreac_evaluate_static``: the library takes every
         workspace from its static device scratch and allocates nothing (pass
         ``out`` to keep the call itself free of PyTorch allocations too).
+        ``neighbor_skin`` > 0 keeps the device-built list (within the cutoff
+        plus twice the skin) on this engine and reuses it on later calls of
+        the same shape until an atom has moved more than the skin; results
+        are bitwise those of a fresh list (``neighbor_list_builds`` counts
+        the builds).
         """
         torch = self.torch
         n = int(positions.shape[0])
@@ -158,7 +163,7 @@ class Engine:
             *(self._csr_fields(nbr)),
         )
         opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision), 0,
-                           1 if full_derivative else 0, 1 if given is not None else 0)
+                           1 if full_derivative else 0, 1 if given is not None else 0, float(neighbor_skin))
         res = _lib.Result()
         res.forces = _ptr(forces)
         res.charges = _ptr(charges)
@@ -221,7 +226,7 @@ class Engine:
 
     def evaluate_batch(self, positions, types, offsets, cells=None, pbc=None, total_charges=None, *, tol=1e-10,
                        max_iter=2000, lammps_angles=True, properties=False, precision="fp64", warm_start=False,
-                       full_derivative=False, out=None, neighbors=None):
+                       full_derivative=False, out=None, neighbors=None, neighbor_skin=0.0):
         """Evaluate independent systems packed atom-wise in one pipeline.
 
         positions (N, 3) float64 and types (N,) int32 are device tensors;
@@ -281,7 +286,7 @@ class Engine:
             t = out.get(name)
             setattr(res, name, _ptr(t) if t is not None else None)
         opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision),
-                           1 if warm_start else 0, 1 if full_derivative else 0, 0)
+                           1 if warm_start else 0, 1 if full_derivative else 0, 0, float(neighbor_skin))
         rc = self.lib.xreac_evaluate_batch(self._ctx, self._ff, C.byref(batch), C.byref(opt), C.byref(res))
         self._check(rc, self._ctx)
         errors = [None if status[k] == _lib.XREAC_OK else self.lib.xreac_batch_error(self._ctx, k).decode()
@@ -305,6 +310,11 @@ class Engine:
                                    _ptr(positions), _ptr(velocities), _ptr(forces), int(stage),
                                    _ptr(kinetic) if kinetic is not None else None)
         self._check(rc, self._ctx)
+
+    @property
+    def neighbor_list_builds(self):
+        """Skin lists built on this engine (one per system and rebuild), as ReaxFFCalculator counts them."""
+        return int(self.lib.xreac_neighbor_list_builds(self._ctx))
 
     def qeq_extrapolate(self, offsets, order, qeq_state, history):
         """Extrapolated QEq starting guesses for the next equal MD step (device tensors: offsets int32
