@@ -78,6 +78,7 @@ class _Job:
     potential: list = field(default_factory=list)
     kinetic: list = field(default_factory=list)
     prep: tuple | None = None  # host arrays for _Group.append, built ahead of admission
+    admitted: bool = False
 
 
 class _Group:
@@ -225,6 +226,7 @@ class Server:
         self.active = {k: _Group(torch, eng) for k, eng in self.engines.items()}
         self._lock = threading.Lock()  # queues, budget and results, shared by the groups' loops
         self._closed = False
+        self._unprepared = deque()  # submitted requests whose host arrays are not built yet, in arrival order
         self.used_bytes = 0
         self.responses = []
         self.atom_evals = 0
@@ -239,6 +241,26 @@ class Server:
         job = _Job(req, n, periodic, footprint(n, periodic), self.clock() if arrival is None else arrival)
         with self._lock:
             self.queues[req["force_field"]].append(job)
+            self._unprepared.append(job)
+
+    def _prepare_ahead(self):
+        """Build submitted requests' host arrays in arrival order while the loops run, so admitting many
+        requests at once does not hold a loop (and the device) for their preparation (about 45 us each);
+        a request admitted before its turn here is prepared by the loop itself."""
+        while True:
+            with self._lock:
+                while self._unprepared and self._unprepared[0].admitted:
+                    self._unprepared.popleft()
+                job = self._unprepared.popleft() if self._unprepared else None
+                closed = self._closed
+            if job is None:
+                if closed:
+                    return
+                time.sleep(2e-4)
+                continue
+            if job.prep is None:
+                name = job.req["force_field"]
+                self.active[name].prepare(job, self._ff[name])
 
     @property
     def queue(self):
@@ -261,6 +283,7 @@ class Server:
                                               atoms + job.n > self.max_batch_atoms):
                 break
             queue.popleft()
+            job.admitted = True
             admitted.append(job)
             self.used_bytes += job.bytes
             active += 1
@@ -386,6 +409,8 @@ class Server:
         times = np.zeros(len(requests)) if arrivals is None else np.asarray(arrivals, dtype=float)
         with self._lock:
             self._closed = False
+        preparer = threading.Thread(target=self._prepare_ahead, daemon=True)
+        preparer.start()
         loops = [self._pool.submit(self._loop, name) for name in self.active]
         i = 0
         try:
@@ -401,4 +426,5 @@ class Server:
                 self._closed = True
             for f in loops:
                 f.result()
+            preparer.join()
         return self.responses
