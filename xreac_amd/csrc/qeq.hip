@@ -684,7 +684,8 @@ struct CgStep {
 //              x += a p, r -= a s, u -= a q, w -= a z;  m = M^-1 w
 //   residual:  r = b - A x;  u = M^-1 r
 //   start:     w = A u;  m = M^-1 w
-// and the wave's dots (r.u, w.u, r.r and, at a start, b.b) accumulate in dots.
+// and the wave's dots (r.u, w.u, r.r and, at a start, b.b; after a residual
+// only r.r and b.b) accumulate in dots.
 template <bool MIXED>
 __device__ inline void chunk_tail(int mode, int r0, int r1, const CgStep& st, const CgVec& v, double2* m_out,
                                   float2* m32_out, ChunkLds& L, double (&dots)[4]) {
@@ -744,6 +745,9 @@ __device__ inline void chunk_tail(int mode, int r0, int r1, const CgStep& st, co
   const double mz = apply_block(L, h, t);
   if (mode == MODE_RESIDUAL) {
     put(v.U, mz);
+    const double b = get(IT_B);
+    dots[2] += r * r;  // the true residual's r.r and b.b, for the start launch's convergence test
+    dots[3] += b * b;
     return;
   }
   put(m_out, mz);
@@ -948,7 +952,28 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       if (ch + 1 < c1) __syncthreads();
     }
   } else {
-    // refresh in FP64: r = b - A x and u = M^-1 r, then w = A u and m = M^-1 w
+    // refresh in FP64: r = b - A x and u = M^-1 r, then w = A u and m = M^-1 w.
+    // A start that follows a residual launch first tests that true residual
+    // (its r.r and b.b partials, reduced in the order an iteration launch
+    // uses): a system that already meets the tolerance settles here, one
+    // launch before the iteration launch would, with the same decision.
+    if (mode == MODE_START && c.kr >= 0) {
+      __shared__ double chk[4];  // r.r and b.b of both right-hand sides
+      const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+      if (w < 4) {
+        const double* src = w < 2 ? part + ((k - 1) & 1) * NDOT * nd + (4 + w) * nd : part_bb + (w - 2) * nd;
+        double a = 0.0;
+        if (l < sd.ndir) a = src[sd.dir0 + l];
+        for (int t = l + 64; t < sd.ndir; t += 64) a += src[sd.dir0 + t];
+        a = xr_wave_sum(a);
+        if (l == 0) chk[w] = a;
+      }
+      __syncthreads();
+      if (chk[0] <= tol2 * chk[2] && chk[1] <= tol2 * chk[3]) {
+        if (lead) settle(&cg[sys], cnt, c, CG_DONE, k);
+        return;
+      }
+    }
     if (lead) {
       CgCtl s = c;
       if (mode == MODE_RESIDUAL) s.restarts = c.restarts + 1;
@@ -969,19 +994,22 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
       if (ch + 1 < c1) __syncthreads();
     }
-    if (mode == MODE_RESIDUAL) return;
   }
   if (threadIdx.x < 128) {
     // wave h holds right-hand side h of every row: one wave sum per series,
-    // lane 0 stores series 2q + h
+    // lane 0 stores series 2q + h (after a residual launch only r.r, and b.b)
     const int h = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < 4; ++q) dots[q] = xr_wave_sum(dots[q]);
     if ((threadIdx.x & 63) == 0) {
       double* out = part + (k & 1) * NDOT * nd;
+      if (mode == MODE_RESIDUAL) {
+        out[(4 + h) * nd + b] = dots[2];
+      } else {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) out[(2 * q + h) * nd + b] = dots[q];
-      if (mode == MODE_START) part_bb[h * nd + b] = dots[3];
+        for (int q = 0; q < 3; ++q) out[(2 * q + h) * nd + b] = dots[q];
+      }
+      if (mode != MODE_ITER) part_bb[h * nd + b] = dots[3];
     }
   }
 }
