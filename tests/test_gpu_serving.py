@@ -235,6 +235,31 @@ def test_extrapolated_qeq_guesses_keep_results_and_cut_iterations():
         np.testing.assert_allclose(b.charges, a.charges, rtol=0, atol=1e-8)
 
 
+def test_served_nve_with_kept_skin_lists_is_bitwise_fresh():
+    """NVE segments served with kept skin lists (neighbor_skin) give bitwise the results of fresh lists,
+    with far fewer list builds than evaluations."""
+    reqs = []
+    for rid, (s, x, cell) in enumerate((
+        (*workloads.water_cluster_of(40, 5), None),
+        workloads.water_cell(50, seed=4),
+        (*workloads.water_cluster_of(20, 9), None),
+    )):
+        reqs.append(dict(rid=rid, force_field="ffield.reax.HO.2015", symbols=list(s), positions=x, cell=cell,
+                         pbc=(cell is not None,) * 3, total_charge=0.0, md_steps=30, dt=0.25, temperature=300.0,
+                         velocity_seed=7 + rid))
+    fresh, _ = _serve(reqs)
+    kept, srv = _serve(reqs, neighbor_skin=0.3)
+    builds = sum(eng.neighbor_list_builds for eng in srv.engines.values())
+    print(f"[serving] kept skin lists: {builds} list builds for {3 * 31} system evaluations", flush=True)
+    assert builds < 31
+    for req in reqs:
+        a, b = fresh[req["rid"]], kept[req["rid"]]
+        np.testing.assert_array_equal(a.positions, b.positions)
+        np.testing.assert_array_equal(a.velocities, b.velocities)
+        np.testing.assert_array_equal(a.potential, b.potential)
+        np.testing.assert_array_equal(a.charges, b.charges)
+
+
 def test_served_relaxation_follows_oracle_fire():
     reqs = [r for r in _requests(20, seed=13, max_atoms=40) if r["md_steps"] == 0][:2]
     for r in reqs:

@@ -191,14 +191,19 @@ class Server:
     precision: "fp64" or "mixed" QEq; warm_start: start each QEq solve from the
     request's previous (s, t); extrapolate: for NVE segments, start instead
     from the polynomial through the last (up to four) solutions, one step
-    ahead, which roughly halves the warm solve's iterations; max_systems: cap on requests per batch (1 gives
+    ahead, which roughly halves the warm solve's iterations; neighbor_skin: keep
+    each group's device-built neighbor list with this per-atom allowance (the
+    reference ASE calculator's neighbor_skin) and reuse it while the group's
+    batch keeps its shape and no atom has moved further (results bitwise those
+    of fresh lists; pays for NVE-heavy work, while a batch that changes at
+    every step rebuilds every step); max_systems: cap on requests per batch (1 gives
     the one-request-at-a-time plain path); budget_bytes / max_batch_atoms bound
     the active set; full_derivative: charge-response forces (the exact gradient
     of the QEq-equilibrated energy, calculator.py:79-86) for every request.
     """
 
     def __init__(self, force_fields=("ffield.reax.HO.2015", "ffield.reax.CHO.2008"), device=0, *,
-                 precision="fp64", warm_start=True, extrapolate=True, tol=1e-10, max_systems=None, max_batch_atoms=262144,
+                 precision="fp64", warm_start=True, extrapolate=True, neighbor_skin=0.0, tol=1e-10, max_systems=None, max_batch_atoms=262144,
                  budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, full_derivative=False,
                  clock=time.perf_counter):
         import torch
@@ -214,6 +219,7 @@ class Server:
         self.precision = precision
         self.warm_start = warm_start
         self.extrapolate = bool(extrapolate)
+        self.neighbor_skin = float(neighbor_skin)
         self.tol = tol
         self.max_systems = max_systems
         self.max_batch_atoms = max_batch_atoms
@@ -344,7 +350,7 @@ class Server:
                    total_bond_orders=None, lone_pairs=None, bond_counts=None, qeq_state=g.QS)
         res = eng.evaluate_batch(g.X, g.T, offsets, g.cells, g.pbcs, g.charges, tol=self.tol,
                                  precision=self.precision, warm_start=self.warm_start,
-                                 full_derivative=self.full_derivative, out=out)
+                                 full_derivative=self.full_derivative, out=out, neighbor_skin=self.neighbor_skin)
         kin = None
         if g.md.any():
             kin_t = torch.zeros(ns, dtype=torch.float64, device=dev)
