@@ -60,6 +60,9 @@ struct FFDev {
   const double* ang;           // [rows][7] (theta00 v1 v2 coa1 v7 pen1 v4), |v1|>0.001
   const double* tor;           // [nt^4][5] keyed (t_i, t_j, t_k, t_l)
   const unsigned char* tor_on; // [nt^4]
+  const unsigned char* tor_mid;    // [nt^2] (t_j, t_k): some torsion row exists
+  const unsigned char* tor_left;   // [nt^3] (t_i, t_j, t_k): some row exists for some t_l
+  const unsigned char* tor_right;  // [nt^3] (t_j, t_k, t_l): some row exists for some t_i
   const double* hb;            // [nt^3][4] keyed (t_donor, t_H, t_acceptor)
   const unsigned char* hb_on;  // [nt^3]
   const double* vdw_b;         // [nt*nt] (1/gamma_w)^p_vdw, the shielding offset of each pair

@@ -157,7 +157,7 @@ struct xreac_ctx {
 namespace {
 
 constexpr int kBondScalars = 27;  // per-bond double arrays carved below
-constexpr int kAtomScalars = 21;  // per-atom double arrays carved below
+constexpr int kAtomScalars = 22;  // per-atom double arrays carved below
 
 int fail(xreac_ctx* c, int code, const std::string& msg) {
   c->message = msg;
@@ -261,6 +261,23 @@ int xreac_ff_create(const xreac_ff_tables* t, xreac_ff** out) {
   if (e == hipSuccess) e = upload(ff, t->angle_rows, nang * ANG_W, &ff->dev.ang);
   if (e == hipSuccess) e = upload(ff, t->torsion, nt * nt * nt * nt * TOR_W, &ff->dev.tor);
   if (e == hipSuccess) e = upload(ff, t->torsion_on, nt * nt * nt * nt, &ff->dev.tor_on);
+  if (e == hipSuccess) {
+    // Which type pairs (j, k) and triples (i, j, k), (j, k, l) have any torsion parameters: the torsion
+    // kernel skips middle bonds and bond loops that no parameter row can match.
+    std::vector<unsigned char> mid(nt * nt, 0), left(nt * nt * nt, 0), right(nt * nt * nt, 0);
+    for (size_t i = 0; i < nt; ++i)
+      for (size_t j = 0; j < nt; ++j)
+        for (size_t k = 0; k < nt; ++k)
+          for (size_t l = 0; l < nt; ++l)
+            if (t->torsion_on[((i * nt + j) * nt + k) * nt + l]) {
+              mid[j * nt + k] = 1;
+              left[(i * nt + j) * nt + k] = 1;
+              right[(j * nt + k) * nt + l] = 1;
+            }
+    e = upload(ff, mid.data(), nt * nt, &ff->dev.tor_mid);
+    if (e == hipSuccess) e = upload(ff, left.data(), nt * nt * nt, &ff->dev.tor_left);
+    if (e == hipSuccess) e = upload(ff, right.data(), nt * nt * nt, &ff->dev.tor_right);
+  }
   if (e == hipSuccess) e = upload(ff, t->hbond, nt * nt * nt * HB_W, &ff->dev.hb);
   if (e == hipSuccess) e = upload(ff, t->hbond_on, nt * nt * nt, &ff->dev.hb_on);
   if (e == hipSuccess) {
@@ -722,6 +739,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   at.A_sbo2 = nxt();
   at.a_ov2 = nxt();
   at.a_totp = nxt();
+  at.bomax = nxt();
   if (sp > scal_end) return fail(ctx, XREAC_INVALID, "internal error: per-atom workspace layout");
   // Per-atom properties the caller asked for are computed in its buffers directly.
   if (out.tbo) at.total = out.tbo;

@@ -280,13 +280,19 @@ __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, 
       const int m = half ? mb : rv;
       const int j = bd.bi[m], k = bd.bj[m];
       const int tj = at.type[j], tk = at.type[k];
+      if (!ff.tor_mid[tj * ff.nt + tk]) continue;  // no torsion row has this middle pair (uniform in the group)
       int msx, msy, msz;
       xr_unpack_shift(bd.code[m], msx, msy, msz);
       const double dbj = at.db[j], dbk = at.db[k];
+      const double bm = bd.BO[m];
       if (half) {
-        // x = j: this lane's left bond a = mine, right bonds c of k
+        // x = j: this lane's left bond a = mine, right bonds c of k.  A lane whose left bond can
+        // satisfy neither the parameter test (some row for (t_i, t_j, t_k)) nor the product test
+        // with the largest bond order of k skips the loop: both bounds are exact (the product
+        // (b_a b_m) b_c is monotone in b_c, rounding included), so no torsion is lost.
         double gm0 = 0.0, gm1 = 0.0, gm2 = 0.0, abm = 0.0, apim = 0.0;
-        if (owns) {
+        if (owns && ff.tor_left[(at.type[bd.bj[mine]] * ff.nt + tj) * ff.nt + tk] &&
+            bd.BO[mine] * bm * at.bomax[k] > THB_CUT) {
           for (int c = bd.ptr[k]; c < bd.ptr[k + 1]; ++c) {
             const int key = torsion_key(at, bd, ff, mine, m, c, tj, tk, msx, msy, msz);
             if (key < 0) continue;
@@ -321,8 +327,10 @@ __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, 
           bd.A_BO[m] += abm;
           bd.A_PI[m] += apim;
         }
-      } else if (owns) {
-        // x = k: this lane's right bond c = mine, left bonds a of j
+      } else if (owns && ff.tor_right[(tj * ff.nt + tk) * ff.nt + at.type[bd.bj[mine]]] &&
+                 at.bomax[j] * bm * bd.BO[mine] > THB_CUT) {
+        // x = k: this lane's right bond c = mine, left bonds a of j (pruned as above: the product
+        // (b_a b_m) b_c is monotone in b_a)
         for (int a = bd.ptr[j]; a < bd.ptr[j + 1]; ++a) {
           const int key = torsion_key(at, bd, ff, a, m, mine, tj, tk, msx, msy, msz);
           if (key < 0) continue;
@@ -511,6 +519,15 @@ __global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, FFDev f
   }
 }
 
+// Largest corrected bond order over each atom's bonds (0 without bonds).
+__global__ void k_atom_bomax(Atoms at, Bonds bd) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= at.n) return;
+  double m = 0.0;
+  for (int b = bd.ptr[x]; b < bd.ptr[x + 1]; ++b) m = fmax(m, bd.BO[b]);
+  at.bomax[x] = m;
+}
+
 // One wave per block: per-atom and per-bond kernels have a few thousand
 // threads, and small blocks spread them over every CU.
 constexpr int BLK = 64;
@@ -522,6 +539,7 @@ hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* ro
                        const int* nbr_s, const unsigned char* eflag, int lammps_angles, EnergyAcc acc,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_angles, grid(at.n), dim3(BLK), 0, s, at, bd, ff, lammps_angles, acc);
+  hipLaunchKernelGGL(k_atom_bomax, grid(at.n), dim3(BLK), 0, s, at, bd);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
   const int per_block = 4 * (64 / TOR_GROUP);
   hipLaunchKernelGGL(k_torsions<TOR_GROUP>, dim3(at.n > 0 ? (at.n + per_block - 1) / per_block : 1), dim3(256), 0,

@@ -50,6 +50,7 @@ struct Atoms {
   double* total, *d, *db, *nlp, *vlpex, *dlp, *dlpt, *sbo, *sbo2, *prod, *ov2;
   int* bond_count;         // bonds with BO > 0.3
   double *A_total, *A_d, *A_db, *A_nlp, *A_dlpt, *A_dlp, *A_sbo2, *a_ov2, *a_totp;
+  double* bomax;           // largest corrected bond order over the atom's bonds (torsion pruning)
   double* F_nb;            // [n*3] nonbonded forces
   double* F_hb;            // [n*3] hydrogen-bond acceptor-vector forces
   double* q;               // charges
