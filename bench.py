@@ -428,7 +428,7 @@ def water_box_bench(args):
 
 
 def system_bench(args, name, ffname, build, steps):
-    """FP64 at 1e-10 against the mixed form on one system (C2, C4)."""
+    """FP64 at 1e-10 against the mixed form on one system (the CHO mixture, the anatase/water slab)."""
     import torch
 
     from xreac_amd import ForceField
