@@ -260,6 +260,27 @@ def test_served_nve_with_kept_skin_lists_is_bitwise_fresh():
         np.testing.assert_array_equal(a.charges, b.charges)
 
 
+def test_served_mix_with_kept_skin_lists_is_bitwise_fresh():
+    """A mixed stream (single evaluations, NVE segments, relaxations) whose batches change at every step:
+    kept skin lists are rebuilt whenever a group's batch changes, and every result stays bitwise that of
+    fresh lists."""
+    reqs = _requests(16, seed=23, relax_every=4)
+    kw = dict(max_batch_atoms=500, relax_iterations=6)
+    fresh, _ = _serve(reqs, **kw)
+    kept, srv = _serve(reqs, neighbor_skin=0.3, **kw)
+    builds = sum(eng.neighbor_list_builds for eng in srv.engines.values())
+    print(f"[serving] mixed stream with kept skin lists: {srv.steps} steps, {builds} list builds", flush=True)
+    assert builds > 0
+    for req in reqs:
+        a, b = fresh[req["rid"]], kept[req["rid"]]
+        assert a.evaluations == b.evaluations
+        np.testing.assert_array_equal(a.forces, b.forces)
+        np.testing.assert_array_equal(a.charges, b.charges)
+        np.testing.assert_array_equal(a.components, b.components)
+        if req["md_steps"] or req.get("relax"):
+            np.testing.assert_array_equal(a.positions, b.positions)
+
+
 def test_served_relaxation_follows_oracle_fire():
     reqs = [r for r in _requests(20, seed=13, max_atoms=40) if r["md_steps"] == 0][:2]
     for r in reqs:
