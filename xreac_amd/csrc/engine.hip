@@ -110,14 +110,16 @@ struct xreac_ctx {
   // (slab of skin_stride keys per atom), their lengths, and the positions and
   // types they were built at; valid for calls of the same shape as the last.
   Buffer skin_slab_buf, skin_counts_buf, skin_xref_buf, skin_tref_buf;
+  // The kept rows are geometry only: they depend on the cutoff, the skin and
+  // the systems' atom ranges and cells (types are compared on the device).
   struct SkinKey {
-    const void* ff = nullptr;
+    double rc = 0.0;
     double skin = 0.0;
     std::vector<int32_t> offsets;
     std::vector<double> cells;
     std::vector<int32_t> pbc;
     bool operator==(const SkinKey& o) const {
-      return ff == o.ff && skin == o.skin && offsets == o.offsets && cells == o.cells && pbc == o.pbc;
+      return rc == o.rc && skin == o.skin && offsets == o.offsets && cells == o.cells && pbc == o.pbc;
     }
   } skin_key;
   bool skin_valid = false;
@@ -526,7 +528,6 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   } active(arena ? nullptr : &ctx->prof);  // no timer events (which would be created) in static scratch
   const double rc = ffh->cutoff;
   const double bond_cut = std::min(5.0, rc);
-  // Skin reuse applies to lists built here (not to supplied ones); bins are then wide enough for rc + skin.
   // ase.py:90-93 (the same check and text)
   if (!(opt->neighbor_skin >= 0.0) || !std::isfinite(opt->neighbor_skin))
     return fail(ctx, XREAC_INVALID, "neighbor_skin must be a finite nonnegative number");
@@ -868,7 +869,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
       }
       const int stride_s = xr_fill_stride(hint_s);
       xreac_ctx::SkinKey key;
-      key.ff = ffh;
+      key.rc = rc;
       key.skin = skin;
       key.offsets.assign(in.offsets, in.offsets + ns + 1);
       if (in.cells) key.cells.assign(in.cells, in.cells + 9 * (size_t)ns);
