@@ -203,8 +203,8 @@ class Server:
     """
 
     def __init__(self, force_fields=("ffield.reax.HO.2015", "ffield.reax.CHO.2008"), device=0, *,
-                 precision="fp64", warm_start=True, extrapolate=True, neighbor_skin=0.0, tol=1e-10, max_systems=None, max_batch_atoms=262144,
-                 budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, full_derivative=False,
+                 precision="fp64", warm_start=True, extrapolate=True, neighbor_skin=0.0, tol=1e-10,
+                 max_systems=None, max_batch_atoms=262144, budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, full_derivative=False,
                  clock=time.perf_counter):
         import torch
 
