@@ -10,9 +10,10 @@
 //   H-bonds   - hydrogen j, donor bond to a heavy atom with BO >= 0.01 and any
 //               heavy-atom edge within min(7.5, rc) other than that same edge.
 // An item's derivative is deposited only into slots owned by the thread that
-// enumerates it (its own bond row and atom).  Torsions are enumerated from
-// both ends of the middle bond and H-bonds from both the hydrogen and the
-// acceptor, so each end writes its own share without atomics.
+// enumerates it (its own bond row and atom).  Angles are enumerated from both
+// of their bonds, torsions from both ends of the middle bond and H-bonds from
+// both the hydrogen and the acceptor, so each end writes its own share
+// without atomics.
 #include "state.h"
 
 namespace {
@@ -30,122 +31,188 @@ __device__ inline void cross3(const double* a, const double* b, double* c) {
 }
 
 // ------------------------------------------------------------------ angles
-// One thread per centre over its unordered bond pairs (a < c), in row order.
-// Centres have few bonds, so a wave per centre (each pair evaluated by the
-// lanes of both bonds) was measured slower on liquid water.
-__global__ void __launch_bounds__(256) k_angles(Atoms at, Bonds bd, FFDev ff, int lammps, EnergyAcc acc) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  double e[3] = {0.0, 0.0, 0.0};
-  if (j < at.n) {
-    const double* g = ff.gen;
-    const int tj = at.type[j];
-    const double* aj = AP(ff, tj);
-    const int nt = ff.nt;
-    const double db = at.db[j], d = at.d[j], totj = at.total[j], sbo2 = at.sbo2[j];
-    const double p3 = aj[AP_PVAL3], p5 = aj[AP_PVAL5];
-    // centre-only factors
-    const double ex3 = exp(-g[20] * d), ex4 = exp(g[21] * d);
-    const double Q = (2 + ex3) / (1 + ex3 + ex4);
-    const double dQ = (-g[20] * ex3 * (1 + ex3 + ex4) - (2 + ex3) * (-g[20] * ex3 + g[21] * ex4)) /
-                      ((1 + ex3 + ex4) * (1 + ex3 + ex4));
-    const double ecj = exp(g[2] * (totj - aj[AP_VALVAL]));
-    const double Cj = 1 / (1 + ecj);
-    const double dCj = -g[2] * ecj * Cj * Cj;
-    const double esb = exp(-g[17] * (2 - sbo2));
-    double A_db = 0.0, A_d = 0.0, A_tot = 0.0, A_sbo2 = 0.0;
-    const int b0 = bd.ptr[j], b1 = bd.ptr[j + 1];
-    for (int a = b0; a < b1; ++a) {
-      const double Ba = bd.BO[a];
-      if (!(Ba > THB_CUT)) continue;
-      for (int c = a + 1; c < b1; ++c) {
-        const double Bc = bd.BO[c];
-        if (!(Bc > THB_CUT)) continue;
-        if (Ba * Bc <= THB_PRODUCT_CUT) continue;
-        const int ia = bd.bj[a], kc = bd.bj[c];
-        const int key = (at.type[ia] * nt + tj) * nt + at.type[kc];
-        const int r0 = ff.ang_off[key], r1 = ff.ang_off[key + 1];
-        if (r0 == r1) continue;
-        const double* va = bd.vec + 3 * a;
-        const double* vc = bd.vec + 3 * c;
-        const double ra = bd.r[a], rc = bd.r[c];
-        const double cosv = dot3(va, vc) / (ra * rc);
-        double theta, dth;
-        if (lammps) {
-          const double cc = fmin(fmax(cosv, -1.0), 1.0);
-          theta = acos(cc);
-          dth = -1.0 / fmax(sin(theta), 1e-5);
-        } else {
-          const double lo = -1 + 1e-14, hi = 1 - 1e-14;
-          const double cc = fmin(fmax(cosv, lo), hi);
-          theta = acos(cc);
-          dth = (cc == lo || cc == hi) ? 0.0 : -1.0 / sqrt(1 - cc * cc);
-        }
-        const double bij = Ba - THB_CUT, bjk = Bc - THB_CUT;
-        const double tota = at.total[ia], totc = at.total[kc];
-        double gA = 0.0, gC = 0.0, dE_dcos = 0.0, a_ta = 0.0, a_tc = 0.0;
-        for (int row = r0; row < r1; ++row) {
-          const double* pr = ff.ang + row * ANG_W;
-          const double th00 = pr[0], v1 = pr[1], v2 = pr[2], coa1 = pr[3], v7 = pr[4], pen1 = pr[5], v4 = pr[6];
-          const double theta0 = (180 - th00 * (1 - esb)) * DEG;
-          const double pa = pow(bij, v4), pc = pow(bjk, v4);
-          const double ea = exp(-p3 * pa), ec = exp(-p3 * pc);
-          const double Fa = 1 - ea, Fc = 1 - ec;
-          const double f7 = Fa * Fc;
-          const double ex6 = exp(g[14] * db), ex7 = exp(-v7 * db);
-          const double den8 = 1 + ex6 + ex7;
-          const double R = (2 + ex6) / den8;
-          const double f8 = p5 - (p5 - 1) * R;
-          const double dR = (g[14] * ex6 * den8 - (2 + ex6) * (g[14] * ex6 - v7 * ex7)) / (den8 * den8);
-          const double dt = theta0 - theta;
-          const double exang = exp(-v2 * (dt * dt));
-          const double s = v1 >= 0 ? 1 - exang : -exang;
-          e[0] += f7 * f8 * v1 * s;
-          const double dE_dth = -f7 * f8 * v1 * 2 * v2 * dt * exang;
-          dE_dcos += dE_dth * dth;
-          A_sbo2 += -dE_dth * th00 * g[17] * esb * DEG;
-          const double dE_df7 = f8 * v1 * s, dE_df8 = f7 * v1 * s;
-          gA += dE_df7 * Fc * p3 * v4 * (bij > 0 ? pa / bij : 0.0) * ea;
-          gC += dE_df7 * Fa * p3 * v4 * (bjk > 0 ? pc / bjk : 0.0) * ec;
-          A_db += dE_df8 * (-(p5 - 1) * dR);
-          // penalty
-          const double Ebb = exp(-g[19] * ((bij - 2) * (bij - 2) + (bjk - 2) * (bjk - 2)));
-          const double Ep = pen1 * Q * Ebb;
-          e[1] += Ep;
-          A_d += pen1 * dQ * Ebb;
-          gA += Ep * (-2 * g[19] * (bij - 2));
-          gC += Ep * (-2 * g[19] * (bjk - 2));
-          // angle conjugation
-          const double E1 = exp(-g[38] * ((tota - bij) * (tota - bij) + (totc - bjk) * (totc - bjk)));
-          const double E2 = exp(-g[30] * ((bij - 1.5) * (bij - 1.5) + (bjk - 1.5) * (bjk - 1.5)));
-          const double Ec = coa1 * Cj * E1 * E2;
-          e[2] += Ec;
-          A_tot += coa1 * dCj * E1 * E2;
-          a_ta += Ec * (-2 * g[38] * (tota - bij));
-          a_tc += Ec * (-2 * g[38] * (totc - bjk));
-          gA += Ec * (2 * g[38] * (tota - bij) - 2 * g[30] * (bij - 1.5));
-          gC += Ec * (2 * g[38] * (totc - bjk) - 2 * g[30] * (bjk - 1.5));
-        }
-        bd.A_BO[a] += gA;
-        bd.A_BO[c] += gC;
-        bd.A_totfar[a] += a_ta;
-        bd.A_totfar[c] += a_tc;
-        const double inv = 1.0 / (ra * rc);
+// Factors of an angle term that depend on the centre j alone.
+struct AngleCentre {
+  double db, d, sbo2, esb, ex6, Q, dQ, Cj, dCj, p3, p5;
+  int tj;
+};
+
+__device__ inline AngleCentre angle_centre(const Atoms& at, const FFDev& ff, int j) {
+  const double* g = ff.gen;
+  AngleCentre c;
+  c.tj = at.type[j];
+  const double* aj = AP(ff, c.tj);
+  c.db = at.db[j];
+  c.d = at.d[j];
+  c.sbo2 = at.sbo2[j];
+  c.p3 = aj[AP_PVAL3];
+  c.p5 = aj[AP_PVAL5];
+  const double ex3 = exp(-g[20] * c.d), ex4 = exp(g[21] * c.d);
+  c.Q = (2 + ex3) / (1 + ex3 + ex4);
+  c.dQ = (-g[20] * ex3 * (1 + ex3 + ex4) - (2 + ex3) * (-g[20] * ex3 + g[21] * ex4)) /
+         ((1 + ex3 + ex4) * (1 + ex3 + ex4));
+  const double ecj = exp(g[2] * (at.total[j] - aj[AP_VALVAL]));
+  c.Cj = 1 / (1 + ecj);
+  c.dCj = -g[2] * ecj * c.Cj * c.Cj;
+  c.esb = exp(-g[17] * (2 - c.sbo2));
+  c.ex6 = exp(g[14] * c.db);
+  return c;
+}
+
+// One angle (a, j, c) of bonds a < c of the centre, summed over its parameter
+// rows: the shares of the two bonds' slots (A_BO, A_totfar, bond-vector
+// derivative) and of the centre (energies and adjoints).
+struct AnglePair {
+  double gA, gC, ta, tc;
+  double GA[3], GC[3];
+  double e[3];
+  double A_db, A_d, A_tot, A_sbo2;
+};
+
+// False when the reference's enumeration (energy.py:248-277) skips the pair:
+// a bond order at or below 0.001, a product at or below 1e-5, or no row.
+__device__ inline bool angle_pair(const Atoms& at, const Bonds& bd, const FFDev& ff, int lammps,
+                                  const AngleCentre& cj, int a, int c, AnglePair& o) {
+  const double Ba = bd.BO[a], Bc = bd.BO[c];
+  if (!(Ba > THB_CUT) || !(Bc > THB_CUT) || Ba * Bc <= THB_PRODUCT_CUT) return false;
+  const int nt = ff.nt;
+  const int ia = bd.bj[a], kc = bd.bj[c];
+  const int key = (at.type[ia] * nt + cj.tj) * nt + at.type[kc];
+  const int r0 = ff.ang_off[key], r1 = ff.ang_off[key + 1];
+  if (r0 == r1) return false;
+  const double* g = ff.gen;
+  const double* va = bd.vec + 3 * a;
+  const double* vc = bd.vec + 3 * c;
+  const double ra = bd.r[a], rc = bd.r[c];
+  const double cosv = dot3(va, vc) / (ra * rc);
+  double theta, dth;
+  if (lammps) {
+    const double cc = fmin(fmax(cosv, -1.0), 1.0);
+    theta = acos(cc);
+    dth = -1.0 / fmax(sin(theta), 1e-5);
+  } else {
+    const double lo = -1 + 1e-14, hi = 1 - 1e-14;
+    const double cc = fmin(fmax(cosv, lo), hi);
+    theta = acos(cc);
+    dth = (cc == lo || cc == hi) ? 0.0 : -1.0 / sqrt(1 - cc * cc);
+  }
+  const double bij = Ba - THB_CUT, bjk = Bc - THB_CUT;
+  const double tota = at.total[ia], totc = at.total[kc];
+  const double p3 = cj.p3, p5 = cj.p5, db = cj.db, esb = cj.esb, ex6 = cj.ex6;
+  o = AnglePair{};
+  double dE_dcos = 0.0;
+  for (int row = r0; row < r1; ++row) {
+    const double* pr = ff.ang + row * ANG_W;
+    const double th00 = pr[0], v1 = pr[1], v2 = pr[2], coa1 = pr[3], v7 = pr[4], pen1 = pr[5], v4 = pr[6];
+    const double theta0 = (180 - th00 * (1 - esb)) * DEG;
+    const double pa = pow(bij, v4), pc = pow(bjk, v4);
+    const double ea = exp(-p3 * pa), ec = exp(-p3 * pc);
+    const double Fa = 1 - ea, Fc = 1 - ec;
+    const double f7 = Fa * Fc;
+    const double ex7 = exp(-v7 * db);
+    const double den8 = 1 + ex6 + ex7;
+    const double R = (2 + ex6) / den8;
+    const double f8 = p5 - (p5 - 1) * R;
+    const double dR = (g[14] * ex6 * den8 - (2 + ex6) * (g[14] * ex6 - v7 * ex7)) / (den8 * den8);
+    const double dt = theta0 - theta;
+    const double exang = exp(-v2 * (dt * dt));
+    const double s = v1 >= 0 ? 1 - exang : -exang;
+    o.e[0] += f7 * f8 * v1 * s;
+    const double dE_dth = -f7 * f8 * v1 * 2 * v2 * dt * exang;
+    dE_dcos += dE_dth * dth;
+    o.A_sbo2 += -dE_dth * th00 * g[17] * esb * DEG;
+    const double dE_df7 = f8 * v1 * s, dE_df8 = f7 * v1 * s;
+    o.gA += dE_df7 * Fc * p3 * v4 * (bij > 0 ? pa / bij : 0.0) * ea;
+    o.gC += dE_df7 * Fa * p3 * v4 * (bjk > 0 ? pc / bjk : 0.0) * ec;
+    o.A_db += dE_df8 * (-(p5 - 1) * dR);
+    // penalty
+    const double Ebb = exp(-g[19] * ((bij - 2) * (bij - 2) + (bjk - 2) * (bjk - 2)));
+    const double Ep = pen1 * cj.Q * Ebb;
+    o.e[1] += Ep;
+    o.A_d += pen1 * cj.dQ * Ebb;
+    o.gA += Ep * (-2 * g[19] * (bij - 2));
+    o.gC += Ep * (-2 * g[19] * (bjk - 2));
+    // angle conjugation
+    const double E1 = exp(-g[38] * ((tota - bij) * (tota - bij) + (totc - bjk) * (totc - bjk)));
+    const double E2 = exp(-g[30] * ((bij - 1.5) * (bij - 1.5) + (bjk - 1.5) * (bjk - 1.5)));
+    const double Ec = coa1 * cj.Cj * E1 * E2;
+    o.e[2] += Ec;
+    o.A_tot += coa1 * cj.dCj * E1 * E2;
+    o.ta += Ec * (-2 * g[38] * (tota - bij));
+    o.tc += Ec * (-2 * g[38] * (totc - bjk));
+    o.gA += Ec * (2 * g[38] * (tota - bij) - 2 * g[30] * (bij - 1.5));
+    o.gC += Ec * (2 * g[38] * (totc - bjk) - 2 * g[30] * (bjk - 1.5));
+  }
+  const double inv = 1.0 / (ra * rc);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          bd.G[3 * a + k] += dE_dcos * (vc[k] * inv - cosv * va[k] / (ra * ra));
-          bd.G[3 * c + k] += dE_dcos * (va[k] * inv - cosv * vc[k] / (rc * rc));
-        }
+  for (int k = 0; k < 3; ++k) {
+    o.GA[k] = dE_dcos * (vc[k] * inv - cosv * va[k] / (ra * ra));
+    o.GC[k] = dE_dcos * (va[k] * inv - cosv * vc[k] / (rc * rc));
+  }
+  return true;
+}
+
+// One group of ANG_GROUP lanes per centre x; lane l owns bond slot p0 + l
+// (rows longer than the group are taken in chunks).  The lane walks every
+// other bond y of the row in row order and evaluates the angle of the pair
+// (min, max): its own slot takes that angle's share for the bond it owns, in
+// the order of the pairs, and the lane of the lower bond adds the centre's
+// share, which is then reduced over the group.  Each angle is evaluated by
+// both of its lanes: the work is doubled, but spread over a lane per bond
+// instead of a thread per centre, which leaves most of the chip idle.  Every
+// slot has one writer and a fixed order, so the result is deterministic and
+// the same for a system alone or in a batch.
+constexpr int ANG_GROUP = 8;
+__global__ void __launch_bounds__(64) k_angles(Atoms at, Bonds bd, FFDev ff, int lammps, EnergyAcc acc) {
+  const int x = blockIdx.x * (64 / ANG_GROUP) + (threadIdx.x & 63) / ANG_GROUP;
+  const int lane = threadIdx.x & (ANG_GROUP - 1);
+  if (x >= at.n) return;  // the whole group
+  const AngleCentre cj = angle_centre(at, ff, x);
+  const int p0 = bd.ptr[x], p1 = bd.ptr[x + 1];
+  double e[3] = {0.0, 0.0, 0.0};
+  double A_db = 0.0, A_d = 0.0, A_tot = 0.0, A_sbo2 = 0.0;
+  for (int chunk = p0; chunk < p1; chunk += ANG_GROUP) {
+    const int mine = chunk + lane;
+    if (mine >= p1 || !(bd.BO[mine] > THB_CUT)) continue;
+    double abo = bd.A_BO[mine], afar = bd.A_totfar[mine];
+    double G[3] = {bd.G[3 * mine], bd.G[3 * mine + 1], bd.G[3 * mine + 2]};
+    for (int y = p0; y < p1; ++y) {
+      if (y == mine) continue;
+      const bool low = mine < y;
+      AnglePair o;
+      if (!angle_pair(at, bd, ff, lammps, cj, low ? mine : y, low ? y : mine, o)) continue;
+      abo += low ? o.gA : o.gC;
+      afar += low ? o.ta : o.tc;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) G[k] += low ? o.GA[k] : o.GC[k];
+      if (low) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e[k] += o.e[k];
+        A_db += o.A_db;
+        A_d += o.A_d;
+        A_tot += o.A_tot;
+        A_sbo2 += o.A_sbo2;
       }
     }
-    at.A_db[j] += A_db;
-    at.A_d[j] += A_d;
-    at.A_total[j] += A_tot;
-    at.A_sbo2[j] += A_sbo2;
+    bd.A_BO[mine] = abo;
+    bd.A_totfar[mine] = afar;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bd.G[3 * mine + k] = G[k];
   }
-  if (j < at.n) {
-    xr_energy(acc, CH_ANGLE, j, e[0]);
-    xr_energy(acc, CH_PEN, j, e[1]);
-    xr_energy(acc, CH_COA, j, e[2]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) e[k] = xr_group_sum<ANG_GROUP>(e[k]);
+  A_db = xr_group_sum<ANG_GROUP>(A_db);
+  A_d = xr_group_sum<ANG_GROUP>(A_d);
+  A_tot = xr_group_sum<ANG_GROUP>(A_tot);
+  A_sbo2 = xr_group_sum<ANG_GROUP>(A_sbo2);
+  if (lane == 0) {
+    at.A_db[x] += A_db;
+    at.A_d[x] += A_d;
+    at.A_total[x] += A_tot;
+    at.A_sbo2[x] += A_sbo2;
+    xr_energy(acc, CH_ANGLE, x, e[0]);
+    xr_energy(acc, CH_PEN, x, e[1]);
+    xr_energy(acc, CH_COA, x, e[2]);
   }
 }
 
@@ -395,18 +462,15 @@ __device__ inline void hbond_eval(const double* prm, double BOd, const double* v
 
 constexpr int HB_QUEUE = 256;  // LDS slots per wave for H-bond candidate edges
 
-// One wave per hydrogen: donor-side energy and derivatives.  The row's
+// Donor side of hydrogen j (one wave): energy and derivatives.  The row's
 // candidate edges (acceptor type within min(7.5, rc), flagged by the edge
-// pass) are queued first, so each donor bond evaluates them in dense batches.
-__global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
-                                                  const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                                  const int* __restrict__ nbr_s,
-                                                  const unsigned char* __restrict__ eflag, EnergyAcc acc) {
-  __shared__ int queue[4][HB_QUEUE];
-  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+// pass) are queued first in q, so each donor bond evaluates them in dense
+// batches.
+__device__ inline void hb_donor(int j, int* q, const Atoms& at, const Bonds& bd, const FFDev& ff,
+                                const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
+                                const int* __restrict__ nbr_s, const unsigned char* __restrict__ eflag,
+                                const EnergyAcc& acc) {
   const int lane = threadIdx.x & 63;
-  if (j >= at.n || AP(ff, at.type[j])[AP_HBOND] != 1.0) return;
-  int* q = queue[threadIdx.x >> 6];
   const CellDev& cell = xr_cell(at.S, j);
   const int nt = ff.nt, tj = at.type[j];
   const double hcut = fmin(7.5, ff.gen[12]);
@@ -464,20 +528,15 @@ __global__ void __launch_bounds__(256) k_hb_donor(Atoms at, Bonds bd, FFDev ff,
   }
 }
 
-// One wave per heavy atom: force from the acceptor-vector derivative.  The
-// H-bond hydrogens within min(7.5, rc) (flagged by the edge pass) are queued,
-// then each lane takes one hydrogen and walks its donor bonds.
-__global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, FFDev ff,
-                                                     const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                                     const int* __restrict__ nbr_s,
-                                                     const unsigned char* __restrict__ eflag) {
-  __shared__ int queue[4][HB_QUEUE];
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+// Acceptor side of heavy atom k (one wave): force from the acceptor-vector
+// derivative.  The H-bond hydrogens within min(7.5, rc) (flagged by the edge
+// pass) are queued in q, then each lane takes one hydrogen and walks its
+// donor bonds.
+__device__ inline void hb_acceptor(int k, int* q, const Atoms& at, const Bonds& bd, const FFDev& ff,
+                                   const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
+                                   const int* __restrict__ nbr_s, const unsigned char* __restrict__ eflag) {
   const int lane = threadIdx.x & 63;
-  if (k >= at.n) return;
   const int tk = at.type[k];
-  if (AP(ff, tk)[AP_HBOND] != 2.0) return;
-  int* q = queue[threadIdx.x >> 6];
   const CellDev& cell = xr_cell(at.S, k);
   const int nt = ff.nt;
   const double hcut = fmin(7.5, ff.gen[12]);
@@ -519,6 +578,22 @@ __global__ void __launch_bounds__(256) k_hb_acceptor(Atoms at, Bonds bd, FFDev f
   }
 }
 
+// One wave per atom: a hydrogen of H-bond type 1 takes its donor side, a
+// heavy atom of type 2 its acceptor side.  The two sides write disjoint slots
+// (the hydrogen's force and donor bonds, the acceptor's force), so one launch
+// holds both.
+__global__ void __launch_bounds__(256) k_hbonds(Atoms at, Bonds bd, FFDev ff, const int* __restrict__ row_ptr,
+                                                const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
+                                                const unsigned char* __restrict__ eflag, EnergyAcc acc) {
+  __shared__ int queue[4][HB_QUEUE];
+  const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (x >= at.n) return;
+  const double role = AP(ff, at.type[x])[AP_HBOND];
+  int* q = queue[threadIdx.x >> 6];
+  if (role == 1.0) hb_donor(x, q, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag, acc);
+  else if (role == 2.0) hb_acceptor(x, q, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag);
+}
+
 // Largest corrected bond order over each atom's bonds (0 without bonds).
 __global__ void k_atom_bomax(Atoms at, Bonds bd) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -538,15 +613,16 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
                        const int* nbr_s, const unsigned char* eflag, int lammps_angles, EnergyAcc acc,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_angles, grid(at.n), dim3(BLK), 0, s, at, bd, ff, lammps_angles, acc);
+  const int angle_groups = 64 / ANG_GROUP;
+  hipLaunchKernelGGL(k_angles, dim3(at.n > 0 ? (at.n + angle_groups - 1) / angle_groups : 1), dim3(64), 0, s, at, bd,
+                     ff, lammps_angles, acc);
   hipLaunchKernelGGL(k_atom_bomax, grid(at.n), dim3(BLK), 0, s, at, bd);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
   const int per_block = 4 * (64 / TOR_GROUP);
   hipLaunchKernelGGL(k_torsions<TOR_GROUP>, dim3(at.n > 0 ? (at.n + per_block - 1) / per_block : 1), dim3(256), 0,
                      s, at, bd, ff, acc);
   hipLaunchKernelGGL(k_torsions<64>, wgrid, dim3(256), 0, s, at, bd, ff, acc);
-  hipLaunchKernelGGL(k_hb_donor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag, acc);
-  hipLaunchKernelGGL(k_hb_acceptor, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag);
+  hipLaunchKernelGGL(k_hbonds, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag, acc);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
