@@ -196,7 +196,7 @@ int xreac_evaluate(xreac_ctx* ctx, const xreac_ff* ff, const xreac_system* sys, 
  * forbid hipMalloc).  Every workspace comes from a file-scope static device
  * array of XREAC_STATIC_SCRATCH_BYTES; the side stream, events and pinned
  * read-back slots are made by xreac_ctx_create.  A system whose workspace
- * (about 15 KB per atom of liquid water, i.e. up to roughly 4,000 such atoms)
+ * (about 18 KB per atom of liquid water, i.e. up to roughly 3,600 such atoms)
  * does not fit returns XREAC_NOMEM with a message naming the size.  Results
  * are bitwise those of xreac_evaluate.  All calls on one device share the
  * array: calls that overlap in time on one device (two contexts, two host

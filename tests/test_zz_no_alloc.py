@@ -55,10 +55,11 @@ def test_static_scratch_matches_evaluate(precision):
 def test_static_scratch_rejects_large_systems():
     ff = ForceField.bundled("ffield.reax.HO.2015")
     eng = Engine(ff)
-    s, x, c = workloads.water_box(side=16, seed=1)  # 12,288 atoms: about 180 MB of workspace
+    s, x, c = workloads.water_box(side=16, seed=1)  # 12,288 atoms: about 220 MB of workspace
     print(f"[no-alloc] oversize n={len(s)}", flush=True)
-    with pytest.raises(DeviceError, match="static scratch"):
+    with pytest.raises(DeviceError, match="static scratch") as err:
         _evaluate(eng, s, x, c, True)
+    print(f"[no-alloc] {err.value}", flush=True)
     # the context still evaluates, with and without the static scratch
     s2, x2 = workloads.water_cluster()
     got, _ = _evaluate(eng, s2, x2, None, True)

@@ -116,9 +116,7 @@ __device__ inline void xr_flag(const Systems& S, int i, unsigned bits) { atomicO
 
 // Per-edge flag byte written by the edge pass.
 enum EdgeFlag : unsigned char {
-  EF_BOND = 1,         // raw bond order at or above the cutoff: an active bond
-  EF_HB_DONOR = 2,     // row atom is an H-bond hydrogen, far atom an acceptor, r <= min(7.5, rc)
-  EF_HB_ACCEPTOR = 4,  // row atom is an acceptor, far atom an H-bond hydrogen, r <= min(7.5, rc)
+  EF_BOND = 1,  // raw bond order at or above the cutoff: an active bond
 };
 
 // ------------------------------------------------------------ shift packing
@@ -188,28 +186,6 @@ __device__ inline void xr_wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Wave-private LDS queue of row entries: scans [e, end) 64 entries at a time
-// and appends, in row order, those for which keep(index) holds, until the row
-// ends or fewer than 64 free slots remain.  Advances e past what it scanned
-// and returns the number queued; q holds cap >= 64 ints.  Sparse selections
-// (H-bond candidates) are then evaluated one dense batch of 64 at a time.
-template <class Keep>
-__device__ inline int xr_queue_edges(int* q, int cap, int& e, int end, Keep keep) {
-  const int lane = threadIdx.x & 63;
-  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  int count = 0;
-  while (e < end && count <= cap - 64) {
-    const int idx = e + lane;
-    const bool take = idx < end && keep(idx);
-    const unsigned long long m = __ballot(take);
-    if (take) q[count + __popcll(m & below)] = idx;
-    count += __popcll(m);
-    e += 64;
-  }
-  xr_wave_sync();
-  return count;
 }
 
 // Block reduction of NV doubles into partials[k * nblocks + blockIdx.x].

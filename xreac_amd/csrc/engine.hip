@@ -638,6 +638,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double>(off, xr_cg_partials(nd, nu));
   carve_size<double>(off, (size_t)kAtomScalars * n);  // atom scalars
   carve_size<int>(off, n);               // bond counts (>0.3)
+  carve_size<int>(off, n);               // H-bond candidates per row
   carve_size<double>(off, 6 * (size_t)n);   // F_nb, F_hb
   carve_size<double>(off, (size_t)NCH * n);  // per-atom energy channels
   carve_size<double2>(off, full ? n : 0);    // charge-response right-hand sides (dE/dq, -1)
@@ -666,6 +667,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   double* scal = cv.take<double>((size_t)kAtomScalars * n);
   double* const scal_end = scal + (size_t)kAtomScalars * n;
   int* bcount03 = cv.take<int>(n);
+  int* hb_count = cv.take<int>(n);
   double* fbuf = cv.take<double>(6 * (size_t)n);
   double* energies = cv.take<double>((size_t)NCH * n);
   double2* dEdq = full ? cv.take<double2>(n) : nullptr;
@@ -833,6 +835,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   const bool narrow = max_sys <= 65536;
   carve_size<unsigned short>(off, narrow ? m_cap : 0);
   carve_size<unsigned char>(off, m_cap);
+  carve_size<int2>(off, m_cap);  // H-bond candidates, at the start of each row's range
   void* edge_ws = nullptr;
   XR_WORKSPACE(ctx->edge_buf, off + 4096, &edge_ws, "edge workspace");
   Carver ce{static_cast<char*>(edge_ws)};
@@ -842,6 +845,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   float* hval32 = mixed ? ce.take<float>(m_cap) : nullptr;
   unsigned short* col16 = narrow ? ce.take<unsigned short>(m_cap) : nullptr;
   unsigned char* bflag = ce.take<unsigned char>(m_cap);
+  int2* hb_cand = ce.take<int2>(m_cap);
   void* slab = nullptr;
   int slab_stride = 0;
   if (supplied) {
@@ -911,7 +915,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
 
   // ------------------------------------------------------- edges and bonds
   HIPCHK(xr_edge_prep(n, in.x, in.types, S, ff, row_ptr, slab, slab_stride, nbr_j, nbr_s, col16, bond_cut, hval,
-                      hval32, bflag, bond_count, join, s),
+                      hval32, bflag, bond_count, join, hb_cand, hb_count, s),
          "edge pass");
   // The bond list (scan of the active-bond counts, compaction) feeds only the
   // bonded terms, so with buffers sized from the last evaluation it is built on
@@ -1017,7 +1021,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     XR_TIMED(XT_BOND_FORWARD, s2, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s2));
     if (e != hipSuccess) return e;
     side_what = "many-body";
-    XR_TIMED(XT_MANYBODY, s2, e = xr_manybody(at, bd, ff, row_ptr, nbr_j, nbr_s, bflag, lammps, acc, s2));
+    XR_TIMED(XT_MANYBODY, s2, e = xr_manybody(at, bd, ff, row_ptr, hb_cand, hb_count, lammps, acc, s2));
     if (e != hipSuccess) return e;
     side_what = "adjoint";
     XR_TIMED(XT_BACKWARD, s2, e = xr_bond_backward(at, bd, ff, s2));

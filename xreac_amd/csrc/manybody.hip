@@ -460,63 +460,53 @@ __device__ inline void hbond_eval(const double* prm, double BOd, const double* v
   }
 }
 
-constexpr int HB_QUEUE = 256;  // LDS slots per wave for H-bond candidate edges
-
 // Donor side of hydrogen j (one wave): energy and derivatives.  The row's
-// candidate edges (acceptor type within min(7.5, rc), flagged by the edge
-// pass) are queued first in q, so each donor bond evaluates them in dense
-// batches.
-__device__ inline void hb_donor(int j, int* q, const Atoms& at, const Bonds& bd, const FFDev& ff,
-                                const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                const int* __restrict__ nbr_s, const unsigned char* __restrict__ eflag,
-                                const EnergyAcc& acc) {
+// candidates (acceptors within min(7.5, rc), compacted by the edge pass in
+// row order) are taken one per lane for each donor bond; a candidate that is
+// the donor bond's own edge (same atom and image) is skipped.
+__device__ inline void hb_donor(int j, const Atoms& at, const Bonds& bd, const FFDev& ff,
+                                const int2* __restrict__ cand, int count, const EnergyAcc& acc) {
   const int lane = threadIdx.x & 63;
   const CellDev& cell = xr_cell(at.S, j);
   const int nt = ff.nt, tj = at.type[j];
   const double hcut = fmin(7.5, ff.gen[12]);
   double etot = 0.0;
   double F[3] = {0, 0, 0};
-  int e = row_ptr[j];
-  const int end = row_ptr[j + 1];
-  while (e < end) {
-    const int count = xr_queue_edges(q, HB_QUEUE, e, end, [&](int x) { return (eflag[x] & EF_HB_DONOR) != 0; });
-    for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
-      const int kd = bd.bj[d];
-      if (AP(ff, at.type[kd])[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
-      const double* vd = bd.vec + 3 * d;
-      const double rd = bd.r[d], BOd = bd.BO[d];
-      const int ed = bd.edge[d];
-      const int kbase = (at.type[kd] * nt + tj) * nt;
-      double Gd[3] = {0, 0, 0}, dBO = 0.0;
-      for (int t = lane; t < count; t += 64) {
-        const int c = q[t];
-        if (c == ed) continue;
-        const int k = nbr_j[c];
-        const int tk = at.type[k];
-        if (!ff.hb_on[kbase + tk]) continue;
-        double va[3];
-        const double ra = xr_edge(at.x, cell, j, k, nbr_s[c], va);
-        if (!(ra <= hcut)) continue;
-        HbTerm o;
-        hbond_eval(ff.hb + (kbase + tk) * HB_W, BOd, vd, rd, va, ra, o);
-        etot += o.E;
-        dBO += o.dBO;
+  for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
+    const int kd = bd.bj[d];
+    if (AP(ff, at.type[kd])[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
+    const double* vd = bd.vec + 3 * d;
+    const double rd = bd.r[d], BOd = bd.BO[d];
+    const int cd = bd.code[d];
+    const int kbase = (at.type[kd] * nt + tj) * nt;
+    double Gd[3] = {0, 0, 0}, dBO = 0.0;
+    for (int t = lane; t < count; t += 64) {
+      const int2 c = cand[t];
+      const int k = c.x;
+      if (k == kd && c.y == cd) continue;
+      const int tk = at.type[k];
+      if (!ff.hb_on[kbase + tk]) continue;
+      double va[3];
+      const double ra = xr_edge(at.x, cell, j, k, c.y, va);
+      if (!(ra <= hcut)) continue;
+      HbTerm o;
+      hbond_eval(ff.hb + (kbase + tk) * HB_W, BOd, vd, rd, va, ra, o);
+      etot += o.E;
+      dBO += o.dBO;
 #pragma unroll
-        for (int k2 = 0; k2 < 3; ++k2) {
-          Gd[k2] += o.Gd[k2];
-          F[k2] += o.Ga[k2];
-        }
-      }
-      dBO = xr_wave_sum(dBO);
-#pragma unroll
-      for (int k2 = 0; k2 < 3; ++k2) Gd[k2] = xr_wave_sum(Gd[k2]);
-      if (lane == 0) {
-        bd.A_BO[d] += dBO;
-#pragma unroll
-        for (int k2 = 0; k2 < 3; ++k2) bd.G[3 * d + k2] += Gd[k2];
+      for (int k2 = 0; k2 < 3; ++k2) {
+        Gd[k2] += o.Gd[k2];
+        F[k2] += o.Ga[k2];
       }
     }
-    xr_wave_sync();  // the queue is read before it is refilled
+    dBO = xr_wave_sum(dBO);
+#pragma unroll
+    for (int k2 = 0; k2 < 3; ++k2) Gd[k2] = xr_wave_sum(Gd[k2]);
+    if (lane == 0) {
+      bd.A_BO[d] += dBO;
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) bd.G[3 * d + k2] += Gd[k2];
+    }
   }
 #pragma unroll
   for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
@@ -529,46 +519,38 @@ __device__ inline void hb_donor(int j, int* q, const Atoms& at, const Bonds& bd,
 }
 
 // Acceptor side of heavy atom k (one wave): force from the acceptor-vector
-// derivative.  The H-bond hydrogens within min(7.5, rc) (flagged by the edge
-// pass) are queued in q, then each lane takes one hydrogen and walks its
-// donor bonds.
-__device__ inline void hb_acceptor(int k, int* q, const Atoms& at, const Bonds& bd, const FFDev& ff,
-                                   const int* __restrict__ row_ptr, const int* __restrict__ nbr_j,
-                                   const int* __restrict__ nbr_s, const unsigned char* __restrict__ eflag) {
+// derivative.  Each lane takes one candidate hydrogen of the row (compacted
+// by the edge pass) and walks its donor bonds.
+__device__ inline void hb_acceptor(int k, const Atoms& at, const Bonds& bd, const FFDev& ff,
+                                   const int2* __restrict__ cand, int count) {
   const int lane = threadIdx.x & 63;
   const int tk = at.type[k];
   const CellDev& cell = xr_cell(at.S, k);
   const int nt = ff.nt;
   const double hcut = fmin(7.5, ff.gen[12]);
   double F[3] = {0, 0, 0};
-  int e = row_ptr[k];
-  const int end = row_ptr[k + 1];
-  while (e < end) {
-    const int count = xr_queue_edges(q, HB_QUEUE, e, end, [&](int x) { return (eflag[x] & EF_HB_ACCEPTOR) != 0; });
-    for (int t = lane; t < count; t += 64) {
-      const int c = q[t];
-      const int j = nbr_j[c];
-      const int tj = at.type[j];
-      const int code = nbr_s[c];
-      double vk[3];
-      const double ra = xr_edge(at.x, cell, k, j, code, vk);
-      if (!(ra <= hcut)) continue;
-      const double va[3] = {-vk[0], -vk[1], -vk[2]};
-      const int back = xr_neg_shift(code);
-      for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
-        const int kd = bd.bj[d];
-        const int td = at.type[kd];
-        if (AP(ff, td)[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
-        if (kd == k && bd.code[d] == back) continue;  // the acceptor edge is the donor edge
-        const int key = (td * nt + tj) * nt + tk;
-        if (!ff.hb_on[key]) continue;
-        HbTerm o;
-        hbond_eval(ff.hb + key * HB_W, bd.BO[d], bd.vec + 3 * d, bd.r[d], va, ra, o);
+  for (int t = lane; t < count; t += 64) {
+    const int2 c = cand[t];
+    const int j = c.x;
+    const int tj = at.type[j];
+    const int code = c.y;
+    double vk[3];
+    const double ra = xr_edge(at.x, cell, k, j, code, vk);
+    if (!(ra <= hcut)) continue;
+    const double va[3] = {-vk[0], -vk[1], -vk[2]};
+    const int back = xr_neg_shift(code);
+    for (int d = bd.ptr[j]; d < bd.ptr[j + 1]; ++d) {
+      const int kd = bd.bj[d];
+      const int td = at.type[kd];
+      if (AP(ff, td)[AP_HBOND] != 2.0 || !(bd.BO[d] >= 0.01)) continue;
+      if (kd == k && bd.code[d] == back) continue;  // the acceptor edge is the donor edge
+      const int key = (td * nt + tj) * nt + tk;
+      if (!ff.hb_on[key]) continue;
+      HbTerm o;
+      hbond_eval(ff.hb + key * HB_W, bd.BO[d], bd.vec + 3 * d, bd.r[d], va, ra, o);
 #pragma unroll
-        for (int k2 = 0; k2 < 3; ++k2) F[k2] -= o.Ga[k2];
-      }
+      for (int k2 = 0; k2 < 3; ++k2) F[k2] -= o.Ga[k2];
     }
-    xr_wave_sync();  // the queue is read before it is refilled
   }
 #pragma unroll
   for (int k2 = 0; k2 < 3; ++k2) F[k2] = xr_wave_sum(F[k2]);
@@ -583,15 +565,16 @@ __device__ inline void hb_acceptor(int k, int* q, const Atoms& at, const Bonds& 
 // (the hydrogen's force and donor bonds, the acceptor's force), so one launch
 // holds both.
 __global__ void __launch_bounds__(256) k_hbonds(Atoms at, Bonds bd, FFDev ff, const int* __restrict__ row_ptr,
-                                                const int* __restrict__ nbr_j, const int* __restrict__ nbr_s,
-                                                const unsigned char* __restrict__ eflag, EnergyAcc acc) {
-  __shared__ int queue[4][HB_QUEUE];
+                                                const int2* __restrict__ hb_cand, const int* __restrict__ hb_count,
+                                                EnergyAcc acc) {
   const int x = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (x >= at.n) return;
   const double role = AP(ff, at.type[x])[AP_HBOND];
-  int* q = queue[threadIdx.x >> 6];
-  if (role == 1.0) hb_donor(x, q, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag, acc);
-  else if (role == 2.0) hb_acceptor(x, q, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag);
+  if (role != 1.0 && role != 2.0) return;
+  const int2* cand = hb_cand + row_ptr[x];
+  const int count = hb_count[x];
+  if (role == 1.0) hb_donor(x, at, bd, ff, cand, count, acc);
+  else hb_acceptor(x, at, bd, ff, cand, count);
 }
 
 // Largest corrected bond order over each atom's bonds (0 without bonds).
@@ -610,9 +593,8 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 
 }  // namespace
 
-hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int* nbr_j,
-                       const int* nbr_s, const unsigned char* eflag, int lammps_angles, EnergyAcc acc,
-                       hipStream_t s) {
+hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int2* hb_cand,
+                       const int* hb_count, int lammps_angles, EnergyAcc acc, hipStream_t s) {
   const int angle_groups = 64 / ANG_GROUP;
   hipLaunchKernelGGL(k_angles, dim3(at.n > 0 ? (at.n + angle_groups - 1) / angle_groups : 1), dim3(64), 0, s, at, bd,
                      ff, lammps_angles, acc);
@@ -622,7 +604,7 @@ hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* ro
   hipLaunchKernelGGL(k_torsions<TOR_GROUP>, dim3(at.n > 0 ? (at.n + per_block - 1) / per_block : 1), dim3(256), 0,
                      s, at, bd, ff, acc);
   hipLaunchKernelGGL(k_torsions<64>, wgrid, dim3(256), 0, s, at, bd, ff, acc);
-  hipLaunchKernelGGL(k_hbonds, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, nbr_j, nbr_s, eflag, acc);
+  hipLaunchKernelGGL(k_hbonds, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, hb_cand, hb_count, acc);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }

@@ -101,11 +101,12 @@ struct CgOutcome {
 // slab: the fill's sorted rows (row i at slab[i * slab_stride]), moved to nbr_j / nbr_s on the way; null
 // when nbr_j / nbr_s already hold the rows (a supplied list).  col16 (or null): receives the
 // neighbor indices relative to the system's first atom as 2-byte columns for the CG sweeps (every
-// system of at most 65,536 atoms).
+// system of at most 65,536 atoms).  hb_cand (one int2 slot per edge) receives each row's H-bond
+// candidates as (j, shift code) at the start of the row's range, hb_count their number per row.
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const void* slab, int slab_stride, int* nbr_j, int* nbr_s, unsigned short* col16,
                         double bond_cut, double* hval, float* hval32, unsigned char* bflag, int* bond_count,
-                        unsigned char* join, hipStream_t s);
+                        unsigned char* join, int2* hb_cand, int* hb_count, hipStream_t s);
 hipError_t xr_bond_fill(int n, const int* row_ptr, const unsigned char* bflag, const int* bond_ptr, int cap,
                         int* bond_edge, int* bond_i, hipStream_t s);
 // Iteration blocks (ndir), chunks (nupd) and nominal rows per chunk of a

@@ -50,14 +50,19 @@ constexpr int BOND_QUEUE = 128;     // edge pass: per-wave queue of edges within
 // ---------------------------------------------------------------- edge pass
 // With a slab (the fill's sorted rows), the pass first moves each row's keys
 // to their CSR places, nbr_j[e] and nbr_s[e]; without one (a supplied list)
-// it reads them there.
+// it reads them there.  The row's H-bond candidates (a hydrogen of H-bond
+// type 1 and an acceptor of type 2 within min(7.5, rc), either way round)
+// are compacted, in row order, as (j, shift code) pairs into the first
+// hb_count[i] slots of the row's own range of hb_cand, so the H-bond terms
+// read only them instead of rescanning the row.
 __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restrict__ x, const int* __restrict__ type,
                                                    Systems S, FFDev ff, const int* __restrict__ row_ptr,
                                                    const unsigned long long* __restrict__ slab, int slab_stride,
                                                    int* __restrict__ nbr_j, int* __restrict__ nbr_s,
                                                    unsigned short* __restrict__ col16, double bond_cut, double* __restrict__ hval,
                                                    float* __restrict__ hval32, unsigned char* __restrict__ bflag,
-                                                   int* __restrict__ bond_count, unsigned char* __restrict__ join) {
+                                                   int* __restrict__ bond_count, unsigned char* __restrict__ join,
+                                                   int2* __restrict__ hb_cand, int* __restrict__ hb_count) {
   const int i = blockIdx.x * ROWS_PER_BLOCK + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (i >= n) return;
@@ -79,8 +84,9 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   const int w = threadIdx.x >> 6;
   __shared__ int q_e[ROWS_PER_BLOCK][BOND_QUEUE], q_j[ROWS_PER_BLOCK][BOND_QUEUE], q_t[ROWS_PER_BLOCK][BOND_QUEUE];
   __shared__ double q_r[ROWS_PER_BLOCK][BOND_QUEUE];
-  __shared__ unsigned char q_f[ROWS_PER_BLOCK][BOND_QUEUE];  // H-bond flags, 0x80: zero image shift
+  __shared__ unsigned char q_f[ROWS_PER_BLOCK][BOND_QUEUE];  // 0x80: zero image shift
   int qn = 0;  // identical in every lane
+  int hn = 0;  // H-bond candidates so far, identical in every lane
   auto flush = [&](int count) {  // the first count (<= 64) queued edges, one per lane
     if (lane < count) {
       const int e = q_e[w][lane], j = q_j[w][lane];
@@ -91,7 +97,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
       const double raw = (b.sig + b.pi) + b.pp;
       const bool active = raw >= cut;
       if (raw >= BLOCK_BOND && j < i && j >= i - 3 && (f & 0x80)) jm |= 1u << (i - j - 1);
-      bflag[e] = (unsigned char)((f & 0x7f) | (active ? EF_BOND : 0));
+      bflag[e] = active ? EF_BOND : 0;
       cnt += active;
     }
   };
@@ -99,10 +105,9 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   for (int base = e_begin; base < e_end; base += 64) {
     const int e = base + lane;
     const bool valid = e < e_end;
-    bool cand = false;
+    bool cand = false, hb = false;
     int j = 0, tj = 0, code = 0;
     double r = 0.0;
-    unsigned char flag = 0;
     if (valid) {
       if (srow) {
         const unsigned long long key = srow[e];
@@ -124,12 +129,14 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
       if (hval32) hval32[e] = (float)(14.4 * s);
       if (r <= hcut) {
         const double hb_j = ff.atom[tj * AP_COUNT + AP_HBOND];
-        if (hb_i == 1.0 && hb_j == 2.0) flag |= EF_HB_DONOR;
-        if (hb_i == 2.0 && hb_j == 1.0) flag |= EF_HB_ACCEPTOR;
+        hb = (hb_i == 1.0 && hb_j == 2.0) || (hb_i == 2.0 && hb_j == 1.0);
       }
       cand = r <= bond_cut;
-      if (!cand) bflag[e] = flag;
+      if (!cand) bflag[e] = 0;
     }
+    const unsigned long long hm = __ballot(hb);
+    if (hb) hb_cand[e_begin + hn + __popcll(hm & below)] = make_int2(j, code);
+    hn += __popcll(hm);
     const unsigned long long cm = __ballot(cand);
     if (cand) {
       const int slot = qn + __popcll(cm & below);
@@ -137,7 +144,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
       q_j[w][slot] = j;
       q_t[w][slot] = tj;
       q_r[w][slot] = r;
-      q_f[w][slot] = (unsigned char)(flag | (code == zero ? 0x80 : 0));
+      q_f[w][slot] = (unsigned char)(code == zero ? 0x80 : 0);
     }
     qn += __popcll(cm);
     if (qn >= 64) {
@@ -164,6 +171,7 @@ __global__ void __launch_bounds__(256) k_edge_prep(int n, const double* __restri
   if (lane == 0) {
     bond_count[i] = cnt;
     join[i] = (unsigned char)jm;
+    hb_count[i] = hn;
   }
 }
 
@@ -1070,11 +1078,11 @@ __global__ void __launch_bounds__(1024) k_qeq_charges(Systems S, const double2* 
 hipError_t xr_edge_prep(int n, const double* x, const int* type, const Systems& S, FFDev ff, const int* row_ptr,
                         const void* slab, int slab_stride, int* nbr_j, int* nbr_s, unsigned short* col16,
                         double bond_cut, double* hval, float* hval32, unsigned char* bflag, int* bond_count,
-                        unsigned char* join, hipStream_t s) {
+                        unsigned char* join, int2* hb_cand, int* hb_count, hipStream_t s) {
   const int blocks = (n + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK;
   XR_LAUNCH(XT_EDGE, k_edge_prep, dim3(blocks), dim3(256), 0, s, n, x, type, S, ff, row_ptr,
             static_cast<const unsigned long long*>(slab), slab_stride, nbr_j, nbr_s, col16, bond_cut, hval, hval32,
-            bflag, bond_count, join);
+            bflag, bond_count, join, hb_cand, hb_count);
   XR_LAUNCH_CHECK();
   return hipSuccess;
 }
