@@ -96,8 +96,7 @@ class Engine:
         raise DeviceError(msg)
 
     def type_ids(self, symbols):
-        idx = self.tables.index
-        return np.array([idx[s] for s in symbols], dtype=np.int32)
+        return np.fromiter(map(self.tables.index.__getitem__, symbols), dtype=np.int32, count=len(symbols))
 
     def outputs(self, n, properties=True):
         """Device output buffers for an n-atom evaluation (reusable across calls via ``out=``)."""

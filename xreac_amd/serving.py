@@ -109,6 +109,7 @@ class _Group:
         self.T = torch.zeros(0, dtype=torch.int32, device=self.dev)
         self.FS = torch.zeros((0, 4), **f64)
         self.ST = torch.zeros(0, dtype=torch.int32, device=self.dev)
+        self._mass = {}  # per force field: element -> mass
 
     def prepare(self, job, ff):
         """Host arrays of one job (positions, type ids, velocities, FIRE state); pure host work."""
@@ -118,7 +119,10 @@ class _Group:
         if req.get("md_steps", 0) > 0:
             v0 = req.get("velocities")
             if v0 is None:
-                masses = [ff.atoms[s]["mass"] for s in req["symbols"]]
+                mass = self._mass.get(id(ff))
+                if mass is None:
+                    mass = self._mass[id(ff)] = {s: a["mass"] for s, a in ff.atoms.items()}
+                masses = np.fromiter(map(mass.__getitem__, req["symbols"]), dtype=np.float64, count=job.n)
                 v0 = maxwell_boltzmann(masses, req.get("temperature", 300.0), req.get("velocity_seed", 0))
             v = np.asarray(v0, dtype=np.float64).reshape(job.n, 3)
         else:
@@ -202,6 +206,8 @@ class Server:
     of the QEq-equilibrated energy, calculator.py:79-86) for every request.
     """
 
+    INLINE_PREPARE = 32  # unprepared requests a loop admits (and prepares itself) per step
+
     def __init__(self, force_fields=("ffield.reax.HO.2015", "ffield.reax.CHO.2008"), device=0, *,
                  precision="fp64", warm_start=True, extrapolate=True, neighbor_skin=0.0, tol=1e-10,
                  max_systems=None, max_batch_atoms=262144, budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, full_derivative=False,
@@ -275,12 +281,17 @@ class Server:
 
     def _admit(self, name):
         """Admit queued requests of force field ``name`` in arrival order while the shared budget allows
-        (the caller holds the lock); a group with nothing active always takes its next request."""
+        (the caller holds the lock); a group with nothing active always takes its next request.  At most
+        INLINE_PREPARE requests whose host arrays are not built yet are admitted per step (the loop builds
+        them before its step): behind a burst of arrivals the device starts on a small batch at once and
+        the batches grow as the preparer thread gets ahead, instead of the device idling while the loop
+        prepares the whole burst."""
         group = self.active[name]
         queue = self.queues[name]
         active = sum(len(g.jobs) for g in self.active.values())
         atoms = sum(int(g.sizes.sum()) for g in self.active.values())
         admitted = []
+        inline = 0
         while queue:
             job = queue[0]
             if self.max_systems is not None and active >= self.max_systems:
@@ -288,6 +299,10 @@ class Server:
             if (group.jobs or admitted) and (self.used_bytes + job.bytes > self.budget or
                                               atoms + job.n > self.max_batch_atoms):
                 break
+            if job.prep is None:
+                if inline >= self.INLINE_PREPARE and (group.jobs or admitted):
+                    break
+                inline += 1
             queue.popleft()
             job.admitted = True
             admitted.append(job)
