@@ -630,7 +630,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double2>(off, 10 * (size_t)n);  // CG vectors: x, r, u, w, z, q, s, p and two m (pairs)
   carve_size<float2>(off, 2 * (size_t)n);    // FP32 copies of the two m (mixed precision)
   carve_size<int2>(off, nu);                 // CG chunk ranges
-  carve_size<int4>(off, nd);                 // CG iteration block descriptors
+  carve_size<CgBlock>(off, nd);              // CG iteration block descriptors and row pointers
   carve_size<double>(off, xr_cg_block_doubles(n));  // block-Jacobi inverse rows
   carve_size<int2>(off, n);              // block placement
   carve_size<unsigned char>(off, n);     // block join flags
@@ -658,7 +658,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   double2* cgv = cv.take<double2>(10 * (size_t)n);
   float2* cgq = cv.take<float2>(2 * (size_t)n);
   int2* cg_chunks = cv.take<int2>(nu);
-  int4* cg_blocks = cv.take<int4>(nd);
+  CgBlock* cg_blocks = cv.take<CgBlock>(nd);
   double* minv = cv.take<double>(xr_cg_block_doubles(n));
   int2* binfo = cv.take<int2>(n);
   unsigned char* join = cv.take<unsigned char>(n);
@@ -827,23 +827,25 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   }
 
   off = 0;
+  // The arrays the CG sweep reads (columns and H values) hold one spare entry
+  // past the list: its batch loads read entry row_ptr[i] of an empty row i.
+  carve_size<int>(off, m_cap + 1);
   carve_size<int>(off, m_cap);
-  carve_size<int>(off, m_cap);
-  carve_size<double>(off, m_cap);
-  carve_size<float>(off, mixed ? m_cap : 0);
+  carve_size<double>(off, m_cap + 1);
+  carve_size<float>(off, mixed ? m_cap + 1 : 0);
   // 2-byte CG columns (relative to the system's first atom): every system has at most 65,536 atoms
   const bool narrow = max_sys <= 65536;
-  carve_size<unsigned short>(off, narrow ? m_cap : 0);
+  carve_size<unsigned short>(off, narrow ? m_cap + 1 : 0);
   carve_size<unsigned char>(off, m_cap);
   carve_size<int2>(off, m_cap);  // H-bond candidates, at the start of each row's range
   void* edge_ws = nullptr;
   XR_WORKSPACE(ctx->edge_buf, off + 4096, &edge_ws, "edge workspace");
   Carver ce{static_cast<char*>(edge_ws)};
-  int* nbr_j = ce.take<int>(m_cap);
+  int* nbr_j = ce.take<int>(m_cap + 1);
   int* nbr_s = ce.take<int>(m_cap);
-  double* hval = ce.take<double>(m_cap);
-  float* hval32 = mixed ? ce.take<float>(m_cap) : nullptr;
-  unsigned short* col16 = narrow ? ce.take<unsigned short>(m_cap) : nullptr;
+  double* hval = ce.take<double>(m_cap + 1);
+  float* hval32 = mixed ? ce.take<float>(m_cap + 1) : nullptr;
+  unsigned short* col16 = narrow ? ce.take<unsigned short>(m_cap + 1) : nullptr;
   unsigned char* bflag = ce.take<unsigned char>(m_cap);
   int2* hb_cand = ce.take<int2>(m_cap);
   void* slab = nullptr;
@@ -1247,6 +1249,8 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
 
 int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out,
               Arena* arena = nullptr) {
+  // The CG sweep addresses gathered vector entries with 32-bit byte offsets (16 B per atom).
+  if (in.offsets[in.ns] >= (1 << 28)) return fail(ctx, XREAC_INVALID, "at most 268,435,455 atoms per call");
   bool overflow = false;
   int rc = evaluate_once(ctx, ffh, in, opt, out, false, &overflow, arena);
   if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ffh, in, opt, out, true, &overflow, arena);

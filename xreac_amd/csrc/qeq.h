@@ -44,6 +44,17 @@ struct CgVec {
   float2* M32[2];  // FP32 copies gathered by the mixed-precision sweep
 };
 
+// Per CG iteration block: its descriptor (system, atom range [y, z) of its
+// first chunk, that chunk's system-local index) and the row pointers of that
+// chunk's rows, rp[t] = row_ptr[y + min(t, z - y)] for t <= CG_CHUNK_ROWS_MAX,
+// so a wave of the block finds its rows' entry ranges with one load indexed by
+// the block, without waiting for the descriptor.
+constexpr int CG_CHUNK_ROWS_MAX = 64;  // rows of one chunk at most
+struct CgBlock {
+  int4 desc;
+  int rp[CG_CHUNK_ROWS_MAX + 12];  // 65 used; 320 bytes per block
+};
+
 struct CgArgs {
   Systems S;
   int nd, nu;          // total iteration blocks / chunks over all systems
@@ -61,7 +72,7 @@ struct CgArgs {
   double* minv;               // block-Jacobi inverse rows, column-major: entry c of row i at minv[c * n + i]
   int2* binfo;                // per atom: (chunk row of its block start, block size)
   int2* chunks;               // per chunk: atom range [x, y)
-  int4* blocks;               // per iteration block: system, first chunk's atom range, its chunk index
+  CgBlock* blocks;            // per iteration block: descriptor and first chunk's row pointers
   double* eta;
   double* partials;  // xr_cg_partials(nd, nu) doubles
   void* state;       // xr_cg_state_bytes(ns) bytes

@@ -228,7 +228,7 @@ constexpr int CG_NARROW_MIN_BLOCKS = 1024;
 constexpr int CG_MAX_BLOCKS_SMALL = 512;
 constexpr int CHUNK_SMALL = 24;
 constexpr int CG_SMALL_ATOMS = 8192;
-constexpr int CHUNK_MAX = 64;             // CHUNK + RUN_MAX - 1 rows at most: one wave holds a chunk
+constexpr int CHUNK_MAX = CG_CHUNK_ROWS_MAX;  // CHUNK + RUN_MAX - 1 rows at most: one wave holds a chunk
 constexpr int INNER_MAX = 64;          // mixed precision: iterations between FP64 restarts
 constexpr double INNER_DROP2 = 1e-8;   // mixed precision: squared residual drop before a restart
 constexpr int MAX_RESTARTS = 12;
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict
                                                 const int* __restrict__ nbr_j, const double* __restrict__ hval,
                                                 const unsigned char* __restrict__ join, double* __restrict__ minv,
                                                 int2* __restrict__ binfo, int2* __restrict__ chunks,
-                                                int4* __restrict__ blocks, double* __restrict__ eta_out,
+                                                CgBlock* __restrict__ blocks, double* __restrict__ eta_out,
                                                 CgSys* __restrict__ cg,
                                                 CgCounters* __restrict__ cnt, const unsigned* __restrict__ abort_flag,
                                                 int n_all) {
@@ -366,12 +366,17 @@ __global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict
   __shared__ ChunkBlocks br;
   __shared__ double hb[CHUNK_MAX][BLOCK_MAX];  // each row's row of its H block, inverted in place
   __shared__ double2 bsh[CHUNK_MAX];
-  if (t == 0) {
-    chunks[blockIdx.x] = make_int2(row0, sd.atom0 + r1);
+  if (t == 0) chunks[blockIdx.x] = make_int2(row0, sd.atom0 + r1);
+  {
     // the iteration block owning this chunk first: its system, this chunk's
-    // rows and the system-local index of this chunk
+    // rows, the system-local index of this chunk and the rows' pointers
     const int per = (sd.nupd + sd.ndir - 1) / sd.ndir;
-    if (c % per == 0) blocks[sd.dir0 + c / per] = make_int4(sys, row0, sd.atom0 + r1, c);
+    if (c % per == 0) {
+      CgBlock& blk = blocks[sd.dir0 + c / per];
+      if (t == 0) blk.desc = make_int4(sys, row0, sd.atom0 + r1, c);
+      blk.rp[t] = row_ptr[row0 + min(t, rows)];
+      if (t == 0) blk.rp[CHUNK_MAX] = row_ptr[row0 + rows];
+    }
   }
   join_sh[t] = mine ? join[i] : 0;
   xr_wave_sync();
@@ -485,88 +490,144 @@ __global__ void __launch_bounds__(64) k_cg_init(Systems S, const int* __restrict
   }
 }
 
-// Eight H entries per lane of one row (coalesced over the wave): columns and
-// values, padded with the row atom and zero.
+// The sweep n = A m over a chunk's rows [r0, r1): wave w owns rows r0 + w,
+// r0 + w + WAVES, ...; lanes 2k and 2k + 1 of the wave hold the entry range
+// [row_ptr[i], row_ptr[i + 1]) of its k-th row, loaded for all of them at
+// once.  Every load below is unconditional, with the index clamped to a valid
+// entry, and the lanes past a row's end are masked where the values are used,
+// so the compiler keeps a batch's loads in flight together instead of waiting
+// behind per-lane branches.
+__device__ inline int wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+template <int WAVES>
+__device__ inline int wave_rows(int r0, int r1) {
+  const int first = r0 + wave_index();
+  return first < r1 ? (r1 - first + WAVES - 1) / WAVES : 0;
+}
+
+template <int WAVES>
+__device__ inline int wave_row_ptrs(int r0, int r1, const int* __restrict__ row_ptr) {
+  const int lane = threadIdx.x & 63;
+  const int i = r0 + wave_index() + (lane >> 1) * WAVES + (lane & 1);
+  return row_ptr[min(i, r1)];  // lanes past the wave's rows read a valid pointer, never used
+}
+
+// Eight entries per lane of one row, coalesced over the wave: entry
+// base + 64 u + lane in slot u, as loaded (slots at or past `end` hold the
+// row's last entry, or entry 0 for an empty row, and are masked by the user).
 template <class HV>
-struct RowBatch {
-  int j[8];
+struct Batch {
+  unsigned j[8];
   HV h[8];
 };
 
 // Columns are the 4-byte neighbor indices or, in batches of at most 65,536
-// atoms, their 2-byte copies (IT = unsigned short).
+// atoms, their 2-byte copies relative to the system's first atom
+// (IT = unsigned short).  Slot u reads entry min(base + 64 u + lane, last)
+// with last = max(end - 1, base): for an empty row that is entry `base`,
+// which lies inside the edge arrays (they are carved with at least one spare
+// entry past the list).  The addresses are the row's first entry (uniform)
+// plus a 32-bit lane offset.
 template <class HV, class IT>
-__device__ inline void load_batch(const IT* __restrict__ cols, const HV* __restrict__ hv, int base, int e1, int i,
-                                  int lane, RowBatch<HV>& b) {
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const int e = base + u * 64 + lane;
-    const bool ok = e < e1;
-    b.j[u] = ok ? (int)cols[e] : i;
-    b.h[u] = ok ? hv[e] : HV(0);
-  }
-}
-
-// sum_e h_e v_j(e) over one batch, both right-hand sides
-template <class HV, class VT, class AT>
-__device__ inline void gather_sum(const RowBatch<HV>& b, const VT* __restrict__ vec, AT& as, AT& at) {
-#pragma unroll
-  for (int u = 0; u < 8; ++u) {
-    const VT w = vec[b.j[u]];
-    as += b.h[u] * w.x;
-    at += b.h[u] * w.y;
-  }
-}
-
-// Off-diagonal row sums sum_e h_e v_j(e) for the rows [r0, r1) of a chunk:
-// wave w takes rows r0 + w, r0 + w + WAVES, ...; each row's first 512 entries
-// are in cur (loaded by the caller for the wave's first row), and the next
-// row's are loaded before the current row's gathers.  Lane 0 leaves each
-// row's pair of sums in out[row - r0].
-template <int WAVES, class HV, class VT, class AT, class IT>
-__device__ inline void chunk_spmv(int r0, int r1, const int* __restrict__ row_ptr, const IT* __restrict__ nbr_j,
-                                  const HV* __restrict__ hv, const VT* __restrict__ vec, int off, RowBatch<HV>& cur,
-                                  int e0, int e1, double2* out) {
+__device__ inline void load_batch(const IT* __restrict__ cols, const HV* __restrict__ hv, int base, int end,
+                                  Batch<HV>& b) {
   const int lane = threadIdx.x & 63;
-  vec += off;  // columns are relative to off (the system's first atom for 2-byte columns, else 0)
-  for (int i = r0 + (threadIdx.x >> 6); i < r1; i += WAVES) {
-    const int inext = i + WAVES;
-    int n0 = 0, n1 = 0;
-    RowBatch<HV> next;
-    if (inext < r1) {
-      n0 = row_ptr[inext];
-      n1 = row_ptr[inext + 1];
-      load_batch(nbr_j, hv, n0, n1, inext - off, lane, next);
-    }
-    AT as = AT(0), at = AT(0);
-    gather_sum(cur, vec, as, at);
-    for (int base = e0 + 512; base < e1; base += 512) {  // rows longer than one batch reuse cur
-      load_batch(nbr_j, hv, base, e1, i - off, lane, cur);
-      gather_sum(cur, vec, as, at);
-    }
-    const double s0 = xr_wave_sum((double)as), s1 = xr_wave_sum((double)at);
-    if (lane == 0) out[i - r0] = make_double2(s0, s1);
-    e0 = n0;
-    e1 = n1;
-    cur = next;
+  const unsigned last = (unsigned)max(end - 1 - base, 0);
+  const char* cb = reinterpret_cast<const char*>(cols + base);
+  const char* hb = reinterpret_cast<const char*>(hv + base);
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const unsigned r = min((unsigned)(u * 64 + lane), last);
+    b.j[u] = (unsigned)*reinterpret_cast<const IT*>(cb + r * (unsigned)sizeof(IT));
+    b.h[u] = *reinterpret_cast<const HV*>(hb + r * (unsigned)sizeof(HV));
   }
 }
 
-// The wave's first row of a chunk: its entry range, then its first batch.
-__device__ inline void first_row(int r0, int r1, const int* __restrict__ row_ptr, int& e0, int& e1) {
-  const int i = r0 + (threadIdx.x >> 6);
-  e0 = e1 = 0;
-  if (i < r1) {
-    e0 = row_ptr[i];
-    e1 = row_ptr[i + 1];
+// m_j for column j: a 32-bit byte offset from the system's base, so the
+// gather addresses stay one register each
+template <class VT>
+__device__ inline VT gather(const char* __restrict__ vbase, unsigned j) {
+  return *reinterpret_cast<const VT*>(vbase + j * (unsigned)sizeof(VT));
+}
+
+// Off-diagonal row sums sum_e h_e m_j(e) of the wave's rows.  `cur` holds the
+// first batch of the wave's first row (loaded by the caller).  The wave works
+// through its rows' batches in order; each step issues the gathers of the
+// current batch (masked slots gather the system's first atom with value 0),
+// then the loads of the next batch, then accumulates, so the next batch is in
+// flight while the gathers return.  Per lane the products are summed in entry
+// order, then over the wave; lane 0 leaves each row's pair of sums in
+// out[row - r0].
+template <int WAVES, class HV, class VT, class AT, class IT>
+__device__ inline void chunk_spmv(int r0, int r1, int rp, const IT* __restrict__ cols, const HV* __restrict__ hv,
+                                  const VT* __restrict__ vec, int off, Batch<HV>& cur, double2* out) {
+  const int lane = threadIdx.x & 63;
+  const int nk = wave_rows<WAVES>(r0, r1);
+  if (nk == 0) return;
+  const int first = wave_index();  // the wave's first row, relative to r0
+  // columns are relative to off (the system's first atom for 2-byte columns, else 0)
+  const char* vbase = reinterpret_cast<const char*>(vec + off);
+  int k = 0;
+  int base = __builtin_amdgcn_readlane(rp, 0), end = __builtin_amdgcn_readlane(rp, 1);
+  AT as = AT(0), at = AT(0);
+  for (;;) {
+    // the next batch: the rest of this row, else the next row's first (an
+    // empty range once the wave's rows are done)
+    int kn = k, nbase = base + 512, nend = end;
+    if (nbase >= end) {
+      kn = k + 1;
+      const int l = kn < nk ? 2 * kn : 0;
+      nbase = __builtin_amdgcn_readlane(rp, l);
+      nend = kn < nk ? __builtin_amdgcn_readlane(rp, l + 1) : nbase;
+    }
+    VT g[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) g[u] = gather<VT>(vbase, base + u * 64 + lane < end ? cur.j[u] : 0u);
+    Batch<HV> nxt;
+    load_batch(cols, hv, nbase, nend, nxt);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const HV h = base + u * 64 + lane < end ? cur.h[u] : HV(0);
+      as += h * g[u].x;
+      at += h * g[u].y;
+    }
+    if (kn != k) {
+      const double s0 = xr_wave_sum((double)as), s1 = xr_wave_sum((double)at);
+      if (lane == 0) out[first + k * WAVES] = make_double2(s0, s1);
+      if (kn >= nk) return;
+      as = AT(0);
+      at = AT(0);
+    }
+    cur = nxt;
+    k = kn;
+    base = nbase;
+    end = nend;
   }
 }
 
-template <class HV, class IT>
-__device__ inline void first_batch(int r0, int r1, const IT* __restrict__ nbr_j, const HV* __restrict__ hv, int e0,
-                                   int e1, int off, RowBatch<HV>& b) {
-  const int i = r0 + (threadIdx.x >> 6);
-  if (i < r1) load_batch(nbr_j, hv, e0, e1, i - off, threadIdx.x & 63, b);
+// The first batch of the wave's first row of a chunk (an empty range for a
+// wave without rows)
+template <int WAVES, class HV, class IT>
+__device__ inline void first_batch(int r0, int r1, int rp, const IT* __restrict__ cols, const HV* __restrict__ hv,
+                                   Batch<HV>& b) {
+  const int base = __builtin_amdgcn_readlane(rp, 0);
+  const int end = wave_rows<WAVES>(r0, r1) > 0 ? __builtin_amdgcn_readlane(rp, 1) : base;
+  load_batch(cols, hv, base, end, b);
+}
+
+// Sum over the series of a system's ndir block partials in p[0..ndir): lane l
+// adds partials l, l + 64, ... in that order (the loads issued together).
+template <int NP>
+__device__ inline double partial_sum(const double* __restrict__ p, int ndir) {
+  const int l = threadIdx.x & 63;
+  double v[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) v[q] = p[min(l + 64 * q, ndir - 1)];
+  double a = l < ndir ? v[0] : 0.0;
+#pragma unroll
+  for (int q = 1; q < NP; ++q) a = l + 64 * q < ndir ? a + v[q] : a;
+  for (int t = l + 64 * NP; t < ndir; t += 64) a += p[t];  // block caps above 64 NP
+  return a;
 }
 
 // What a system's block does in launch k, from the state the previous launch
@@ -626,32 +687,24 @@ __device__ inline void stage_load(int mode, int r0, int r1, const CgVec& v, cons
                                   const double* __restrict__ minv, int n_all, const int2* __restrict__ binfo,
                                   const double* __restrict__ eta, const double2* __restrict__ rhs,
                                   const int* __restrict__ type, const FFDev& ff, Staged<WAVES>& s) {
-  const int row = threadIdx.x & 63;
-  const int i = r0 + row;
+  // lane t stages row r0 + t; lanes past the chunk load row r1 - 1 (never stored)
+  const int i = min(r0 + (int)(threadIdx.x & 63), r1 - 1);
 #pragma unroll
   for (int u = 0; u < STAGE_ITEMS / WAVES; ++u) {
-    const int item = (threadIdx.x >> 6) + u * WAVES;
+    const int item = (threadIdx.x >> 6) + u * WAVES;  // uniform over the wave
     double2 val = make_double2(0.0, 0.0);
-    double mv = 0.0;
-    if (i < r1) {
-      mv = minv[(size_t)item * n_all + i];  // entries past the block size are never used
-      if (item < NITEM && item_used(mode, item)) {
-        switch (item) {
-          case IT_Z: val = v.Z[i]; break;
-          case IT_Q: val = v.Q[i]; break;
-          case IT_S: val = v.S[i]; break;
-          case IT_P: val = v.P[i]; break;
-          case IT_X: val = v.X[i]; break;
-          case IT_R: val = v.R[i]; break;
-          case IT_U: val = v.U[i]; break;
-          case IT_W: val = v.W[i]; break;
-          case IT_M: val = m_in[i]; break;
-          default: val = cg_rhs(rhs, type, ff, i); break;
-        }
+    const double mv = minv[(size_t)item * n_all + i];  // entries past the block size are never used
+    if (item < NITEM && item_used(mode, item)) {
+      if (item == IT_B) {
+        val = cg_rhs(rhs, type, ff, i);
+      } else {
+        const double2* src = item == IT_Z ? v.Z : item == IT_Q ? v.Q : item == IT_S ? v.S : item == IT_P ? v.P
+                           : item == IT_X ? v.X : item == IT_R ? v.R : item == IT_U ? v.U : item == IT_W ? v.W : m_in;
+        val = src[i];
       }
-      if (item == NITEM) val.x = eta[i];
-      if (item == NITEM + 1) val.x = __longlong_as_double(*reinterpret_cast<const long long*>(binfo + i));
     }
+    if (item == NITEM) val.x = eta[i];
+    if (item == NITEM + 1) val.x = __longlong_as_double(*reinterpret_cast<const long long*>(binfo + i));
     s.val[u] = val;
     s.mv[u] = mv;
   }
@@ -805,7 +858,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
                                                            const double* __restrict__ minv,
                                                            const int2* __restrict__ binfo,
                                                            const int2* __restrict__ chunks,
-                                                           const int4* __restrict__ blocks,
+                                                           const CgBlock* __restrict__ blocks,
                                                            const int* __restrict__ type, FFDev ff,
                                                            const double2* __restrict__ rhs, double* __restrict__ part,
                                                            double* __restrict__ part_bb, int nd, double tol2,
@@ -814,13 +867,16 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
   constexpr int WAVES = BLOCK / 64;
   constexpr int QUARTER = BLOCK / 4;  // threads per series pair of partials
   static_assert(WAVES % 4 == 0, "a quarter of the block per series pair");
+  // partial loads per lane in flight together: the block cap of the systems this shape runs
+  constexpr int NP = (BLOCK == CG_BLOCK ? CG_MAX_BLOCKS : CG_MAX_BLOCKS_SMALL) / 64;
   const int b = base + blockIdx.x;
-  // The block's descriptor gives its system and first chunk; the wave's first
-  // row range is loaded alongside the system's control state.
-  const int4 desc = blocks[b];
+  // The block's descriptor gives its system and first chunk, and its record
+  // holds that chunk's row pointers, so the waves' first loads do not wait on
+  // each other.
+  const int4 desc = blocks[b].desc;
   const int sys = desc.x;
-  int e0, e1;
-  first_row(desc.y, desc.z, row_ptr, e0, e1);
+  // the wave's first-chunk row pointers: loaded by block index alongside the descriptor
+  int rp = blocks[b].rp[min(wave_index() + (int)((threadIdx.x & 63) >> 1) * WAVES + (int)(threadIdx.x & 1), CHUNK_MAX)];
   const CgCtl c = cg[sys].ctl[(k - 1) & 1];
   int j;
   const int mode = cg_mode(c, k, j);
@@ -860,8 +916,8 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     // lanes: the same order whatever the block shape, so a system's result
     // does not depend on the shape its batch runs in.
     int2 cr = make_int2(desc.y, desc.z);
-    RowBatch<HV> cur;
-    first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, coff, cur);
+    Batch<HV> cur;
+    first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hv, cur);
     Staged<WAVES> sv;
     stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
     double pa = 0.0, pb = 0.0;
@@ -869,95 +925,93 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     if (series_wave) {
       const int series = (threadIdx.x / QUARTER) * 2;
       const double* pbase = series < NDOT ? part + ((k - 1) & 1) * NDOT * nd + series * nd : part_bb;
-      const int l = threadIdx.x & 63;
-      if (l < sd.ndir) {
-        pa = pbase[sd.dir0 + l];
-        pb = pbase[nd + sd.dir0 + l];
-      }
-      for (int t = l + 64; t < sd.ndir; t += 64) {  // systems with more than 64 blocks
-        pa += pbase[sd.dir0 + t];
-        pb += pbase[nd + sd.dir0 + t];
-      }
+      pa = partial_sum<NP>(pbase + sd.dir0, sd.ndir);
+      pb = partial_sum<NP>(pbase + nd + sd.dir0, sd.ndir);
     }
     __shared__ double red[8];
-    bool stepped = false;
-    for (int ch = c0; ch < c1; ++ch) {
-      if (ch > c0) {
-        cr = chunks[sd.upd0 + ch];
-        first_row(cr.x, cr.y, row_ptr, e0, e1);
-        first_batch(cr.x, cr.y, nbr_j, hv, e0, e1, coff, cur);
-        stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+    // The first chunk: its sweep, then this launch's step from the reduced
+    // partials, then its tail.  Straight-line code, so the first gathers wait
+    // only for the first batch, not for the row operands and partials loaded
+    // after it.
+    chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, rp, nbr_j, hv, vin, coff, cur, L.sums);
+    stage_store<WAVES>(cr.x, cr.y, sv, L);
+    if (series_wave) {
+      pa = xr_wave_sum(pa);
+      pb = xr_wave_sum(pb);
+      if ((threadIdx.x & 63) == 0) {
+        red[(threadIdx.x / QUARTER) * 2] = pa;
+        red[(threadIdx.x / QUARTER) * 2 + 1] = pb;
       }
-      chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, row_ptr, nbr_j, hv, vin, coff, cur, e0, e1, L.sums);
-      stage_store<WAVES>(cr.x, cr.y, sv, L);
-      if (!stepped && series_wave) {
-        pa = xr_wave_sum(pa);
-        pb = xr_wave_sum(pb);
-        if ((threadIdx.x & 63) == 0) {
-          red[(threadIdx.x / QUARTER) * 2] = pa;
-          red[(threadIdx.x / QUARTER) * 2 + 1] = pb;
-        }
+    }
+    __syncthreads();
+    {
+      double res[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) res[q] = red[q];
+      bool conv[2];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        double target = tol2 * res[6 + r];
+        if (MIXED && j > 0) target = fmax(target, INNER_DROP2 * c.rr0[r]);
+        conv[r] = res[4 + r] <= target;
       }
-      __syncthreads();
-      if (!stepped) {
-        stepped = true;
-        double res[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) res[q] = red[q];
-        bool conv[2];
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          double target = tol2 * res[6 + r];
-          if (MIXED && j > 0) target = fmax(target, INNER_DROP2 * c.rr0[r]);
-          conv[r] = res[4 + r] <= target;
-        }
-        if ((conv[0] && conv[1]) || c.it >= max_iter || (MIXED && j >= INNER_MAX)) {
-          if (lead) {
-            int state;
-            if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
-            else if (c.it >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
-            else state = CG_REFRESH;
-            settle(&cg[sys], cnt, c, state, k);
-          }
-          return;
-        }
-        st.fresh = j == 0;
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          const double g = res[r], d = res[2 + r];
-          double a = 0.0, b = 0.0;
-          if (!conv[r]) {
-            if (st.fresh || c.alp[r] == 0.0 || c.gam[r] == 0.0) {
-              a = d != 0.0 ? g / d : 0.0;
-            } else {
-              b = g / c.gam[r];
-              const double den = d - b * g / c.alp[r];
-              a = den != 0.0 ? g / den : 0.0;
-            }
-          }
-          st.alpha[r] = a;
-          st.beta[r] = b;
-        }
+      if ((conv[0] && conv[1]) || c.it >= max_iter || (MIXED && j >= INNER_MAX)) {
         if (lead) {
-          CgCtl s = c;
-          s.done = CG_RUNNING;
-          if (st.fresh) {
-            s.k0 = k;
-            s.rr0[0] = res[4];
-            s.rr0[1] = res[5];
-          }
-          s.it = c.it + 1;
-#pragma unroll
-          for (int r = 0; r < 2; ++r) {
-            // a converged right-hand side keeps its last step (alpha = 0 stops it)
-            s.gam[r] = conv[r] ? c.gam[r] : res[r];
-            s.alp[r] = conv[r] ? c.alp[r] : st.alpha[r];
-          }
-          cg[sys].ctl[k & 1] = s;
+          int state;
+          if (conv[0] && conv[1] && (!MIXED || j == 0)) state = CG_DONE;
+          else if (c.it >= max_iter || c.restarts > MAX_RESTARTS) state = CG_STALLED;
+          else state = CG_REFRESH;
+          settle(&cg[sys], cnt, c, state, k);
         }
+        return;
       }
+      st.fresh = j == 0;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const double g = res[r], d = res[2 + r];
+        double a = 0.0, b = 0.0;
+        if (!conv[r]) {
+          if (st.fresh || c.alp[r] == 0.0 || c.gam[r] == 0.0) {
+            a = d != 0.0 ? g / d : 0.0;
+          } else {
+            b = g / c.gam[r];
+            const double den = d - b * g / c.alp[r];
+            a = den != 0.0 ? g / den : 0.0;
+          }
+        }
+        st.alpha[r] = a;
+        st.beta[r] = b;
+      }
+      if (lead) {
+        CgCtl s = c;
+        s.done = CG_RUNNING;
+        if (st.fresh) {
+          s.k0 = k;
+          s.rr0[0] = res[4];
+          s.rr0[1] = res[5];
+        }
+        s.it = c.it + 1;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          // a converged right-hand side keeps its last step (alpha = 0 stops it)
+          s.gam[r] = conv[r] ? c.gam[r] : res[r];
+          s.alp[r] = conv[r] ? c.alp[r] : st.alpha[r];
+        }
+        cg[sys].ctl[k & 1] = s;
+      }
+    }
+    if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
+    // further chunks of a block (systems with more chunks than the block cap)
+    for (int ch = c0 + 1; ch < c1; ++ch) {
+      __syncthreads();
+      cr = chunks[sd.upd0 + ch];
+      rp = wave_row_ptrs<WAVES>(cr.x, cr.y, row_ptr);
+      first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hv, cur);
+      stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+      chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, rp, nbr_j, hv, vin, coff, cur, L.sums);
+      stage_store<WAVES>(cr.x, cr.y, sv, L);
+      __syncthreads();
       if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
-      if (ch + 1 < c1) __syncthreads();
     }
   } else {
     // refresh in FP64: r = b - A x and u = M^-1 r, then w = A u and m = M^-1 w.
@@ -970,9 +1024,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
       if (w < 4) {
         const double* src = w < 2 ? part + ((k - 1) & 1) * NDOT * nd + (4 + w) * nd : part_bb + (w - 2) * nd;
-        double a = 0.0;
-        if (l < sd.ndir) a = src[sd.dir0 + l];
-        for (int t = l + 64; t < sd.ndir; t += 64) a += src[sd.dir0 + t];
+        double a = partial_sum<NP>(src + sd.dir0, sd.ndir);
         a = xr_wave_sum(a);
         if (l == 0) chk[w] = a;
       }
@@ -990,13 +1042,12 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     const double2* vin = mode == MODE_RESIDUAL ? v.X : v.U;
     for (int ch = c0; ch < c1; ++ch) {
       const int2 cr = ch == c0 ? make_int2(desc.y, desc.z) : chunks[sd.upd0 + ch];
-      if (ch > c0) first_row(cr.x, cr.y, row_ptr, e0, e1);
-      RowBatch<double> cur;
-      first_batch(cr.x, cr.y, nbr_j, hval, e0, e1, coff, cur);
+      if (ch > c0) rp = wave_row_ptrs<WAVES>(cr.x, cr.y, row_ptr);
+      Batch<double> cur;
+      first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hval, cur);
       Staged<WAVES> sv;
       stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
-      chunk_spmv<WAVES, double, double2, double, IT>(cr.x, cr.y, row_ptr, nbr_j, hval, vin, coff, cur, e0, e1,
-                                                     L.sums);
+      chunk_spmv<WAVES, double, double2, double, IT>(cr.x, cr.y, rp, nbr_j, hval, vin, coff, cur, L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
       __syncthreads();
       if (threadIdx.x < 128) chunk_tail<MIXED>(mode, cr.x, cr.y, st, v, m_out, m32_out, L, dots);
