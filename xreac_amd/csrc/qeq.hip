@@ -683,10 +683,32 @@ __device__ inline bool item_used(int mode, int item) {
 }
 
 template <int WAVES>
+__device__ inline void stage_write(int r0, int r1, const Staged<WAVES>& s, ChunkLds& L) {
+  const int row = threadIdx.x & 63;
+  if (r0 + row >= r1) return;
+#pragma unroll
+  for (int u = 0; u < STAGE_ITEMS / WAVES; ++u) {
+    const int item = (threadIdx.x >> 6) + u * WAVES;
+    L.mv[item][row] = s.mv[u];
+    if (item < NITEM) L.st[item][row] = s.val[u];
+    if (item == NITEM) L.eta[row] = s.val[u].x;
+    if (item == NITEM + 1) *reinterpret_cast<long long*>(&L.bi[row]) = __double_as_longlong(s.val[u].x);
+  }
+}
+
+// The 1024-thread shape (one item per thread) keeps the staged values in
+// registers over the sweep and writes them to LDS after it, so their loads
+// finish under the gathers; the smaller shapes (two or four items per
+// thread) write them at once, which costs a wait before the sweep but keeps
+// their registers for the sweep.
+template <int WAVES>
+constexpr bool stage_held() { return STAGE_ITEMS / WAVES == 1; }
+
+template <int WAVES>
 __device__ inline void stage_load(int mode, int r0, int r1, const CgVec& v, const double2* __restrict__ m_in,
                                   const double* __restrict__ minv, int n_all, const int2* __restrict__ binfo,
                                   const double* __restrict__ eta, const double2* __restrict__ rhs,
-                                  const int* __restrict__ type, const FFDev& ff, Staged<WAVES>& s) {
+                                  const int* __restrict__ type, const FFDev& ff, Staged<WAVES>& s, ChunkLds& L) {
   // lane t stages row r0 + t; lanes past the chunk load row r1 - 1 (never stored)
   const int i = min(r0 + (int)(threadIdx.x & 63), r1 - 1);
 #pragma unroll
@@ -708,21 +730,15 @@ __device__ inline void stage_load(int mode, int r0, int r1, const CgVec& v, cons
     s.val[u] = val;
     s.mv[u] = mv;
   }
+  if constexpr (!stage_held<WAVES>()) stage_write<WAVES>(r0, r1, s, L);
 }
 
+// After the sweep: the held values into LDS (the smaller shapes wrote theirs before it)
 template <int WAVES>
 __device__ inline void stage_store(int r0, int r1, const Staged<WAVES>& s, ChunkLds& L) {
-  const int row = threadIdx.x & 63;
-  if (r0 + row >= r1) return;
-#pragma unroll
-  for (int u = 0; u < STAGE_ITEMS / WAVES; ++u) {
-    const int item = (threadIdx.x >> 6) + u * WAVES;
-    L.mv[item][row] = s.mv[u];
-    if (item < NITEM) L.st[item][row] = s.val[u];
-    if (item == NITEM) L.eta[row] = s.val[u].x;
-    if (item == NITEM + 1) *reinterpret_cast<long long*>(&L.bi[row]) = __double_as_longlong(s.val[u].x);
-  }
+  if constexpr (stage_held<WAVES>()) stage_write<WAVES>(r0, r1, s, L);
 }
+
 
 // M^-1 applied to the exchanged vector of right-hand side h for row t
 __device__ inline double apply_block(const ChunkLds& L, int h, int t) {
@@ -919,7 +935,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     Batch<HV> cur;
     first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hv, cur);
     Staged<WAVES> sv;
-    stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+    stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv, L);
     double pa = 0.0, pb = 0.0;
     const bool series_wave = threadIdx.x % QUARTER < 64;  // the first wave of its quarter
     if (series_wave) {
@@ -1007,7 +1023,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       cr = chunks[sd.upd0 + ch];
       rp = wave_row_ptrs<WAVES>(cr.x, cr.y, row_ptr);
       first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hv, cur);
-      stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+      stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv, L);
       chunk_spmv<WAVES, HV, VT, AT, IT>(cr.x, cr.y, rp, nbr_j, hv, vin, coff, cur, L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
       __syncthreads();
@@ -1046,7 +1062,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
       Batch<double> cur;
       first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hval, cur);
       Staged<WAVES> sv;
-      stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv);
+      stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv, L);
       chunk_spmv<WAVES, double, double2, double, IT>(cr.x, cr.y, rp, nbr_j, hval, vin, coff, cur, L.sums);
       stage_store<WAVES>(cr.x, cr.y, sv, L);
       __syncthreads();
