@@ -512,14 +512,23 @@ __device__ inline int wave_row_ptrs(int r0, int r1, const int* __restrict__ row_
   return row_ptr[min(i, r1)];  // lanes past the wave's rows read a valid pointer, never used
 }
 
-// Eight entries per lane of one row, coalesced over the wave: entry
+// U entries per lane of one row, coalesced over the wave: entry
 // base + 64 u + lane in slot u, as loaded (slots at or past `end` hold the
 // row's last entry, or entry 0 for an empty row, and are masked by the user).
-template <class HV>
+// A lane sums its entries in entry order whatever U is, so U changes no result.
+template <class HV, int U>
 struct Batch {
-  unsigned j[8];
-  HV h[8];
+  unsigned j[U];
+  HV h[U];
 };
+
+// Entries per lane and batch: 8 in the 1024-thread shape, which holds one
+// block per CU whatever its register count; 4 in the smaller shapes, whose
+// freed registers let a CU hold 5-6 waves per SIMD instead of 4.
+template <int BLOCK>
+constexpr int batch_entries() {
+  return BLOCK == 1024 ? 8 : 4;
+}
 
 // Columns are the 4-byte neighbor indices or, in batches of at most 65,536
 // atoms, their 2-byte copies relative to the system's first atom
@@ -528,15 +537,15 @@ struct Batch {
 // which lies inside the edge arrays (they are carved with at least one spare
 // entry past the list).  The addresses are the row's first entry (uniform)
 // plus a 32-bit lane offset.
-template <class HV, class IT>
+template <class HV, class IT, int U>
 __device__ inline void load_batch(const IT* __restrict__ cols, const HV* __restrict__ hv, int base, int end,
-                                  Batch<HV>& b) {
+                                  Batch<HV, U>& b) {
   const int lane = threadIdx.x & 63;
   const unsigned last = (unsigned)max(end - 1 - base, 0);
   const char* cb = reinterpret_cast<const char*>(cols + base);
   const char* hb = reinterpret_cast<const char*>(hv + base);
 #pragma unroll
-  for (int u = 0; u < 8; ++u) {
+  for (int u = 0; u < U; ++u) {
     const unsigned r = min((unsigned)(u * 64 + lane), last);
     b.j[u] = (unsigned)*reinterpret_cast<const IT*>(cb + r * (unsigned)sizeof(IT));
     b.h[u] = *reinterpret_cast<const HV*>(hb + r * (unsigned)sizeof(HV));
@@ -558,9 +567,9 @@ __device__ inline VT gather(const char* __restrict__ vbase, unsigned j) {
 // flight while the gathers return.  Per lane the products are summed in entry
 // order, then over the wave; lane 0 leaves each row's pair of sums in
 // out[row - r0].
-template <int WAVES, class HV, class VT, class AT, class IT>
+template <int WAVES, class HV, class VT, class AT, class IT, int U>
 __device__ inline void chunk_spmv(int r0, int r1, int rp, const IT* __restrict__ cols, const HV* __restrict__ hv,
-                                  const VT* __restrict__ vec, int off, Batch<HV>& cur, double2* out) {
+                                  const VT* __restrict__ vec, int off, Batch<HV, U>& cur, double2* out) {
   const int lane = threadIdx.x & 63;
   const int nk = wave_rows<WAVES>(r0, r1);
   if (nk == 0) return;
@@ -573,20 +582,20 @@ __device__ inline void chunk_spmv(int r0, int r1, int rp, const IT* __restrict__
   for (;;) {
     // the next batch: the rest of this row, else the next row's first (an
     // empty range once the wave's rows are done)
-    int kn = k, nbase = base + 512, nend = end;
+    int kn = k, nbase = base + 64 * U, nend = end;
     if (nbase >= end) {
       kn = k + 1;
       const int l = kn < nk ? 2 * kn : 0;
       nbase = __builtin_amdgcn_readlane(rp, l);
       nend = kn < nk ? __builtin_amdgcn_readlane(rp, l + 1) : nbase;
     }
-    VT g[8];
+    VT g[U];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) g[u] = gather<VT>(vbase, base + u * 64 + lane < end ? cur.j[u] : 0u);
-    Batch<HV> nxt;
+    for (int u = 0; u < U; ++u) g[u] = gather<VT>(vbase, base + u * 64 + lane < end ? cur.j[u] : 0u);
+    Batch<HV, U> nxt;
     load_batch(cols, hv, nbase, nend, nxt);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < U; ++u) {
       const HV h = base + u * 64 + lane < end ? cur.h[u] : HV(0);
       as += h * g[u].x;
       at += h * g[u].y;
@@ -607,9 +616,9 @@ __device__ inline void chunk_spmv(int r0, int r1, int rp, const IT* __restrict__
 
 // The first batch of the wave's first row of a chunk (an empty range for a
 // wave without rows)
-template <int WAVES, class HV, class IT>
+template <int WAVES, class HV, class IT, int U>
 __device__ inline void first_batch(int r0, int r1, int rp, const IT* __restrict__ cols, const HV* __restrict__ hv,
-                                   Batch<HV>& b) {
+                                   Batch<HV, U>& b) {
   const int base = __builtin_amdgcn_readlane(rp, 0);
   const int end = wave_rows<WAVES>(r0, r1) > 0 ? __builtin_amdgcn_readlane(rp, 1) : base;
   load_batch(cols, hv, base, end, b);
@@ -932,7 +941,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     // lanes: the same order whatever the block shape, so a system's result
     // does not depend on the shape its batch runs in.
     int2 cr = make_int2(desc.y, desc.z);
-    Batch<HV> cur;
+    Batch<HV, batch_entries<BLOCK>()> cur;
     first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hv, cur);
     Staged<WAVES> sv;
     stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv, L);
@@ -1059,7 +1068,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_cg_iteration(Systems S, int k, CgV
     for (int ch = c0; ch < c1; ++ch) {
       const int2 cr = ch == c0 ? make_int2(desc.y, desc.z) : chunks[sd.upd0 + ch];
       if (ch > c0) rp = wave_row_ptrs<WAVES>(cr.x, cr.y, row_ptr);
-      Batch<double> cur;
+      Batch<double, batch_entries<BLOCK>()> cur;
       first_batch<WAVES>(cr.x, cr.y, rp, nbr_j, hval, cur);
       Staged<WAVES> sv;
       stage_load<WAVES>(mode, cr.x, cr.y, v, m_in, minv, n_all, binfo, eta, rhs, type, ff, sv, L);
