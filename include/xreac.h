@@ -257,6 +257,51 @@ int xreac_fire_step(xreac_ctx* ctx, int32_t num_systems, const int32_t* offsets,
                     double* velocities, const double* forces, double* state, int32_t* status,
                     double force_tolerance, int32_t max_iterations);
 
+/* Integrator stages of a recorded step (xreac_step_create), run in this order
+ * around its evaluation: velocity-Verlet stage 0, QEq extrapolation, the
+ * evaluation, velocity-Verlet stage 1, one FIRE iteration.  A NULL dt_fs,
+ * qeq_order or fire_state leaves that stage out.  Arrays as in xreac_verlet,
+ * xreac_qeq_extrapolate and xreac_fire_step (device memory); stage 0 reads the
+ * forces array's content (the previous step's forces). */
+typedef struct {
+  const double* dt_fs;        /* [num_systems]; 0 keeps a system still */
+  double* velocities;         /* [N*3] */
+  double* kinetic;            /* [num_systems] or NULL */
+  const int32_t* qeq_order;   /* [num_systems] */
+  double* qeq_history;        /* [3][N*2] */
+  double* fire_state;         /* [num_systems*4] */
+  int32_t* fire_status;       /* [num_systems] */
+  double fire_force_tolerance;
+  int32_t fire_max_iterations;
+} xreac_step_stages;
+
+typedef struct xreac_step xreac_step;
+
+/* A step of a fixed batch recorded once as a HIP graph and then replayed: no
+ * reference counterpart (the reference evaluates one system per call, host
+ * driven).  It is for serving and dynamics loops that step the same systems
+ * many times: a replay is one graph launch and one wait instead of about a
+ * hundred host-issued launches and the QEq convergence read-backs.  The
+ * batch's host arrays are copied; its device arrays, the result's device
+ * arrays and the stages' arrays are used in place by every run, so their
+ * contents may change between runs (positions, dt, extrapolation orders) but
+ * not their addresses or sizes.  The step owns its workspaces.  The context
+ * needs its own stream (xreac_ctx_create with a non-NULL stream) and must
+ * outlive the step.  Neither supplied neighbor lists nor neighbor_skin. */
+int xreac_step_create(xreac_ctx* ctx, const xreac_ff* ff, const xreac_batch* batch, const xreac_options* opt,
+                      const xreac_step_stages* stages, const xreac_batch_result* result, xreac_step** out);
+/* Run the step once and wait for it; the host arrays of result (components,
+ * status, qeq_iterations) receive the evaluation's, as xreac_evaluate_batch.
+ * The first run executes the step unrecorded and records it.  A replay whose
+ * edge or bond buffers overflowed, or whose QEq needed more launches than were
+ * recorded, puts back every array the step modifies and runs the step
+ * unrecorded, then records it again; either way the results are bitwise those
+ * of the unrecorded step. */
+int xreac_step_run(xreac_step* step, xreac_batch_result* result);
+/* Replays, fallbacks to an unrecorded run after a replay, and recordings so far. */
+int xreac_step_stats(const xreac_step* step, int64_t* replays, int64_t* fallbacks, int64_t* recordings);
+void xreac_step_destroy(xreac_step* step);
+
 /* Kernel timing with hipEvents on the launching stream (no reference
  * counterpart; measurement only).  timer_mask bit k enables timer k (0: off);
  * each sampled launch of an enabled kernel records start/stop events with

@@ -35,6 +35,10 @@ EXPORTED = (
     "xreac_profile_enable",
     "xreac_profile_stride",
     "xreac_profile_read",
+    "xreac_step_create",
+    "xreac_step_run",
+    "xreac_step_stats",
+    "xreac_step_destroy",
 )
 TIMERS = (
     "neighbor_count",
@@ -125,6 +129,20 @@ class BatchResult(C.Structure):
     ]
 
 
+class StepStages(C.Structure):
+    _fields_ = [
+        ("dt_fs", C.c_void_p),
+        ("velocities", C.c_void_p),
+        ("kinetic", C.c_void_p),
+        ("qeq_order", C.c_void_p),
+        ("qeq_history", C.c_void_p),
+        ("fire_state", C.c_void_p),
+        ("fire_status", C.c_void_p),
+        ("fire_force_tolerance", C.c_double),
+        ("fire_max_iterations", C.c_int32),
+    ]
+
+
 class Result(C.Structure):
     _fields_ = [
         ("components", C.c_double * NCOMP),
@@ -191,5 +209,14 @@ def load():
     lib.xreac_profile_stride.restype = C.c_int
     lib.xreac_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int]
     lib.xreac_profile_read.restype = C.c_int
+    lib.xreac_step_create.argtypes = [vp, vp, C.POINTER(Batch), C.POINTER(Options), C.POINTER(StepStages),
+                                      C.POINTER(BatchResult), C.POINTER(vp)]
+    lib.xreac_step_create.restype = C.c_int
+    lib.xreac_step_run.argtypes = [vp, C.POINTER(BatchResult)]
+    lib.xreac_step_run.restype = C.c_int
+    lib.xreac_step_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.xreac_step_stats.restype = C.c_int
+    lib.xreac_step_destroy.argtypes = [vp]
+    lib.xreac_step_destroy.restype = None
     _LIB = lib
     return lib

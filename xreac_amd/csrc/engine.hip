@@ -98,50 +98,39 @@ static __device__ __align__(256) unsigned char scratch[XREAC_STATIC_SCRATCH_BYTE
 constexpr int kMaxDevices = 64;
 void* g_scratch_ptr[kMaxDevices] = {};  // device address of scratch on each device, once resolved
 
-}  // namespace
+// The kept skin rows are geometry only: they depend on the cutoff, the skin
+// and the systems' atom ranges and cells (types are compared on the device).
+struct SkinKey {
+  double rc = 0.0;
+  double skin = 0.0;
+  std::vector<int32_t> offsets;
+  std::vector<double> cells;
+  std::vector<int32_t> pbc;
+  bool operator==(const SkinKey& o) const {
+    return rc == o.rc && skin == o.skin && offsets == o.offsets && cells == o.cells && pbc == o.pbc;
+  }
+};
 
-struct xreac_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string message;
-  std::vector<std::string> sys_messages;  // per system of the last batch
+// Workspaces and sizing history of one evaluation pipeline: the context's
+// own, and one per captured step (whose recorded addresses must stay valid
+// while the context evaluates other batches).
+struct EvalState {
   Buffer atom_buf, bin_buf, edge_buf, bond_buf, ctl_buf, slab_buf;
   // Skin reuse (xreac_options.neighbor_skin): the kept rows within rc + 2 skin
   // (slab of skin_stride keys per atom), their lengths, and the positions and
   // types they were built at; valid for calls of the same shape as the last.
   Buffer skin_slab_buf, skin_counts_buf, skin_xref_buf, skin_tref_buf;
-  // The kept rows are geometry only: they depend on the cutoff, the skin and
-  // the systems' atom ranges and cells (types are compared on the device).
-  struct SkinKey {
-    double rc = 0.0;
-    double skin = 0.0;
-    std::vector<int32_t> offsets;
-    std::vector<double> cells;
-    std::vector<int32_t> pbc;
-    bool operator==(const SkinKey& o) const {
-      return rc == o.rc && skin == o.skin && offsets == o.offsets && cells == o.cells && pbc == o.pbc;
-    }
-  } skin_key;
+  SkinKey skin_key;
   bool skin_valid = false;
   int skin_stride = 0;
-  int skin_max_row = 0;
-  int64_t neighbor_list_builds = 0;
   char* ctl_host = nullptr;  // pinned mirror of the control block
   size_t ctl_host_bytes = 0;
-  XrProfiler prof;
   int last_cg_launches = 0;
   int cg_history[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // launches of the last eight solves (0: none yet)
   int cg_history_next = 0;
   double m_per_atom = 0.0;  // edges and bonds per atom of the last evaluation
   double b_per_atom = 0.0;
   int max_row = 0;  // longest neighbor row of the last built list
-  // Second stream (default priority) for the charge-independent bonded terms,
-  // which run while the QEq solve occupies the context's stream.
-  hipStream_t side = nullptr;
-  hipEvent_t bonds_ready = nullptr, side_done = nullptr;
-  // QEq read-back: pinned slots and their copy events
-  int* poll_host = nullptr;
-  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   // last evaluation
   int n = 0;
   int64_t m = 0, nb = 0;
@@ -153,6 +142,40 @@ struct xreac_ctx {
   int* bond_j = nullptr;
   int* bond_code = nullptr;
   double* bond_BO = nullptr;
+  EvalState() = default;
+  EvalState(const EvalState&) = delete;
+  EvalState& operator=(const EvalState&) = delete;
+  ~EvalState() {
+    if (ctl_host) (void)hipHostFree(ctl_host);
+  }
+  // Sizing history only (what the next evaluation's buffers are sized from).
+  void take_hints(const EvalState& o) {
+    last_cg_launches = o.last_cg_launches;
+    std::copy(o.cg_history, o.cg_history + 8, cg_history);
+    cg_history_next = o.cg_history_next;
+    m_per_atom = o.m_per_atom;
+    b_per_atom = o.b_per_atom;
+    max_row = o.max_row;
+  }
+};
+
+}  // namespace
+
+struct xreac_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string message;
+  std::vector<std::string> sys_messages;  // per system of the last batch
+  EvalState es;
+  int64_t neighbor_list_builds = 0;
+  XrProfiler prof;
+  // Second stream (default priority) for the charge-independent bonded terms,
+  // which run while the QEq solve occupies the context's stream.
+  hipStream_t side = nullptr;
+  hipEvent_t bonds_ready = nullptr, side_done = nullptr;
+  // QEq read-back: pinned slots and their copy events
+  int* poll_host = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   void* scratch = nullptr;  // this device's scratch array (resolved at creation)
 };
 
@@ -334,7 +357,8 @@ int xreac_ctx_create(int device, void* stream, xreac_ctx** out) {
   for (hipEvent_t& ev : c->poll_ev) ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
   ok = ok && hipHostMalloc(reinterpret_cast<void**>(&c->poll_host), 16 * sizeof(int)) == hipSuccess;
   const size_t ctl_bytes = 64 * 1024;
-  if (ok && hipHostMalloc(reinterpret_cast<void**>(&c->ctl_host), ctl_bytes) == hipSuccess) c->ctl_host_bytes = ctl_bytes;
+  if (ok && hipHostMalloc(reinterpret_cast<void**>(&c->es.ctl_host), ctl_bytes) == hipSuccess)
+    c->es.ctl_host_bytes = ctl_bytes;
   else ok = false;
   if (ok && !g_scratch_ptr[device]) {
     void* p = nullptr;
@@ -362,7 +386,6 @@ void xreac_ctx_destroy(xreac_ctx* c) {
   for (hipEvent_t ev : c->poll_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->poll_host) (void)hipHostFree(c->poll_host);
-  if (c->ctl_host) (void)hipHostFree(c->ctl_host);
   delete c;
 }
 
@@ -496,13 +519,128 @@ hipError_t workspace(Arena* arena, Buffer& buf, size_t need, void** out) {
   return hipSuccess;
 }
 
+// What the host side of an evaluation needs once the device is done.
+struct Finish {
+  CtlLayout L{};
+  int ns = 0, n = 0;
+  bool given_q = false, full = false, supplied = false;
+  bool fixed = false;  // a fixed number of CG launches (a recorded step)
+  double skin = 0.0;
+  int64_t m_cap = 0, nbond = 0, nbr_count = 0;
+  int* row_ptr = nullptr;
+  int* nbr_j = nullptr;
+  int* nbr_s = nullptr;
+  int* bond_ptr = nullptr;
+  int* bond_i = nullptr;
+  int* bond_j = nullptr;
+  int* bond_code = nullptr;
+  double* bond_BO = nullptr;
+};
+
+// Recording an evaluation into a graph (stream capture): no read-back, a
+// fixed number of CG launches per solve, and no buffer may be (re)allocated
+// while the stream captures; a workspace that would grow is named instead
+// and the recording is abandoned, to be grown and recorded again.
+struct Capture {
+  int cg_launches = 0;
+  Finish fin;
+  Buffer* grow = nullptr;
+  size_t grow_bytes = 0;
+  size_t grow_pinned = 0;  // bytes of pinned control block needed
+};
+
+bool cg_final(int state) { return state == CG_DONE || state == CG_STALLED || state == CG_FAILED; }
+
+// Host side of an evaluation once the device is done with it: overflow
+// detection, per-system status, components and QEq iterations, and the
+// sizing history the next evaluation's buffers are taken from.
+int finish_eval(xreac_ctx* ctx, EvalState& st, const Finish& f, BatchOut& out, bool* overflow) {
+  *overflow = false;
+  char* hc = st.ctl_host;
+  const CtlLayout& L = f.L;
+  const int ns = f.ns;
+  const unsigned* h_flags = reinterpret_cast<const unsigned*>(hc + L.flags);
+  const int* host_counts = reinterpret_cast<const int*>(h_flags) + 1;
+  const int64_t m = host_counts[0], nb_total = host_counts[1];
+  if (f.supplied && m != f.nbr_count)
+    return fail(ctx, XREAC_INVALID, "neighbor_row_ptr does not match neighbor_count");
+  if (m > f.m_cap || nb_total > f.nbond || (h_flags[0] & XR_ERR_CAPACITY)) {
+    *overflow = true;
+    return XREAC_OK;
+  }
+  const unsigned* h_serr = reinterpret_cast<const unsigned*>(hc + L.serr);
+  const double* h_comps = reinterpret_cast<const double*>(hc + L.comps);
+  const CgSys* h_cg = xr_cg_systems(hc + L.cg);
+  const CgSys* h_cg2 = xr_cg_systems(hc + L.cg2);
+  // With a fixed launch count a system still running when the launches ran
+  // out is not a failure: the step is repeated with read-backs.
+  if (f.fixed)
+    for (int q = 0; q < ns; ++q)
+      if (!h_serr[q] && ((!f.given_q && !cg_final(h_cg[q].ctl[0].done)) || (f.full && !cg_final(h_cg2[q].ctl[0].done)))) {
+        *overflow = true;
+        return XREAC_OK;
+      }
+  ctx->sys_messages.assign(ns, std::string());
+  int launches = 0;
+  for (int q = 0; q < ns; ++q) {
+    out.iters[q] = f.given_q ? 0 : h_cg[q].iters;
+    if (!f.given_q && h_cg[q].ctl[0].done == CG_DONE) launches = std::max(launches, h_cg[q].ks + 1);
+    if (f.full && h_cg2[q].ctl[0].done == CG_DONE) launches = std::max(launches, h_cg2[q].ks + 1);
+    xr_components(h_comps + (size_t)q * NCH, out.comps + (size_t)q * NCOMP);
+    if (h_serr[q]) {
+      out.status[q] = XREAC_INVALID;
+      ctx->sys_messages[q] = error_text(h_serr[q]);
+    } else if ((!f.given_q && h_cg[q].ctl[0].done != CG_DONE) || (f.full && h_cg2[q].ctl[0].done != CG_DONE)) {
+      out.status[q] = XREAC_INVALID;
+      // calculator.py:149-150 raises this text when the dense solve fails
+      ctx->sys_messages[q] =
+          "Singular QEq system; check the geometry and parameters (the QEq conjugate gradient did not converge)";
+    } else {
+      out.status[q] = XREAC_OK;
+    }
+  }
+  if (f.skin > 0.0) {
+    const unsigned* h_gate = reinterpret_cast<const unsigned*>(hc + L.gate);
+    bool clean = true;
+    for (int q = 0; q < ns; ++q) {
+      ctx->neighbor_list_builds += h_gate[q] ? 1 : 0;
+      clean = clean && h_serr[q] == 0;
+    }
+    st.skin_valid = clean;  // a failed system may have left its kept rows incomplete
+  }
+  out.num_edges = m;
+  out.num_bonds = nb_total;
+  st.m_per_atom = (double)m / f.n;
+  st.b_per_atom = (double)nb_total / f.n;
+  if (!f.supplied) st.max_row = host_counts[2];
+  st.last_cg_launches = launches;
+  if (launches > 0) {
+    st.cg_history[st.cg_history_next] = launches;
+    st.cg_history_next = (st.cg_history_next + 1) % 8;
+  }
+
+  st.n = f.n;
+  st.m = m;
+  st.nb = nb_total;
+  st.row_ptr = f.row_ptr;
+  st.nbr_j = f.nbr_j;
+  st.nbr_s = f.nbr_s;
+  st.bond_ptr = f.bond_ptr;
+  st.bond_i = f.bond_i;
+  st.bond_j = f.bond_j;
+  st.bond_code = f.bond_code;
+  st.bond_BO = f.bond_BO;
+  return XREAC_OK;
+}
+
 // One attempt.  With exact == false, edge and bond buffers are sized from the
 // previous evaluation's per-atom counts and no mid-pipeline read-back happens;
 // if a count exceeds its buffer, *overflow is set and the caller repeats the
 // evaluation with exact == true (sizes read back before allocation).
-int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out,
-                  bool exact, bool* overflow, Arena* arena) {
+int evaluate_once(xreac_ctx* ctx, EvalState& st, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt,
+                  BatchOut& out, bool exact, bool* overflow, Arena* arena, Capture* cap = nullptr) {
   *overflow = false;
+  if (cap && (exact || arena)) return fail(ctx, XREAC_INVALID, "internal error: a recorded evaluation is sized ahead");
   if (arena) {
     arena->used = 0;
     arena->wanted = 0;
@@ -525,7 +663,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
       }
       xr_active_profiler = nullptr;
     }
-  } active(arena ? nullptr : &ctx->prof);  // no timer events (which would be created) in static scratch
+  } active(arena || cap ? nullptr : &ctx->prof);  // no timer events in static scratch or in a recording
   const double rc = ffh->cutoff;
   const double bond_cut = std::min(5.0, rc);
   // ase.py:90-93 (the same check and text)
@@ -534,7 +672,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // Skin reuse applies to lists built here (not to supplied ones).  As ASE's neighbor_skin (ase.py:39-40,
   // 112-124): a per-atom displacement allowance, so the kept rows reach rc + 2 skin, and bins are that wide.
   const double skin =
-      (in.nbr_row_ptr == nullptr && arena == nullptr && opt->neighbor_skin > 0.0) ? opt->neighbor_skin : 0.0;
+      (in.nbr_row_ptr == nullptr && arena == nullptr && cap == nullptr && opt->neighbor_skin > 0.0)
+          ? opt->neighbor_skin
+          : 0.0;
   const double rc_keep = rc + 2.0 * skin;
   const double rcb = rc_keep * (1 + 1e-8) + 1e-12;
   const bool mixed = opt->precision == XREAC_PRECISION_MIXED;
@@ -547,6 +687,11 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // The arena's workspaces are checked against its capacity as they are taken.
 #define XR_WORKSPACE(buf, need, out, what)                                                                    \
   do {                                                                                                        \
+    if (cap && (size_t)(need) > (buf).bytes) {                                                                \
+      cap->grow = &(buf);                                                                                     \
+      cap->grow_bytes = (need);                                                                               \
+      return XREAC_OK;                                                                                        \
+    }                                                                                                         \
     hipError_t e_ = workspace(arena, (buf), (need), (out));                                                   \
     if (e_ != hipSuccess && arena)                                                                            \
       return fail(ctx, XREAC_NOMEM,                                                                           \
@@ -557,20 +702,24 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     if (e_ != hipSuccess) return hip_fail(ctx, e_, what);                                                     \
   } while (0)
   void* ctl_dev = nullptr;
-  XR_WORKSPACE(ctx->ctl_buf, L.total, &ctl_dev, "control block");
-  if (arena && ctx->ctl_host_bytes < L.total)
+  XR_WORKSPACE(st.ctl_buf, L.total, &ctl_dev, "control block");
+  if (arena && st.ctl_host_bytes < L.total)
     return fail(ctx, XREAC_NOMEM, "xreac_evaluate_static: the pinned control block made at context creation is "
                                   "too small for this call");
-  if (ctx->ctl_host_bytes < L.total) {
-    if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
-    ctx->ctl_host = nullptr;
-    ctx->ctl_host_bytes = 0;
+  if (cap && st.ctl_host_bytes < L.total) {
+    cap->grow_pinned = L.total;
+    return XREAC_OK;
+  }
+  if (st.ctl_host_bytes < L.total) {
+    if (st.ctl_host) (void)hipHostFree(st.ctl_host);
+    st.ctl_host = nullptr;
+    st.ctl_host_bytes = 0;
     const size_t want = L.total + L.total / 2;
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&ctx->ctl_host), want), "pinned control block");
-    ctx->ctl_host_bytes = want;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&st.ctl_host), want), "pinned control block");
+    st.ctl_host_bytes = want;
   }
   char* dc = static_cast<char*>(ctl_dev);
-  char* hc = ctx->ctl_host;
+  char* hc = st.ctl_host;
   unsigned* d_flags = reinterpret_cast<unsigned*>(dc + L.flags);
   SysDev* hsys = reinterpret_cast<SysDev*>(hc + L.sys);
   std::vector<double> heights(3 * (size_t)ns);
@@ -644,7 +793,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double2>(off, full ? n : 0);    // charge-response right-hand sides (dE/dq, -1)
   carve_size<double>(off, full ? n : 0);     // charge-response adjoint lambda
   void* atom_ws = nullptr;
-  XR_WORKSPACE(ctx->atom_buf, off + 4096, &atom_ws, "atom workspace");
+  XR_WORKSPACE(st.atom_buf, off + 4096, &atom_ws, "atom workspace");
   Carver cv{static_cast<char*>(atom_ws)};
   int* asys = cv.take<int>(n);
   int* home = cv.take<int>(3 * (size_t)n);
@@ -703,7 +852,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<int>(off, nbins + 1);
   carve_size<int>(off, xr_scan_scratch_ints((int)nbins));
   void* bin_ws = nullptr;
-  XR_WORKSPACE(ctx->bin_buf, off + 4096, &bin_ws, "bin workspace");
+  XR_WORKSPACE(st.bin_buf, off + 4096, &bin_ws, "bin workspace");
   Carver cb{static_cast<char*>(bin_ws)};
   int* bin_count = cb.take<int>(nbins + 1);
   int* bin_start = cb.take<int>(nbins + 1);
@@ -792,7 +941,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // supplied list gets the count pass too: it checks every pair for
   // coincident atoms, as validate_input does whatever list is supplied.
   const bool supplied = in.nbr_row_ptr != nullptr;
-  int max_row = ctx->max_row;
+  int max_row = st.max_row;
   int skin_max_row = 0;
   int64_t m_cap = 0;
   if (supplied) {
@@ -804,8 +953,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     HIPCHK(xr_clamp_supplied(row_ptr, n, (int)m_cap, dev_counts, d_flags, S, s), "clamp");
   } else {
     const double slack = 1.10;
-    m_cap = exact || ctx->m_per_atom <= 0 ? 0 : (int64_t)std::ceil(ctx->m_per_atom * n * slack) + 1024;
+    m_cap = exact || st.m_per_atom <= 0 ? 0 : (int64_t)std::ceil(st.m_per_atom * n * slack) + 1024;
     exact = exact || m_cap == 0;
+    if (cap && exact) return fail(ctx, XREAC_INVALID, "recording a step needs an earlier evaluation to size it");
     if (m_cap > INT32_MAX - 1) m_cap = INT32_MAX - 1;
     if (exact) {
       if (skin > 0.0) {  // the longest row within rc + 2 skin sizes the kept rows
@@ -839,7 +989,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<unsigned char>(off, m_cap);
   carve_size<int2>(off, m_cap);  // H-bond candidates, at the start of each row's range
   void* edge_ws = nullptr;
-  XR_WORKSPACE(ctx->edge_buf, off + 4096, &edge_ws, "edge workspace");
+  XR_WORKSPACE(st.edge_buf, off + 4096, &edge_ws, "edge workspace");
   Carver ce{static_cast<char*>(edge_ws)};
   int* nbr_j = ce.take<int>(m_cap + 1);
   int* nbr_s = ce.take<int>(m_cap);
@@ -859,7 +1009,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     // is repeated exactly.
     const int hint = exact ? max_row : (max_row > 0 ? max_row + 64 : 2048);
     slab_stride = xr_fill_stride(hint);
-    XR_WORKSPACE(ctx->slab_buf, sizeof(unsigned long long) * (size_t)slab_stride * n + 4096, &slab,
+    XR_WORKSPACE(st.slab_buf, sizeof(unsigned long long) * (size_t)slab_stride * n + 4096, &slab,
                  "neighbor slab");
     if (skin > 0.0) {
       // Skin reuse.  The kept rows' stride comes from an exact count, else stays as kept, else from the
@@ -867,42 +1017,42 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
       int hint_s;
       if (exact) {
         hint_s = skin_max_row;
-      } else if (ctx->skin_stride > 0) {
-        hint_s = ctx->skin_stride;
+      } else if (st.skin_stride > 0) {
+        hint_s = st.skin_stride;
       } else {
         const double grow = std::pow(rc_keep / rc, 3.0);
         hint_s = max_row > 0 ? (int)std::min(2048.0, std::ceil(max_row * grow * 1.1) + 64) : 2048;
       }
       const int stride_s = xr_fill_stride(hint_s);
-      xreac_ctx::SkinKey key;
+      SkinKey key;
       key.rc = rc;
       key.skin = skin;
       key.offsets.assign(in.offsets, in.offsets + ns + 1);
       if (in.cells) key.cells.assign(in.cells, in.cells + 9 * (size_t)ns);
       if (in.pbc) key.pbc.assign(in.pbc, in.pbc + 3 * (size_t)ns);
-      bool force = exact || !ctx->skin_valid || stride_s != ctx->skin_stride || !(key == ctx->skin_key);
-      const void* kept[4] = {ctx->skin_slab_buf.ptr, ctx->skin_counts_buf.ptr, ctx->skin_xref_buf.ptr,
-                             ctx->skin_tref_buf.ptr};
-      HIPCHK(ctx->skin_slab_buf.ensure(sizeof(unsigned long long) * (size_t)stride_s * n + 4096), "skin rows");
-      HIPCHK(ctx->skin_counts_buf.ensure(sizeof(int) * (size_t)n + 256), "skin rows");
-      HIPCHK(ctx->skin_xref_buf.ensure(sizeof(double) * 3 * (size_t)n + 256), "skin rows");
-      HIPCHK(ctx->skin_tref_buf.ensure(sizeof(int) * (size_t)n + 256), "skin rows");
-      force = force || kept[0] != ctx->skin_slab_buf.ptr || kept[1] != ctx->skin_counts_buf.ptr ||
-              kept[2] != ctx->skin_xref_buf.ptr || kept[3] != ctx->skin_tref_buf.ptr;
-      ctx->skin_valid = false;  // until this call completes cleanly
-      ctx->skin_stride = stride_s;
-      ctx->skin_key = std::move(key);
+      bool force = exact || !st.skin_valid || stride_s != st.skin_stride || !(key == st.skin_key);
+      const void* kept[4] = {st.skin_slab_buf.ptr, st.skin_counts_buf.ptr, st.skin_xref_buf.ptr,
+                             st.skin_tref_buf.ptr};
+      HIPCHK(st.skin_slab_buf.ensure(sizeof(unsigned long long) * (size_t)stride_s * n + 4096), "skin rows");
+      HIPCHK(st.skin_counts_buf.ensure(sizeof(int) * (size_t)n + 256), "skin rows");
+      HIPCHK(st.skin_xref_buf.ensure(sizeof(double) * 3 * (size_t)n + 256), "skin rows");
+      HIPCHK(st.skin_tref_buf.ensure(sizeof(int) * (size_t)n + 256), "skin rows");
+      force = force || kept[0] != st.skin_slab_buf.ptr || kept[1] != st.skin_counts_buf.ptr ||
+              kept[2] != st.skin_xref_buf.ptr || kept[3] != st.skin_tref_buf.ptr;
+      st.skin_valid = false;  // until this call completes cleanly
+      st.skin_stride = stride_s;
+      st.skin_key = std::move(key);
       unsigned* gate = reinterpret_cast<unsigned*>(dc + L.gate);
-      int* skin_counts = static_cast<int*>(ctx->skin_counts_buf.ptr);
-      double* xref = static_cast<double*>(ctx->skin_xref_buf.ptr);
-      int* tref = static_cast<int*>(ctx->skin_tref_buf.ptr);
+      int* skin_counts = static_cast<int*>(st.skin_counts_buf.ptr);
+      double* xref = static_cast<double*>(st.skin_xref_buf.ptr);
+      int* tref = static_cast<int*>(st.skin_tref_buf.ptr);
       // atoms that moved at most skin (less a rounding margin) keep every pair within rc in the kept rows
       HIPCHK(xr_skin_check(S, in.x, in.types, xref, tref, skin * (1.0 - 1e-9), force, gate, s), "skin check");
       HIPCHK(xr_neighbor_fill(n, in.x, S, home, bin, bin_start, binned, rc_keep, skin_counts,
-                              ctx->skin_slab_buf.ptr, stride_s, d_flags, s, gate),
+                              st.skin_slab_buf.ptr, stride_s, d_flags, s, gate),
              "neighbor fill");
       HIPCHK(xr_skin_commit(n, S, in.x, in.types, xref, tref, gate, s), "skin commit");
-      HIPCHK(xr_skin_filter(n, in.x, S, ctx->skin_slab_buf.ptr, stride_s, skin_counts, rc, counts, slab,
+      HIPCHK(xr_skin_filter(n, in.x, S, st.skin_slab_buf.ptr, stride_s, skin_counts, rc, counts, slab,
                             slab_stride, d_flags, s),
              "skin filter");
     } else {
@@ -925,7 +1075,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   const bool bonds_on_side = !exact;
   int64_t b_cap = 0;
   if (!exact) {
-    b_cap = (int64_t)std::ceil(ctx->b_per_atom * n * 1.25) + 1024;
+    b_cap = (int64_t)std::ceil(st.b_per_atom * n * 1.25) + 1024;
     if (b_cap > m_cap) b_cap = m_cap;
   } else {
     HIPCHK(xr_scan_clamped(bond_count, n, bond_ptr, scan_tmp, INT32_MAX, dev_counts + 1, d_flags, nullptr, s),
@@ -943,7 +1093,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   carve_size<double>(off, 3 * (size_t)nbond);               // G
   carve_size<double>(off, (size_t)kBondScalars * nbond);    // per-bond scalars
   void* bond_ws = nullptr;
-  XR_WORKSPACE(ctx->bond_buf, off + 4096, &bond_ws, "bond workspace");
+  XR_WORKSPACE(st.bond_buf, off + 4096, &bond_ws, "bond workspace");
 #undef XR_WORKSPACE
   Carver cbd{static_cast<char*>(bond_ws)};
   Bonds bd{};
@@ -1070,9 +1220,9 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
   // the next, easier one queue a long run of idle launches before its first read-back.
   {
     int recent[8], count = 0;
-    for (int h : ctx->cg_history)
+    for (int h : st.cg_history)
       if (h > 0) recent[count++] = h;
-    int first = ctx->last_cg_launches > 0 ? ctx->last_cg_launches + 1 : 16;
+    int first = st.last_cg_launches > 0 ? st.last_cg_launches + 1 : 16;
     if (count > 0) {
       std::sort(recent, recent + count);
       first = std::min(first, 2 * recent[count / 2] + 1);
@@ -1080,6 +1230,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     cg.first_check = std::max(2, std::min(first, 128));
   }
   cg.check_every = 2;
+  cg.fixed_launches = cap ? cap->cg_launches : 0;
   cg.guess = (opt->warm_start && out.qeq_state) ? reinterpret_cast<const double2*>(out.qeq_state) : nullptr;
   cg.keep = reinterpret_cast<double2*>(out.qeq_state);
   cg.q = out.charges;
@@ -1131,7 +1282,7 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     TailAll* fn;
     hipError_t err;
   } tail{&tail_all, hipSuccess};
-  if (!full && ns == 1 && ctx->last_cg_launches > 0) {
+  if (!cap && !full && ns == 1 && st.last_cg_launches > 0) {
     cg.tail = [](void* p) {
       TailHook* t = static_cast<TailHook*>(p);
       return t->err = (*t->fn)();
@@ -1180,71 +1331,33 @@ int evaluate_once(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const 
     }
     if (hipError_t e = tail_back(); e != hipSuccess) return hip_fail(ctx, e, tail_what);
   }
-  HIPCHK(hipStreamSynchronize(s), "final sync");
-  if (!arena) ctx->prof.resolve();
-  const unsigned* h_flags = reinterpret_cast<const unsigned*>(hc + L.flags);
-  const int* host_counts = reinterpret_cast<const int*>(h_flags) + 1;
-  const int64_t m = host_counts[0], nb_total = host_counts[1];
-  if (supplied && m != in.nbr_count)
-    return fail(ctx, XREAC_INVALID, "neighbor_row_ptr does not match neighbor_count");
-  if (m > m_cap || nb_total > nbond || (h_flags[0] & XR_ERR_CAPACITY)) {
-    *overflow = true;
+  Finish fin;
+  fin.L = L;
+  fin.ns = ns;
+  fin.n = n;
+  fin.given_q = given_q;
+  fin.full = full;
+  fin.supplied = supplied;
+  fin.fixed = cap != nullptr;
+  fin.skin = skin;
+  fin.m_cap = m_cap;
+  fin.nbond = nbond;
+  fin.nbr_count = in.nbr_count;
+  fin.row_ptr = row_ptr;
+  fin.nbr_j = nbr_j;
+  fin.nbr_s = nbr_s;
+  fin.bond_ptr = bond_ptr;
+  fin.bond_i = b_i;
+  fin.bond_j = bd.bj;
+  fin.bond_code = bd.code;
+  fin.bond_BO = bd.BO;
+  if (cap) {  // recorded, not run: the replay's host side reads the control block
+    cap->fin = fin;
     return XREAC_OK;
   }
-  const unsigned* h_serr = reinterpret_cast<const unsigned*>(hc + L.serr);
-  const double* h_comps = reinterpret_cast<const double*>(hc + L.comps);
-  const CgSys* h_cg = xr_cg_systems(hc + L.cg);
-  const CgSys* h_cg2 = xr_cg_systems(hc + L.cg2);
-  ctx->sys_messages.assign(ns, std::string());
-  int launches = 0;
-  for (int q = 0; q < ns; ++q) {
-    out.iters[q] = given_q ? 0 : h_cg[q].iters;
-    if (!given_q && h_cg[q].ctl[0].done == CG_DONE) launches = std::max(launches, h_cg[q].ks + 1);
-    xr_components(h_comps + (size_t)q * NCH, out.comps + (size_t)q * NCOMP);
-    if (h_serr[q]) {
-      out.status[q] = XREAC_INVALID;
-      ctx->sys_messages[q] = error_text(h_serr[q]);
-    } else if ((!given_q && h_cg[q].ctl[0].done != CG_DONE) || (full && h_cg2[q].ctl[0].done != CG_DONE)) {
-      out.status[q] = XREAC_INVALID;
-      // calculator.py:149-150 raises this text when the dense solve fails
-      ctx->sys_messages[q] =
-          "Singular QEq system; check the geometry and parameters (the QEq conjugate gradient did not converge)";
-    } else {
-      out.status[q] = XREAC_OK;
-    }
-  }
-  if (skin > 0.0) {
-    const unsigned* h_gate = reinterpret_cast<const unsigned*>(hc + L.gate);
-    bool clean = true;
-    for (int q = 0; q < ns; ++q) {
-      ctx->neighbor_list_builds += h_gate[q] ? 1 : 0;
-      clean = clean && h_serr[q] == 0;
-    }
-    ctx->skin_valid = clean;  // a failed system may have left its kept rows incomplete
-  }
-  out.num_edges = m;
-  out.num_bonds = nb_total;
-  ctx->m_per_atom = (double)m / n;
-  ctx->b_per_atom = (double)nb_total / n;
-  if (!supplied) ctx->max_row = host_counts[2];
-  ctx->last_cg_launches = launches;
-  if (launches > 0) {
-    ctx->cg_history[ctx->cg_history_next] = launches;
-    ctx->cg_history_next = (ctx->cg_history_next + 1) % 8;
-  }
-
-  ctx->n = n;
-  ctx->m = m;
-  ctx->nb = nb_total;
-  ctx->row_ptr = row_ptr;
-  ctx->nbr_j = nbr_j;
-  ctx->nbr_s = nbr_s;
-  ctx->bond_ptr = bond_ptr;
-  ctx->bond_i = b_i;
-  ctx->bond_j = bd.bj;
-  ctx->bond_code = bd.code;
-  ctx->bond_BO = bd.BO;
-  return XREAC_OK;
+  HIPCHK(hipStreamSynchronize(s), "final sync");
+  if (!arena) ctx->prof.resolve();
+  return finish_eval(ctx, st, fin, out, overflow);
 }
 
 int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xreac_options* opt, BatchOut& out,
@@ -1252,8 +1365,8 @@ int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xrea
   // The CG sweep addresses gathered vector entries with 32-bit byte offsets (16 B per atom).
   if (in.offsets[in.ns] >= (1 << 28)) return fail(ctx, XREAC_INVALID, "at most 268,435,455 atoms per call");
   bool overflow = false;
-  int rc = evaluate_once(ctx, ffh, in, opt, out, false, &overflow, arena);
-  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ffh, in, opt, out, true, &overflow, arena);
+  int rc = evaluate_once(ctx, ctx->es, ffh, in, opt, out, false, &overflow, arena);
+  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ctx->es, ffh, in, opt, out, true, &overflow, arena);
   if (rc == XREAC_OK && overflow) return fail(ctx, XREAC_DEVICE, "internal error: buffer sizing");
   return rc;
 }
@@ -1430,15 +1543,15 @@ int xreac_profile_read(xreac_ctx* ctx, int timer, double* total_ms, int64_t* lau
 
 int xreac_copy_neighbors(xreac_ctx* ctx, int32_t* row_ptr, int32_t* j, int32_t* shifts, int64_t capacity) {
   if (!ctx || !row_ptr || !j || !shifts) return XREAC_INVALID;
-  if (capacity < ctx->m) return fail(ctx, XREAC_INVALID, "neighbor capacity too small");
+  if (capacity < ctx->es.m) return fail(ctx, XREAC_INVALID, "neighbor capacity too small");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-  std::vector<int> codes(ctx->m);
-  HIPCHK(hipMemcpy(row_ptr, ctx->row_ptr, sizeof(int) * (ctx->n + 1), hipMemcpyDeviceToHost), "copy");
-  if (ctx->m) {
-    HIPCHK(hipMemcpy(j, ctx->nbr_j, sizeof(int) * ctx->m, hipMemcpyDeviceToHost), "copy");
-    HIPCHK(hipMemcpy(codes.data(), ctx->nbr_s, sizeof(int) * ctx->m, hipMemcpyDeviceToHost), "copy");
+  std::vector<int> codes(ctx->es.m);
+  HIPCHK(hipMemcpy(row_ptr, ctx->es.row_ptr, sizeof(int) * (ctx->es.n + 1), hipMemcpyDeviceToHost), "copy");
+  if (ctx->es.m) {
+    HIPCHK(hipMemcpy(j, ctx->es.nbr_j, sizeof(int) * ctx->es.m, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(codes.data(), ctx->es.nbr_s, sizeof(int) * ctx->es.m, hipMemcpyDeviceToHost), "copy");
   }
-  for (int64_t e = 0; e < ctx->m; ++e) {
+  for (int64_t e = 0; e < ctx->es.m; ++e) {
     int sx, sy, sz;
     xr_unpack_shift(codes[e], sx, sy, sz);
     shifts[3 * e] = sx;
@@ -1450,16 +1563,16 @@ int xreac_copy_neighbors(xreac_ctx* ctx, int32_t* row_ptr, int32_t* j, int32_t* 
 
 int xreac_copy_bonds(xreac_ctx* ctx, int32_t* i, int32_t* j, int32_t* shifts, double* bo, int64_t capacity) {
   if (!ctx || !i || !j || !shifts || !bo) return XREAC_INVALID;
-  if (capacity < ctx->nb) return fail(ctx, XREAC_INVALID, "bond capacity too small");
+  if (capacity < ctx->es.nb) return fail(ctx, XREAC_INVALID, "bond capacity too small");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-  std::vector<int> codes(ctx->nb);
-  if (ctx->nb) {
-    HIPCHK(hipMemcpy(i, ctx->bond_i, sizeof(int) * ctx->nb, hipMemcpyDeviceToHost), "copy");
-    HIPCHK(hipMemcpy(j, ctx->bond_j, sizeof(int) * ctx->nb, hipMemcpyDeviceToHost), "copy");
-    HIPCHK(hipMemcpy(codes.data(), ctx->bond_code, sizeof(int) * ctx->nb, hipMemcpyDeviceToHost), "copy");
-    HIPCHK(hipMemcpy(bo, ctx->bond_BO, sizeof(double) * ctx->nb, hipMemcpyDeviceToHost), "copy");
+  std::vector<int> codes(ctx->es.nb);
+  if (ctx->es.nb) {
+    HIPCHK(hipMemcpy(i, ctx->es.bond_i, sizeof(int) * ctx->es.nb, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(j, ctx->es.bond_j, sizeof(int) * ctx->es.nb, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(codes.data(), ctx->es.bond_code, sizeof(int) * ctx->es.nb, hipMemcpyDeviceToHost), "copy");
+    HIPCHK(hipMemcpy(bo, ctx->es.bond_BO, sizeof(double) * ctx->es.nb, hipMemcpyDeviceToHost), "copy");
   }
-  for (int64_t e = 0; e < ctx->nb; ++e) {
+  for (int64_t e = 0; e < ctx->es.nb; ++e) {
     int sx, sy, sz;
     xr_unpack_shift(codes[e], sx, sy, sz);
     shifts[3 * e] = sx;
@@ -1467,6 +1580,302 @@ int xreac_copy_bonds(xreac_ctx* ctx, int32_t* i, int32_t* j, int32_t* shifts, do
     shifts[3 * e + 2] = sz;
   }
   return XREAC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ recorded steps
+// One step of a fixed batch (integrator stages around an evaluation) recorded
+// once as a HIP graph and replayed, so a step costs one graph launch and one
+// wait instead of a hundred host-issued launches and the QEq read-backs.  The
+// step owns its workspaces (their addresses are baked into the graph); the
+// context's own are untouched by replays.  A replay whose buffers overflowed
+// or whose QEq needed more launches than were recorded restores what the
+// step modifies and runs the step again with the context's read-backs, then
+// records it afresh, so a run always completes the step with the results the
+// unrecorded step gives.
+struct xreac_step {
+  xreac_ctx* ctx = nullptr;
+  const xreac_ff* ff = nullptr;
+  EvalState st;
+  int ns = 0, n = 0;
+  std::vector<int32_t> offsets, pbc;
+  std::vector<double> cells, charges;
+  const double* x = nullptr;
+  const int32_t* types = nullptr;
+  xreac_options opt{};
+  xreac_step_stages stg{};
+  double* forces = nullptr;
+  double* q = nullptr;
+  double* tbo = nullptr;
+  double* nlp = nullptr;
+  int32_t* bcounts = nullptr;
+  double* qeq_state = nullptr;
+  Buffer dev_offsets;  // device copy of offsets (integrator stages)
+  Buffer backup;       // what the step modifies, as it was before the replay
+  struct Saved {
+    void* at;
+    size_t bytes;
+    size_t off;
+  };
+  std::vector<Saved> saved;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  Finish fin;
+  int cg_launches = 0;
+  int64_t replays = 0, fallbacks = 0, recordings = 0;
+  ~xreac_step() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+
+BatchIn step_batch(const xreac_step* p) {
+  BatchIn in{p->ns, p->offsets.data(), p->x, p->types, p->cells.empty() ? nullptr : p->cells.data(),
+             p->pbc.empty() ? nullptr : p->pbc.data(), p->charges.empty() ? nullptr : p->charges.data()};
+  return in;
+}
+
+BatchOut step_out(xreac_step* p, xreac_batch_result* r) {
+  BatchOut bo{};
+  bo.comps = r->components;
+  bo.status = r->status;
+  bo.iters = r->qeq_iterations;
+  bo.forces = p->forces;
+  bo.charges = p->q;
+  bo.tbo = p->tbo;
+  bo.nlp = p->nlp;
+  bo.bcounts = p->bcounts;
+  bo.qeq_state = p->qeq_state;
+  return bo;
+}
+
+hipError_t stage_before(xreac_step* p, hipStream_t s) {
+  const int* off = static_cast<const int*>(p->dev_offsets.ptr);
+  hipError_t e = hipSuccess;
+  if (p->stg.dt_fs)
+    e = xr_verlet(p->ff->dev, p->ns, off, p->types, p->stg.dt_fs, const_cast<double*>(p->x), p->stg.velocities,
+                  p->forces, 0, nullptr, s);
+  if (e == hipSuccess && p->stg.qeq_order)
+    e = xr_qeq_extrapolate(p->ns, p->n, off, p->stg.qeq_order, p->qeq_state, p->stg.qeq_history, s);
+  return e;
+}
+
+hipError_t stage_after(xreac_step* p, hipStream_t s) {
+  const int* off = static_cast<const int*>(p->dev_offsets.ptr);
+  hipError_t e = hipSuccess;
+  if (p->stg.dt_fs)
+    e = xr_verlet(p->ff->dev, p->ns, off, p->types, p->stg.dt_fs, const_cast<double*>(p->x), p->stg.velocities,
+                  p->forces, 1, p->stg.kinetic, s);
+  if (e == hipSuccess && p->stg.fire_state)
+    e = xr_fire(p->ns, off, const_cast<double*>(p->x), p->stg.velocities, p->forces, p->stg.fire_state,
+                p->stg.fire_status, p->stg.fire_force_tolerance, p->stg.fire_max_iterations, s);
+  return e;
+}
+
+// The step with the context's read-backs (the first run, and a replay that
+// did not complete).
+int step_unrecorded(xreac_step* p, xreac_batch_result* r) {
+  xreac_ctx* ctx = p->ctx;
+  HIPCHK(stage_before(p, ctx->stream), "step stages");
+  BatchIn in = step_batch(p);
+  BatchOut bo = step_out(p, r);
+  const int rc = run_batch(ctx, p->ff, in, &p->opt, bo);
+  if (rc != XREAC_OK) return rc;
+  r->num_edges = bo.num_edges;
+  r->num_bonds = bo.num_bonds;
+  HIPCHK(stage_after(p, ctx->stream), "step stages");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "step sync");
+  return XREAC_OK;
+}
+
+// Record the step from the context's latest sizing history.  A workspace that
+// would have to grow while the stream records is grown first and the
+// recording repeated (each buffer grows at most once per recording).
+int step_record(xreac_step* p) {
+  xreac_ctx* ctx = p->ctx;
+  hipStream_t s = ctx->stream;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  p->st.take_hints(ctx->es);
+  // QEq launches: a quarter above the most any recent solve needed, plus four.
+  int most = 0;
+  for (int h : p->st.cg_history) most = std::max(most, h);
+  p->cg_launches = std::min(most + most / 4 + 4, 4096);
+  size_t need = 0;
+  for (const auto& v : p->saved) need = std::max(need, v.off + v.bytes);
+  HIPCHK(p->backup.ensure(need + 256), "step backup");
+  BatchIn in = step_batch(p);
+  std::vector<double> comps((size_t)p->ns * NCOMP);
+  std::vector<int32_t> status(p->ns), iters(p->ns);
+  xreac_batch_result r{};
+  r.components = comps.data();
+  r.status = status.data();
+  r.qeq_iterations = iters.data();
+  BatchOut bo = step_out(p, &r);
+  for (int attempt = 0; attempt < 16; ++attempt) {
+    Capture cap;
+    cap.cg_launches = p->cg_launches;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "step recording");
+    hipError_t e = hipSuccess;
+    for (const auto& v : p->saved)
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(static_cast<char*>(p->backup.ptr) + v.off, v.at, v.bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = stage_before(p, s);
+    bool overflow = false;
+    int rc = XREAC_OK;
+    if (e == hipSuccess) rc = evaluate_once(ctx, p->st, p->ff, in, &p->opt, bo, false, &overflow, nullptr, &cap);
+    const bool complete = e == hipSuccess && rc == XREAC_OK && !cap.grow && !cap.grow_pinned;
+    if (complete) e = stage_after(p, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(s, &g);
+    if (complete && e == hipSuccess && ee == hipSuccess) {
+      p->graph = g;
+      const hipError_t ei = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+      if (ei != hipSuccess) return hip_fail(ctx, ei, "step graph instantiation");
+      p->fin = cap.fin;
+      ++p->recordings;
+      return XREAC_OK;
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != XREAC_OK) return rc;
+    if (e != hipSuccess) return hip_fail(ctx, e, "step recording");
+    if (ee != hipSuccess) return hip_fail(ctx, ee, "step recording");
+    if (cap.grow) HIPCHK(cap.grow->ensure(cap.grow_bytes), "step workspace");
+    if (cap.grow_pinned) {
+      if (p->st.ctl_host) (void)hipHostFree(p->st.ctl_host);
+      p->st.ctl_host = nullptr;
+      p->st.ctl_host_bytes = 0;
+      HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p->st.ctl_host), cap.grow_pinned), "pinned control block");
+      p->st.ctl_host_bytes = cap.grow_pinned;
+    }
+  }
+  return fail(ctx, XREAC_DEVICE, "internal error: step workspaces kept growing while recording");
+}
+
+}  // namespace
+
+extern "C" {
+
+int xreac_step_create(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_batch* b, const xreac_options* opt,
+                      const xreac_step_stages* stages, const xreac_batch_result* result, xreac_step** out) {
+  if (!ctx || !ffh || !b || !opt || !result || !out) return XREAC_INVALID;
+  ctx->message.clear();
+  *out = nullptr;
+  if (b->num_systems <= 0 || !b->atom_offsets || !b->positions || !b->types || !result->forces || !result->charges)
+    return fail(ctx, XREAC_INVALID, "A nonempty batch with offsets, device positions, types, forces and charges is "
+                                    "required");
+  if (b->atom_offsets[0] != 0) return fail(ctx, XREAC_INVALID, "atom_offsets[0] must be 0");
+  for (int q = 0; q < b->num_systems; ++q)
+    if (b->atom_offsets[q + 1] <= b->atom_offsets[q])
+      return fail(ctx, XREAC_INVALID, "every system of a batch needs at least one atom");
+  if (b->atom_offsets[b->num_systems] >= (1 << 28)) return fail(ctx, XREAC_INVALID, "at most 268,435,455 atoms per call");
+  if (b->neighbor_row_ptr) return fail(ctx, XREAC_INVALID, "a recorded step builds its neighbor lists on the device");
+  if (opt->neighbor_skin != 0.0)
+    return fail(ctx, XREAC_INVALID, "a recorded step builds fresh neighbor lists: neighbor_skin must be 0");
+  if (opt->warm_start && !result->qeq_state) return fail(ctx, XREAC_INVALID, "warm_start needs result->qeq_state");
+  if (!ctx->stream) return fail(ctx, XREAC_INVALID, "a recorded step needs a context created with its own stream");
+  xreac_step_stages none{};
+  const xreac_step_stages& sg = stages ? *stages : none;
+  if (sg.dt_fs && !sg.velocities) return fail(ctx, XREAC_INVALID, "velocity Verlet stages need velocities");
+  if (sg.qeq_order && (!sg.qeq_history || !result->qeq_state))
+    return fail(ctx, XREAC_INVALID, "QEq extrapolation needs qeq_history and result->qeq_state");
+  if (sg.fire_state && (!sg.velocities || !sg.fire_status || !(sg.fire_force_tolerance > 0) ||
+                        sg.fire_max_iterations < 0))
+    return fail(ctx, XREAC_INVALID, "a FIRE stage needs velocities, status, a positive tolerance and a limit");
+  auto* p = new xreac_step();
+  p->ctx = ctx;
+  p->ff = ffh;
+  p->ns = b->num_systems;
+  p->n = b->atom_offsets[b->num_systems];
+  p->offsets.assign(b->atom_offsets, b->atom_offsets + p->ns + 1);
+  if (b->cells) p->cells.assign(b->cells, b->cells + 9 * (size_t)p->ns);
+  if (b->pbc) p->pbc.assign(b->pbc, b->pbc + 3 * (size_t)p->ns);
+  if (b->total_charges) p->charges.assign(b->total_charges, b->total_charges + p->ns);
+  p->x = b->positions;
+  p->types = b->types;
+  p->opt = *opt;
+  p->stg = sg;
+  p->forces = result->forces;
+  p->q = result->charges;
+  p->tbo = result->total_bond_orders;
+  p->nlp = result->lone_pairs;
+  p->bcounts = result->bond_counts;
+  p->qeq_state = result->qeq_state;
+  const size_t n = (size_t)p->n, ns = (size_t)p->ns;
+  size_t off = 0;
+  auto save = [&](const void* at, size_t bytes) {
+    if (!at) return;
+    p->saved.push_back({const_cast<void*>(at), bytes, off});
+    off = (off + bytes + 255) & ~size_t(255);
+  };
+  const bool moves = sg.dt_fs || sg.fire_state;
+  if (moves) save(p->x, 24 * n);
+  if (moves) save(sg.velocities, 24 * n);
+  if (sg.dt_fs) save(p->forces, 24 * n);  // stage 0 reads the previous step's forces
+  if (opt->warm_start || sg.qeq_order) save(p->qeq_state, 16 * n);
+  if (sg.qeq_order) save(sg.qeq_history, 48 * n);
+  if (sg.fire_state) save(sg.fire_state, 32 * ns);
+  if (sg.fire_state) save(sg.fire_status, 4 * ns);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = p->dev_offsets.ensure(sizeof(int32_t) * (ns + 1));
+  if (e == hipSuccess)
+    e = hipMemcpy(p->dev_offsets.ptr, b->atom_offsets, sizeof(int32_t) * (ns + 1), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete p;
+    return hip_fail(ctx, e, "step set-up");
+  }
+  *out = p;
+  return XREAC_OK;
+}
+
+int xreac_step_run(xreac_step* p, xreac_batch_result* r) {
+  if (!p || !r || !r->components || !r->status || !r->qeq_iterations) return XREAC_INVALID;
+  xreac_ctx* ctx = p->ctx;
+  ctx->message.clear();
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  if (p->exec) {
+    HIPCHK(hipGraphLaunch(p->exec, ctx->stream), "step replay");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "step replay");
+    BatchOut bo = step_out(p, r);
+    bool incomplete = false;
+    const int rc = finish_eval(ctx, p->st, p->fin, bo, &incomplete);
+    if (rc != XREAC_OK) return rc;
+    if (!incomplete) {
+      ++p->replays;
+      r->num_edges = bo.num_edges;
+      r->num_bonds = bo.num_bonds;
+      return XREAC_OK;
+    }
+    // put back what the replay changed, then run the step with read-backs
+    for (const auto& v : p->saved)
+      HIPCHK(hipMemcpyAsync(v.at, static_cast<char*>(p->backup.ptr) + v.off, v.bytes, hipMemcpyDeviceToDevice,
+                            ctx->stream),
+             "step restore");
+    ++p->fallbacks;
+  }
+  const int rc = step_unrecorded(p, r);
+  if (rc != XREAC_OK) return rc;
+  return step_record(p);
+}
+
+int xreac_step_stats(const xreac_step* p, int64_t* replays, int64_t* fallbacks, int64_t* recordings) {
+  if (!p) return XREAC_INVALID;
+  if (replays) *replays = p->replays;
+  if (fallbacks) *fallbacks = p->fallbacks;
+  if (recordings) *recordings = p->recordings;
+  return XREAC_OK;
+}
+
+void xreac_step_destroy(xreac_step* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  delete p;
 }
 
 }  // extern "C"

@@ -80,6 +80,7 @@ struct CgArgs {
   int max_iter;      // per system
   int first_check;   // launches before the first read-back
   int check_every;   // launches between later read-backs
+  int fixed_launches;  // > 0: exactly this many iteration launches and no read-back (a recorded solve)
   const double2* rhs;    // optional right-hand sides per atom (default (-chi, -1))
   bool neutral;          // normalise the result to zero sum instead of the systems' total charges
   const double2* guess;  // optional starting (s, t) per atom

@@ -1278,17 +1278,30 @@ hipError_t xr_cg_solve(const CgArgs& a, hipStream_t s, CgOutcome* out) {
     if (e == hipSuccess) pending[slot] = true;
     return e;
   };
-  // A few launches go out before the caller's side-stream work, so the device
-  // is busy while the host issues it, and the rest of the first chunk after.
-  const int lead = a.after_start ? std::min(a.first_check, CG_LEAD_LAUNCHES) : 0;
-  hipError_t e = enqueue(lead);
-  if (e == hipSuccess && a.after_start) e = a.after_start(a.after_start_arg);
-  if (e == hipSuccess) e = issue(0, a.first_check - lead);
-  if (e != hipSuccess) return e;
   auto charges = [&]() {
     hipLaunchKernelGGL(k_qeq_charges, dim3(ns), dim3(1024), 0, s, a.S, a.v.X, a.q, a.mu, a.keep, a.neutral ? 1 : 0);
     return hipGetLastError();
   };
+  // A few launches go out before the caller's side-stream work, so the device
+  // is busy while the host issues it, and the rest of the first chunk after.
+  const int lead = a.after_start ? std::min(a.fixed_launches > 0 ? a.fixed_launches : a.first_check,
+                                            CG_LEAD_LAUNCHES)
+                                 : 0;
+  hipError_t e = enqueue(lead);
+  if (e == hipSuccess && a.after_start) e = a.after_start(a.after_start_arg);
+  if (e != hipSuccess) return e;
+  if (a.fixed_launches > 0) {
+    // A recorded solve: a fixed number of launches and no read-back.  Systems
+    // that settle earlier sit out the rest; the caller reads every system's
+    // final state from the control block afterwards.
+    e = enqueue(a.fixed_launches - lead);
+    if (e == hipSuccess) e = charges();
+    out->sweeps = k;
+    out->tail_done = false;
+    return e;
+  }
+  e = issue(0, a.first_check - lead);
+  if (e != hipSuccess) return e;
   if (a.tail) {
     e = charges();
     if (e == hipSuccess) e = a.tail(a.tail_arg);

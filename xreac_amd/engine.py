@@ -296,6 +296,61 @@ class Engine:
                       num_edges=int(res.num_edges), num_bonds=int(res.num_bonds), offsets=offsets)
         return result
 
+    def record_step(self, positions, types, offsets, cells=None, pbc=None, total_charges=None, *, out, tol=1e-10,
+                    max_iter=2000, lammps_angles=True, precision="fp64", warm_start=False, full_derivative=False,
+                    dt_fs=None, velocities=None, kinetic=None, qeq_order=None, qeq_history=None, fire_state=None,
+                    fire_status=None, fire_force_tolerance=0.0, fire_max_iterations=0):
+        """One step of a fixed batch, recorded as a HIP graph on its first run and replayed after.
+
+        The step runs, in order: velocity-Verlet stage 0 (``dt_fs`` given),
+        QEq extrapolation (``qeq_order`` given), the batch evaluation as
+        ``evaluate_batch`` into ``out``, velocity-Verlet stage 1, and one FIRE
+        iteration (``fire_state`` given).  Every tensor is used in place by
+        every run: contents may change between runs, shapes and storage not.
+        The engine needs its own stream (``Engine(..., stream=torch.cuda.Stream())``).
+        """
+        torch = self.torch
+        if self.stream.cuda_stream == 0:
+            raise ValueError("record_step needs an Engine created with its own stream")
+        offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        ns = len(offsets) - 1
+        n = int(offsets[-1])
+        assert positions.shape == (n, 3) and positions.dtype == torch.float64 and positions.is_contiguous()
+        assert types.shape == (n,) and types.dtype == torch.int32 and types.is_contiguous()
+        keep = [offsets, positions, types, out]
+        cells_arr = pbc_arr = q_arr = None
+        if cells is not None:
+            cells_arr = np.ascontiguousarray(cells, dtype=np.float64).reshape(ns, 9)
+            pbc_arr = np.ones((ns, 3), dtype=np.int32) if pbc is None else np.ascontiguousarray(pbc, dtype=np.int32)
+        if total_charges is not None:
+            q_arr = np.ascontiguousarray(total_charges, dtype=np.float64).reshape(ns)
+        batch = _lib.Batch(ns, _ptr(offsets), _ptr(positions), _ptr(types),
+                           _ptr(cells_arr) if cells_arr is not None else None,
+                           _ptr(pbc_arr) if pbc_arr is not None else None,
+                           _ptr(q_arr) if q_arr is not None else None, None, None, None, 0)
+        res = _lib.BatchResult()
+        res.forces = _ptr(out["forces"])
+        res.charges = _ptr(out["charges"])
+        for name in ("total_bond_orders", "lone_pairs", "bond_counts", "qeq_state"):
+            t = out.get(name)
+            setattr(res, name, _ptr(t) if t is not None else None)
+        stages = _lib.StepStages()
+        for name, t in (("dt_fs", dt_fs), ("velocities", velocities), ("kinetic", kinetic), ("qeq_order", qeq_order),
+                        ("qeq_history", qeq_history), ("fire_state", fire_state), ("fire_status", fire_status)):
+            if t is not None:
+                assert t.is_cuda and t.is_contiguous(), name
+                setattr(stages, name, _ptr(t))
+                keep.append(t)
+        stages.fire_force_tolerance = float(fire_force_tolerance)
+        stages.fire_max_iterations = int(fire_max_iterations)
+        opt = _lib.Options(float(tol), int(max_iter), 1 if lammps_angles else 0, _precision(precision),
+                           1 if warm_start else 0, 1 if full_derivative else 0, 0, 0.0)
+        handle = C.c_void_p()
+        rc = self.lib.xreac_step_create(self._ctx, self._ff, C.byref(batch), C.byref(opt), C.byref(stages),
+                                        C.byref(res), C.byref(handle))
+        self._check(rc, self._ctx)
+        return RecordedStep(self, handle, ns, offsets, out, keep)
+
     def verlet(self, offsets, types, dt_fs, positions, velocities, forces, stage, kinetic=None):
         """One velocity-Verlet stage on device tensors (offsets int32 [ns+1], dt_fs float64 [ns]).
 
@@ -373,3 +428,48 @@ class Engine:
             self.lib.xreac_copy_bonds(self._ctx, _ptr(i), _ptr(j), _ptr(S), _ptr(bo), max(nb, 1)), self._ctx
         )
         return i[:nb], j[:nb], S[:nb], bo[:nb]
+
+
+class RecordedStep:
+    """A batch step recorded as a HIP graph (``Engine.record_step``)."""
+
+    def __init__(self, engine, handle, ns, offsets, out, keep):
+        self.engine, self._h, self.ns, self.offsets, self.out, self._keep = engine, handle, ns, offsets, out, keep
+        self.components = np.zeros((ns, _lib.NCOMP), dtype=np.float64)
+        self.status = np.zeros(ns, dtype=np.int32)
+        self.iters = np.zeros(ns, dtype=np.int32)
+        self._res = _lib.BatchResult()
+        self._res.components = _ptr(self.components)
+        self._res.status = _ptr(self.status)
+        self._res.qeq_iterations = _ptr(self.iters)
+
+    def run(self):
+        """Run the step once (replayed after the first run) and wait; results as ``Engine.evaluate_batch``."""
+        eng = self.engine
+        rc = eng.lib.xreac_step_run(self._h, C.byref(self._res))
+        eng._check(rc, eng._ctx)
+        errors = [None if self.status[k] == _lib.XREAC_OK else eng.lib.xreac_batch_error(eng._ctx, k).decode()
+                  for k in range(self.ns)]
+        result = {k: self.out.get(k) for k in ("forces", "charges", "total_bond_orders", "lone_pairs", "bond_counts",
+                                               "qeq_state")}
+        result.update(components=self.components.copy(), status=self.status.copy(), errors=errors,
+                      qeq_iterations=self.iters.copy(), num_edges=int(self._res.num_edges),
+                      num_bonds=int(self._res.num_bonds), offsets=self.offsets)
+        return result
+
+    def stats(self):
+        """(replays, fallbacks to an unrecorded run, recordings) so far."""
+        r, f, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.engine._check(self.engine.lib.xreac_step_stats(self._h, C.byref(r), C.byref(f), C.byref(c)), None)
+        return r.value, f.value, c.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.engine.lib.xreac_step_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
