@@ -17,7 +17,8 @@ time, cold QEq), Poisson arrivals at three loads with p50/p99 latency, the
 12,288-atom water box (plain FP64, tolerance 1e-10, the roofline of its
 dominant kernel and its error against the committed oracle fixture), the
 3,150-atom CHO mixture and the 24,360-atom anatase/water slab (mixed against
-FP64 at tolerance 1e-10), and the FP64 CPU oracle timed on this host.
+FP64 at tolerance 1e-10), NVE steps of the 192-atom cluster run unrecorded and
+as recorded HIP-graph replays, and the FP64 CPU oracle timed on this host.
 Positions are resident in HBM when every timed region starts.
 """
 
@@ -205,7 +206,8 @@ def stream_bench(args, reqs):
     modes = (("batched_mixed", "mixed", args.mixed_tol), ("batched_fp64", "fp64", args.qeq_tol),
              ("batched_fp64_at_mixed_tol", "fp64", args.mixed_tol))
     for mode, precision, tol in modes:
-        srv = Server(precision=precision, tol=tol, max_batch_atoms=args.stream_batch_atoms)
+        srv = Server(precision=precision, tol=tol, max_batch_atoms=args.stream_batch_atoms,
+                     recorded_steps=args.stream_recorded)
         run(srv, reqs[:32])  # warm-up
         out, stats = run(srv, reqs)
         responses[mode] = {r.rid: r for r in out}
@@ -229,7 +231,7 @@ def stream_bench(args, reqs):
                     "the FP64 oracle at 1e-9 relative energy, 1e-6 forces, 1e-7 charges (pytest -m gpu)",
         }
     # plain: one request at a time, FP64, cold QEq, over a prefix bounded in time
-    plain = Server(precision="fp64", warm_start=False, tol=args.qeq_tol, max_systems=1)
+    plain = Server(precision="fp64", warm_start=False, tol=args.qeq_tol, max_systems=1, recorded_steps=False)
     run(plain, reqs[:4])
     plain.responses, plain.atom_evals, plain.steps = [], 0, 0
     torch.cuda.synchronize()
@@ -254,7 +256,8 @@ def stream_bench(args, reqs):
         rate = load * result["batched_mixed"]["requests_per_s"]
         rng = np.random.default_rng(7)
         arrivals = np.cumsum(rng.exponential(1.0 / rate, len(sub)))
-        srv = Server(precision="mixed", tol=args.mixed_tol, max_batch_atoms=args.stream_batch_atoms)
+        srv = Server(precision="mixed", tol=args.mixed_tol, max_batch_atoms=args.stream_batch_atoms,
+                     recorded_steps=args.stream_recorded)
         run(srv, reqs[:32])
         _, stats = run(srv, sub, arrivals)
         stats.update(arrival_rate_per_s=rate, load=load, qeq_tolerance=args.mixed_tol)
@@ -453,6 +456,85 @@ def system_bench(args, name, ffname, build, steps):
     return result
 
 
+def recorded_bench(args, steps=100, warmup=5):
+    """NVE steps run unrecorded (host-issued launches, QEq read-backs) and as recorded graph replays.
+
+    Each step is velocity-Verlet stage 0, extrapolated QEq guesses, the batch
+    evaluation (FP64, warm start, tolerance 1e-10) and stage 1, as the server
+    runs an NVE segment; the two runs start from the same state and must end
+    bitwise equal.  Cases: the 64-water, 192-atom cluster, alone and as 64
+    seeded copies in one batch.
+    """
+    import torch
+
+    from xreac_amd import ForceField, workloads
+    from xreac_amd.engine import Engine
+
+    ff = ForceField.bundled("ffield.reax.HO.2015")
+    eng = Engine(ff, 0, stream=torch.cuda.Stream(0))
+    out = {}
+    for label, copies in (("water_cluster_192", 1), ("water_cluster_192_x64", 64)):
+        systems = [workloads.water_cluster(seed=k) for k in range(copies)]
+        sym = [a for s, _ in systems for a in s]
+        x0 = np.vstack([x for _, x in systems])
+        offsets = np.array([0] + list(np.cumsum([len(s) for s, _ in systems])), dtype=np.int32)
+        masses = np.array([ff.atoms[a]["mass"] for a in sym])
+        v0 = workloads.maxwell_boltzmann(masses, 300.0, seed=3)
+        n, ns = len(sym), copies
+        finals, rates = {}, {}
+        with torch.cuda.stream(eng.stream):
+            dev = eng.device
+            f64 = dict(dtype=torch.float64, device=dev)
+            for mode in ("unrecorded", "recorded"):
+                x = torch.as_tensor(x0, **f64).contiguous()
+                v = torch.as_tensor(v0, **f64).contiguous()
+                t = torch.as_tensor(eng.type_ids(sym), device=dev)
+                o = eng.batch_outputs(n, qeq_state=True)
+                o["forces"].zero_()
+                qh = torch.zeros((3, n, 2), **f64)
+                kin = torch.zeros(ns, **f64)
+                off = torch.as_tensor(offsets, device=dev)
+                dt = torch.zeros(ns, **f64)
+                order = torch.zeros(ns, dtype=torch.int32, device=dev)
+                # the first evaluation of the segment (nothing moves), then the steps
+                eng.evaluate_batch(x, t, offsets, tol=1e-10, warm_start=True, out=o)
+                dt.fill_(0.25)
+                step = None
+                if mode == "recorded":
+                    step = eng.record_step(x, t, offsets, out=o, tol=1e-10, warm_start=True, dt_fs=dt, velocities=v,
+                                           kinetic=kin, qeq_order=order, qeq_history=qh)
+                t0 = 0.0
+                for k in range(warmup + steps):
+                    if k == warmup:
+                        torch.cuda.synchronize()
+                        t0 = time.perf_counter()
+                    order.fill_(min(k + 1, 4) - 1)
+                    if step is not None:
+                        step.run()
+                    else:
+                        eng.verlet(off, t, dt, x, v, o["forces"], 0)
+                        eng.qeq_extrapolate(off, order, o["qeq_state"], qh)
+                        eng.evaluate_batch(x, t, offsets, tol=1e-10, warm_start=True, out=o)
+                        eng.verlet(off, t, dt, x, v, o["forces"], 1, kin)
+                    kin.cpu()  # the server reads the kinetic energies back every step
+                torch.cuda.synchronize()
+                sec = time.perf_counter() - t0
+                rates[mode] = 1e3 * sec / steps
+                finals[mode] = (x.cpu().numpy(), o["forces"].cpu().numpy())
+                if step is not None:
+                    replays, fallbacks, recordings = step.stats()
+                    step.close()
+        same = all(np.array_equal(a, b) for a, b in zip(finals["unrecorded"], finals["recorded"]))
+        out[label] = {"atoms": n, "systems": ns, "steps": steps,
+                      "ms_per_step_unrecorded": rates["unrecorded"], "ms_per_step_recorded": rates["recorded"],
+                      "speedup": rates["unrecorded"] / rates["recorded"], "bitwise_equal": same,
+                      "replays": replays, "fallbacks": fallbacks, "recordings": recordings}
+        log(f"[recorded] {label}: {rates['unrecorded']:.3f} -> {rates['recorded']:.3f} ms/step "
+            f"(bitwise equal: {same}; replays {replays}, fallbacks {fallbacks})")
+    eng.close()
+    return out
+
+
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc, from profiles/, if recorded."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
@@ -492,6 +574,8 @@ def main():
     ap.add_argument("--no-timers", action="store_true", help="do not bracket kernels with hipEvents")
     ap.add_argument("--no-stream", action="store_true", help="skip the serving stream (value is then the box's)")
     ap.add_argument("--no-systems", action="store_true", help="skip the CHO mixture and the anatase slab")
+    ap.add_argument("--no-recorded", action="store_true", help="skip the recorded-step (HIP graph) NVE comparison")
+    ap.add_argument("--only-recorded", action="store_true", help="measure only the recorded-step NVE comparison")
     ap.add_argument("--only-box", action="store_true", help="measure only the 12,288-atom box")
     ap.add_argument("--only-stream", action="store_true", help="measure only the serving stream (for profiling)")
     ap.add_argument("--stream-requests", type=int, default=20000)
@@ -499,6 +583,8 @@ def main():
     ap.add_argument("--stream-loads", type=float, nargs="+", default=[0.5, 0.7, 0.9],
                     help="Poisson rates as fractions of the saturated request rate")
     ap.add_argument("--poisson-requests", type=int, default=5000)
+    ap.add_argument("--stream-recorded", action="store_true",
+                    help="serve the stream with recorded steps (HIP-graph replays of unchanged groups)")
     ap.add_argument("--plain-seconds", type=float, default=15.0)
     ap.add_argument("--breakdown-steps", type=int, default=2,
                     help="untimed evaluations with every kernel timed, before the timed region")
@@ -521,10 +607,13 @@ def main():
         )
         sys.exit(2)
     if args.only_box:
-        args.no_stream = args.no_systems = args.no_cpu_baseline = True
+        args.no_stream = args.no_systems = args.no_cpu_baseline = args.no_recorded = True
     if args.only_stream:
-        args.no_systems = args.no_cpu_baseline = True
+        args.no_systems = args.no_cpu_baseline = args.no_recorded = True
 
+    if args.only_recorded:
+        print(json.dumps({"recorded_steps": recorded_bench(args)}), flush=True)
+        return
     # Process pools and CPU children start before this process touches the GPU.
     reqs = None if args.no_stream else build_stream(args)
     baseline = None
@@ -546,6 +635,7 @@ def main():
         systems["anatase_slab_24360"] = system_bench(
             args, "anatase_slab_24360: anatase 16x16x5 + 3000 H2O + 15 A vacuum, periodic, ffield.reax.TiOH.Kim2013",
             "ffield.reax.TiOH.Kim2013", workloads.anatase_water_slab, 5)
+    recorded = None if args.no_recorded else recorded_bench(args)
     stream = None if args.no_stream else stream_bench(args, reqs)
 
     if stream is not None:
@@ -588,6 +678,7 @@ def main():
         "stream": stream,
         "water_box_12288": box,
         "systems": systems,
+        "recorded_steps": recorded,
         "cpu_baseline": baseline,
     }
     print(json.dumps(result), flush=True)

@@ -300,6 +300,7 @@ int xreac_step_create(xreac_ctx* ctx, const xreac_ff* ff, const xreac_batch* bat
 int xreac_step_run(xreac_step* step, xreac_batch_result* result);
 /* Replays, fallbacks to an unrecorded run after a replay, and recordings so far. */
 int xreac_step_stats(const xreac_step* step, int64_t* replays, int64_t* fallbacks, int64_t* recordings);
+/* Frees the step's graph and workspaces; it does not use the context. */
 void xreac_step_destroy(xreac_step* step);
 
 /* Kernel timing with hipEvents on the launching stream (no reference

@@ -211,7 +211,8 @@ def test_extrapolated_qeq_guesses_keep_results_and_cut_iterations():
     for extrapolate in (False, True):
         from xreac_amd.serving import Server
 
-        srv = Server(extrapolate=extrapolate, tol=1e-10)
+        # unrecorded steps: the count wraps evaluate_batch, which recorded replays do not call
+        srv = Server(extrapolate=extrapolate, tol=1e-10, recorded_steps=False)
         count = [0]
         for eng in srv.engines.values():
             run = eng.evaluate_batch
@@ -299,3 +300,39 @@ def test_served_relaxation_follows_oracle_fire():
               f"converged={r.converged}", flush=True)
         assert r.converged == converged and r.evaluations == it + 1
         np.testing.assert_allclose(r.positions, x, rtol=0, atol=1e-8)
+
+
+def test_recorded_steps_change_no_result():
+    # NVE segments and relaxations admitted together: the group keeps its
+    # composition for most steps, which then run as recorded graph replays
+    reqs = [r for r in _requests(40, seed=17, md_steps=12, max_atoms=150) if r["md_steps"]][:4]
+    relax = [r for r in _requests(20, seed=18, max_atoms=60) if r["md_steps"] == 0][:2]
+    for r in relax:
+        r["relax"] = True
+    reqs = reqs + relax
+    for k, r in enumerate(reqs):  # the two plans number their requests from 0 each
+        r["rid"] = k
+    print(f"[serving] recorded steps: {len(reqs)} requests", flush=True)
+    out = {}
+    for rec in (True, False):
+        for precision in ("fp64", "mixed"):
+            res, srv = _serve(reqs, precision=precision, relax_iterations=12, recorded_steps=rec)
+            out[rec, precision] = res
+            print(f"[serving] recorded={rec} {precision}: steps {srv.steps}, recorded runs {srv.recorded_runs}",
+                  flush=True)
+            if rec:
+                assert srv.recorded_runs >= 8, srv.recorded_runs
+            else:
+                assert srv.recorded_runs == 0
+    for precision in ("fp64", "mixed"):
+        for req in reqs:
+            a, b = out[True, precision][req["rid"]], out[False, precision][req["rid"]]
+            assert a.error is None and b.error is None, (a.error, b.error)
+            assert a.evaluations == b.evaluations
+            np.testing.assert_array_equal(a.forces, b.forces)
+            np.testing.assert_array_equal(a.charges, b.charges)
+            np.testing.assert_array_equal(a.positions, b.positions)
+            np.testing.assert_array_equal(a.potential, b.potential)
+            if req["md_steps"]:
+                np.testing.assert_array_equal(a.velocities, b.velocities)
+                np.testing.assert_array_equal(a.kinetic, b.kinetic)

@@ -248,6 +248,21 @@ __global__ void k_zero_ranges(ZeroRanges z) {
   }
 }
 
+// Copy up to eight byte ranges (multiples of 4 bytes) in one launch.
+struct CopyRanges {
+  const unsigned* src[8];
+  unsigned* dst[8];
+  long long words[8];
+  int count;
+};
+
+__global__ void k_copy_ranges(CopyRanges c) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (int r = 0; r < c.count; ++r)
+    for (long long w = blockIdx.x * (long long)blockDim.x + threadIdx.x; w < c.words[r]; w += stride)
+      c.dst[r][w] = c.src[r][w];
+}
+
 // Record total = ptr[n] and clamp every pointer to `cap`, so later kernels
 // stay inside buffers sized for `cap` entries even when the total exceeds it.
 template <class T>
@@ -1596,6 +1611,7 @@ int xreac_copy_bonds(xreac_ctx* ctx, int32_t* i, int32_t* j, int32_t* shifts, do
 // unrecorded step gives.
 struct xreac_step {
   xreac_ctx* ctx = nullptr;
+  int device = 0;
   const xreac_ff* ff = nullptr;
   EvalState st;
   int ns = 0, n = 0;
@@ -1652,6 +1668,22 @@ BatchOut step_out(xreac_step* p, xreac_batch_result* r) {
   return bo;
 }
 
+// The arrays a step modifies, to (save) or from (restore) the step's backup, in one launch.
+hipError_t step_backup(xreac_step* p, bool save, hipStream_t s) {
+  CopyRanges c{};
+  for (const auto& v : p->saved) {
+    auto* b = reinterpret_cast<unsigned*>(static_cast<char*>(p->backup.ptr) + v.off);
+    auto* a = static_cast<unsigned*>(v.at);
+    c.src[c.count] = save ? a : b;
+    c.dst[c.count] = save ? b : a;
+    c.words[c.count] = (long long)(v.bytes / 4);
+    ++c.count;
+  }
+  if (c.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_ranges, dim3(256), dim3(256), 0, s, c);
+  return hipGetLastError();
+}
+
 hipError_t stage_before(xreac_step* p, hipStream_t s) {
   const int* off = static_cast<const int*>(p->dev_offsets.ptr);
   hipError_t e = hipSuccess;
@@ -1702,10 +1734,13 @@ int step_record(xreac_step* p) {
   p->exec = nullptr;
   p->graph = nullptr;
   p->st.take_hints(ctx->es);
-  // QEq launches: a quarter above the most any recent solve needed, plus four.
-  int most = 0;
-  for (int h : p->st.cg_history) most = std::max(most, h);
-  p->cg_launches = std::min(most + most / 4 + 4, 4096);
+  // QEq launches: an eighth above what the step's unrecorded run just needed,
+  // plus three (not the most of recent solves: one cold start before a run of
+  // warm ones would set the count for all of them; every launch past the last
+  // system's settling is an idle dispatch in each replay).  A replay that
+  // needs more falls back and records again.
+  const int last = p->st.last_cg_launches;
+  p->cg_launches = std::min(last + last / 8 + 3, 4096);
   size_t need = 0;
   for (const auto& v : p->saved) need = std::max(need, v.off + v.bytes);
   HIPCHK(p->backup.ensure(need + 256), "step backup");
@@ -1721,10 +1756,7 @@ int step_record(xreac_step* p) {
     Capture cap;
     cap.cg_launches = p->cg_launches;
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "step recording");
-    hipError_t e = hipSuccess;
-    for (const auto& v : p->saved)
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(static_cast<char*>(p->backup.ptr) + v.off, v.at, v.bytes, hipMemcpyDeviceToDevice, s);
+    hipError_t e = step_backup(p, true, s);
     if (e == hipSuccess) e = stage_before(p, s);
     bool overflow = false;
     int rc = XREAC_OK;
@@ -1789,6 +1821,7 @@ int xreac_step_create(xreac_ctx* ctx, const xreac_ff* ffh, const xreac_batch* b,
     return fail(ctx, XREAC_INVALID, "a FIRE stage needs velocities, status, a positive tolerance and a limit");
   auto* p = new xreac_step();
   p->ctx = ctx;
+  p->device = ctx->device;
   p->ff = ffh;
   p->ns = b->num_systems;
   p->n = b->atom_offsets[b->num_systems];
@@ -1852,10 +1885,7 @@ int xreac_step_run(xreac_step* p, xreac_batch_result* r) {
       return XREAC_OK;
     }
     // put back what the replay changed, then run the step with read-backs
-    for (const auto& v : p->saved)
-      HIPCHK(hipMemcpyAsync(v.at, static_cast<char*>(p->backup.ptr) + v.off, v.bytes, hipMemcpyDeviceToDevice,
-                            ctx->stream),
-             "step restore");
+    HIPCHK(step_backup(p, false, ctx->stream), "step restore");
     ++p->fallbacks;
   }
   const int rc = step_unrecorded(p, r);
@@ -1873,8 +1903,9 @@ int xreac_step_stats(const xreac_step* p, int64_t* replays, int64_t* fallbacks, 
 
 void xreac_step_destroy(xreac_step* p) {
   if (!p) return;
-  (void)hipSetDevice(p->ctx->device);
-  (void)hipStreamSynchronize(p->ctx->stream);
+  // Every run waits for its work, so nothing of the step is in flight here;
+  // the context is not touched (it may already be gone).
+  (void)hipSetDevice(p->device);
   delete p;
 }
 

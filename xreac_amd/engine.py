@@ -7,6 +7,7 @@ device tensors; PyTorch only supplies device memory and the stream.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -69,11 +70,14 @@ class Engine:
         # the context launches on `stream` (a torch.cuda.Stream; default: the current stream)
         self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
         self._ctx = C.c_void_p()
+        self._steps = weakref.WeakSet()  # recorded steps, closed before the context
         with torch.cuda.device(self.device):
             self._check(self.lib.xreac_ctx_create(device, C.c_void_p(self.stream.cuda_stream), C.byref(self._ctx)),
                         None)
 
     def close(self):
+        for step in list(getattr(self, "_steps", ())):
+            step.close()
         if getattr(self, "_ctx", None):
             self.lib.xreac_ctx_destroy(self._ctx)
             self._ctx = None
@@ -349,7 +353,9 @@ class Engine:
         rc = self.lib.xreac_step_create(self._ctx, self._ff, C.byref(batch), C.byref(opt), C.byref(stages),
                                         C.byref(res), C.byref(handle))
         self._check(rc, self._ctx)
-        return RecordedStep(self, handle, ns, offsets, out, keep)
+        step = RecordedStep(self, handle, ns, offsets, out, keep)
+        self._steps.add(step)
+        return step
 
     def verlet(self, offsets, types, dt_fs, positions, velocities, forces, stage, kinetic=None):
         """One velocity-Verlet stage on device tensors (offsets int32 [ns+1], dt_fs float64 [ns]).

@@ -110,6 +110,14 @@ class _Group:
         self.FS = torch.zeros((0, 4), **f64)
         self.ST = torch.zeros(0, dtype=torch.int32, device=self.dev)
         self._mass = {}  # per force field: element -> mass
+        # Composition: bumped by every append and drop.  A step whose group has
+        # the composition of its previous step runs as a recorded step (one
+        # graph replay), which needs fixed per-step buffers.
+        self.version = 0
+        self.stepped_version = -1
+        self.rec = None
+        self.Q = self.K = self.DT = self.ORD = None
+        self._off, self._off_version = None, -1
 
     def prepare(self, job, ff):
         """Host arrays of one job (positions, type ids, velocities, FIRE state); pure host work."""
@@ -168,6 +176,7 @@ class _Group:
                                                          [int(bool(p)) for p in job.req["pbc"]] for job in jobs],
                                                         dtype=np.int32).reshape(-1, 3)])
         self.charges = np.concatenate([self.charges, [job.req.get("total_charge", 0.0) for job in jobs]])
+        self.version += 1
 
     def drop(self, done):
         """Remove the jobs flagged in ``done`` (bool per job) with one gather per array."""
@@ -187,6 +196,19 @@ class _Group:
         self.jobs = [j for j, k in zip(self.jobs, keep) if k]
         for name in ("sizes", "md", "relax", "dt", "cells", "pbcs", "charges"):
             setattr(self, name, getattr(self, name)[keep])
+        self.version += 1
+
+    def offsets_dev(self, offsets):
+        """Device copy of the packed offsets, made once per composition."""
+        if self._off_version != self.version:
+            self._off = self.torch.as_tensor(offsets, device=self.dev)
+            self._off_version = self.version
+        return self._off
+
+    def forget_step(self):
+        if self.rec is not None:
+            self.rec.close()
+            self.rec = None
 
 
 class Server:
@@ -200,7 +222,11 @@ class Server:
     reference ASE calculator's neighbor_skin) and reuse it while the group's
     batch keeps its shape and no atom has moved further (results bitwise those
     of fresh lists; pays for NVE-heavy work, while a batch that changes at
-    every step rebuilds every step); max_systems: cap on requests per batch (1 gives
+    every step rebuilds every step); recorded_steps: a group whose requests are
+    the same as at its previous step runs the step as a recorded HIP graph
+    (``Engine.record_step``; results bitwise those of the unrecorded step; off
+    by default: replays measured slower than host-issued steps);
+    max_systems: cap on requests per batch (1 gives
     the one-request-at-a-time plain path); budget_bytes / max_batch_atoms bound
     the active set; full_derivative: charge-response forces (the exact gradient
     of the QEq-equilibrated energy, calculator.py:79-86) for every request.
@@ -210,8 +236,8 @@ class Server:
 
     def __init__(self, force_fields=("ffield.reax.HO.2015", "ffield.reax.CHO.2008"), device=0, *,
                  precision="fp64", warm_start=True, extrapolate=True, neighbor_skin=0.0, tol=1e-10,
-                 max_systems=None, max_batch_atoms=262144, budget_bytes=32 << 30, relax_tolerance=1e-4, relax_iterations=500, full_derivative=False,
-                 clock=time.perf_counter):
+                 max_systems=None, max_batch_atoms=262144, budget_bytes=32 << 30, relax_tolerance=1e-4,
+                 relax_iterations=500, full_derivative=False, recorded_steps=False, clock=time.perf_counter):
         import torch
 
         self.torch = torch
@@ -233,6 +259,8 @@ class Server:
         self.relax_tolerance = relax_tolerance
         self.relax_iterations = relax_iterations
         self.full_derivative = bool(full_derivative)
+        self.recorded_steps = bool(recorded_steps)
+        self.recorded_runs = 0  # group steps run as recorded steps (graph replays after the first)
         self.clock = clock
         self.queues = {k: deque() for k in self.engines}  # per force field, in arrival order
         self.active = {k: _Group(torch, eng) for k, eng in self.engines.items()}
@@ -352,30 +380,58 @@ class Server:
         n = int(offsets[-1])
         evals = np.fromiter((j.evals for j in g.jobs), dtype=np.int64, count=ns)
         moving = g.md & (evals > 0)
-        off_dev = torch.as_tensor(offsets, device=dev) if (g.md.any() or g.relax.any()) else None
-        dt_dev = torch.as_tensor(np.where(moving, g.dt, 0.0), dtype=torch.float64, device=dev) if g.md.any() else None
-        if moving.any():
-            eng.verlet(off_dev, g.T, dt_dev, g.X, g.V, g.F, 0)
-            if self.warm_start and self.extrapolate:
-                # degree min(evaluations, 4) - 1 through the segment's solutions so far (fixed dt)
-                order = np.where(moving, np.minimum(evals, 4) - 1, 0).astype(np.int32)
-                eng.qeq_extrapolate(off_dev, torch.as_tensor(order, device=dev), g.QS, g.QH)
-        out = dict(forces=torch.empty((n, 3), dtype=torch.float64, device=dev),
-                   charges=torch.empty(n, dtype=torch.float64, device=dev),
-                   total_bond_orders=None, lone_pairs=None, bond_counts=None, qeq_state=g.QS)
-        res = eng.evaluate_batch(g.X, g.T, offsets, g.cells, g.pbcs, g.charges, tol=self.tol,
-                                 precision=self.precision, warm_start=self.warm_start,
-                                 full_derivative=self.full_derivative, out=out, neighbor_skin=self.neighbor_skin)
-        kin = None
-        if g.md.any():
-            kin_t = torch.zeros(ns, dtype=torch.float64, device=dev)
-            eng.verlet(off_dev, g.T, dt_dev, g.X, g.V, out["forces"], 1, kin_t)
-            kin = kin_t.cpu().numpy()
-        fire_status = None
-        if g.relax.any():
-            eng.fire_step(off_dev, g.X, g.V, out["forces"], g.FS, g.ST, self.relax_tolerance, self.relax_iterations)
-            fire_status = g.ST.cpu().numpy()
-        g.F = out["forces"]
+        md_any, relax_any = bool(g.md.any()), bool(g.relax.any())
+        extrap = self.warm_start and self.extrapolate
+        order = np.where(moving, np.minimum(evals, 4) - 1, 0).astype(np.int32)
+        dt = np.where(moving, g.dt, 0.0)
+        stable = g.version == g.stepped_version
+        g.stepped_version = g.version
+        if not stable:  # a new composition: fresh per-step buffers, no recorded step
+            g.forget_step()
+            f64 = dict(dtype=torch.float64, device=dev)
+            g.Q = torch.empty(n, **f64)
+            g.K = torch.zeros(ns, **f64)
+            g.DT = torch.zeros(ns, **f64)
+            g.ORD = torch.zeros(ns, dtype=torch.int32, device=dev)
+        g.DT.copy_(torch.from_numpy(dt))
+        g.ORD.copy_(torch.from_numpy(order))
+        out = dict(forces=g.F, charges=g.Q, total_bond_orders=None, lone_pairs=None, bond_counts=None,
+                   qeq_state=g.QS)
+        # The same composition as the last step: the step runs recorded (its
+        # first run records it, later ones replay it).  With every MD request
+        # past its first evaluation the recorded verlet and extrapolation
+        # stages are exactly the ones the unrecorded step below issues.
+        recorded = (stable and self.recorded_steps and self.neighbor_skin == 0.0 and
+                    (not md_any or bool(moving[g.md].all())))
+        if recorded:
+            if g.rec is None:
+                g.rec = eng.record_step(
+                    g.X, g.T, offsets, g.cells, g.pbcs, g.charges, out=out, tol=self.tol, precision=self.precision,
+                    warm_start=self.warm_start, full_derivative=self.full_derivative,
+                    dt_fs=g.DT if md_any else None, velocities=g.V if (md_any or relax_any) else None,
+                    kinetic=g.K if md_any else None, qeq_order=g.ORD if (md_any and extrap) else None,
+                    qeq_history=g.QH if (md_any and extrap) else None, fire_state=g.FS if relax_any else None,
+                    fire_status=g.ST if relax_any else None, fire_force_tolerance=self.relax_tolerance,
+                    fire_max_iterations=self.relax_iterations)
+            res = g.rec.run()
+            self.recorded_runs += 1
+        else:
+            g.forget_step()
+            if moving.any():
+                eng.verlet(g.offsets_dev(offsets), g.T, g.DT, g.X, g.V, g.F, 0)
+                if extrap:
+                    # degree min(evaluations, 4) - 1 through the segment's solutions so far (fixed dt)
+                    eng.qeq_extrapolate(g.offsets_dev(offsets), g.ORD, g.QS, g.QH)
+            res = eng.evaluate_batch(g.X, g.T, offsets, g.cells, g.pbcs, g.charges, tol=self.tol,
+                                     precision=self.precision, warm_start=self.warm_start,
+                                     full_derivative=self.full_derivative, out=out, neighbor_skin=self.neighbor_skin)
+            if md_any:
+                eng.verlet(g.offsets_dev(offsets), g.T, g.DT, g.X, g.V, g.F, 1, g.K)
+            if relax_any:
+                eng.fire_step(g.offsets_dev(offsets), g.X, g.V, g.F, g.FS, g.ST, self.relax_tolerance,
+                              self.relax_iterations)
+        kin = g.K.cpu().numpy() if md_any else None
+        fire_status = g.ST.cpu().numpy() if relax_any else None
         comps = res["components"]
         pot = comps.sum(axis=1)
         errors = res["errors"]
