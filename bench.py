@@ -503,6 +503,8 @@ def recorded_bench(args, steps=100, warmup=5):
                 if mode == "recorded":
                     step = eng.record_step(x, t, offsets, out=o, tol=1e-10, warm_start=True, dt_fs=dt, velocities=v,
                                            kinetic=kin, qeq_order=order, qeq_history=qh)
+                torch.cuda.synchronize()
+                time.sleep(0.2)  # a gap that separates the modes in a kernel trace
                 t0 = 0.0
                 for k in range(warmup + steps):
                     if k == warmup:

@@ -79,7 +79,7 @@ def test_recorded_nve_matches_unrecorded(precision, tol):
     step = rec.record_step(B.x, B.t, offsets, cells, pbc, charges, out=B.out, tol=tol, precision=precision,
                            warm_start=True, dt_fs=dt_b, velocities=B.v, kinetic=kin_b, qeq_order=order_b,
                            qeq_history=B.qh)
-    steps = 12
+    steps = 20
     for e in range(steps):
         moving = e > 0
         dt = torch.as_tensor(dt_host if moving else np.zeros(ns), dtype=torch.float64, device=dev)
@@ -107,7 +107,9 @@ def test_recorded_nve_matches_unrecorded(precision, tol):
         assert torch.equal(B.x, A.x) and torch.equal(B.v, A.v), e
         assert torch.equal(kin_b, kin_a), e
     replays, fallbacks, recordings = step.stats()
-    assert (replays, fallbacks, recordings) == (steps - 1, 0, 1)
+    # recorded at the first run; recorded again with fewer QEq launches once
+    # eight replays settled well inside the first recording's count
+    assert (replays, fallbacks) == (steps - 1, 0) and recordings >= 1
     step.close()
 
 

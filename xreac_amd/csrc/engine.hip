@@ -1639,6 +1639,8 @@ struct xreac_step {
   hipGraphExec_t exec = nullptr;
   Finish fin;
   int cg_launches = 0;
+  int need_max = 0;      // most QEq launches a replay needed since the last recording
+  int since_record = 0;  // replays since the last recording
   int64_t replays = 0, fallbacks = 0, recordings = 0;
   ~xreac_step() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -1723,27 +1725,36 @@ int step_unrecorded(xreac_step* p, xreac_batch_result* r) {
   return XREAC_OK;
 }
 
-// Record the step from the context's latest sizing history.  A workspace that
-// would have to grow while the stream records is grown first and the
-// recording repeated (each buffer grows at most once per recording).
-int step_record(xreac_step* p) {
+// QEq launches of a recording that most recently needed `need`: an eighth
+// more plus three.  Every launch past the last system's settling is an idle
+// dispatch in each replay; a replay that needs more falls back and records
+// again.
+int recorded_launches(int need) { return std::min(need + need / 8 + 3, 4096); }
+
+// Record the step, sized from the context's latest sizing history (after an
+// unrecorded run on it) or from the step's own (replays since the last
+// recording; `need` then gives the QEq launches).  A workspace that would
+// have to grow while the stream records is grown first and the recording
+// repeated (each buffer grows at most once per recording).
+int step_record(xreac_step* p, int need = -1) {
   xreac_ctx* ctx = p->ctx;
   hipStream_t s = ctx->stream;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   p->exec = nullptr;
   p->graph = nullptr;
-  p->st.take_hints(ctx->es);
-  // QEq launches: an eighth above what the step's unrecorded run just needed,
-  // plus three (not the most of recent solves: one cold start before a run of
-  // warm ones would set the count for all of them; every launch past the last
-  // system's settling is an idle dispatch in each replay).  A replay that
-  // needs more falls back and records again.
-  const int last = p->st.last_cg_launches;
-  p->cg_launches = std::min(last + last / 8 + 3, 4096);
-  size_t need = 0;
-  for (const auto& v : p->saved) need = std::max(need, v.off + v.bytes);
-  HIPCHK(p->backup.ensure(need + 256), "step backup");
+  if (need < 0) {
+    // from the unrecorded run just made (not the most of recent solves: one
+    // cold start before a run of warm ones would set the count for all)
+    p->st.take_hints(ctx->es);
+    need = p->st.last_cg_launches;
+  }
+  p->cg_launches = recorded_launches(need);
+  p->need_max = 0;
+  p->since_record = 0;
+  size_t backup_bytes = 0;
+  for (const auto& v : p->saved) backup_bytes = std::max(backup_bytes, v.off + v.bytes);
+  HIPCHK(p->backup.ensure(backup_bytes + 256), "step backup");
   BatchIn in = step_batch(p);
   std::vector<double> comps((size_t)p->ns * NCOMP);
   std::vector<int32_t> status(p->ns), iters(p->ns);
@@ -1882,6 +1893,15 @@ int xreac_step_run(xreac_step* p, xreac_batch_result* r) {
       ++p->replays;
       r->num_edges = bo.num_edges;
       r->num_bonds = bo.num_bonds;
+      // Eight replays that all settled well inside the recorded launch count
+      // (the recording run was the slowest, e.g. the first warm step of a
+      // segment): record again with fewer launches.
+      p->need_max = std::max(p->need_max, p->st.last_cg_launches);
+      if (++p->since_record >= 8) {
+        if (recorded_launches(p->need_max) < p->cg_launches) return step_record(p, p->need_max);
+        p->need_max = 0;
+        p->since_record = 0;
+      }
       return XREAC_OK;
     }
     // put back what the replay changed, then run the step with read-backs
