@@ -455,7 +455,8 @@ struct CtlLayout {
 CtlLayout ctl_layout(int ns) {
   CtlLayout c{};
   size_t off = 0;
-  c.flags = carve_at<unsigned>(off, 16);  // [0] buffer-overflow bit, int [1] edges, int [2] bonds
+  c.flags = carve_at<unsigned>(off, 16);  // [0] buffer-overflow bit, int [1] edges, [2] bonds, [3] longest row,
+                                          // [5], [6] skin edges and longest row, [8] torsion long-atom count
   c.serr = carve_at<unsigned>(off, ns);
   c.comps = carve_at<double>(off, (size_t)ns * NCH);  // channel sums; components on the host
   c.bounds = carve_at<double>(off, (size_t)ns * 6);
@@ -803,6 +804,7 @@ int evaluate_once(xreac_ctx* ctx, EvalState& st, const xreac_ff* ffh, const Batc
   carve_size<double>(off, (size_t)kAtomScalars * n);  // atom scalars
   carve_size<int>(off, n);               // bond counts (>0.3)
   carve_size<int>(off, n);               // H-bond candidates per row
+  carve_size<int>(off, n);               // atoms with more bonds than a torsion lane group takes
   carve_size<double>(off, 6 * (size_t)n);   // F_nb, F_hb
   carve_size<double>(off, (size_t)NCH * n);  // per-atom energy channels
   carve_size<double2>(off, full ? n : 0);    // charge-response right-hand sides (dE/dq, -1)
@@ -832,6 +834,7 @@ int evaluate_once(xreac_ctx* ctx, EvalState& st, const xreac_ff* ffh, const Batc
   double* const scal_end = scal + (size_t)kAtomScalars * n;
   int* bcount03 = cv.take<int>(n);
   int* hb_count = cv.take<int>(n);
+  int* tor_long = cv.take<int>(n);
   double* fbuf = cv.take<double>(6 * (size_t)n);
   double* energies = cv.take<double>((size_t)NCH * n);
   double2* dEdq = full ? cv.take<double2>(n) : nullptr;
@@ -1188,7 +1191,8 @@ int evaluate_once(xreac_ctx* ctx, EvalState& st, const xreac_ff* ffh, const Batc
     XR_TIMED(XT_BOND_FORWARD, s2, e = xr_bond_forward(at, bd, ff, nbr_j, nbr_s, acc, s2));
     if (e != hipSuccess) return e;
     side_what = "many-body";
-    XR_TIMED(XT_MANYBODY, s2, e = xr_manybody(at, bd, ff, row_ptr, hb_cand, hb_count, lammps, acc, s2));
+    XR_TIMED(XT_MANYBODY, s2,
+             e = xr_manybody(at, bd, ff, row_ptr, hb_cand, hb_count, lammps, acc, tor_long, d_flags + 8, s2));
     if (e != hipSuccess) return e;
     side_what = "adjoint";
     XR_TIMED(XT_BACKWARD, s2, e = xr_bond_backward(at, bd, ff, s2));

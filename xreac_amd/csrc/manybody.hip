@@ -327,13 +327,8 @@ __device__ inline int torsion_key(const Atoms& at, const Bonds& bd, const FFDev&
 // chunks.
 constexpr int TOR_GROUP = 8;
 template <int G>
-__global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc) {
-  const int x = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / G) + ((threadIdx.x & 63) / G);
-  const int lane = threadIdx.x & (G - 1);
-  if (x >= at.n) return;
-  const int p0 = bd.ptr[x], p1 = bd.ptr[x + 1];
-  // atoms with at most TOR_GROUP bonds go to the 8-lane groups, the rest to whole waves
-  if (G < 64 ? p1 - p0 > G : p1 - p0 <= TOR_GROUP) return;
+__device__ inline void torsion_atom(const Atoms& at, const Bonds& bd, const FFDev& ff, const EnergyAcc& acc, int x,
+                                    int lane, int p0, int p1) {
   double e_tor = 0.0, e_conj = 0.0, a_db = 0.0;
   unsigned err = 0;
   for (int chunk = p0; chunk < p1; chunk += G) {
@@ -431,6 +426,35 @@ __global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, 
     if (err) xr_flag(at.S, x, err);
     xr_energy(acc, CH_TOR, x, e_tor);
     xr_energy(acc, CH_CONJ, x, e_conj);
+  }
+}
+
+// Atoms with at most TOR_GROUP bonds, eight per wave.  An atom with more is
+// appended to `long_atoms` (order immaterial: each atom's work writes only its
+// own slots) for k_torsions_long, so that launch does not sweep every atom.
+__global__ void __launch_bounds__(256) k_torsions(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc,
+                                                  int* __restrict__ long_atoms, unsigned* __restrict__ long_count) {
+  constexpr int G = TOR_GROUP;
+  const int x = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / G) + ((threadIdx.x & 63) / G);
+  const int lane = threadIdx.x & (G - 1);
+  if (x >= at.n) return;
+  const int p0 = bd.ptr[x], p1 = bd.ptr[x + 1];
+  if (p1 - p0 > G) {
+    if (lane == 0) long_atoms[atomicAdd(long_count, 1u)] = x;
+    return;
+  }
+  torsion_atom<G>(at, bd, ff, acc, x, lane, p0, p1);
+}
+
+// The listed atoms, a whole wave each; every wave strides over the list.
+__global__ void __launch_bounds__(256) k_torsions_long(Atoms at, Bonds bd, FFDev ff, EnergyAcc acc,
+                                                       const int* __restrict__ long_atoms,
+                                                       const unsigned* __restrict__ long_count) {
+  const int lane = threadIdx.x & 63;
+  const int count = (int)*long_count;
+  for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < count; w += gridDim.x * 4) {
+    const int x = long_atoms[w];
+    torsion_atom<64>(at, bd, ff, acc, x, lane, bd.ptr[x], bd.ptr[x + 1]);
   }
 }
 
@@ -594,16 +618,19 @@ inline dim3 grid(int count) { return dim3(count > 0 ? (count + BLK - 1) / BLK : 
 }  // namespace
 
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int2* hb_cand,
-                       const int* hb_count, int lammps_angles, EnergyAcc acc, hipStream_t s) {
+                       const int* hb_count, int lammps_angles, EnergyAcc acc, int* long_atoms,
+                       unsigned* long_count, hipStream_t s) {
   const int angle_groups = 64 / ANG_GROUP;
   hipLaunchKernelGGL(k_angles, dim3(at.n > 0 ? (at.n + angle_groups - 1) / angle_groups : 1), dim3(64), 0, s, at, bd,
                      ff, lammps_angles, acc);
   hipLaunchKernelGGL(k_atom_bomax, grid(at.n), dim3(BLK), 0, s, at, bd);
   const dim3 wgrid((at.n + 3) / 4 > 0 ? (at.n + 3) / 4 : 1);
   const int per_block = 4 * (64 / TOR_GROUP);
-  hipLaunchKernelGGL(k_torsions<TOR_GROUP>, dim3(at.n > 0 ? (at.n + per_block - 1) / per_block : 1), dim3(256), 0,
-                     s, at, bd, ff, acc);
-  hipLaunchKernelGGL(k_torsions<64>, wgrid, dim3(256), 0, s, at, bd, ff, acc);
+  hipLaunchKernelGGL(k_torsions, dim3(at.n > 0 ? (at.n + per_block - 1) / per_block : 1), dim3(256), 0, s, at, bd,
+                     ff, acc, long_atoms, long_count);
+  // at most 4,096 waves stride over the atoms with more than TOR_GROUP bonds
+  const int long_blocks = std::max(1, std::min((at.n + 3) / 4, 1024));
+  hipLaunchKernelGGL(k_torsions_long, dim3(long_blocks), dim3(256), 0, s, at, bd, ff, acc, long_atoms, long_count);
   hipLaunchKernelGGL(k_hbonds, wgrid, dim3(256), 0, s, at, bd, ff, row_ptr, hb_cand, hb_count, acc);
   XR_LAUNCH_CHECK();
   return hipSuccess;

@@ -65,7 +65,7 @@ hipError_t xr_force_assembly(const Atoms& at, const Bonds& bd, double* forces, h
 // manybody.hip
 // hb_cand / hb_count: the edge pass's H-bond candidate lists (xr_edge_prep).
 hipError_t xr_manybody(const Atoms& at, const Bonds& bd, FFDev ff, const int* row_ptr, const int2* hb_cand,
-                       const int* hb_count, int lammps_angles, EnergyAcc acc, hipStream_t s);
+                       const int* hb_count, int lammps_angles, EnergyAcc acc, int* long_atoms, unsigned* long_count, hipStream_t s);
 // pairs.hip
 // van der Waals (positions only; second stream): forces written into F_hb, before the H-bond kernels add theirs.
 hipError_t xr_vdw(const Atoms& at, FFDev ff, const int* row_ptr, const int* nbr_j, const int* nbr_s, EnergyAcc acc,
