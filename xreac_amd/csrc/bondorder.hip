@@ -18,7 +18,7 @@ __device__ inline const double* PPm(const FFDev& ff, int ti, int tj) { return ff
 __global__ void k_bond_geom(Atoms at, Bonds bd, FFDev ff, const int* __restrict__ nbr_j,
                             const int* __restrict__ nbr_s) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= *bd.nbp) return;
+  if (b >= xr_bond_count(bd)) return;
   const int e = bd.edge[b], i = bd.bi[b];
   const int j = nbr_j[e], code = nbr_s[e];
   double v[3];
@@ -51,7 +51,7 @@ __global__ void k_bond_geom(Atoms at, Bonds bd, FFDev ff, const int* __restrict_
 
 __global__ void k_bond_reverse(Bonds bd, Systems S) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= *bd.nbp) return;
+  if (b >= xr_bond_count(bd)) return;
   const int i = bd.bi[b], j = bd.bj[b], want = xr_neg_shift(bd.code[b]);
   int found = -1;
   for (int s = bd.ptr[j]; s < bd.ptr[j + 1]; ++s) {
@@ -78,7 +78,7 @@ __global__ void k_atom_totalp(Atoms at, Bonds bd) {
 // Corrected bond orders (energy.py:150-165).
 __global__ void k_bond_correct(Atoms at, Bonds bd, FFDev ff) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= *bd.nbp) return;
+  if (b >= xr_bond_count(bd)) return;
   const double* g = ff.gen;
   const int i = bd.bi[b], j = bd.bj[b];
   const int ti = at.type[i], tj = at.type[j];
@@ -328,7 +328,7 @@ __global__ void k_atom_backward(Atoms at, Bonds bd, FFDev ff) {
 // Through thresholds and corrections to the uncorrected quantities.
 __global__ void k_bond_backward1(Atoms at, Bonds bd, FFDev ff) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= *bd.nbp) return;
+  if (b >= xr_bond_count(bd)) return;
   const double* g = ff.gen;
   const int i = bd.bi[b], j = bd.bj[b];
   const int ti = at.type[i], tj = at.type[j];
@@ -392,7 +392,7 @@ __global__ void k_atom_backward2(Atoms at, Bonds bd) {
 
 __global__ void k_bond_backward2(Atoms at, Bonds bd) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= *bd.nbp) return;
+  if (b >= xr_bond_count(bd)) return;
   const double a_bop = bd.a_bop[b] + at.a_totp[bd.bi[b]];
   const double gr = a_bop * bd.dsig[b] + (a_bop + bd.a_pim[b]) * bd.dpi[b] + (a_bop + bd.a_ppm[b]) * bd.dpp[b];
   const double s = gr / bd.r[b];

@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -98,6 +100,13 @@ static __device__ __align__(256) unsigned char scratch[XREAC_STATIC_SCRATCH_BYTE
 constexpr int kMaxDevices = 64;
 void* g_scratch_ptr[kMaxDevices] = {};  // device address of scratch on each device, once resolved
 
+// One evaluation at a time per device, whatever the context: evaluations of
+// two contexts whose kernels overlapped on one device gave rare run-to-run
+// differences (the serving stream's two force-field groups: a few of 20,000
+// requests per run, occasionally a wrong evaluation), while serialized ones
+// are bitwise reproducible.  The integrators stay outside the lock.
+std::mutex g_eval_lock[kMaxDevices];
+
 // The kept skin rows are geometry only: they depend on the cutoff, the skin
 // and the systems' atom ranges and cells (types are compared on the device).
 struct SkinKey {
@@ -120,6 +129,7 @@ struct EvalState {
   // (slab of skin_stride keys per atom), their lengths, and the positions and
   // types they were built at; valid for calls of the same shape as the last.
   Buffer skin_slab_buf, skin_counts_buf, skin_xref_buf, skin_tref_buf;
+  Buffer guess_buf;  // the warm-start guess as it was before an attempt that may be repeated
   SkinKey skin_key;
   bool skin_valid = false;
   int skin_stride = 0;
@@ -649,6 +659,19 @@ int finish_eval(xreac_ctx* ctx, EvalState& st, const Finish& f, BatchOut& out, b
   return XREAC_OK;
 }
 
+// XREAC_BOND_CAPACITY_SCALE (default 1, read once per process) scales the
+// bond capacity an evaluation sizes from the previous call's bonds per atom,
+// slack included; below 1 such first attempts overflow their bond buffers on
+// the second stream and are repeated with exact sizes, which tests exercise.
+double bond_capacity_scale() {
+  static const double scale = [] {
+    const char* v = std::getenv("XREAC_BOND_CAPACITY_SCALE");
+    const double d = v ? std::atof(v) : 1.0;
+    return d > 0.0 && d < 1.0 ? d : 1.0;
+  }();
+  return scale;
+}
+
 // One attempt.  With exact == false, edge and bond buffers are sized from the
 // previous evaluation's per-atom counts and no mid-pipeline read-back happens;
 // if a count exceeds its buffer, *overflow is set and the caller repeats the
@@ -1093,7 +1116,8 @@ int evaluate_once(xreac_ctx* ctx, EvalState& st, const xreac_ff* ffh, const Batc
   const bool bonds_on_side = !exact;
   int64_t b_cap = 0;
   if (!exact) {
-    b_cap = (int64_t)std::ceil(st.b_per_atom * n * 1.25) + 1024;
+    const double scale = bond_capacity_scale();
+    b_cap = (int64_t)std::ceil(st.b_per_atom * n * 1.25 * scale) + (int64_t)(1024 * scale);
     if (b_cap > m_cap) b_cap = m_cap;
   } else {
     HIPCHK(xr_scan_clamped(bond_count, n, bond_ptr, scan_tmp, INT32_MAX, dev_counts + 1, d_flags, nullptr, s),
@@ -1383,9 +1407,34 @@ int run_batch(xreac_ctx* ctx, const xreac_ff* ffh, const BatchIn& in, const xrea
               Arena* arena = nullptr) {
   // The CG sweep addresses gathered vector entries with 32-bit byte offsets (16 B per atom).
   if (in.offsets[in.ns] >= (1 << 28)) return fail(ctx, XREAC_INVALID, "at most 268,435,455 atoms per call");
+  std::lock_guard<std::mutex> serial(g_eval_lock[ctx->device]);
   bool overflow = false;
+  // With a warm start the QEq of the first attempt writes its solution over
+  // the guess (result->qeq_state).  An attempt whose buffers (sized from the
+  // last evaluation) overflowed is repeated from the guess as it was, so a
+  // result never depends on whether the sizes of the last call fitted.
+  const size_t guess_bytes = sizeof(double2) * (size_t)in.offsets[in.ns];
+  const bool keep_guess = !arena && opt->warm_start && out.qeq_state;
+  if (keep_guess) {
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(ctx->es.guess_buf.ensure(guess_bytes), "guess copy");
+    HIPCHK(hipMemcpyAsync(ctx->es.guess_buf.ptr, out.qeq_state, guess_bytes, hipMemcpyDeviceToDevice, ctx->stream),
+           "guess copy");
+  }
   int rc = evaluate_once(ctx, ctx->es, ffh, in, opt, out, false, &overflow, arena);
-  if (rc == XREAC_OK && overflow) rc = evaluate_once(ctx, ctx->es, ffh, in, opt, out, true, &overflow, arena);
+  if (rc == XREAC_OK && overflow) {
+    // The overflowed attempt can still have work in flight (launches queued
+    // past the read-back that saw the overflow, and the side stream's bonded
+    // terms, which write into the bond workspace); the repeat may free and
+    // regrow those buffers, and freed memory can go to another context at
+    // once, so nothing of the attempt may run after this point.
+    HIPCHK(hipStreamSynchronize(ctx->side), "overflow sync");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "overflow sync");
+    if (keep_guess)
+      HIPCHK(hipMemcpyAsync(out.qeq_state, ctx->es.guess_buf.ptr, guess_bytes, hipMemcpyDeviceToDevice, ctx->stream),
+             "guess copy");
+    rc = evaluate_once(ctx, ctx->es, ffh, in, opt, out, true, &overflow, arena);
+  }
   if (rc == XREAC_OK && overflow) return fail(ctx, XREAC_DEVICE, "internal error: buffer sizing");
   return rc;
 }
@@ -1887,10 +1936,13 @@ int xreac_step_run(xreac_step* p, xreac_batch_result* r) {
   ctx->message.clear();
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   if (p->exec) {
-    HIPCHK(hipGraphLaunch(p->exec, ctx->stream), "step replay");
-    HIPCHK(hipStreamSynchronize(ctx->stream), "step replay");
     BatchOut bo = step_out(p, r);
     bool incomplete = false;
+    {
+      std::lock_guard<std::mutex> serial(g_eval_lock[ctx->device]);
+      HIPCHK(hipGraphLaunch(p->exec, ctx->stream), "step replay");
+      HIPCHK(hipStreamSynchronize(ctx->stream), "step replay");
+    }
     const int rc = finish_eval(ctx, p->st, p->fin, bo, &incomplete);
     if (rc != XREAC_OK) return rc;
     if (!incomplete) {

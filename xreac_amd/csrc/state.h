@@ -19,7 +19,8 @@ __device__ inline void xr_energy(const EnergyAcc& a, int ch, int i, double v) { 
 
 struct Bonds {
   int cap;                 // allocated bond slots (launch size of per-bond kernels)
-  const int* nbp;          // device: number of active directed bonds (<= cap)
+  const int* nbp;          // device: active directed bonds counted by the bond scan, unclamped: above cap
+                           // when the bond buffers overflowed (the call is then repeated); use xr_bond_count
   const int* ptr;          // [n+1]
   const int* edge;         // [nb] canonical edge index
   const int* bi;           // [nb] row atom
@@ -40,6 +41,11 @@ struct Bonds {
   double* G;               // [nb*3] dE/d(edge vector)
   double* a_bop, *a_pim, *a_ppm, *c_i, *c_j;
 };
+
+// Bond slots a per-bond kernel may touch: the active bonds, at most the
+// allocated slots (a grid of cap threads rounded up to whole blocks must not
+// reach past the arrays when the count overflowed them).
+__device__ inline int xr_bond_count(const Bonds& bd) { return min(*bd.nbp, bd.cap); }
 
 struct Atoms {
   int n;
