@@ -11,7 +11,11 @@
  * Plain pointers and sizes only.  Arrays marked "device" live in GPU memory
  * (hipMalloc or a framework allocator); arrays marked "host" in CPU memory.
  * All calls on one context are issued on the context's stream and are not
- * thread-safe with respect to each other.
+ * thread-safe with respect to each other.  Calls on different contexts may
+ * come from different threads; the library runs one evaluation (including a
+ * recorded step's replay) at a time per device and returns to the other
+ * threads' calls as each finishes, so such calls overlap only in host work
+ * and in the integrator kernels.
  */
 #ifndef XREAC_H
 #define XREAC_H
