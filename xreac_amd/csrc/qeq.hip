@@ -522,12 +522,13 @@ struct Batch {
   HV h[U];
 };
 
-// Entries per lane and batch: 8 in the 1024-thread shape, which holds one
-// block per CU whatever its register count; 4 in the smaller shapes, whose
-// freed registers let a CU hold 5-6 waves per SIMD instead of 4.
+// Entries per lane and batch: 4 in every shape.  The smaller shapes then
+// hold 5-6 waves per SIMD instead of 4; the 1024-thread block (85 VGPRs in
+// FP64 against 125 with 8 entries) leaves room on its CU for the side
+// stream's waves, so it no longer waits for a whole CU to drain.
 template <int BLOCK>
 constexpr int batch_entries() {
-  return BLOCK == 1024 ? 8 : 4;
+  return 4;
 }
 
 // Columns are the 4-byte neighbor indices or, in batches of at most 65,536
